@@ -1,0 +1,162 @@
+#!/usr/bin/env python3
+"""EpsilonDivDiv apply throughput on one MI355X (GDoF/s).
+
+Workload: level 6 icosahedral shell, r in [0.55, 1.0], 10 diamonds x 64 x 64
+lateral x 64 radial cells (5,242,880 wedges, 2,662,530 unique nodes,
+7,987,590 velocity DoF), free-slip on both surfaces with the rotation
+penalty (eps = 1e-7), viscosity 1e3 below r = 0.85 times log-normal(0.5)
+noise, seeded uniform [-1, 1] velocity; FP64.  A step is one operator apply
+(penalty moment reduction + apply kernel + additive halo sum).
+
+GDoF/s = 3 * unique nodes / seconds per apply (the reference's definition,
+apps/benchmarks/performance/benchmark_operators.cpp:213-214).
+
+Prints one JSON line.  Also reports the roofline of the apply kernel (HIP
+events on the library stream) and the CPU oracle timed on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "terra-ng_amd"))
+
+METRIC = "EpsilonDivDiv GDoF/s; Stokes solve s to 1e-8 rel, FP32-MG vs FP64 (1 GPU)"
+HBM_PEAK_GBS = 8000.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--level", type=int, default=6)
+    ap.add_argument("--bcs", default="freeslip", choices=["freeslip", "dirichlet"])
+    ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="budget of the CPU oracle sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import tng
+    from tng import inputs
+
+    L = args.level
+    bc = tng.FREESLIP if args.bcs == "freeslip" else tng.DIRICHLET
+    dt = tng.F64 if args.dtype == "f64" else tng.F32
+    esize = 8 if dt == tng.F64 else 4
+    print(f"[bench] building level-{L} shell", file=sys.stderr, flush=True)
+    dom = tng.DistributedDomain.create_uniform(L, L, 0.55, 1.0)
+    ctx = dom.ctx
+    w = inputs.Workload(dom.coords(), dom.radii(), L, L)
+    k = w.eta_layered(102, jump=1e3, r_jump=0.85, sigma=0.5)
+    u = w.velocity(101)
+    kv = tng.VectorQ1Scalar(dom, dt).upload(k)
+    op = tng.EpsilonDivDivKerngen(dom, kv, (bc, bc))
+    src = tng.VectorQ1Vec(dom, dt).upload(u)
+    dst = tng.VectorQ1Vec(dom, dt)
+
+    for _ in range(args.warmup):
+        op.apply(src, dst)
+    ctx.synchronize()
+    print(f"[bench] warmup done ({args.warmup} applies); timing {args.steps}", file=sys.stderr, flush=True)
+
+    # timed region: inputs resident in HBM, K applies between two synchronisations
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        op.apply(src, dst)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    sec = (t1 - t0) / args.steps
+    dofs = 3 * dom.num_owned
+    gdofs = dofs / sec / 1e9
+
+    # kernel durations from HIP events on the library stream, same workload
+    ctx.set_profiling(True)
+    for _ in range(args.steps):
+        op.apply(src, dst)
+    napp, main_ms, total_ms = ctx.profile_read()
+    ctx.set_profiling(False)
+
+    nalloc = dom.num_alloc
+    nlat = dom.nsd * dom.nx * dom.nx
+    # algorithmic bytes of the apply kernel: src (3) + k (1) read, dst (3) written once,
+    # lateral coordinates and radii read once
+    alg_bytes = (7 * esize) * nalloc + 3 * esize * nlat + esize * dom.nsd * dom.nr
+    achieved = alg_bytes / (main_ms * 1e-3) / 1e9 if main_ms > 0 else None
+    # whole apply (adds the penalty's src re-read and the halo traffic)
+    pen_bytes = (3 * esize + 1) * nalloc if bc == tng.FREESLIP else 0
+
+    result = {
+        "metric": METRIC,
+        "value": round(gdofs, 3),
+        "unit": "GDoF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic (seeded SplitMix64 fields)",
+        "config": {
+            "workload": f"EpsilonDivDiv apply, level {L} shell r in [0.55,1.0] (10 x {2**L}x{2**L} x {2**L} cells, "
+                        f"{20 * 8**L} wedges), {args.bcs} both surfaces"
+                        + (" + rotation penalty eps=1e-7" if bc == tng.FREESLIP else "")
+                        + ", eta = 1e3 (r<0.85) x lognormal(0.5)",
+            "dofs": dofs,
+            "unique_nodes": dom.num_owned,
+            "allocated_node_copies": nalloc,
+            "apply_kernel_ms": round(main_ms, 5),
+            "apply_total_ms_events": round(total_ms, 5),
+            "whole_apply_bytes": alg_bytes + pen_bytes,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "epsdivdiv_apply_kernel",
+            "achieved": round(achieved, 1) if achieved else None,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+            "traffic": None,
+            "algorithmic_bytes": alg_bytes,
+        },
+    }
+
+    if not args.no_cpu:
+        print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        om = O.Mesh(L, L, 0.55, 1.0)
+        ocb = O.FREESLIP if bc == tng.FREESLIP else O.DIRICHLET
+        modes = om.penalty_modes() if bc == tng.FREESLIP else None
+        om.epsdivdiv(k, u, ocb, ocb, path="fast", penalty_modes=modes)  # warm-up
+        n = 0
+        c0 = time.perf_counter()
+        while True:
+            om.epsdivdiv(k, u, ocb, ocb, path="fast", penalty_modes=modes)
+            n += 1
+            if time.perf_counter() - c0 > args.cpu_seconds or n >= 50:
+                break
+        csec = (time.perf_counter() - c0) / n
+        result["cpu_baseline"] = {
+            "value": round(dofs / csec / 1e9, 5),
+            "unit": "GDoF/s",
+            "cores": O.num_threads(),
+            "kind": "port",
+            "sample": f"{n} FP64 applies of the same level-{L} workload with the oracle's restatement of the "
+                      f"reference fast path (OpenMP), {csec:.4f} s/apply",
+        }
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,166 @@
+"""ctypes front end of the CPU oracle (oracle/tng_oracle.c).
+
+TEST INFRASTRUCTURE: imported only by tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of bench.py, always as the checker.  The product path
+(terra-ng_amd/) never imports this module.
+
+Arrays follow the reference's unpadded Kokkos layouts:
+scalar (sd, x, y, r); vector 3 x (sd, x, y, r); coords (sd, x, y, 3); radii (sd, r).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "build", "libtng_oracle.so")
+
+NEUMANN, DIRICHLET, FREESLIP = 0, 1, 2
+REPLACE, ADD = 0, 1
+SKIP, COMMUNICATE = 0, 1
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {_LIB_PATH} (run make -C oracle)")
+        L = C.CDLL(_LIB_PATH)
+        vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.c_int
+        L.orc_mesh_create.restype = vp
+        L.orc_mesh_create.argtypes = [ip, ip, C.c_double, C.c_double, ip, ip]
+        L.orc_mesh_destroy.argtypes = [vp]
+        L.orc_mesh_dims.argtypes = [vp, C.POINTER(C.c_int64)]
+        for name, rt in [("orc_mesh_coords", dp), ("orc_mesh_radii", dp),
+                         ("orc_mesh_boundary_mask", C.POINTER(C.c_uint8)),
+                         ("orc_mesh_ownership_mask", C.POINTER(C.c_uint8)),
+                         ("orc_mesh_canonical_ids", C.POINTER(C.c_int64)),
+                         ("orc_mesh_subdomain_info", C.POINTER(C.c_int))]:
+            getattr(L, name).restype = rt
+            getattr(L, name).argtypes = [vp]
+        L.orc_halo_sum.argtypes = [vp, dp, ip]
+        L.orc_masked_dot.restype = C.c_double
+        L.orc_masked_dot.argtypes = [vp, dp, dp, ip]
+        L.orc_uniform_shell_radii.argtypes = [C.c_double, C.c_double, ip, dp]
+        L.orc_trafo_nt.argtypes = [dp, dp]
+        L.orc_epsdivdiv_local_matrix.argtypes = [vp, dp, ip, ip, ip, ip, ip, dp]
+        L.orc_epsdivdiv_apply_slow.argtypes = [vp, dp, ip, ip, ip, ip, ip, C.c_double, dp, dp]
+        L.orc_epsdivdiv_apply_fast.argtypes = [vp, dp, ip, ip, ip, ip, ip, C.c_double, dp, dp, dp]
+        L.orc_penalty_modes.argtypes = [vp, dp]
+        L.orc_num_threads.restype = ip
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Mesh:
+    """Oracle mesh: DistributedDomain::create_uniform(level, level_r, r_min, r_max, sdr, sdr_r)."""
+
+    def __init__(self, lat_level, rad_level=None, r_min=0.55, r_max=1.0, lat_sdr=0, rad_sdr=0):
+        L = lib()
+        if rad_level is None:
+            rad_level = lat_level
+        self.h = L.orc_mesh_create(lat_level, rad_level, r_min, r_max, lat_sdr, rad_sdr)
+        if not self.h:
+            raise ValueError("invalid mesh parameters")
+        d = (C.c_int64 * 5)()
+        L.orc_mesh_dims(self.h, d)
+        self.nsd, self.nx, self.nr, self.nalloc, self.nunique = [int(v) for v in d]
+        self.shape = (self.nsd, self.nx, self.nx, self.nr)
+        self.lat_level, self.rad_level = lat_level, rad_level
+        self.r_min, self.r_max = r_min, r_max
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_mesh_destroy(self.h)
+            self.h = None
+
+    def _arr(self, fn, n, dtype, shape):
+        p = getattr(lib(), fn)(self.h)
+        return np.ctypeslib.as_array(p, shape=(n,)).astype(dtype, copy=True).reshape(shape)
+
+    @property
+    def coords(self):
+        return self._arr("orc_mesh_coords", self.nsd * self.nx * self.nx * 3, np.float64,
+                         (self.nsd, self.nx, self.nx, 3))
+
+    @property
+    def radii(self):
+        return self._arr("orc_mesh_radii", self.nsd * self.nr, np.float64, (self.nsd, self.nr))
+
+    @property
+    def boundary_mask(self):
+        return self._arr("orc_mesh_boundary_mask", self.nalloc, np.uint8, self.shape)
+
+    @property
+    def ownership_mask(self):
+        return self._arr("orc_mesh_ownership_mask", self.nalloc, np.uint8, self.shape)
+
+    @property
+    def canonical_ids(self):
+        return self._arr("orc_mesh_canonical_ids", self.nalloc, np.int64, self.shape)
+
+    @property
+    def subdomain_info(self):
+        return self._arr("orc_mesh_subdomain_info", self.nsd * 4, np.int32, (self.nsd, 4))
+
+    def halo_sum(self, data):
+        a = np.ascontiguousarray(data, dtype=np.float64).copy()
+        ncomp = a.size // self.nalloc
+        lib().orc_halo_sum(self.h, _dp(a), ncomp)
+        return a
+
+    def masked_dot(self, a, b):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        return lib().orc_masked_dot(self.h, _dp(a), _dp(b), a.size // self.nalloc)
+
+    def local_matrix(self, k, sd, xc, yc, rc, wedge):
+        k = np.ascontiguousarray(k, dtype=np.float64)
+        A = np.zeros((18, 18))
+        lib().orc_epsdivdiv_local_matrix(self.h, _dp(k), sd, xc, yc, rc, wedge, _dp(A))
+        return A
+
+    def penalty_modes(self):
+        out = np.zeros((3, 3) + self.shape)
+        lib().orc_penalty_modes(self.h, _dp(out))
+        return out
+
+    def epsdivdiv(self, k, src, bc_cmb, bc_surf, diagonal=False, apply_mode=REPLACE, comm=COMMUNICATE,
+                  penalty_eps=1e-7, dst=None, path="slow", penalty_modes=None):
+        """EpsilonDivDivKerngen::apply_impl restated (slow or fast path)."""
+        k = np.ascontiguousarray(k, dtype=np.float64)
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        out = np.zeros((3,) + self.shape) if dst is None else np.array(dst, dtype=np.float64, copy=True)
+        if path == "slow":
+            lib().orc_epsdivdiv_apply_slow(self.h, _dp(k), bc_cmb, bc_surf, int(diagonal), apply_mode, comm,
+                                           penalty_eps, _dp(src), _dp(out))
+        else:
+            pm = None if penalty_modes is None else _dp(np.ascontiguousarray(penalty_modes))
+            lib().orc_epsdivdiv_apply_fast(self.h, _dp(k), bc_cmb, bc_surf, int(diagonal), apply_mode, comm,
+                                           penalty_eps, _dp(src), _dp(out), pm)
+        return out
+
+
+def uniform_shell_radii(r_min, r_max, n):
+    out = np.zeros(n)
+    lib().orc_uniform_shell_radii(r_min, r_max, n, _dp(out))
+    return out
+
+
+def trafo_nt(n):
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    R = np.zeros(9)
+    lib().orc_trafo_nt(_dp(n), _dp(R))
+    return R.reshape(3, 3)
+
+
+def num_threads():
+    return lib().orc_num_threads()

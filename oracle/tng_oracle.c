@@ -1,0 +1,964 @@
+/*
+ * tng_oracle.c -- CPU restatement of TERRA-NG's matrix-free wedge-element
+ * Stokes hot path, used ONLY as a checker.
+ *
+ * TEST INFRASTRUCTURE.  This file is the oracle: only tests/, the smoke()
+ * entry of __graft_entry__.py and the cpu_baseline leg of bench.py may load
+ * it.  The product library (terra-ng_amd/) never links, loads or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the TERRA-NG source tree).  The restatement is deliberately literal (the
+ * reference's own loop structure, node orderings and floating-point
+ * expression order); it is compiled with -ffp-contract=off so that no fused
+ * multiply-add changes the rounding of the source expressions.
+ *
+ * Layout conventions (identical to the reference's Kokkos views, unpadded):
+ *   scalar field  : (sd, x, y, r)            r fastest
+ *   vector field  : comp-major SoA, 3 x (sd, x, y, r)
+ *   lateral coords: (sd, x, y, 3)            unit sphere
+ *   radii         : (sd, r)
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* flags (grid/shell/bit_masks.hpp:11-29, grid/bit_masks.hpp:16-20)          */
+/* ------------------------------------------------------------------------- */
+enum { SBF_INNER = 1, SBF_BOUNDARY = 2, SBF_CMB = 2 | 4, SBF_SURFACE = 2 | 8 };
+enum { BC_NEUMANN = 0, BC_DIRICHLET = 1, BC_FREESLIP = 2 };
+enum { OWN_OWNED = 1 };
+
+/* util/bit_masking.hpp:43-53 */
+static int has_flag(uint8_t v, uint8_t f) { return f == 0 ? v == 0 : ((v & f) == f); }
+
+typedef struct { double v[3]; } vec3;
+
+static vec3 v3(double a, double b, double c) { vec3 r = {{a, b, c}}; return r; }
+static vec3 vadd(vec3 a, vec3 b) { return v3(a.v[0] + b.v[0], a.v[1] + b.v[1], a.v[2] + b.v[2]); }
+static vec3 vsub(vec3 a, vec3 b) { return v3(a.v[0] - b.v[0], a.v[1] - b.v[1], a.v[2] - b.v[2]); }
+static vec3 vscale(vec3 a, double s) { return v3(a.v[0] * s, a.v[1] * s, a.v[2] * s); }
+static vec3 vcross(vec3 a, vec3 b)
+{   /* dense/vec.hpp:58-65 */
+    return v3(a.v[1] * b.v[2] - a.v[2] * b.v[1], a.v[2] * b.v[0] - a.v[0] * b.v[2], a.v[0] * b.v[1] - a.v[1] * b.v[0]);
+}
+static double vnorm(vec3 a)
+{   /* dense/vec.hpp:88-96 */
+    double s = 0;
+    for (int i = 0; i < 3; ++i) s += a.v[i] * a.v[i];
+    return sqrt(s);
+}
+static vec3 vnormalized(vec3 a)
+{   /* dense/vec.hpp:98-118: divide only if the norm exceeds 1e-15 */
+    double m = vnorm(a);
+    if (m > 1e-15)
+        for (int i = 0; i < 3; ++i) a.v[i] /= m;
+    return a;
+}
+
+/* ------------------------------------------------------------------------- */
+/* mesh                                                                      */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int lat_level, rad_level, lat_sdr, rad_sdr;
+    int S, Sr;                 /* subdomains per diamond side / radially */
+    int nsd, nx, nr;           /* nx == ny */
+    int64_t nalloc;            /* nsd*nx*nx*nr */
+    int *sd_info;              /* nsd x 4 : diamond, sx, sy, sr */
+    double *coords;            /* nsd x nx x nx x 3 */
+    double *radii;             /* nsd x nr */
+    uint8_t *bmask;            /* boundary flags, nalloc */
+    uint8_t *omask;            /* ownership flags, nalloc */
+    int64_t *canon;            /* canonical unique node id per copy, nalloc */
+    int64_t nunique;
+    int64_t *uniq_first;       /* CSR over unique ids -> copies */
+    int64_t *uniq_copies;
+} orc_mesh;
+
+static int64_t lin(const orc_mesh *m, int sd, int x, int y, int r)
+{
+    return (((int64_t)sd * m->nx + x) * m->nx + y) * m->nr + r;
+}
+
+/* spherical_shell.hpp:27-52 uniform_shell_radii */
+void orc_uniform_shell_radii(double r_min, double r_max, int n, double *out)
+{
+    const double r_step = (r_max - r_min) / (n - 1);
+    for (int i = 0; i < n; ++i) out[i] = r_min + i * r_step;
+    out[n - 1] = r_max;
+}
+
+/* spherical_shell.hpp:465-604 : the twelve icosahedral nodes and the diamond map */
+static void icosahedron(double i_node[12][3])
+{
+    double fifthpi = 0.4 * asin(1.0);
+    double w = 2.0 * acos(1.0 / (2.0 * sin(fifthpi)));
+    double cosw = cos(w), sinw = sin(w), phi;
+    i_node[0][0] = 0.0; i_node[0][1] = 0.0; i_node[0][2] = +1.0;
+    i_node[11][0] = 0.0; i_node[11][1] = 0.0; i_node[11][2] = -1.0;
+    for (int k = 1; k <= 5; k++) {
+        phi = 2.0 * ((double)k - 0.5) * fifthpi;
+        i_node[k][0] = sinw * cos(phi);
+        i_node[k][1] = sinw * sin(phi);
+        i_node[k][2] = cosw;
+    }
+    for (int k = 1; k <= 5; k++) {
+        phi = 2.0 * ((double)k - 1) * fifthpi;
+        i_node[k + 5][0] = sinw * cos(phi);
+        i_node[k + 5][1] = sinw * sin(phi);
+        i_node[k + 5][2] = -cosw;
+    }
+}
+
+static const int D_NODE[10][4] = {{0, 5, 6, 1},  {0, 1, 7, 2},  {0, 2, 8, 3},  {0, 3, 9, 4},  {0, 4, 10, 5},
+                                  {11, 7, 1, 6}, {11, 8, 2, 7}, {11, 9, 3, 8}, {11, 10, 4, 9}, {11, 6, 5, 10}};
+
+typedef struct { vec3 p00, p0N, pN0, pNN; int N; } corners_t;
+
+/* spherical_shell.hpp:276-392 compute_node_recursive, restated WITHOUT the
+ * memoisation cache (the cache only changes speed: every node is a pure
+ * function of (i, j) and the corners). */
+static vec3 node_recursive(int i, int j, const corners_t *c)
+{
+    const int N = c->N;
+    if (i == 0 && j == 0) return c->p00;
+    if (i == 0 && j == N) return c->p0N;
+    if (i == N && j == 0) return c->pN0;
+    if (i == N && j == N) return c->pNN;
+    int l2 = 1, l = 2;
+    while (l <= N) {
+        if (i % l != 0 || j % l != 0) break;
+        l2 *= 2;
+        l = 2 * l2;
+    }
+    vec3 p1, p2;
+    if (i % l == 0 && j % l == l2) {
+        p1 = node_recursive(i, j - l2, c);
+        p2 = node_recursive(i, j + l2, c);
+    } else if (i % l == l2 && j % l == 0) {
+        p1 = node_recursive(i - l2, j, c);
+        p2 = node_recursive(i + l2, j, c);
+    } else {
+        p1 = node_recursive(i - l2, j + l2, c);
+        p2 = node_recursive(i + l2, j - l2, c);
+    }
+    return vnormalized(vadd(p1, p2));
+}
+
+static int cmp_key(const void *a, const void *b)
+{
+    const uint64_t *x = (const uint64_t *)a, *y = (const uint64_t *)b;
+    for (int i = 0; i < 4; ++i) {
+        if (x[i] < y[i]) return -1;
+        if (x[i] > y[i]) return 1;
+    }
+    return 0;
+}
+
+void orc_mesh_destroy(orc_mesh *m);
+
+/* DistributedDomain::create_uniform (spherical_shell.hpp:2599-2640) with one
+ * rank: local subdomain index = enumeration order (diamond, x, y, r)
+ * (DomainInfo::local_subdomains, :1034-1063). */
+orc_mesh *orc_mesh_create(int lat_level, int rad_level, double r_min, double r_max, int lat_sdr, int rad_sdr)
+{
+    orc_mesh *m = (orc_mesh *)calloc(1, sizeof(orc_mesh));
+    m->lat_level = lat_level; m->rad_level = rad_level; m->lat_sdr = lat_sdr; m->rad_sdr = rad_sdr;
+    m->S = 1 << lat_sdr; m->Sr = 1 << rad_sdr;
+    const int E = 1 << lat_level, L = 1 << rad_level;
+    if (E % m->S || L % m->Sr) { free(m); return NULL; }
+    m->nsd = 10 * m->S * m->S * m->Sr;
+    m->nx = E / m->S + 1;
+    m->nr = L / m->Sr + 1;
+    m->nalloc = (int64_t)m->nsd * m->nx * m->nx * m->nr;
+    m->sd_info = (int *)malloc(sizeof(int) * 4 * m->nsd);
+    int idx = 0;
+    for (int d = 0; d < 10; ++d)
+        for (int x = 0; x < m->S; ++x)
+            for (int y = 0; y < m->S; ++y)
+                for (int r = 0; r < m->Sr; ++r) {
+                    m->sd_info[4 * idx + 0] = d; m->sd_info[4 * idx + 1] = x;
+                    m->sd_info[4 * idx + 2] = y; m->sd_info[4 * idx + 3] = r;
+                    idx++;
+                }
+
+    /* lateral coordinates: spherical_shell.hpp:465-633, 2827-2853 */
+    double i_node[12][3];
+    icosahedron(i_node);
+    const int nx = m->nx;
+    m->coords = (double *)malloc(sizeof(double) * 3 * (size_t)m->nsd * nx * nx);
+    for (int sd = 0; sd < m->nsd; ++sd) {
+        const int d = m->sd_info[4 * sd], sx = m->sd_info[4 * sd + 1], sy = m->sd_info[4 * sd + 2];
+        int Lc, Rc;
+        if (d < 5) { Lc = 1; Rc = 3; } else { Rc = 1; Lc = 3; }
+        corners_t c;
+        c.N = E;
+        for (int i = 0; i < 3; ++i) {
+            c.p00.v[i] = i_node[D_NODE[d][0]][i];
+            c.pN0.v[i] = i_node[D_NODE[d][Lc]][i];
+            c.pNN.v[i] = i_node[D_NODE[d][2]][i];
+            c.p0N.v[i] = i_node[D_NODE[d][Rc]][i];
+        }
+        const int base = E / m->S;
+        const int i0 = sx * base, j0 = sy * base;
+        for (int x = 0; x < nx; ++x)
+            for (int y = 0; y < nx; ++y) {
+                vec3 p = node_recursive(i0 + x, j0 + y, &c);
+                double *o = m->coords + 3 * (((size_t)sd * nx + x) * nx + y);
+                o[0] = p.v[0]; o[1] = p.v[1]; o[2] = p.v[2];
+            }
+    }
+
+    /* radii: uniform_shell_radii + subdomain_shell_radii (:2862-2887) */
+    double *all_r = (double *)malloc(sizeof(double) * (L + 1));
+    orc_uniform_shell_radii(r_min, r_max, L + 1, all_r);
+    m->radii = (double *)malloc(sizeof(double) * (size_t)m->nsd * m->nr);
+    for (int sd = 0; sd < m->nsd; ++sd) {
+        const int sr = m->sd_info[4 * sd + 3];
+        for (int j = 0; j < m->nr; ++j) m->radii[(size_t)sd * m->nr + j] = all_r[sr * (m->nr - 1) + j];
+    }
+    free(all_r);
+
+    /* boundary mask: grid/shell/bit_masks.hpp:82-136 */
+    m->bmask = (uint8_t *)malloc((size_t)m->nalloc);
+    memset(m->bmask, SBF_INNER, (size_t)m->nalloc);
+    for (int sd = 0; sd < m->nsd; ++sd) {
+        const int sr = m->sd_info[4 * sd + 3];
+        for (int x = 0; x < nx; ++x)
+            for (int y = 0; y < nx; ++y) {
+                if (sr == 0) m->bmask[lin(m, sd, x, y, 0)] = SBF_CMB;
+                if (sr == m->Sr - 1) m->bmask[lin(m, sd, x, y, m->nr - 1)] = SBF_SURFACE;
+            }
+    }
+
+    /* Duplicated node copies.  Copies of one physical node sit at bit-identical
+     * lateral coordinates (the recursion is a pure function of the global
+     * (i, j) index and is symmetric under the diamond-edge reversals) and at
+     * the same global shell index.  The additive halo exchange of the
+     * reference (communication_plan.hpp:48-71) leaves every copy holding the
+     * sum over all copies; grouping by (coordinate bits, shell) reproduces
+     * exactly those groups. */
+    const int64_t nlat = (int64_t)m->nsd * nx * nx;
+    uint64_t *keys = (uint64_t *)malloc(sizeof(uint64_t) * 4 * nlat);
+    for (int64_t i = 0; i < nlat; ++i) {
+        memcpy(&keys[4 * i], &m->coords[3 * i], 3 * sizeof(double));
+        keys[4 * i + 3] = (uint64_t)i;
+    }
+    qsort(keys, (size_t)nlat, 4 * sizeof(uint64_t), cmp_key);
+    int64_t *lat_id = (int64_t *)malloc(sizeof(int64_t) * nlat);
+    int64_t nlat_unique = 0;
+    for (int64_t i = 0; i < nlat; ++i) {
+        if (i > 0 && memcmp(&keys[4 * i], &keys[4 * (i - 1)], 3 * sizeof(uint64_t)) != 0) nlat_unique++;
+        lat_id[keys[4 * i + 3]] = nlat_unique;
+    }
+    nlat_unique++;
+    free(keys);
+    const int nshell = L + 1;
+    m->canon = (int64_t *)malloc(sizeof(int64_t) * m->nalloc);
+    for (int sd = 0; sd < m->nsd; ++sd) {
+        const int sr = m->sd_info[4 * sd + 3];
+        for (int x = 0; x < nx; ++x)
+            for (int y = 0; y < nx; ++y) {
+                const int64_t lid = lat_id[((int64_t)sd * nx + x) * nx + y];
+                for (int r = 0; r < m->nr; ++r) m->canon[lin(m, sd, x, y, r)] = lid * nshell + sr * (m->nr - 1) + r;
+            }
+    }
+    free(lat_id);
+    /* compact canonical ids */
+    const int64_t nmax = nlat_unique * nshell;
+    int64_t *remap = (int64_t *)malloc(sizeof(int64_t) * nmax);
+    for (int64_t i = 0; i < nmax; ++i) remap[i] = -1;
+    int64_t nu = 0;
+    for (int64_t i = 0; i < m->nalloc; ++i) {
+        int64_t c = m->canon[i];
+        if (remap[c] < 0) remap[c] = nu++;
+        m->canon[i] = remap[c];
+    }
+    free(remap);
+    m->nunique = nu;
+    m->uniq_first = (int64_t *)calloc((size_t)nu + 1, sizeof(int64_t));
+    for (int64_t i = 0; i < m->nalloc; ++i) m->uniq_first[m->canon[i] + 1]++;
+    for (int64_t u = 0; u < nu; ++u) m->uniq_first[u + 1] += m->uniq_first[u];
+    m->uniq_copies = (int64_t *)malloc(sizeof(int64_t) * m->nalloc);
+    int64_t *fill = (int64_t *)calloc((size_t)nu, sizeof(int64_t));
+    for (int64_t i = 0; i < m->nalloc; ++i) {
+        int64_t u = m->canon[i];
+        m->uniq_copies[m->uniq_first[u] + fill[u]++] = i;
+    }
+    free(fill);
+
+    /* ownership: grid/bit_masks.hpp:29-85 -- the copy whose global subdomain
+     * id (SubdomainInfo::global_id, spherical_shell.hpp:696-722) is the
+     * minimum over the copies of the node is OWNED. */
+    m->omask = (uint8_t *)calloc((size_t)m->nalloc, 1);
+    for (int64_t u = 0; u < nu; ++u) {
+        int64_t best = -1;
+        int64_t best_gid = INT64_MAX;
+        for (int64_t k = m->uniq_first[u]; k < m->uniq_first[u + 1]; ++k) {
+            int64_t c = m->uniq_copies[k];
+            int sd = (int)(c / ((int64_t)nx * nx * m->nr));
+            const int *si = &m->sd_info[4 * sd];
+            int64_t gid = ((int64_t)si[0] << 57) | ((int64_t)si[3] << 38) | ((int64_t)si[2] << 19) | (int64_t)si[1];
+            if (gid < best_gid) { best_gid = gid; best = sd; }
+        }
+        for (int64_t k = m->uniq_first[u]; k < m->uniq_first[u + 1]; ++k) {
+            int64_t c = m->uniq_copies[k];
+            int sd = (int)(c / ((int64_t)nx * nx * m->nr));
+            if (sd == best) m->omask[c] = OWN_OWNED;
+        }
+    }
+    return m;
+}
+
+void orc_mesh_destroy(orc_mesh *m)
+{
+    if (!m) return;
+    free(m->sd_info); free(m->coords); free(m->radii); free(m->bmask); free(m->omask);
+    free(m->canon); free(m->uniq_first); free(m->uniq_copies);
+    free(m);
+}
+
+void orc_mesh_dims(const orc_mesh *m, int64_t *out)
+{
+    out[0] = m->nsd; out[1] = m->nx; out[2] = m->nr; out[3] = m->nalloc; out[4] = m->nunique;
+}
+const double *orc_mesh_coords(const orc_mesh *m) { return m->coords; }
+const double *orc_mesh_radii(const orc_mesh *m) { return m->radii; }
+const uint8_t *orc_mesh_boundary_mask(const orc_mesh *m) { return m->bmask; }
+const uint8_t *orc_mesh_ownership_mask(const orc_mesh *m) { return m->omask; }
+const int64_t *orc_mesh_canonical_ids(const orc_mesh *m) { return m->canon; }
+const int *orc_mesh_subdomain_info(const orc_mesh *m) { return m->sd_info; }
+
+/* ------------------------------------------------------------------------- */
+/* halo: additive exchange (communication_plan.hpp:48-71, 471-517)           */
+/* every copy of a node ends up holding the sum over all copies              */
+/* ------------------------------------------------------------------------- */
+void orc_halo_sum(const orc_mesh *m, double *data, int ncomp)
+{
+    for (int c = 0; c < ncomp; ++c) {
+        double *d = data + (size_t)c * m->nalloc;
+        for (int64_t u = 0; u < m->nunique; ++u) {
+            const int64_t b = m->uniq_first[u], e = m->uniq_first[u + 1];
+            if (e - b < 2) continue;
+            double s = 0.0;
+            for (int64_t k = b; k < e; ++k) s += d[m->uniq_copies[k]];
+            for (int64_t k = b; k < e; ++k) d[m->uniq_copies[k]] = s;
+        }
+    }
+}
+
+/* linalg/vector_q1.hpp:96-100, grid_operations.hpp masked_dot_product */
+double orc_masked_dot(const orc_mesh *m, const double *a, const double *b, int ncomp)
+{
+    double s = 0.0;
+    for (int c = 0; c < ncomp; ++c) {
+        double sc = 0.0;
+        for (int64_t i = 0; i < m->nalloc; ++i)
+            if (m->omask[i] == OWN_OWNED) sc += a[(size_t)c * m->nalloc + i] * b[(size_t)c * m->nalloc + i];
+        s += sc;
+    }
+    return s;
+}
+
+/* ------------------------------------------------------------------------- */
+/* 3x3 helpers (dense/mat.hpp)                                               */
+/* ------------------------------------------------------------------------- */
+typedef struct { double a[3][3]; } mat3;
+
+static double det3(const mat3 *J)
+{   /* mat.hpp:246-251 */
+    const double(*d)[3] = J->a;
+    return d[0][0] * (d[1][1] * d[2][2] - d[1][2] * d[2][1]) - d[0][1] * (d[1][0] * d[2][2] - d[1][2] * d[2][0]) +
+           d[0][2] * (d[1][0] * d[2][1] - d[1][1] * d[2][0]);
+}
+
+static mat3 inv_transposed3(const mat3 *J, double det)
+{   /* cofactor matrix / det == (J^{-1})^T  (mat.hpp:339-375) */
+    const double(*d)[3] = J->a;
+    const double id = 1.0 / det;
+    mat3 r;
+    r.a[0][0] = (d[1][1] * d[2][2] - d[1][2] * d[2][1]) * id;
+    r.a[0][1] = (d[1][2] * d[2][0] - d[1][0] * d[2][2]) * id;
+    r.a[0][2] = (d[1][0] * d[2][1] - d[1][1] * d[2][0]) * id;
+    r.a[1][0] = (d[0][2] * d[2][1] - d[0][1] * d[2][2]) * id;
+    r.a[1][1] = (d[0][0] * d[2][2] - d[0][2] * d[2][0]) * id;
+    r.a[1][2] = (d[0][1] * d[2][0] - d[0][0] * d[2][1]) * id;
+    r.a[2][0] = (d[0][1] * d[1][2] - d[0][2] * d[1][1]) * id;
+    r.a[2][1] = (d[0][2] * d[1][0] - d[0][0] * d[1][2]) * id;
+    r.a[2][2] = (d[0][0] * d[1][1] - d[0][1] * d[1][0]) * id;
+    return r;
+}
+
+/* linalg/trafo/local_basis_trafo_normal_tangential.hpp:34-76 */
+static mat3 trafo_nt(vec3 n_in)
+{
+    vec3 n = vnormalized(n_in);
+    const double nx = fabs(n.v[0]), ny = fabs(n.v[1]), nz = fabs(n.v[2]);
+    vec3 ref;
+    if (nx <= ny && nx <= nz) ref = v3(1, 0, 0);
+    else if (ny <= nx && ny <= nz) ref = v3(0, 1, 0);
+    else ref = v3(0, 0, 1);
+    vec3 t1 = vnormalized(vcross(ref, n));
+    if (vnorm(t1) < 1e-15) t1 = vnormalized(v3(n.v[1], -n.v[0], 0));
+    vec3 t2 = vnormalized(vcross(n, t1));
+    mat3 R;
+    for (int i = 0; i < 3; ++i) { R.a[0][i] = n.v[i]; R.a[1][i] = t1.v[i]; R.a[2][i] = t2.v[i]; }
+    return R;
+}
+
+void orc_trafo_nt(const double *n, double *R_out)
+{
+    mat3 R = trafo_nt(v3(n[0], n[1], n[2]));
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) R_out[3 * i + j] = R.a[i][j];
+}
+
+/* ------------------------------------------------------------------------- */
+/* EpsilonDivDivKerngen -- slow (local matrix) path                          */
+/* ------------------------------------------------------------------------- */
+/* wedge node offsets (dx, dy, dr): kernel_helpers.hpp:305-345 / 386-398 */
+static const int WOFF[2][6][3] = {{{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}},
+                                  {{1, 1, 0}, {0, 1, 0}, {1, 0, 0}, {1, 1, 1}, {0, 1, 1}, {1, 0, 1}}};
+
+/* integrands.hpp:146-258 shape / grad_shape at a reference point */
+static double shape_w(int k, double xi, double eta, double zeta)
+{
+    const double N_lat[3] = {1.0 - xi - eta, xi, eta};
+    const double N_rad[2] = {0.5 * (1 - zeta), 0.5 * (1 + zeta)};
+    return N_lat[k % 3] * N_rad[k / 3];
+}
+static vec3 grad_shape_w(int k, double xi, double eta, double zeta)
+{
+    static const double gxi[3] = {-1.0, 1.0, 0.0}, geta[3] = {-1.0, 0.0, 1.0}, grad_rad[2] = {-0.5, 0.5};
+    const double N_rad[2] = {0.5 * (1 - zeta), 0.5 * (1 + zeta)};
+    const double N_lat[3] = {1.0 - xi - eta, xi, eta};
+    return v3(gxi[k % 3] * N_rad[k / 3], geta[k % 3] * N_rad[k / 3], N_lat[k % 3] * grad_rad[k / 3]);
+}
+
+/* integrands.hpp:596-661 jac(): columns (p2-p1, p3-p1, forward_map_lat) x diag(r, r, dr/2) */
+static mat3 jac_w(vec3 p1, vec3 p2, vec3 p3, double r_1, double r_2, double xi, double eta, double zeta)
+{
+    vec3 c0 = vsub(p2, p1), c1 = vsub(p3, p1);
+    vec3 c2 = vadd(vadd(vscale(p1, (1 - xi - eta)), vscale(p2, xi)), vscale(p3, eta));
+    const double r = r_1 + 0.5 * (r_2 - r_1) * (1 + zeta);
+    const double gr = 0.5 * (r_2 - r_1);
+    mat3 J;
+    for (int i = 0; i < 3; ++i) {
+        /* (jac_lat * diag(r, r, gr))(i, j) = sum_k lat(i,k) diag(k,j) */
+        J.a[i][0] = c0.v[i] * r + c1.v[i] * 0.0 + c2.v[i] * 0.0;
+        J.a[i][1] = c0.v[i] * 0.0 + c1.v[i] * r + c2.v[i] * 0.0;
+        J.a[i][2] = c0.v[i] * 0.0 + c1.v[i] * 0.0 + c2.v[i] * gr;
+    }
+    return J;
+}
+
+typedef struct {
+    const orc_mesh *m;
+    const double *k;
+    int bc_cmb, bc_surf;
+} op_ctx;
+
+static int bc_of(const op_ctx *o, int sbf) { return sbf == SBF_CMB ? o->bc_cmb : o->bc_surf; }
+
+/* epsilon_divdiv_kerngen.hpp:2251-2392 assemble_trial_test_vecs + assemble_local_matrix
+ * A(i + 6 dimi, j + 6 dimj) = |det J| w k_eval (2 eps_j : eps_i - 2/3 div_j div_i) */
+static void assemble_local_matrix(const op_ctx *o, int sd, int xc, int yc, int rc, int w, double A[18][18])
+{
+    const orc_mesh *m = o->m;
+    const int nx = m->nx;
+    vec3 q[2][2];
+    for (int x = 0; x < 2; ++x)
+        for (int y = 0; y < 2; ++y) {
+            const double *c = m->coords + 3 * (((size_t)sd * nx + xc + x) * nx + yc + y);
+            q[x][y] = v3(c[0], c[1], c[2]);
+        }
+    vec3 P[3];
+    if (w == 0) { P[0] = q[0][0]; P[1] = q[1][0]; P[2] = q[0][1]; }
+    else { P[0] = q[1][1]; P[1] = q[0][1]; P[2] = q[1][0]; }
+    const double r_1 = m->radii[(size_t)sd * m->nr + rc], r_2 = m->radii[(size_t)sd * m->nr + rc + 1];
+    const double xi = 1.0 / 3.0, eta = 1.0 / 3.0, zeta = 0.0, qw = 1.0; /* quadrature.hpp:32-46 */
+    mat3 J = jac_w(P[0], P[1], P[2], r_1, r_2, xi, eta, zeta);
+    const double det = det3(&J);
+    const double abs_det = fabs(det);
+    mat3 JiT = inv_transposed3(&J, det);
+    double k_eval = 0.0;
+    for (int n = 0; n < 6; ++n)
+        k_eval += shape_w(n, xi, eta, zeta) *
+                  o->k[lin(m, sd, xc + WOFF[w][n][0], yc + WOFF[w][n][1], rc + WOFF[w][n][2])];
+    const double jkw = qw * k_eval * abs_det;
+    /* physical gradients g_n = J^{-T} grad_ref N_n; sym_grad(n, dim) = sym(g_n e_dim^T) */
+    double g[6][3];
+    for (int n = 0; n < 6; ++n) {
+        vec3 gr = grad_shape_w(n, xi, eta, zeta);
+        for (int i = 0; i < 3; ++i) g[n][i] = JiT.a[i][0] * gr.v[0] + JiT.a[i][1] * gr.v[1] + JiT.a[i][2] * gr.v[2];
+    }
+    for (int di = 0; di < 3; ++di)
+        for (int dj = 0; dj < 3; ++dj)
+            for (int i = 0; i < 6; ++i)
+                for (int j = 0; j < 6; ++j) {
+                    double Ei[3][3] = {{0}}, Ej[3][3] = {{0}};
+                    for (int a = 0; a < 3; ++a) {
+                        /* grad = g e_dim^T : column dim holds g */
+                        Ei[a][di] += 0.5 * g[i][a];
+                        Ei[di][a] += 0.5 * g[i][a];
+                        Ej[a][dj] += 0.5 * g[j][a];
+                        Ej[dj][a] += 0.5 * g[j][a];
+                    }
+                    double dc = 0.0;
+                    for (int a = 0; a < 3; ++a)
+                        for (int b = 0; b < 3; ++b) dc += Ej[a][b] * Ei[a][b];
+                    A[i + 6 * di][j + 6 * dj] += jkw * (2 * dc - 2.0 / 3.0 * Ej[dj][dj] * Ei[di][di]);
+                }
+}
+
+void orc_epsdivdiv_local_matrix(const orc_mesh *m, const double *k, int sd, int xc, int yc, int rc, int w,
+                                double *A_out)
+{
+    op_ctx o = {m, k, 0, 0};
+    double A[18][18];
+    memset(A, 0, sizeof(A));
+    assemble_local_matrix(&o, sd, xc, yc, rc, w, A);
+    memcpy(A_out, A, sizeof(A));
+}
+
+static void matmul18(const double A[18][18], const double B[18][18], double C[18][18])
+{
+    for (int i = 0; i < 18; ++i)
+        for (int j = 0; j < 18; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < 18; ++k) s += A[i][k] * B[k][j];
+            C[i][j] = s;
+        }
+}
+
+/* epsilon_divdiv_kerngen.hpp:1013-1242 operator_slow_path, one hex cell */
+static void slow_cell(const op_ctx *o, int sd, int xc, int yc, int rc, int diagonal, const double *src, double *dst)
+{
+    const orc_mesh *m = o->m;
+    const int64_t N = m->nalloc;
+    const int at_cmb = has_flag(m->bmask[lin(m, sd, xc, yc, rc)], SBF_CMB);
+    const int at_surface = has_flag(m->bmask[lin(m, sd, xc, yc, rc + 1)], SBF_SURFACE);
+    double A[2][18][18];
+    memset(A, 0, sizeof(A));
+    assemble_local_matrix(o, sd, xc, yc, rc, 0, A[0]);
+    assemble_local_matrix(o, sd, xc, yc, rc, 1, A[1]);
+    double s[2][18];
+    int64_t idx[2][6];
+    for (int w = 0; w < 2; ++w)
+        for (int n = 0; n < 6; ++n) idx[w][n] = lin(m, sd, xc + WOFF[w][n][0], yc + WOFF[w][n][1], rc + WOFF[w][n][2]);
+    for (int w = 0; w < 2; ++w)
+        for (int d = 0; d < 3; ++d)
+            for (int n = 0; n < 6; ++n) s[w][d * 6 + n] = src[(size_t)d * N + idx[w][n]];
+
+    double bm[18][18];
+    for (int i = 0; i < 18; ++i)
+        for (int j = 0; j < 18; ++j) bm[i][j] = 1.0;
+    int fs_reorder = 0;
+    double R[2][18][18];
+    if (at_cmb || at_surface) {
+        const int sbf = at_cmb ? SBF_CMB : SBF_SURFACE;
+        const int bcf = bc_of(o, sbf);
+        if (bcf == BC_DIRICHLET) {
+            for (int di = 0; di < 3; ++di)
+                for (int dj = 0; dj < 3; ++dj)
+                    for (int i = 0; i < 6; i++)
+                        for (int j = 0; j < 6; j++)
+                            if ((at_cmb && ((di == dj && i != j && (i < 3 || j < 3)) || (di != dj && (i < 3 || j < 3)))) ||
+                                (at_surface &&
+                                 ((di == dj && i != j && (i >= 3 || j >= 3)) || (di != dj && (i >= 3 || j >= 3)))))
+                                bm[i + di * 6][j + dj * 6] = 0.0;
+        } else if (bcf == BC_FREESLIP) {
+            fs_reorder = 1;
+            static const int offx[2][3] = {{0, 1, 0}, {1, 0, 1}}, offy[2][3] = {{0, 0, 1}, {1, 1, 0}};
+            for (int w = 0; w < 2; ++w) {
+                double At[18][18];
+                for (int di = 0; di < 3; ++di)
+                    for (int ni = 0; ni < 6; ++ni)
+                        for (int dj = 0; dj < 3; ++dj)
+                            for (int nj = 0; nj < 6; ++nj) At[ni * 3 + di][nj * 3 + dj] = A[w][ni + di * 6][nj + dj * 6];
+                /* reorder_local_dofs DIMENSIONWISE -> NODEWISE (kernel_helpers.hpp:617-651) */
+                double tmp[18];
+                for (int n = 0; n < 6; ++n)
+                    for (int d = 0; d < 3; ++d) tmp[n * 3 + d] = s[w][d * 6 + n];
+                memcpy(s[w], tmp, sizeof(tmp));
+                memset(R[w], 0, sizeof(R[w]));
+                for (int i = 0; i < 18; ++i) R[w][i][i] = 1.0;
+                for (int b = 0; b < 3; ++b) {
+                    const int x = xc + offx[w][b], y = yc + offy[w][b], r = rc + (at_cmb ? 0 : 1);
+                    const double *c = m->coords + 3 * (((size_t)sd * m->nx + x) * m->nx + y);
+                    const double rad = m->radii[(size_t)sd * m->nr + r];
+                    /* grid::shell::coords(): unit coords times radius (spherical_shell.hpp:2890-2914) */
+                    mat3 Ri = trafo_nt(v3(c[0] * rad, c[1] * rad, c[2] * rad));
+                    const int off = at_cmb ? 0 : 9;
+                    for (int a = 0; a < 3; ++a)
+                        for (int bb = 0; bb < 3; ++bb) R[w][off + b * 3 + a][off + b * 3 + bb] = Ri.a[a][bb];
+                }
+                double RA[18][18], RT[18][18];
+                for (int i = 0; i < 18; ++i)
+                    for (int j = 0; j < 18; ++j) RT[i][j] = R[w][j][i];
+                matmul18(R[w], At, RA);
+                matmul18(RA, RT, A[w]);
+                double st[18];
+                for (int i = 0; i < 18; ++i) {
+                    double acc = 0.0;
+                    for (int j = 0; j < 18; ++j) acc += R[w][i][j] * s[w][j];
+                    st[i] = acc;
+                }
+                memcpy(s[w], st, sizeof(st));
+                const int ns = at_surface ? 3 : 0, ne = at_surface ? 6 : 3;
+                for (int nd = ns; nd < ne; nd++) {
+                    const int ix = nd * 3;
+                    for (int kk = 0; kk < 18; ++kk)
+                        if (kk != ix) { bm[ix][kk] = 0.0; bm[kk][ix] = 0.0; }
+                }
+            }
+        }
+    }
+    for (int w = 0; w < 2; ++w)
+        for (int i = 0; i < 18; ++i)
+            for (int j = 0; j < 18; ++j) A[w][i][j] *= bm[i][j];
+    if (diagonal)
+        for (int w = 0; w < 2; ++w)
+            for (int i = 0; i < 18; ++i)
+                for (int j = 0; j < 18; ++j)
+                    if (i != j) A[w][i][j] = 0.0;
+    double d[2][18];
+    for (int w = 0; w < 2; ++w)
+        for (int i = 0; i < 18; ++i) {
+            double acc = 0.0;
+            for (int j = 0; j < 18; ++j) acc += A[w][i][j] * s[w][j];
+            d[w][i] = acc;
+        }
+    if (fs_reorder) {
+        for (int w = 0; w < 2; ++w) {
+            double t[18];
+            for (int i = 0; i < 18; ++i) {
+                double acc = 0.0;
+                for (int j = 0; j < 18; ++j) acc += R[w][j][i] * d[w][j];
+                t[i] = acc;
+            }
+            /* NODEWISE -> DIMENSIONWISE */
+            for (int n = 0; n < 6; ++n)
+                for (int dd = 0; dd < 3; ++dd) d[w][dd * 6 + n] = t[n * 3 + dd];
+        }
+    }
+    for (int w = 0; w < 2; ++w)
+        for (int dd = 0; dd < 3; ++dd)
+            for (int n = 0; n < 6; ++n) dst[(size_t)dd * N + idx[w][n]] += d[w][dd * 6 + n];
+}
+
+/* epsilon_divdiv_kerngen.hpp:324-445 init_penalty: rotation modes e_i x x,
+ * free-slip enforcement (the flag test requires CMB|SURFACE together, i.e.
+ * it never fires on a shell with more than one layer), then Gram-Schmidt
+ * with OWNED-masked dots. */
+static void penalty_modes(const orc_mesh *m, double *nm /* 3 modes x 3 comps x N */)
+{
+    const int64_t N = m->nalloc;
+    const uint8_t fsmask = (uint8_t)(SBF_CMB | SBF_SURFACE);
+    for (int axis = 0; axis < 3; ++axis) {
+        double *md = nm + (size_t)axis * 3 * N;
+        for (int sd = 0; sd < m->nsd; ++sd)
+            for (int x = 0; x < m->nx; ++x)
+                for (int y = 0; y < m->nx; ++y) {
+                    const double *c0 = m->coords + 3 * (((size_t)sd * m->nx + x) * m->nx + y);
+                    for (int r = 0; r < m->nr; ++r) {
+                        const double rad = m->radii[(size_t)sd * m->nr + r];
+                        const double c[3] = {c0[0] * rad, c0[1] * rad, c0[2] * rad};
+                        const int64_t i = lin(m, sd, x, y, r);
+                        const int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
+                        for (int d = 0; d < 3; ++d) md[(size_t)d * N + i] = 0.0;
+                        md[(size_t)a1 * N + i] = c[a2];
+                        md[(size_t)a2 * N + i] = -c[a1];
+                        if (has_flag(m->bmask[i], fsmask)) {
+                            const double nrm[3] = {c0[0], c0[1], c0[2]};
+                            mat3 R = trafo_nt(v3(nrm[0], nrm[1], nrm[2]));
+                            double vv[3] = {md[i], md[N + i], md[2 * N + i]}, nt[3], ca[3];
+                            for (int a = 0; a < 3; ++a) nt[a] = R.a[a][0] * vv[0] + R.a[a][1] * vv[1] + R.a[a][2] * vv[2];
+                            nt[0] = 0.0;
+                            for (int a = 0; a < 3; ++a) ca[a] = R.a[0][a] * nt[0] + R.a[1][a] * nt[1] + R.a[2][a] * nt[2];
+                            for (int a = 0; a < 3; ++a) md[(size_t)a * N + i] = ca[a];
+                        }
+                    }
+                }
+    }
+    for (int i = 0; i < 3; ++i) {
+        double *ni = nm + (size_t)i * 3 * N;
+        for (int j = 0; j < i; ++j) {
+            double *nj = nm + (size_t)j * 3 * N;
+            const double dij = orc_masked_dot(m, ni, nj, 3);
+            for (int64_t t = 0; t < 3 * N; ++t) ni[t] -= dij * nj[t];
+        }
+        const double nsq = orc_masked_dot(m, ni, ni, 3);
+        const double inv = 1.0 / sqrt(nsq);
+        for (int64_t t = 0; t < 3 * N; ++t) ni[t] *= inv;
+    }
+}
+
+void orc_penalty_modes(const orc_mesh *m, double *out) { penalty_modes(m, out); }
+
+/* epsilon_divdiv_kerngen.hpp:711-785: dst += eps * sum_i (n_i . src)_owned n_i on every copy */
+static void penalty_apply(const orc_mesh *m, const double *nm, double eps, const double *src, double *dst)
+{
+    const int64_t N = m->nalloc;
+    double a[3] = {0, 0, 0};
+    for (int64_t i = 0; i < N; ++i) {
+        const double mk = (m->omask[i] == OWN_OWNED) ? 1.0 : 0.0;
+        double s0 = 0, s1 = 0, s2 = 0;
+        for (int d = 0; d < 3; ++d) {
+            const double sv = src[(size_t)d * N + i] * mk;
+            s0 += nm[(size_t)(0 * 3 + d) * N + i] * sv;
+            s1 += nm[(size_t)(1 * 3 + d) * N + i] * sv;
+            s2 += nm[(size_t)(2 * 3 + d) * N + i] * sv;
+        }
+        a[0] += s0; a[1] += s1; a[2] += s2;
+    }
+    for (int i = 0; i < 3; ++i) a[i] *= eps;
+    for (int64_t i = 0; i < N; ++i)
+        for (int d = 0; d < 3; ++d)
+            dst[(size_t)d * N + i] += a[0] * nm[(size_t)(0 * 3 + d) * N + i] + a[1] * nm[(size_t)(1 * 3 + d) * N + i] +
+                                      a[2] * nm[(size_t)(2 * 3 + d) * N + i];
+}
+
+/* epsilon_divdiv_kerngen.hpp:577-792 apply_impl, slow path.
+ * apply_mode: 0 Replace, 1 Add; comm: 0 SkipCommunication, 1 CommunicateAdditively */
+void orc_epsdivdiv_apply_slow(const orc_mesh *m, const double *k, int bc_cmb, int bc_surf, int diagonal,
+                              int apply_mode, int comm, double penalty_eps, const double *src, double *dst)
+{
+    const int64_t N = m->nalloc;
+    op_ctx o = {m, k, bc_cmb, bc_surf};
+    if (apply_mode == 0) memset(dst, 0, sizeof(double) * 3 * N);
+    for (int sd = 0; sd < m->nsd; ++sd)
+        for (int xc = 0; xc < m->nx - 1; ++xc)
+            for (int yc = 0; yc < m->nx - 1; ++yc)
+                for (int rc = 0; rc < m->nr - 1; ++rc) slow_cell(&o, sd, xc, yc, rc, diagonal, src, dst);
+    if (bc_cmb == BC_FREESLIP && bc_surf == BC_FREESLIP && !diagonal) {
+        double *nm = (double *)malloc(sizeof(double) * 9 * N);
+        penalty_modes(m, nm);
+        penalty_apply(m, nm, penalty_eps, src, dst);
+        free(nm);
+    }
+    if (comm) orc_halo_sum(m, dst, 3);
+}
+
+/* ------------------------------------------------------------------------- */
+/* EpsilonDivDivKerngen -- fast matrix-free paths, restated per hex cell     */
+/* (Dirichlet/Neumann: epsilon_divdiv_kerngen.hpp:1245-1607;                 */
+/*  free-slip:         epsilon_divdiv_kerngen.hpp:1638-2198).                */
+/* Used as the CPU baseline; OpenMP over lateral cells of one parity class   */
+/* (cells of equal (x%2, y%2) share no node), whole radial columns per       */
+/* thread, so no two threads ever update the same dst entry.                 */
+/* ------------------------------------------------------------------------- */
+static const double ONE_THIRD = 1.0 / 3.0, ONE_SIXTH = 1.0 / 6.0, NEG_TWO_THIRDS = -0.66666666666666663;
+static const double dN_ref[6][3] = {{-0.5, -0.5, -1.0 / 6.0}, {0.5, 0.0, -1.0 / 6.0}, {0.0, 0.5, -1.0 / 6.0},
+                                    {-0.5, -0.5, 1.0 / 6.0},  {0.5, 0.0, 1.0 / 6.0},  {0.0, 0.5, 1.0 / 6.0}};
+static const int WEDGE_TO_UNIQUE[2][6] = {{0, 1, 2, 3, 4, 5}, {6, 2, 1, 7, 5, 4}};
+/* unique hex node u -> (dx, dy, dr) */
+static const int HEX_NODE[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 0}, {1, 1, 1}};
+
+static void cell_geom(const double *cv[3], double r_0, double r_1, double *kdet_abs, double inv[9])
+{
+    const double half_dr = 0.5 * (r_1 - r_0), r_mid = 0.5 * (r_0 + r_1);
+    double J[3][3];
+    for (int i = 0; i < 3; ++i) {
+        J[i][0] = r_mid * (-cv[0][i] + cv[1][i]);
+        J[i][1] = r_mid * (-cv[0][i] + cv[2][i]);
+        J[i][2] = half_dr * (ONE_THIRD * (cv[0][i] + cv[1][i] + cv[2][i]));
+    }
+    const double J_det = J[0][0] * J[1][1] * J[2][2] - J[0][0] * J[1][2] * J[2][1] - J[0][1] * J[1][0] * J[2][2] +
+                         J[0][1] * J[1][2] * J[2][0] + J[0][2] * J[1][0] * J[2][1] - J[0][2] * J[1][1] * J[2][0];
+    *kdet_abs = fabs(J_det);
+    const double id = 1.0 / J_det;
+    inv[0] = id * (J[1][1] * J[2][2] - J[1][2] * J[2][1]);
+    inv[1] = id * (-J[1][0] * J[2][2] + J[1][2] * J[2][0]);
+    inv[2] = id * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    inv[3] = id * (-J[0][1] * J[2][2] + J[0][2] * J[2][1]);
+    inv[4] = id * (J[0][0] * J[2][2] - J[0][2] * J[2][0]);
+    inv[5] = id * (-J[0][0] * J[2][1] + J[0][1] * J[2][0]);
+    inv[6] = id * (J[0][1] * J[1][2] - J[0][2] * J[1][1]);
+    inv[7] = id * (-J[0][0] * J[1][2] + J[0][2] * J[1][0]);
+    inv[8] = id * (J[0][0] * J[1][1] - J[0][1] * J[1][0]);
+}
+
+static void fast_cell(const op_ctx *o, int sd, int xc, int yc, int rc, int diagonal, const double *src, double *dst)
+{
+    const orc_mesh *m = o->m;
+    const int64_t N = m->nalloc;
+    const int nx = m->nx;
+    const double r_0 = m->radii[(size_t)sd * m->nr + rc], r_1 = m->radii[(size_t)sd * m->nr + rc + 1];
+    const int at_cmb = has_flag(m->bmask[lin(m, sd, xc, yc, rc)], SBF_CMB);
+    const int at_surface = has_flag(m->bmask[lin(m, sd, xc, yc, rc + 1)], SBF_SURFACE);
+    const int cmb_fs = at_cmb && o->bc_cmb == BC_FREESLIP;
+    const int surf_fs = at_surface && o->bc_surf == BC_FREESLIP;
+    const int cmb_dir = at_cmb && o->bc_cmb == BC_DIRICHLET;
+    const int surf_dir = at_surface && o->bc_surf == BC_DIRICHLET;
+    const int cmb_shift = (cmb_dir && !diagonal) ? 3 : 0;
+    const int surf_shift = (surf_dir && !diagonal) ? 3 : 0;
+    /* lateral corners c: 0 -> (0,0), 1 -> (1,0), 2 -> (0,1), 3 -> (1,1) */
+    const double *cc[4];
+    double nrm[4][3];
+    static const int CX[4] = {0, 1, 0, 1}, CY[4] = {0, 0, 1, 1};
+    for (int c = 0; c < 4; ++c) {
+        cc[c] = m->coords + 3 * (((size_t)sd * nx + xc + CX[c]) * nx + yc + CY[c]);
+        const double n2 = cc[c][0] * cc[c][0] + cc[c][1] * cc[c][1] + cc[c][2] * cc[c][2];
+        const double inv = 1.0 / sqrt(n2);
+        for (int d = 0; d < 3; ++d) nrm[c][d] = cc[c][d] * inv;
+    }
+    static const int CORNER_OF[2][3] = {{0, 1, 2}, {3, 2, 1}};
+    static const int CMB_CORNER_TO_UNIQUE[4] = {0, 1, 2, 6}, SURF_CORNER_TO_UNIQUE[4] = {3, 4, 5, 7};
+    double dst8[3][8];
+    memset(dst8, 0, sizeof(dst8));
+    double Ann_cmb[4] = {0, 0, 0, 0}, Ann_surf[4] = {0, 0, 0, 0};
+    int64_t hidx[8];
+    for (int u = 0; u < 8; ++u) hidx[u] = lin(m, sd, xc + HEX_NODE[u][0], yc + HEX_NODE[u][1], rc + HEX_NODE[u][2]);
+
+    for (int w = 0; w < 2; ++w) {
+        const double *cv[3] = {cc[CORNER_OF[w][0]], cc[CORNER_OF[w][1]], cc[CORNER_OF[w][2]]};
+        double k_sum = 0.0;
+        for (int n = 0; n < 6; ++n) k_sum += o->k[hidx[WEDGE_TO_UNIQUE[w][n]]];
+        const double k_eval = ONE_SIXTH * k_sum;
+        double adet, iv[9];
+        cell_geom(cv, r_0, r_1, &adet, iv);
+        const double kwJ = k_eval * adet;
+        double g[6][3];
+        for (int n = 0; n < 6; ++n)
+            for (int i = 0; i < 3; ++i)
+                g[n][i] = iv[3 * i + 0] * dN_ref[n][0] + iv[3 * i + 1] * dN_ref[n][1] + iv[3 * i + 2] * dN_ref[n][2];
+        if (!diagonal) {
+            double gu00 = 0, gu10 = 0, gu11 = 0, gu20 = 0, gu21 = 0, gu22 = 0, div_u = 0;
+            for (int n = cmb_shift; n < 6 - surf_shift; ++n) {
+                const int u = WEDGE_TO_UNIQUE[w][n];
+                double s0 = src[hidx[u]], s1 = src[N + hidx[u]], s2 = src[2 * N + hidx[u]];
+                if ((cmb_fs && n < 3) || (surf_fs && n >= 3)) {
+                    const double *nn = nrm[CORNER_OF[w][n % 3]];
+                    const double dot = nn[0] * s0 + nn[1] * s1 + nn[2] * s2;
+                    s0 -= dot * nn[0]; s1 -= dot * nn[1]; s2 -= dot * nn[2];
+                }
+                const double g0 = g[n][0], g1 = g[n][1], g2 = g[n][2];
+                gu00 += g0 * s0; gu11 += g1 * s1; gu22 += g2 * s2;
+                gu10 += 0.5 * (g1 * s0 + g0 * s1);
+                gu20 += 0.5 * (g2 * s0 + g0 * s2);
+                gu21 += 0.5 * (g2 * s1 + g1 * s2);
+                div_u += g0 * s0 + g1 * s1 + g2 * s2;
+            }
+            for (int n = cmb_shift; n < 6 - surf_shift; ++n) {
+                const int u = WEDGE_TO_UNIQUE[w][n];
+                const double g0 = g[n][0], g1 = g[n][1], g2 = g[n][2];
+                dst8[0][u] += kwJ * (2.0 * (g0 * gu00 + g1 * gu10 + g2 * gu20) + NEG_TWO_THIRDS * g0 * div_u);
+                dst8[1][u] += kwJ * (2.0 * (g0 * gu10 + g1 * gu11 + g2 * gu21) + NEG_TWO_THIRDS * g1 * div_u);
+                dst8[2][u] += kwJ * (2.0 * (g0 * gu20 + g1 * gu21 + g2 * gu22) + NEG_TWO_THIRDS * g2 * div_u);
+                if ((cmb_fs && n < 3) || (surf_fs && n >= 3)) {
+                    const int c = CORNER_OF[w][n % 3];
+                    const double gg = g0 * g0 + g1 * g1 + g2 * g2;
+                    const double ng = nrm[c][0] * g0 + nrm[c][1] * g1 + nrm[c][2] * g2;
+                    if (n < 3) Ann_cmb[c] += kwJ * (gg + ONE_THIRD * ng * ng);
+                    else Ann_surf[c] += kwJ * (gg + ONE_THIRD * ng * ng);
+                }
+            }
+        }
+        if (diagonal || cmb_dir || surf_dir) {
+            for (int n = surf_shift; n < 6 - cmb_shift; ++n) {
+                if (diagonal && cmb_fs && n < 3) continue;
+                if (diagonal && surf_fs && n >= 3) continue;
+                const int u = WEDGE_TO_UNIQUE[w][n];
+                const double g0 = g[n][0], g1 = g[n][1], g2 = g[n][2];
+                const double gg = g0 * g0 + g1 * g1 + g2 * g2;
+                dst8[0][u] += kwJ * src[hidx[u]] * (gg + ONE_THIRD * g0 * g0);
+                dst8[1][u] += kwJ * src[N + hidx[u]] * (gg + ONE_THIRD * g1 * g1);
+                dst8[2][u] += kwJ * src[2 * N + hidx[u]] * (gg + ONE_THIRD * g2 * g2);
+            }
+            if (diagonal) {
+                for (int pass = 0; pass < 2; ++pass) {
+                    if (pass == 0 && !cmb_fs) continue;
+                    if (pass == 1 && !surf_fs) continue;
+                    for (int ni = 0; ni < 3; ++ni) {
+                        const int n = ni + 3 * pass;
+                        const int c = CORNER_OF[w][ni];
+                        const int u = WEDGE_TO_UNIQUE[w][n];
+                        const double g0 = g[n][0], g1 = g[n][1], g2 = g[n][2];
+                        const double gg = g0 * g0 + g1 * g1 + g2 * g2;
+                        mat3 R = trafo_nt(v3(nrm[c][0], nrm[c][1], nrm[c][2]));
+                        const double s0 = src[hidx[u]], s1 = src[N + hidx[u]], s2 = src[2 * N + hidx[u]];
+                        double Rg[3], Rs[3], vv[3];
+                        for (int a = 0; a < 3; ++a) {
+                            Rg[a] = R.a[a][0] * g0 + R.a[a][1] * g1 + R.a[a][2] * g2;
+                            Rs[a] = R.a[a][0] * s0 + R.a[a][1] * s1 + R.a[a][2] * s2;
+                        }
+                        for (int a = 0; a < 3; ++a) vv[a] = kwJ * (gg + ONE_THIRD * Rg[a] * Rg[a]) * Rs[a];
+                        for (int a = 0; a < 3; ++a) dst8[a][u] += R.a[0][a] * vv[0] + R.a[1][a] * vv[1] + R.a[2][a] * vv[2];
+                    }
+                }
+            }
+        }
+    }
+    if (!diagonal) {
+        for (int pass = 0; pass < 2; ++pass) {
+            if (pass == 0 && !cmb_fs) continue;
+            if (pass == 1 && !surf_fs) continue;
+            const int *C2U = pass == 0 ? CMB_CORNER_TO_UNIQUE : SURF_CORNER_TO_UNIQUE;
+            const double *Ann = pass == 0 ? Ann_cmb : Ann_surf;
+            for (int c = 0; c < 4; ++c) {
+                const int u = C2U[c];
+                const double dot = nrm[c][0] * dst8[0][u] + nrm[c][1] * dst8[1][u] + nrm[c][2] * dst8[2][u];
+                for (int a = 0; a < 3; ++a) dst8[a][u] -= dot * nrm[c][a];
+            }
+            for (int c = 0; c < 4; ++c) {
+                const int u = C2U[c];
+                const double un = nrm[c][0] * src[hidx[u]] + nrm[c][1] * src[N + hidx[u]] + nrm[c][2] * src[2 * N + hidx[u]];
+                const double corr = Ann[c] * un;
+                for (int a = 0; a < 3; ++a) dst8[a][u] += corr * nrm[c][a];
+            }
+        }
+    }
+    for (int a = 0; a < 3; ++a)
+        for (int u = 0; u < 8; ++u) dst[(size_t)a * N + hidx[u]] += dst8[a][u];
+}
+
+void orc_epsdivdiv_apply_fast(const orc_mesh *m, const double *k, int bc_cmb, int bc_surf, int diagonal,
+                              int apply_mode, int comm, double penalty_eps, const double *src, double *dst,
+                              const double *penalty_modes_in)
+{
+    const int64_t N = m->nalloc;
+    op_ctx o = {m, k, bc_cmb, bc_surf};
+    if (apply_mode == 0) {
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < 3 * N; ++i) dst[i] = 0.0;
+    }
+    const int ncl = m->nx - 1;
+    for (int color = 0; color < 4; ++color) {
+        const int cx = color & 1, cy = color >> 1;
+        const int hx = (ncl - cx + 1) / 2, hy = (ncl - cy + 1) / 2;
+        const int64_t ntask = (int64_t)m->nsd * hx * hy;
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int64_t t = 0; t < ntask; ++t) {
+            const int sd = (int)(t / ((int64_t)hx * hy));
+            const int rem = (int)(t % ((int64_t)hx * hy));
+            const int xc = cx + 2 * (rem / hy), yc = cy + 2 * (rem % hy);
+            for (int rc = 0; rc < m->nr - 1; ++rc) fast_cell(&o, sd, xc, yc, rc, diagonal, src, dst);
+        }
+    }
+    if (bc_cmb == BC_FREESLIP && bc_surf == BC_FREESLIP && !diagonal) {
+        if (penalty_modes_in) {
+            penalty_apply(m, penalty_modes_in, penalty_eps, src, dst);
+        } else {
+            double *nm = (double *)malloc(sizeof(double) * 9 * N);
+            penalty_modes(m, nm);
+            penalty_apply(m, nm, penalty_eps, src, dst);
+            free(nm);
+        }
+    }
+    if (comm) orc_halo_sum(m, dst, 3);
+}
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

@@ -1,0 +1,386 @@
+// C ABI entry points (include/tng.h).  Every call converts library errors
+// into a tng_status and records the message on the context.
+#include <cstring>
+#include <new>
+
+#include "tng_internal.hpp"
+
+using namespace tng;
+
+namespace {
+
+thread_local std::string g_orphan_error;
+
+template <class F> tng_status guarded(tng_context *ctx, F &&f)
+{
+    try {
+        f();
+        return TNG_OK;
+    } catch (const Error &e) {
+        if (ctx) ctx->last_error = e.what();
+        else g_orphan_error = e.what();
+        return e.status;
+    } catch (const std::bad_alloc &) {
+        if (ctx) ctx->last_error = "host allocation failed";
+        return TNG_OUT_OF_MEMORY;
+    } catch (const std::exception &e) {
+        if (ctx) ctx->last_error = e.what();
+        else g_orphan_error = e.what();
+        return TNG_DEVICE_ERROR;
+    }
+}
+
+void set_device(tng_context *ctx) { TNG_HIP_CHECK(hipSetDevice(ctx->device)); }
+
+size_t dtype_size(tng_dtype t) { return t == TNG_F64 ? 8 : 4; }
+
+}  // namespace
+
+extern "C" {
+
+const char *tng_version(void) { return "tng-mi355x 0.1 (gfx950)"; }
+
+const char *tng_last_error(tng_ctx ctx) { return ctx ? ctx->last_error.c_str() : g_orphan_error.c_str(); }
+
+tng_status tng_ctx_create(int device_ordinal, tng_ctx *out)
+{
+    if (!out) return TNG_INVALID_ARGUMENT;
+    *out = nullptr;
+    auto *ctx = new (std::nothrow) tng_context();
+    if (!ctx) return TNG_OUT_OF_MEMORY;
+    tng_status st = guarded(nullptr, [&] {
+        int n = 0;
+        TNG_HIP_CHECK(hipGetDeviceCount(&n));
+        TNG_REQUIRE(device_ordinal >= 0 && device_ordinal < n, "device ordinal out of range");
+        ctx->device = device_ordinal;
+        set_device(ctx);
+        TNG_HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        ctx->red_partials.alloc(4096 * 3 * sizeof(double));
+        ctx->red_counter.alloc(64);
+        TNG_HIP_CHECK(hipMemset(ctx->red_counter.p, 0, 64));
+        ctx->red_result.alloc(4096);
+        TNG_HIP_CHECK(hipDeviceSynchronize());
+    });
+    if (st != TNG_OK) {
+        delete ctx;
+        return st;
+    }
+    *out = ctx;
+    return TNG_OK;
+}
+
+tng_status tng_ctx_destroy(tng_ctx ctx)
+{
+    if (!ctx) return TNG_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &e : ctx->prof_events)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return TNG_OK;
+}
+
+tng_status tng_ctx_synchronize(tng_ctx ctx)
+{
+    if (!ctx) return TNG_INVALID_ARGUMENT;
+    return guarded(ctx, [&] { TNG_HIP_CHECK(hipStreamSynchronize(ctx->stream)); });
+}
+
+tng_status tng_ctx_set_profiling(tng_ctx ctx, int on)
+{
+    if (!ctx) return TNG_INVALID_ARGUMENT;
+    ctx->profiling = on != 0;
+    ctx->prof_used = 0;
+    return TNG_OK;
+}
+
+tng_status tng_ctx_profile_read(tng_ctx ctx, int *num_applies, double *avg_main_kernel_ms, double *avg_total_ms)
+{
+    if (!ctx) return TNG_INVALID_ARGUMENT;
+    return guarded(ctx, [&] {
+        TNG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        double sm = 0, st = 0;
+        for (size_t i = 0; i < ctx->prof_used; ++i) {
+            hipEvent_t *e = &ctx->prof_events[4 * i];
+            float a = 0, b = 0;
+            TNG_HIP_CHECK(hipEventElapsedTime(&a, e[1], e[2]));
+            TNG_HIP_CHECK(hipEventElapsedTime(&b, e[0], e[3]));
+            sm += a;
+            st += b;
+        }
+        const size_t n = ctx->prof_used;
+        if (num_applies) *num_applies = (int)n;
+        if (avg_main_kernel_ms) *avg_main_kernel_ms = n ? sm / n : 0.0;
+        if (avg_total_ms) *avg_total_ms = n ? st / n : 0.0;
+        ctx->prof_used = 0;
+    });
+}
+
+tng_status tng_mesh_create_uniform(tng_ctx ctx, int lateral_level, int radial_level, double r_min, double r_max,
+                                   int lateral_sdr, int radial_sdr, tng_mesh *out)
+{
+    if (!out) return TNG_INVALID_ARGUMENT;
+    *out = nullptr;
+    auto *m = new (std::nothrow) tng_domain();
+    if (!m) return TNG_OUT_OF_MEMORY;
+    m->ctx = ctx;
+    tng_status st = guarded(ctx, [&] {
+        if (ctx) set_device(ctx);
+        mesh_build(*m, lateral_level, radial_level, r_min, r_max, lateral_sdr, radial_sdr, ctx != nullptr);
+        if (ctx) mesh_upload(*m);
+    });
+    if (st != TNG_OK) {
+        delete m;
+        return st;
+    }
+    *out = m;
+    return TNG_OK;
+}
+
+tng_status tng_mesh_destroy(tng_mesh mesh)
+{
+    if (!mesh) return TNG_INVALID_ARGUMENT;
+    if (mesh->ctx) {
+        (void)hipSetDevice(mesh->ctx->device);
+        (void)hipStreamSynchronize(mesh->ctx->stream);
+    }
+    delete mesh;
+    return TNG_OK;
+}
+
+tng_status tng_mesh_get_info(tng_mesh m, tng_mesh_info *info)
+{
+    if (!m || !info) return TNG_INVALID_ARGUMENT;
+    info->num_subdomains = m->nsd;
+    info->nodes_lateral = m->nx;
+    info->nodes_radial = m->nr;
+    info->lateral_level = m->lat_level;
+    info->radial_level = m->rad_level;
+    info->lateral_sdr = m->lat_sdr;
+    info->radial_sdr = m->rad_sdr;
+    info->num_alloc = m->nalloc;
+    info->num_owned = m->num_owned;
+    info->num_interface = m->num_interface;
+    return TNG_OK;
+}
+
+tng_status tng_mesh_copy_coords(tng_mesh m, double *out)
+{
+    if (!m || !out) return TNG_INVALID_ARGUMENT;
+    std::memcpy(out, m->coords.data(), m->coords.size() * sizeof(double));
+    return TNG_OK;
+}
+tng_status tng_mesh_copy_radii(tng_mesh m, double *out)
+{
+    if (!m || !out) return TNG_INVALID_ARGUMENT;
+    std::memcpy(out, m->radii.data(), m->radii.size() * sizeof(double));
+    return TNG_OK;
+}
+tng_status tng_mesh_copy_boundary_mask(tng_mesh m, uint8_t *out)
+{
+    if (!m || !out) return TNG_INVALID_ARGUMENT;
+    std::memcpy(out, m->bmask.data(), m->bmask.size());
+    return TNG_OK;
+}
+tng_status tng_mesh_copy_ownership_mask(tng_mesh m, uint8_t *out)
+{
+    if (!m || !out) return TNG_INVALID_ARGUMENT;
+    std::memcpy(out, m->omask.data(), m->omask.size());
+    return TNG_OK;
+}
+
+tng_status tng_vec_create(tng_mesh mesh, tng_dtype dtype, int ncomp, tng_vec *out)
+{
+    if (!mesh || !out || (ncomp != 1 && ncomp != 3) || (dtype != TNG_F64 && dtype != TNG_F32))
+        return TNG_INVALID_ARGUMENT;
+    if (!mesh->ctx) return TNG_INVALID_ARGUMENT;  // host-only mesh
+    *out = nullptr;
+    auto *v = new (std::nothrow) tng_vector();
+    if (!v) return TNG_OUT_OF_MEMORY;
+    tng_status st = guarded(mesh->ctx, [&] {
+        set_device(mesh->ctx);
+        v->mesh = mesh;
+        v->dtype = dtype;
+        v->ncomp = ncomp;
+        // components start on 256-byte boundaries
+        const int64_t align = 256 / (int64_t)dtype_size(dtype);
+        v->comp_stride = (mesh->nalloc + align - 1) / align * align;
+        v->buf.alloc((size_t)v->comp_stride * ncomp * dtype_size(dtype));
+        launch_fill(*v, 0.0, mesh->ctx->stream);
+    });
+    if (st != TNG_OK) {
+        delete v;
+        return st;
+    }
+    *out = v;
+    return TNG_OK;
+}
+
+tng_status tng_vec_destroy(tng_vec v)
+{
+    if (!v) return TNG_INVALID_ARGUMENT;
+    (void)hipSetDevice(v->mesh->ctx->device);
+    (void)hipStreamSynchronize(v->mesh->ctx->stream);
+    delete v;
+    return TNG_OK;
+}
+
+tng_status tng_vec_upload(tng_vec v, const void *host)
+{
+    if (!v || !host) return TNG_INVALID_ARGUMENT;
+    return guarded(v->mesh->ctx, [&] {
+        set_device(v->mesh->ctx);
+        const size_t bytes = (size_t)v->mesh->nalloc * v->elem_size();
+        for (int c = 0; c < v->ncomp; ++c)
+            TNG_HIP_CHECK(hipMemcpyAsync(v->comp(c), static_cast<const char *>(host) + c * bytes, bytes,
+                                         hipMemcpyHostToDevice, v->mesh->ctx->stream));
+        TNG_HIP_CHECK(hipStreamSynchronize(v->mesh->ctx->stream));
+    });
+}
+
+tng_status tng_vec_download(tng_vec v, void *host)
+{
+    if (!v || !host) return TNG_INVALID_ARGUMENT;
+    return guarded(v->mesh->ctx, [&] {
+        set_device(v->mesh->ctx);
+        const size_t bytes = (size_t)v->mesh->nalloc * v->elem_size();
+        for (int c = 0; c < v->ncomp; ++c)
+            TNG_HIP_CHECK(hipMemcpyAsync(static_cast<char *>(host) + c * bytes, v->comp(c), bytes,
+                                         hipMemcpyDeviceToHost, v->mesh->ctx->stream));
+        TNG_HIP_CHECK(hipStreamSynchronize(v->mesh->ctx->stream));
+    });
+}
+
+tng_status tng_vec_device_ptr(tng_vec v, int comp, void **ptr)
+{
+    if (!v || !ptr || comp < 0 || comp >= v->ncomp) return TNG_INVALID_ARGUMENT;
+    *ptr = v->comp(comp);
+    return TNG_OK;
+}
+
+tng_status tng_vec_fill(tng_vec v, double value)
+{
+    if (!v) return TNG_INVALID_ARGUMENT;
+    return guarded(v->mesh->ctx, [&] {
+        set_device(v->mesh->ctx);
+        launch_fill(*v, value, v->mesh->ctx->stream);
+    });
+}
+
+tng_status tng_vec_copy(tng_vec dst, tng_vec src)
+{
+    if (!dst || !src) return TNG_INVALID_ARGUMENT;
+    return guarded(dst->mesh->ctx, [&] {
+        set_device(dst->mesh->ctx);
+        launch_copy(*dst, *src, dst->mesh->ctx->stream);
+    });
+}
+
+tng_status tng_epsdivdiv_create(tng_mesh mesh, tng_vec k, tng_bc bc_cmb, tng_bc bc_surface, int diagonal, tng_op *out)
+{
+    if (!mesh || !k || !out || !mesh->ctx) return TNG_INVALID_ARGUMENT;
+    *out = nullptr;
+    auto *op = new (std::nothrow) tng_operator();
+    if (!op) return TNG_OUT_OF_MEMORY;
+    tng_status st = guarded(mesh->ctx, [&] {
+        TNG_REQUIRE(k->mesh == mesh && k->ncomp == 1, "coefficient must be a scalar field on the operator's mesh");
+        TNG_REQUIRE(bc_cmb >= TNG_NEUMANN && bc_cmb <= TNG_FREESLIP && bc_surface >= TNG_NEUMANN &&
+                        bc_surface <= TNG_FREESLIP,
+                    "invalid boundary condition");
+        set_device(mesh->ctx);
+        op->kind = OP_EPSDIVDIV;
+        op->mesh = mesh;
+        op->k = k;
+        op->bc_cmb = bc_cmb;
+        op->bc_surf = bc_surface;
+        op->diagonal = diagonal != 0;
+        op->penalty_active = bc_cmb == TNG_FREESLIP && bc_surface == TNG_FREESLIP;
+        op->d_penalty_b.alloc(4 * sizeof(double));
+        TNG_HIP_CHECK(hipMemset(op->d_penalty_b.p, 0, 4 * sizeof(double)));
+        epsdivdiv_update_boundary_table(*op, mesh->ctx->stream);
+    });
+    if (st != TNG_OK) {
+        delete op;
+        return st;
+    }
+    *out = op;
+    return TNG_OK;
+}
+
+tng_status tng_op_set_modes(tng_op op, tng_apply_mode apply_mode, tng_comm_mode comm_mode)
+{
+    if (!op || (apply_mode != TNG_REPLACE && apply_mode != TNG_ADD) ||
+        (comm_mode != TNG_SKIP_COMMUNICATION && comm_mode != TNG_COMMUNICATE_ADDITIVELY))
+        return TNG_INVALID_ARGUMENT;
+    op->apply_mode = apply_mode;
+    op->comm_mode = comm_mode;
+    return TNG_OK;
+}
+
+tng_status tng_op_set_diagonal(tng_op op, int diagonal)
+{
+    if (!op) return TNG_INVALID_ARGUMENT;
+    op->diagonal = diagonal != 0;
+    return TNG_OK;
+}
+
+tng_status tng_op_set_boundary_conditions(tng_op op, tng_bc bc_cmb, tng_bc bc_surface)
+{
+    if (!op || bc_cmb < TNG_NEUMANN || bc_cmb > TNG_FREESLIP || bc_surface < TNG_NEUMANN || bc_surface > TNG_FREESLIP)
+        return TNG_INVALID_ARGUMENT;
+    op->bc_cmb = bc_cmb;
+    op->bc_surf = bc_surface;
+    op->btab_valid = false;  // the penalty stays as armed at creation, as in the reference
+    return TNG_OK;
+}
+
+tng_status tng_op_set_penalty_epsilon(tng_op op, double eps)
+{
+    if (!op) return TNG_INVALID_ARGUMENT;
+    op->penalty_eps = eps;
+    return TNG_OK;
+}
+
+tng_status tng_op_coefficient_updated(tng_op op)
+{
+    if (!op) return TNG_INVALID_ARGUMENT;
+    op->btab_valid = false;
+    return TNG_OK;
+}
+
+tng_status tng_op_apply(tng_op op, tng_vec src, tng_vec dst)
+{
+    if (!op || !src || !dst) return TNG_INVALID_ARGUMENT;
+    tng_context *ctx = op->mesh->ctx;
+    return guarded(ctx, [&] {
+        TNG_REQUIRE(src != dst, "apply: src and dst must be different vectors");
+        TNG_REQUIRE(src->mesh == op->mesh && dst->mesh == op->mesh, "apply: vectors must live on the operator's mesh");
+        TNG_REQUIRE(src->ncomp == 3 && dst->ncomp == 3, "apply: EpsilonDivDiv acts on 3-component vectors");
+        TNG_REQUIRE(src->dtype == op->k->dtype && dst->dtype == op->k->dtype,
+                    "apply: src, dst and coefficient must share one dtype");
+        set_device(ctx);
+        epsdivdiv_apply(*op, *src, *dst, ctx->stream);
+    });
+}
+
+tng_status tng_op_destroy(tng_op op)
+{
+    if (!op) return TNG_INVALID_ARGUMENT;
+    (void)hipSetDevice(op->mesh->ctx->device);
+    (void)hipStreamSynchronize(op->mesh->ctx->stream);
+    delete op;
+    return TNG_OK;
+}
+
+}  // extern "C"
+
+hipEvent_t *tng_context::prof_next_set()
+{
+    if (4 * (prof_used + 1) > prof_events.size()) {
+        const size_t old = prof_events.size();
+        prof_events.resize(old + 4 * 64, nullptr);
+        for (size_t i = old; i < prof_events.size(); ++i) TNG_HIP_CHECK(hipEventCreate(&prof_events[i]));
+    }
+    return &prof_events[4 * prof_used++];
+}

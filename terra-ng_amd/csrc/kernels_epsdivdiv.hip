@@ -1,0 +1,731 @@
+// EpsilonDivDiv viscous operator (2 eta eps(u):eps(v) - 2/3 eta div u div v) on
+// Q1 wedges of the thick spherical shell, one-point quadrature at the wedge
+// centroid (reference: fe/wedge/operators/shell/epsilon_divdiv_kerngen.hpp,
+// fast paths :1245-1607 (Dirichlet/Neumann) and :1638-2198 (free-slip)).
+//
+// Formulation used here.  A wedge with lateral unit-sphere corners p0, p1, p2
+// between shells r0 < r1 has, at the centroid, the Jacobian
+//     J = L * diag(r_mid, r_mid, h),  L = [p1-p0, p2-p0, (p0+p1+p2)/3],
+//     r_mid = (r0+r1)/2, h = (r1-r0)/2,
+// so J^{-T} = L^{-T} diag(1/r_mid, 1/r_mid, 1/h) and |det J| = |det L| r_mid^2 h.
+// L^{-T} (columns M0, M1, M2) and |det L| depend only on the lateral triangle
+// and are precomputed once per mesh.  With sigma_k = u_k(bottom) + u_k(top)
+// and delta_k = u_k(top) - u_k(bottom) at the three lateral corners, the
+// velocity gradient at the centroid is
+//     grad u = (1/(2 r_mid)) [(sigma_1-sigma_0) M0^T + (sigma_2-sigma_0) M1^T]
+//            + (1/(6 h)) (delta_0+delta_1+delta_2) M2^T,
+// the stress is tau = k |det J| (2 sym(grad u) - 2/3 div u I) with k the mean
+// of the six nodal coefficients, and the test-side contribution to corner k is
+// tau g_k with g_k = J^{-T} grad N_k.  This is the reference's arithmetic
+// regrouped (about 130 flops per wedge instead of about 450).
+//
+// Work decomposition.  Lanes run along r (one radial cell per lane, K cells
+// per lane for more than 64 layers) so every global access is a contiguous
+// run of the r-fastest layout.  A "virtual column" is a group of R lanes that
+// owns one lateral cell column (xc) and sweeps y.  The VX virtual columns of
+// a workgroup cover the cell columns XG-1 .. XG+VX-2; virtual column v
+// outputs node column XG+v.  The contributions of a cell to its left corners
+// belong to the neighbour on the left and are passed through LDS once per
+// y-step; radial neighbours are combined with a lane shuffle.  Every dst
+// entry is written exactly once with a plain store: no atomics, no zeroing
+// pass, deterministic results.
+#include "tng_internal.hpp"
+
+namespace tng {
+
+namespace {
+
+template <typename T> struct ApplyArgs {
+    const T *src0, *src1, *src2;
+    T *dst0, *dst1, *dst2;
+    const T *k;
+    const T *radii;     // (nsd, nr)
+    const T *latgeom;   // (nsd, nc, nc, 2, 10)
+    const T *coords;    // (nsd, nx, nx, 3)
+    const T *btab;      // (nsd, nx, nx, 2, 3) boundary rows, or nullptr
+    const double *pen;  // 3 doubles (penalty axis b), or nullptr
+    const uint8_t *sd_cmb, *sd_surf;
+    int nsd, nx, nr, ncells;
+    int bc_cmb, bc_surf;
+    int add;
+    int strips, segs, seg_len;
+};
+
+template <typename T> __device__ __forceinline__ void normal_of(const T *c, T n[3])
+{
+    const T n2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+    const T inv = T(1) / sqrt(n2);
+    n[0] = c[0] * inv;
+    n[1] = c[1] * inv;
+    n[2] = c[2] * inv;
+}
+
+// Per-lane data of one node column for the cells of the lane.
+template <typename T, int K> struct NodeCol {
+    T su[K][3];  // u(bottom) + u(top)
+    T du[K][3];  // u(top) - u(bottom)
+    T kk[K];     // k(bottom) + k(top)
+};
+
+template <typename T, int K>
+__device__ __forceinline__ void load_col(const ApplyArgs<T> &a, int sd, int X, int Y, int rl, bool has_cmb,
+                                         bool has_surf, NodeCol<T, K> &o)
+{
+    const bool ok = X >= 0 && X < a.nx && Y >= 0 && Y < a.nx;
+    const int64_t base = ((((int64_t)sd * a.nx + X) * a.nx) + Y) * a.nr;
+    T nrm[3] = {T(0), T(0), T(0)};
+    const bool need_n = ok && ((has_cmb && a.bc_cmb == TNG_FREESLIP) || (has_surf && a.bc_surf == TNG_FREESLIP));
+    if (need_n) normal_of(a.coords + 3 * (((int64_t)sd * a.nx + X) * a.nx + Y), nrm);
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+        const int l = rl * K + p;
+        T lo[3] = {T(0), T(0), T(0)}, hi[3] = {T(0), T(0), T(0)}, klo = T(0), khi = T(0);
+        if (ok && l < a.ncells) {
+            lo[0] = a.src0[base + l]; hi[0] = a.src0[base + l + 1];
+            lo[1] = a.src1[base + l]; hi[1] = a.src1[base + l + 1];
+            lo[2] = a.src2[base + l]; hi[2] = a.src2[base + l + 1];
+            klo = a.k[base + l]; khi = a.k[base + l + 1];
+            if (l == 0 && has_cmb) {
+                if (a.bc_cmb == TNG_DIRICHLET) {
+                    lo[0] = lo[1] = lo[2] = T(0);
+                } else if (a.bc_cmb == TNG_FREESLIP) {
+                    const T d = nrm[0] * lo[0] + nrm[1] * lo[1] + nrm[2] * lo[2];
+                    lo[0] -= d * nrm[0]; lo[1] -= d * nrm[1]; lo[2] -= d * nrm[2];
+                }
+            }
+            if (l + 1 == a.nr - 1 && has_surf) {
+                if (a.bc_surf == TNG_DIRICHLET) {
+                    hi[0] = hi[1] = hi[2] = T(0);
+                } else if (a.bc_surf == TNG_FREESLIP) {
+                    const T d = nrm[0] * hi[0] + nrm[1] * hi[1] + nrm[2] * hi[2];
+                    hi[0] -= d * nrm[0]; hi[1] -= d * nrm[1]; hi[2] -= d * nrm[2];
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            o.su[p][c] = lo[c] + hi[c];
+            o.du[p][c] = hi[c] - lo[c];
+        }
+        o.kk[p] = klo + khi;
+    }
+}
+
+// One wedge: corners (P0, P1, P2) in wedge order; writes the lateral part
+// (per corner) and the radial part (common to the three corners).
+template <typename T>
+__device__ __forceinline__ void wedge_stress(const T *g /*M0,M1,M2,detL*/, const T su0[3], const T su1[3],
+                                             const T su2[3], const T du0[3], const T du1[3], const T du2[3],
+                                             T kksum, T hs, T t, T q, T cvol, T A0[3], T A1[3], T Cc[3])
+{
+    T e1[3], e2[3], D[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        e1[d] = hs * (su1[d] - su0[d]);
+        e2[d] = hs * (su2[d] - su0[d]);
+        D[d] = t * (du0[d] + du1[d] + du2[d]);
+    }
+    const T *M0 = g, *M1 = g + 3, *M2 = g + 6;
+    T G[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int e = 0; e < 3; ++e) G[d][e] = e1[d] * M0[e] + e2[d] * M1[e] + D[d] * M2[e];
+    const T div = G[0][0] + G[1][1] + G[2][2];
+    // f = hs * k_eval * |det J| = hs * (kksum / 6) * |det L| * r_mid^2 h
+    const T f = cvol * kksum * g[9];
+    const T f2 = T(2) * f;
+    const T fd = T(-2.0 / 3.0) * f * div;
+    T tau[3][3];
+    tau[0][0] = f2 * G[0][0] + fd;
+    tau[1][1] = f2 * G[1][1] + fd;
+    tau[2][2] = f2 * G[2][2] + fd;
+    tau[0][1] = tau[1][0] = f * (G[0][1] + G[1][0]);
+    tau[0][2] = tau[2][0] = f * (G[0][2] + G[2][0]);
+    tau[1][2] = tau[2][1] = f * (G[1][2] + G[2][1]);
+    T qM2[3] = {q * M2[0], q * M2[1], q * M2[2]};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        A0[d] = tau[d][0] * M0[0] + tau[d][1] * M0[1] + tau[d][2] * M0[2];
+        A1[d] = tau[d][0] * M1[0] + tau[d][1] * M1[1] + tau[d][2] * M1[2];
+        Cc[d] = tau[d][0] * qM2[0] + tau[d][1] * qM2[1] + tau[d][2] * qM2[2];
+    }
+}
+
+template <typename T, int R, int K, int NW>
+__global__ __launch_bounds__(NW * 64, 2) void epsdivdiv_apply_kernel(const ApplyArgs<T> a)
+{
+    constexpr int VPW = 64 / R;
+    constexpr int VX = NW * VPW;
+    constexpr int KL = K + 1;
+    // [buf][v][corner 00|01][level][comp][lane]
+    __shared__ T lds[2][VX][2][KL][3][R];
+
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int vw = lane / R, rl = lane % R;
+    const int v = wave * VPW + vw;
+
+    int b = blockIdx.x;
+    const int seg = b % a.segs;
+    b /= a.segs;
+    const int strip = b % a.strips;
+    const int sd = b / a.strips;
+    const int nx = a.nx, nc = nx - 1;
+    const int XG = strip * (VX - 1);
+    const int xc = XG - 1 + v;
+    const int X = xc + 1;
+    const int Y0 = seg * a.seg_len;
+    const int Yend = min(Y0 + a.seg_len, nx);
+    const bool has_cmb = a.sd_cmb[sd] != 0, has_surf = a.sd_surf[sd] != 0;
+
+    // per-lane radial constants
+    T hs[K], tt[K], qq[K], cv[K];
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+        const int l = rl * K + p;
+        hs[p] = tt[p] = qq[p] = cv[p] = T(0);
+        if (l < a.ncells) {
+            const T r0 = a.radii[(int64_t)sd * a.nr + l], r1 = a.radii[(int64_t)sd * a.nr + l + 1];
+            const T rmid = T(0.5) * (r0 + r1), h = T(0.5) * (r1 - r0);
+            hs[p] = T(0.5) / rmid;
+            tt[p] = T(1) / (T(6) * h);
+            qq[p] = tt[p] / hs[p];
+            cv[p] = hs[p] * rmid * rmid * h * T(1.0 / 6.0);
+        }
+    }
+
+    NodeCol<T, K> B0, B1, T0, T1;
+    load_col<T, K>(a, sd, xc, Y0 - 1, rl, has_cmb, has_surf, B0);
+    load_col<T, K>(a, sd, xc + 1, Y0 - 1, rl, has_cmb, has_surf, B1);
+
+    T acc_cur[KL][3], acc_next[KL][3];
+#pragma unroll
+    for (int i = 0; i < KL; ++i)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc_cur[i][c] = acc_next[i][c] = T(0);
+
+    int step = 0;
+    for (int yc = Y0 - 1; yc < Yend; ++yc, ++step) {
+        load_col<T, K>(a, sd, xc, yc + 1, rl, has_cmb, has_surf, T0);
+        load_col<T, K>(a, sd, xc + 1, yc + 1, rl, has_cmb, has_surf, T1);
+        T s00[KL][3], s01[KL][3];
+#pragma unroll
+        for (int i = 0; i < KL; ++i)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) s00[i][c] = s01[i][c] = T(0);
+
+        const bool cell_ok = xc >= 0 && xc < nc && yc >= 0 && yc < nc;
+        if (cell_ok) {
+            const T *gw = a.latgeom + ((((int64_t)sd * nc + xc) * nc + yc) * 2) * 10;
+            T g0[10], g1[10];
+#pragma unroll
+            for (int i = 0; i < 10; ++i) {
+                g0[i] = gw[i];
+                g1[i] = gw[10 + i];
+            }
+#pragma unroll
+            for (int p = 0; p < K; ++p) {
+                if (rl * K + p >= a.ncells) continue;
+                T A0[3], A1[3], Cc[3];
+                // wedge 0: (c00, c10, c01)
+                wedge_stress<T>(g0, B0.su[p], B1.su[p], T0.su[p], B0.du[p], B1.du[p], T0.du[p],
+                                B0.kk[p] + B1.kk[p] + T0.kk[p], hs[p], tt[p], qq[p], cv[p], A0, A1, Cc);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const T x00 = -(A0[c] + A1[c]);
+                    s00[p][c] += x00 - Cc[c];
+                    s00[p + 1][c] += x00 + Cc[c];
+                    acc_cur[p][c] += A0[c] - Cc[c];
+                    acc_cur[p + 1][c] += A0[c] + Cc[c];
+                    s01[p][c] += A1[c] - Cc[c];
+                    s01[p + 1][c] += A1[c] + Cc[c];
+                }
+                // wedge 1: (c11, c01, c10)
+                wedge_stress<T>(g1, T1.su[p], T0.su[p], B1.su[p], T1.du[p], T0.du[p], B1.du[p],
+                                T1.kk[p] + T0.kk[p] + B1.kk[p], hs[p], tt[p], qq[p], cv[p], A0, A1, Cc);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const T x11 = -(A0[c] + A1[c]);
+                    acc_next[p][c] += x11 - Cc[c];
+                    acc_next[p + 1][c] += x11 + Cc[c];
+                    s01[p][c] += A0[c] - Cc[c];
+                    s01[p + 1][c] += A0[c] + Cc[c];
+                    acc_cur[p][c] += A1[c] - Cc[c];
+                    acc_cur[p + 1][c] += A1[c] + Cc[c];
+                }
+            }
+        }
+
+        // left-corner contributions -> neighbour v-1
+        const int buf = step & 1;
+#pragma unroll
+        for (int i = 0; i < KL; ++i)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                lds[buf][v][0][i][c][rl] = s00[i][c];
+                lds[buf][v][1][i][c][rl] = s01[i][c];
+            }
+        __syncthreads();
+        if (v + 1 < VX) {
+#pragma unroll
+            for (int i = 0; i < KL; ++i)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    acc_cur[i][c] += lds[buf][v + 1][0][i][c][rl];
+                    acc_next[i][c] += lds[buf][v + 1][1][i][c][rl];
+                }
+        }
+
+        // node row yc is complete: combine radially and store
+        T below[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const T up = __shfl_up(acc_cur[K][c], 1, R);
+            below[c] = rl > 0 ? up : T(0);
+        }
+        const int Y = yc;
+        if (Y >= Y0 && v < VX - 1 && X < nx && rl * K < a.ncells) {
+            const int64_t lcol = ((int64_t)sd * nx + X) * nx + Y;
+            const int64_t base = lcol * a.nr;
+#pragma unroll
+            for (int p = 0; p < KL; ++p) {
+                const int l = rl * K + p;
+                if (l > a.ncells) continue;
+                if (p == K && l != a.ncells) continue;
+                T val[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) val[c] = acc_cur[p][c] + (p == 0 ? below[c] : T(0));
+                const bool at_cmb = (l == 0) && has_cmb;
+                const bool at_surf = (l == a.nr - 1) && has_surf;
+                if (at_cmb || at_surf) {
+                    const int bc = at_cmb ? a.bc_cmb : a.bc_surf;
+                    if (bc != TNG_NEUMANN) {
+                        const T *bt = a.btab + (lcol * 2 + (at_cmb ? 0 : 1)) * 3;
+                        const T u0 = a.src0[base + l], u1 = a.src1[base + l], u2 = a.src2[base + l];
+                        if (bc == TNG_DIRICHLET) {
+                            val[0] = bt[0] * u0;
+                            val[1] = bt[1] * u1;
+                            val[2] = bt[2] * u2;
+                        } else {
+                            T n[3];
+                            normal_of(a.coords + 3 * lcol, n);
+                            const T dv = n[0] * val[0] + n[1] * val[1] + n[2] * val[2];
+                            const T un = bt[0] * (n[0] * u0 + n[1] * u1 + n[2] * u2);
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) val[c] = val[c] - dv * n[c] + un * n[c];
+                        }
+                    }
+                }
+                if (a.pen) {
+                    const T rad = a.radii[(int64_t)sd * a.nr + l];
+                    const T *cc = a.coords + 3 * lcol;
+                    const T x0 = cc[0] * rad, x1 = cc[1] * rad, x2 = cc[2] * rad;
+                    const T b0 = (T)a.pen[0], b1 = (T)a.pen[1], b2 = (T)a.pen[2];
+                    val[0] += b1 * x2 - b2 * x1;
+                    val[1] += b2 * x0 - b0 * x2;
+                    val[2] += b0 * x1 - b1 * x0;
+                }
+                if (a.add) {
+                    val[0] += a.dst0[base + l];
+                    val[1] += a.dst1[base + l];
+                    val[2] += a.dst2[base + l];
+                }
+                a.dst0[base + l] = val[0];
+                a.dst1[base + l] = val[1];
+                a.dst2[base + l] = val[2];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < KL; ++i)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                acc_cur[i][c] = acc_next[i][c];
+                acc_next[i][c] = T(0);
+            }
+        B0 = T0;
+        B1 = T1;
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Diagonal (per node, gather over the <= 12 adjacent wedges).
+//   mode 0: dst = D(src) for every node (Diagonal mode of the reference):
+//           D = sum kwJ (|g|^2 + g_d^2/3) per Cartesian component; at
+//           free-slip nodes R^T diag(R A_nn R^T) R (reference :2018-2073).
+//   mode 1: boundary rows for the matvec: per lateral node and side
+//           (0 = CMB at r = 0, 1 = surface at r = nr-1): Dirichlet diagonal
+//           (3 values) or free-slip normal-normal entry Ann (value 0).
+// ----------------------------------------------------------------------------
+template <typename T> struct DiagArgs {
+    const T *src0, *src1, *src2;
+    T *dst0, *dst1, *dst2;
+    const T *k, *radii, *latgeom, *coords;
+    T *btab;
+    const uint8_t *sd_cmb, *sd_surf;
+    int nsd, nx, nr, ncells, bc_cmb, bc_surf, add, mode;
+};
+
+// linalg/trafo/local_basis_trafo_normal_tangential.hpp:34-76
+template <typename T> __device__ void trafo_nt(const T nin[3], T R[3][3])
+{
+    T n[3] = {nin[0], nin[1], nin[2]};
+    {
+        T s = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+        T mag = sqrt(s);
+        if (mag > T(1e-15))
+            for (int i = 0; i < 3; ++i) n[i] /= mag;
+    }
+    const T ax = fabs(n[0]), ay = fabs(n[1]), az = fabs(n[2]);
+    T ref[3] = {T(0), T(0), T(0)};
+    if (ax <= ay && ax <= az) ref[0] = T(1);
+    else if (ay <= ax && ay <= az) ref[1] = T(1);
+    else ref[2] = T(1);
+    auto cross = [](const T *u, const T *w, T *o) {
+        o[0] = u[1] * w[2] - u[2] * w[1];
+        o[1] = u[2] * w[0] - u[0] * w[2];
+        o[2] = u[0] * w[1] - u[1] * w[0];
+    };
+    auto normalize = [](T *v) {
+        T s = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        T mag = sqrt(s);
+        if (mag > T(1e-15))
+            for (int i = 0; i < 3; ++i) v[i] /= mag;
+    };
+    T t1[3], t2[3];
+    cross(ref, n, t1);
+    normalize(t1);
+    const T eps = sizeof(T) == 8 ? T(1e-15) : T(1e-6);
+    if (sqrt(t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2]) < eps) {
+        t1[0] = n[1]; t1[1] = -n[0]; t1[2] = T(0);
+        normalize(t1);
+    }
+    cross(n, t1, t2);
+    normalize(t2);
+    for (int i = 0; i < 3; ++i) { R[0][i] = n[i]; R[1][i] = t1[i]; R[2][i] = t2[i]; }
+}
+
+template <typename T> __global__ void epsdivdiv_diag_kernel(const DiagArgs<T> a)
+{
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx = a.nx, nr = a.nr, nc = nx - 1;
+    int sd, X, Y, l;
+    int side = -1;
+    if (a.mode == 0) {
+        const int64_t n = (int64_t)a.nsd * nx * nx * nr;
+        if (gid >= n) return;
+        l = (int)(gid % nr);
+        int64_t t = gid / nr;
+        Y = (int)(t % nx); t /= nx;
+        X = (int)(t % nx);
+        sd = (int)(t / nx);
+    } else {
+        const int64_t n = (int64_t)a.nsd * nx * nx * 2;
+        if (gid >= n) return;
+        side = (int)(gid % 2);
+        int64_t t = gid / 2;
+        Y = (int)(t % nx); t /= nx;
+        X = (int)(t % nx);
+        sd = (int)(t / nx);
+        l = side == 0 ? 0 : nr - 1;
+    }
+    const bool at_cmb = l == 0 && a.sd_cmb[sd];
+    const bool at_surf = l == nr - 1 && a.sd_surf[sd];
+    const int bc = at_cmb ? a.bc_cmb : (at_surf ? a.bc_surf : TNG_NEUMANN);
+    const bool fs = (at_cmb || at_surf) && bc == TNG_FREESLIP;
+    const int64_t lcol = ((int64_t)sd * nx + X) * nx + Y;
+    T R[3][3] = {{T(1), T(0), T(0)}, {T(0), T(1), T(0)}, {T(0), T(0), T(1)}};
+    if (fs) {
+        T n[3];
+        normal_of(a.coords + 3 * lcol, n);
+        trafo_nt(n, R);
+    }
+    T dd[3] = {T(0), T(0), T(0)}, dr[3] = {T(0), T(0), T(0)};
+    // lateral corner index of (dx, dy) in wedge w; -1 if not a corner
+    const int corner_of[2][2][2] = {{{0, 2}, {1, -1}}, {{-1, 1}, {2, 0}}};  // [w][dx][dy]
+    const int wc[2][3][2] = {{{0, 0}, {1, 0}, {0, 1}}, {{1, 1}, {0, 1}, {1, 0}}};
+    for (int dxc = -1; dxc <= 0; ++dxc) {
+        const int xc = X + dxc;
+        if (xc < 0 || xc >= nc) continue;
+        for (int dyc = -1; dyc <= 0; ++dyc) {
+            const int yc = Y + dyc;
+            if (yc < 0 || yc >= nc) continue;
+            for (int w = 0; w < 2; ++w) {
+                const int kc = corner_of[w][X - xc][Y - yc];
+                if (kc < 0) continue;
+                const T *g = a.latgeom + ((((int64_t)sd * nc + xc) * nc + yc) * 2 + w) * 10;
+                const T l0 = kc == 0 ? T(-0.5) : (kc == 1 ? T(0.5) : T(0));
+                const T l1 = kc == 0 ? T(-0.5) : (kc == 1 ? T(0) : T(0.5));
+                for (int rc = l - 1; rc <= l; ++rc) {
+                    if (rc < 0 || rc >= a.ncells) continue;
+                    const T r0 = a.radii[(int64_t)sd * nr + rc], r1 = a.radii[(int64_t)sd * nr + rc + 1];
+                    const T rmid = T(0.5) * (r0 + r1), h = T(0.5) * (r1 - r0);
+                    const T s = T(1) / rmid;
+                    const T dz = (rc == l) ? T(-1.0 / 6.0) : T(1.0 / 6.0);
+                    T gv[3];
+                    for (int d = 0; d < 3; ++d) gv[d] = s * (l0 * g[d] + l1 * g[3 + d]) + (dz / h) * g[6 + d];
+                    T ks = T(0);
+                    for (int q = 0; q < 3; ++q) {
+                        const int64_t cb = (((int64_t)sd * nx + xc + wc[w][q][0]) * nx + yc + wc[w][q][1]) * nr;
+                        ks += a.k[cb + rc] + a.k[cb + rc + 1];
+                    }
+                    const T kwJ = ks * T(1.0 / 6.0) * g[9] * rmid * rmid * h;
+                    const T gg = gv[0] * gv[0] + gv[1] * gv[1] + gv[2] * gv[2];
+                    for (int d = 0; d < 3; ++d) dd[d] += kwJ * (gg + T(1.0 / 3.0) * gv[d] * gv[d]);
+                    if (fs)
+                        for (int c = 0; c < 3; ++c) {
+                            const T rg = R[c][0] * gv[0] + R[c][1] * gv[1] + R[c][2] * gv[2];
+                            dr[c] += kwJ * (gg + T(1.0 / 3.0) * rg * rg);
+                        }
+                }
+            }
+        }
+    }
+    if (a.mode == 1) {
+        T *o = a.btab + (lcol * 2 + side) * 3;
+        if (fs) {
+            o[0] = dr[0]; o[1] = T(0); o[2] = T(0);
+        } else {
+            o[0] = dd[0]; o[1] = dd[1]; o[2] = dd[2];
+        }
+        return;
+    }
+    const int64_t i = lcol * nr + l;
+    const T s0 = a.src0[i], s1 = a.src1[i], s2 = a.src2[i];
+    T out[3];
+    if (fs) {
+        T rs[3];
+        for (int c = 0; c < 3; ++c) rs[c] = dr[c] * (R[c][0] * s0 + R[c][1] * s1 + R[c][2] * s2);
+        for (int d = 0; d < 3; ++d) out[d] = R[0][d] * rs[0] + R[1][d] * rs[1] + R[2][d] * rs[2];
+    } else {
+        out[0] = dd[0] * s0; out[1] = dd[1] * s1; out[2] = dd[2] * s2;
+    }
+    if (a.add) {
+        out[0] += a.dst0[i]; out[1] += a.dst1[i]; out[2] += a.dst2[i];
+    }
+    a.dst0[i] = out[0]; a.dst1[i] = out[1]; a.dst2[i] = out[2];
+}
+
+// ----------------------------------------------------------------------------
+// fs/fs rotation penalty, first half: m = sum_OWNED x cross u, then on the
+// last block b = eps * G^{-1} m (G = Gram matrix of the rotation fields).
+// Two-stage deterministic reduction: per-block partials in fixed order, the
+// block that takes the last ticket sums them and resets the ticket.
+// ----------------------------------------------------------------------------
+template <typename T> struct PenArgs {
+    const T *src0, *src1, *src2;
+    const T *coords, *radii;
+    const uint8_t *omask;
+    double *partials;
+    unsigned int *counter;
+    double *out_b;
+    double eps;
+    double ginv[9];
+    int nx, nr;
+    int64_t n;
+};
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <typename T> __global__ __launch_bounds__(256) void penalty_moment_kernel(const PenArgs<T> a)
+{
+    __shared__ double red[4][3];
+    __shared__ bool last;
+    double m0 = 0, m1 = 0, m2 = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (a.omask[i] != OWNED) continue;
+        const int l = (int)(i % a.nr);
+        const int64_t lcol = i / a.nr;
+        const int sd = (int)(lcol / ((int64_t)a.nx * a.nx));
+        const double rad = (double)a.radii[(int64_t)sd * a.nr + l];
+        const double x0 = (double)a.coords[3 * lcol] * rad, x1 = (double)a.coords[3 * lcol + 1] * rad,
+                     x2 = (double)a.coords[3 * lcol + 2] * rad;
+        const double u0 = a.src0[i], u1 = a.src1[i], u2 = a.src2[i];
+        m0 += x1 * u2 - x2 * u1;
+        m1 += x2 * u0 - x0 * u2;
+        m2 += x0 * u1 - x1 * u0;
+    }
+    m0 = wave_sum(m0); m1 = wave_sum(m1); m2 = wave_sum(m2);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) { red[wave][0] = m0; red[wave][1] = m1; red[wave][2] = m2; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s[3] = {0, 0, 0};
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+            for (int c = 0; c < 3; ++c) s[c] += red[w][c];
+        for (int c = 0; c < 3; ++c) a.partials[3 * blockIdx.x + c] = s[c];
+        __threadfence();
+        const unsigned int ticket = atomicAdd(a.counter, 1u);
+        last = (ticket == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        __threadfence();
+        double m[3] = {0, 0, 0};
+        for (unsigned int bb = 0; bb < gridDim.x; ++bb)
+            for (int c = 0; c < 3; ++c) {
+                const unsigned long long bits = __hip_atomic_load(
+                    reinterpret_cast<unsigned long long *>(&a.partials[3 * bb + c]), __ATOMIC_RELAXED,
+                    __HIP_MEMORY_SCOPE_AGENT);
+                m[c] += __longlong_as_double((long long)bits);
+            }
+        for (int r = 0; r < 3; ++r)
+            a.out_b[r] = a.eps * (a.ginv[3 * r] * m[0] + a.ginv[3 * r + 1] * m[1] + a.ginv[3 * r + 2] * m[2]);
+        atomicExch(a.counter, 0u);
+    }
+}
+
+constexpr int kPenBlocks = 512;
+
+template <typename T> struct MeshPtrs {
+    const T *coords, *radii, *latgeom;
+};
+template <typename T> MeshPtrs<T> mesh_ptrs(const tng_domain &m);
+template <> MeshPtrs<double> mesh_ptrs<double>(const tng_domain &m)
+{
+    return {m.d_coords.as<double>(), m.d_radii.as<double>(), m.d_latgeom.as<double>()};
+}
+template <> MeshPtrs<float> mesh_ptrs<float>(const tng_domain &m)
+{
+    return {m.d_coords_f.as<float>(), m.d_radii_f.as<float>(), m.d_latgeom_f.as<float>()};
+}
+
+template <typename T, int R, int K, int NW> void launch_apply_rk(const ApplyArgs<T> &a, int blocks, hipStream_t s)
+{
+    hipLaunchKernelGGL((epsdivdiv_apply_kernel<T, R, K, NW>), dim3(blocks), dim3(NW * 64), 0, s, a);
+}
+
+template <typename T> void launch_apply(ApplyArgs<T> a, hipStream_t s)
+{
+    constexpr int NW = 4;
+    int R, K;
+    if (a.ncells <= 64) {
+        K = 1;
+        R = 4;
+        while (R < a.ncells) R *= 2;
+    } else {
+        R = 64;
+        K = (a.ncells + 63) / 64;
+        if (K == 3) K = 4;
+        TNG_REQUIRE(K <= 4, "more than 256 radial layers per subdomain is not supported");
+    }
+    const int VX = NW * (64 / R);
+    a.strips = (a.nx + (VX - 1) - 1) / (VX - 1);
+    // segment length: aim for enough workgroups to fill the chip
+    const int64_t lateral_groups = (int64_t)a.nsd * a.strips;
+    int seg = a.nx;
+    while (seg > 8 && lateral_groups * ((a.nx + seg - 1) / seg) < 2048) seg = std::max(8, (seg + 1) / 2);
+    a.seg_len = seg;
+    a.segs = (a.nx + seg - 1) / seg;
+    const int blocks = (int)(lateral_groups * a.segs);
+    if (K == 1) {
+        switch (R) {
+        case 4: launch_apply_rk<T, 4, 1, NW>(a, blocks, s); break;
+        case 8: launch_apply_rk<T, 8, 1, NW>(a, blocks, s); break;
+        case 16: launch_apply_rk<T, 16, 1, NW>(a, blocks, s); break;
+        case 32: launch_apply_rk<T, 32, 1, NW>(a, blocks, s); break;
+        default: launch_apply_rk<T, 64, 1, NW>(a, blocks, s); break;
+        }
+    } else if (K == 2) {
+        launch_apply_rk<T, 64, 2, NW>(a, blocks, s);
+    } else {
+        launch_apply_rk<T, 64, 4, NW>(a, blocks, s);
+    }
+    TNG_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T> void run_diag_kernel(tng_operator &op, const tng_vector *src, tng_vector *dst, int mode, int add,
+                                           hipStream_t s)
+{
+    const tng_domain &m = *op.mesh;
+    DiagArgs<T> a{};
+    const MeshPtrs<T> mp = mesh_ptrs<T>(m);
+    if (src) {
+        a.src0 = (const T *)src->comp(0); a.src1 = (const T *)src->comp(1); a.src2 = (const T *)src->comp(2);
+    }
+    if (dst) {
+        a.dst0 = (T *)dst->comp(0); a.dst1 = (T *)dst->comp(1); a.dst2 = (T *)dst->comp(2);
+    }
+    a.k = (const T *)op.k->comp(0);
+    a.radii = mp.radii; a.latgeom = mp.latgeom; a.coords = mp.coords;
+    a.btab = op.d_btab.as<T>();
+    a.sd_cmb = m.d_sd_cmb.as<uint8_t>(); a.sd_surf = m.d_sd_surf.as<uint8_t>();
+    a.nsd = m.nsd; a.nx = m.nx; a.nr = m.nr; a.ncells = m.nr - 1;
+    a.bc_cmb = op.bc_cmb; a.bc_surf = op.bc_surf; a.add = add; a.mode = mode;
+    const int64_t n = mode == 0 ? m.nalloc : (int64_t)m.nlat * 2;
+    const int bs = 256;
+    hipLaunchKernelGGL(epsdivdiv_diag_kernel<T>, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s, a);
+    TNG_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T> void apply_impl(tng_operator &op, const tng_vector &src, tng_vector &dst, hipStream_t s)
+{
+    tng_domain &m = *op.mesh;
+    tng_context &ctx = *m.ctx;
+    const bool add = op.apply_mode == TNG_ADD;
+    hipEvent_t *ev = ctx.profiling ? ctx.prof_next_set() : nullptr;
+    if (ev) TNG_HIP_CHECK(hipEventRecord(ev[0], s));
+    if (op.diagonal) {
+        if (ev) TNG_HIP_CHECK(hipEventRecord(ev[1], s));
+        run_diag_kernel<T>(op, &src, &dst, 0, add, s);
+        if (ev) TNG_HIP_CHECK(hipEventRecord(ev[2], s));
+    } else {
+        if (!op.btab_valid) epsdivdiv_update_boundary_table(op, s);
+        const bool pen = op.penalty_active;
+        if (pen) {
+            PenArgs<T> p{};
+            const MeshPtrs<T> mp = mesh_ptrs<T>(m);
+            p.src0 = (const T *)src.comp(0); p.src1 = (const T *)src.comp(1); p.src2 = (const T *)src.comp(2);
+            p.coords = mp.coords; p.radii = mp.radii; p.omask = m.d_omask.as<uint8_t>();
+            p.partials = ctx.red_partials.as<double>();
+            p.counter = ctx.red_counter.as<unsigned int>();
+            p.out_b = op.d_penalty_b.as<double>();
+            p.eps = op.penalty_eps;
+            for (int i = 0; i < 9; ++i) p.ginv[i] = m.rot_gram_inv[i];
+            p.nx = m.nx; p.nr = m.nr; p.n = m.nalloc;
+            hipLaunchKernelGGL(penalty_moment_kernel<T>, dim3(kPenBlocks), dim3(256), 0, s, p);
+            TNG_HIP_CHECK(hipGetLastError());
+        }
+        ApplyArgs<T> a{};
+        const MeshPtrs<T> mp = mesh_ptrs<T>(m);
+        a.src0 = (const T *)src.comp(0); a.src1 = (const T *)src.comp(1); a.src2 = (const T *)src.comp(2);
+        a.dst0 = (T *)dst.comp(0); a.dst1 = (T *)dst.comp(1); a.dst2 = (T *)dst.comp(2);
+        a.k = (const T *)op.k->comp(0);
+        a.radii = mp.radii; a.latgeom = mp.latgeom; a.coords = mp.coords;
+        a.btab = op.d_btab.as<T>();
+        a.pen = pen ? op.d_penalty_b.as<double>() : nullptr;
+        a.sd_cmb = m.d_sd_cmb.as<uint8_t>(); a.sd_surf = m.d_sd_surf.as<uint8_t>();
+        a.nsd = m.nsd; a.nx = m.nx; a.nr = m.nr; a.ncells = m.nr - 1;
+        a.bc_cmb = op.bc_cmb; a.bc_surf = op.bc_surf; a.add = add;
+        if (ev) TNG_HIP_CHECK(hipEventRecord(ev[1], s));
+        launch_apply<T>(a, s);
+        if (ev) TNG_HIP_CHECK(hipEventRecord(ev[2], s));
+    }
+    if (op.comm_mode == TNG_COMMUNICATE_ADDITIVELY) launch_halo_sum(m, dst, s);
+    if (ev) TNG_HIP_CHECK(hipEventRecord(ev[3], s));
+}
+
+}  // namespace
+
+void epsdivdiv_update_boundary_table(tng_operator &op, hipStream_t s)
+{
+    const tng_domain &m = *op.mesh;
+    const size_t es = op.k->elem_size();
+    const size_t bytes = (size_t)m.nlat * 2 * 3 * es;
+    if (op.d_btab.bytes != bytes) op.d_btab.alloc(bytes);
+    if (op.k->dtype == TNG_F64) run_diag_kernel<double>(op, nullptr, nullptr, 1, 0, s);
+    else run_diag_kernel<float>(op, nullptr, nullptr, 1, 0, s);
+    op.btab_valid = true;
+}
+
+void epsdivdiv_apply(tng_operator &op, const tng_vector &src, tng_vector &dst, hipStream_t s)
+{
+    if (src.dtype == TNG_F64) apply_impl<double>(op, src, dst, s);
+    else apply_impl<float>(op, src, dst, s);
+}
+
+}  // namespace tng

@@ -1,0 +1,159 @@
+// Internal declarations of the tng library (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "tng.h"
+
+namespace tng {
+
+// Status-carrying exception used inside the library; converted to tng_status
+// at the C ABI boundary.
+struct Error : std::runtime_error {
+    tng_status status;
+    Error(tng_status s, const std::string &msg) : std::runtime_error(msg), status(s) {}
+};
+
+#define TNG_HIP_CHECK(expr)                                                                                    \
+    do {                                                                                                       \
+        hipError_t e_ = (expr);                                                                                \
+        if (e_ != hipSuccess)                                                                                  \
+            throw ::tng::Error(e_ == hipErrorOutOfMemory ? TNG_OUT_OF_MEMORY : TNG_DEVICE_ERROR,               \
+                               std::string(#expr) + ": " + hipGetErrorString(e_));                            \
+    } while (0)
+
+#define TNG_REQUIRE(cond, msg)                                                                                 \
+    do {                                                                                                       \
+        if (!(cond)) throw ::tng::Error(TNG_INVALID_ARGUMENT, msg);                                            \
+    } while (0)
+
+// Shell boundary flags (grid/shell/bit_masks.hpp:11-19) and ownership flag
+// (grid/bit_masks.hpp:16-20), same bit values as the reference.
+enum : uint8_t { SBF_INNER = 1, SBF_CMB = 2 | 4, SBF_SURFACE = 2 | 8 };
+enum : uint8_t { OWNED = 1 };
+
+// Device buffer RAII.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    void alloc(size_t n)
+    {
+        release();
+        if (n) TNG_HIP_CHECK(hipMalloc(&p, n));
+        bytes = n;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+}  // namespace tng
+
+struct tng_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    bool profiling = false;
+    // Scratch for two-stage reductions: partial sums + a ticket counter.
+    tng::DevBuf red_partials;   // doubles
+    tng::DevBuf red_counter;    // unsigned int, kept at zero between launches
+    tng::DevBuf red_result;     // small device result slots (doubles)
+    // Profiling: per apply four events (begin, main kernel begin/end, end)
+    // recorded on the stream without synchronising; read back in bulk.
+    std::vector<hipEvent_t> prof_events;
+    size_t prof_used = 0;  // event sets recorded since the last read
+    hipEvent_t *prof_next_set();
+};
+
+// One multigrid level of the thick spherical shell: DistributedDomain plus
+// coordinates, radii, masks, halo plan and the precomputed lateral wedge
+// geometry.
+struct tng_domain {
+    tng_context *ctx = nullptr;
+    int lat_level = 0, rad_level = 0, lat_sdr = 0, rad_sdr = 0;
+    int S = 1, Sr = 1;          // subdomains per diamond side / radially
+    int nsd = 0, nx = 0, nr = 0; // nx == ny
+    int64_t nlat = 0;           // nsd * nx * nx
+    int64_t nalloc = 0;         // nlat * nr
+    int64_t num_owned = 0, num_interface = 0;
+    double r_min = 0, r_max = 0;
+
+    std::vector<int> sd_info;      // (diamond, sx, sy, sr) per local subdomain
+    std::vector<double> coords;    // (sd, x, y, 3)
+    std::vector<double> radii;     // (sd, r)
+    std::vector<uint8_t> bmask;    // (sd, x, y, r)
+    std::vector<uint8_t> omask;    // (sd, x, y, r)
+    std::vector<uint8_t> sd_has_cmb, sd_has_surf;
+
+    // device copies
+    tng::DevBuf d_coords, d_radii, d_omask, d_sd_cmb, d_sd_surf;
+    tng::DevBuf d_coords_f, d_radii_f;
+    // lateral wedge geometry: (sd, xc, yc, wedge, 10) = (M0, M1, M2, |det L|),
+    // M = columns of L^{-T}, L = [p1-p0, p2-p0, (p0+p1+p2)/3]
+    tng::DevBuf d_latgeom, d_latgeom_f;
+    // halo plan: one entry per node that has >= 2 copies; copy slots in SoA
+    // [maxc][nmulti] of int32 linear indices (-1 = unused).
+    int halo_maxc = 0;
+    int64_t halo_n = 0;
+    tng::DevBuf d_halo_idx;
+    // Gram matrix of the rotation fields e_j x x over OWNED nodes (inverse).
+    double rot_gram_inv[9] = {0};
+
+    int64_t lin(int sd, int x, int y, int r) const { return (((int64_t)sd * nx + x) * nx + y) * nr + r; }
+};
+
+struct tng_vector {
+    tng_domain *mesh = nullptr;
+    tng_dtype dtype = TNG_F64;
+    int ncomp = 1;
+    int64_t comp_stride = 0;  // elements between components
+    tng::DevBuf buf;
+    size_t elem_size() const { return dtype == TNG_F64 ? 8 : 4; }
+    void *comp(int c) const { return static_cast<char *>(buf.p) + (size_t)c * comp_stride * elem_size(); }
+};
+
+enum OpKind { OP_EPSDIVDIV = 1 };
+
+struct tng_operator {
+    OpKind kind = OP_EPSDIVDIV;
+    tng_domain *mesh = nullptr;
+    tng_vector *k = nullptr;
+    int bc_cmb = TNG_NEUMANN, bc_surf = TNG_NEUMANN;
+    bool diagonal = false;
+    int apply_mode = TNG_REPLACE, comm_mode = TNG_COMMUNICATE_ADDITIVELY;
+    bool penalty_active = false;
+    double penalty_eps = 1e-7;
+    // boundary-row data (sd, x, y, side, 3): Dirichlet diagonal or free-slip Ann
+    tng::DevBuf d_btab;
+    bool btab_valid = false;
+    tng::DevBuf d_penalty_b;  // 3 doubles: eps * G^{-1} m, written on device
+};
+
+namespace tng {
+
+void mesh_build(tng_domain &m, int lat_level, int rad_level, double r_min, double r_max, int lat_sdr, int rad_sdr,
+                bool device);
+void mesh_upload(tng_domain &m);
+
+// kernel launchers (kernels_*.hip)
+void launch_fill(tng_vector &v, double value, hipStream_t s);
+void launch_copy(tng_vector &dst, const tng_vector &src, hipStream_t s);
+void launch_halo_sum(const tng_domain &m, tng_vector &v, hipStream_t s);
+void epsdivdiv_update_boundary_table(tng_operator &op, hipStream_t s);
+void epsdivdiv_apply(tng_operator &op, const tng_vector &src, tng_vector &dst, hipStream_t s);
+
+}  // namespace tng

@@ -1,0 +1,293 @@
+"""Python front end of libtng (ctypes over the C ABI in include/tng.h).
+
+Mirrors the reference's C++ interface for the Stokes hot path so that tests
+read like the reference's own: ``DistributedDomain.create_uniform``,
+``VectorQ1Scalar`` / ``VectorQ1Vec``, ``EpsilonDivDivKerngen`` with
+``set_operator_apply_and_communication_modes``, ``set_diagonal``,
+``set_penalty_epsilon`` and ``apply``.
+
+The library must be built (terra-ng_amd/build/libtng.so); there is no
+fallback: loading fails loudly when it is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "build", "libtng.so"))
+
+# grid/shell/bit_masks.hpp BoundaryConditionFlag values
+NEUMANN, DIRICHLET, FREESLIP = 0, 1, 2
+# linalg/operator.hpp OperatorApplyMode / OperatorCommunicationMode
+REPLACE, ADD = 0, 1
+SKIP_COMMUNICATION, COMMUNICATE_ADDITIVELY = 0, 1
+F64, F32 = 0, 1
+
+# grid/shell/bit_masks.hpp ShellBoundaryFlag, grid/bit_masks.hpp NodeOwnershipFlag
+INNER, CMB, SURFACE, OWNED = 1, 6, 10, 1
+
+_STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "DEVICE_ERROR", 4: "NOT_CONVERGED",
+           5: "NONFINITE", 6: "UNSUPPORTED"}
+
+
+class TngError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{_STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class _MeshInfo(C.Structure):
+    _fields_ = [("num_subdomains", C.c_int), ("nodes_lateral", C.c_int), ("nodes_radial", C.c_int),
+                ("lateral_level", C.c_int), ("radial_level", C.c_int), ("lateral_sdr", C.c_int),
+                ("radial_sdr", C.c_int), ("num_alloc", C.c_longlong), ("num_owned", C.c_longlong),
+                ("num_interface", C.c_longlong)]
+
+
+# (name, restype, argtypes) of every exported entry point of include/tng.h
+_vp, _i, _d = C.c_void_p, C.c_int, C.c_double
+_pv = C.POINTER(C.c_void_p)
+EXPORTS = [
+    ("tng_version", C.c_char_p, []),
+    ("tng_last_error", C.c_char_p, [_vp]),
+    ("tng_ctx_create", _i, [_i, _pv]),
+    ("tng_ctx_destroy", _i, [_vp]),
+    ("tng_ctx_synchronize", _i, [_vp]),
+    ("tng_ctx_set_profiling", _i, [_vp, _i]),
+    ("tng_mesh_create_uniform", _i, [_vp, _i, _i, _d, _d, _i, _i, _pv]),
+    ("tng_mesh_destroy", _i, [_vp]),
+    ("tng_mesh_get_info", _i, [_vp, C.POINTER(_MeshInfo)]),
+    ("tng_mesh_copy_coords", _i, [_vp, _vp]),
+    ("tng_mesh_copy_radii", _i, [_vp, _vp]),
+    ("tng_mesh_copy_boundary_mask", _i, [_vp, _vp]),
+    ("tng_mesh_copy_ownership_mask", _i, [_vp, _vp]),
+    ("tng_vec_create", _i, [_vp, _i, _i, _pv]),
+    ("tng_vec_destroy", _i, [_vp]),
+    ("tng_vec_upload", _i, [_vp, _vp]),
+    ("tng_vec_download", _i, [_vp, _vp]),
+    ("tng_vec_device_ptr", _i, [_vp, _i, _pv]),
+    ("tng_vec_fill", _i, [_vp, _d]),
+    ("tng_vec_copy", _i, [_vp, _vp]),
+    ("tng_epsdivdiv_create", _i, [_vp, _vp, _i, _i, _i, _pv]),
+    ("tng_op_set_modes", _i, [_vp, _i, _i]),
+    ("tng_op_set_diagonal", _i, [_vp, _i]),
+    ("tng_op_set_boundary_conditions", _i, [_vp, _i, _i]),
+    ("tng_op_set_penalty_epsilon", _i, [_vp, _d]),
+    ("tng_op_coefficient_updated", _i, [_vp]),
+    ("tng_op_apply", _i, [_vp, _vp, _vp]),
+    ("tng_op_destroy", _i, [_vp]),
+    ("tng_ctx_profile_read", _i, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+]
+
+_lib = None
+
+
+def load_library():
+    """Load libtng.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libtng.so not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, rt, args in EXPORTS:
+            f = getattr(L, name)
+            f.restype = rt
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class Context:
+    """Device + stream (util/init.hpp:75-80 terra_initialize)."""
+
+    def __init__(self, device=0):
+        L = load_library()
+        h = C.c_void_p()
+        st = L.tng_ctx_create(device, C.byref(h))
+        if st != 0:
+            raise TngError(st, L.tng_last_error(None).decode())
+        self.h = h
+
+    def check(self, st):
+        if st != 0:
+            raise TngError(st, load_library().tng_last_error(self.h).decode())
+
+    def synchronize(self):
+        self.check(load_library().tng_ctx_synchronize(self.h))
+
+    def set_profiling(self, on=True):
+        self.check(load_library().tng_ctx_set_profiling(self.h, int(on)))
+
+    def profile_read(self):
+        """(applies, avg main-kernel ms, avg whole-apply ms) since the last read."""
+        n, a, b = C.c_int(), C.c_double(), C.c_double()
+        self.check(load_library().tng_ctx_profile_read(self.h, C.byref(n), C.byref(a), C.byref(b)))
+        return n.value, a.value, b.value
+
+    def close(self):
+        if self.h:
+            load_library().tng_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class DistributedDomain:
+    """One level of the thick spherical shell (spherical_shell.hpp:2563-2711)."""
+
+    def __init__(self, ctx, h):
+        self.ctx, self.h = ctx, h
+        info = _MeshInfo()
+        ctx.check(load_library().tng_mesh_get_info(h, C.byref(info)))
+        self.info = info
+        self.nsd, self.nx, self.nr = info.num_subdomains, info.nodes_lateral, info.nodes_radial
+        self.shape = (self.nsd, self.nx, self.nx, self.nr)
+        self.num_alloc, self.num_owned = info.num_alloc, info.num_owned
+
+    @classmethod
+    def create_uniform(cls, lateral_level, radial_level, r_min, r_max, lateral_sdr=0, radial_sdr=0, ctx=None):
+        ctx = ctx or default_context()
+        h = C.c_void_p()
+        ctx.check(load_library().tng_mesh_create_uniform(ctx.h, lateral_level, radial_level, r_min, r_max,
+                                                         lateral_sdr, radial_sdr, C.byref(h)))
+        return cls(ctx, h)
+
+    def _copy(self, fn, shape, dtype):
+        out = np.empty(shape, dtype=dtype)
+        self.ctx.check(getattr(load_library(), fn)(self.h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def coords(self):
+        return self._copy("tng_mesh_copy_coords", (self.nsd, self.nx, self.nx, 3), np.float64)
+
+    def radii(self):
+        return self._copy("tng_mesh_copy_radii", (self.nsd, self.nr), np.float64)
+
+    def boundary_mask(self):
+        return self._copy("tng_mesh_copy_boundary_mask", self.shape, np.uint8)
+
+    def ownership_mask(self):
+        return self._copy("tng_mesh_copy_ownership_mask", self.shape, np.uint8)
+
+    def __del__(self):
+        try:
+            if self.h:
+                load_library().tng_mesh_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class _Vector:
+    ncomp = 1
+
+    def __init__(self, domain, dtype=F64):
+        self.domain, self.dtype = domain, dtype
+        h = C.c_void_p()
+        domain.ctx.check(load_library().tng_vec_create(domain.h, dtype, self.ncomp, C.byref(h)))
+        self.h = h
+
+    @property
+    def np_dtype(self):
+        return np.float64 if self.dtype == F64 else np.float32
+
+    @property
+    def host_shape(self):
+        return ((self.ncomp,) if self.ncomp > 1 else ()) + self.domain.shape
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(np.asarray(arr, dtype=self.np_dtype).reshape(self.host_shape))
+        self.domain.ctx.check(load_library().tng_vec_upload(self.h, a.ctypes.data_as(C.c_void_p)))
+        return self
+
+    def download(self):
+        out = np.empty(self.host_shape, dtype=self.np_dtype)
+        self.domain.ctx.check(load_library().tng_vec_download(self.h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def fill(self, value):
+        self.domain.ctx.check(load_library().tng_vec_fill(self.h, float(value)))
+
+    def assign(self, other):
+        self.domain.ctx.check(load_library().tng_vec_copy(self.h, other.h))
+
+    def device_ptr(self, comp=0):
+        p = C.c_void_p()
+        self.domain.ctx.check(load_library().tng_vec_device_ptr(self.h, comp, C.byref(p)))
+        return p.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                load_library().tng_vec_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class VectorQ1Scalar(_Vector):
+    """linalg/vector_q1.hpp:19-155"""
+    ncomp = 1
+
+
+class VectorQ1Vec(_Vector):
+    """linalg/vector_q1.hpp:166-304"""
+    ncomp = 3
+
+
+class EpsilonDivDivKerngen:
+    """fe/wedge/operators/shell/epsilon_divdiv_kerngen.hpp:70-2393 (the HIP path)."""
+
+    def __init__(self, domain, k, bcs=(DIRICHLET, DIRICHLET), diagonal=False):
+        self.domain, self.k = domain, k
+        h = C.c_void_p()
+        domain.ctx.check(load_library().tng_epsdivdiv_create(domain.h, k.h, int(bcs[0]), int(bcs[1]),
+                                                             int(diagonal), C.byref(h)))
+        self.h = h
+
+    def set_operator_apply_and_communication_modes(self, apply_mode, comm_mode):
+        self.domain.ctx.check(load_library().tng_op_set_modes(self.h, apply_mode, comm_mode))
+
+    def set_diagonal(self, v):
+        self.domain.ctx.check(load_library().tng_op_set_diagonal(self.h, int(v)))
+
+    def set_boundary_conditions(self, bcs):
+        self.domain.ctx.check(load_library().tng_op_set_boundary_conditions(self.h, int(bcs[0]), int(bcs[1])))
+
+    def set_penalty_epsilon(self, eps):
+        self.domain.ctx.check(load_library().tng_op_set_penalty_epsilon(self.h, float(eps)))
+
+    def coefficient_updated(self):
+        self.domain.ctx.check(load_library().tng_op_coefficient_updated(self.h))
+
+    def apply(self, src, dst):
+        self.domain.ctx.check(load_library().tng_op_apply(self.h, src.h, dst.h))
+
+    def __del__(self):
+        try:
+            if self.h:
+                load_library().tng_op_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def apply(op, src, dst):
+    """linalg::apply (linalg/operator.hpp:155-159)."""
+    op.apply(src, dst)

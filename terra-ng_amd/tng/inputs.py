@@ -1,0 +1,105 @@
+"""Seeded synthetic node fields for the benchmark and test workloads.
+
+A value belongs to a physical node, not to a node copy: every copy of a
+duplicated interface node gets the value of the node, so fields are
+consistent across subdomains.  The value of a node is a function of
+SplitMix64(seed, component) and a canonical node id that does not depend on
+the subdomain split:
+
+    id = ((d * (E+1) + i) * (E+1) + j) * (Lr+1) + g
+
+with d the lowest diamond holding the node, (i, j) the node's global lateral
+index in that diamond (E = 2^lateral_level) and g its global shell index
+(Lr = 2^radial_level).
+
+uniform: 53-bit mantissa mapped to [-1, 1).
+normal : Box-Muller on two uniforms.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+_M = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(x):
+    x = np.asarray(x, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = x + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def subdomain_info(lateral_sdr, radial_sdr):
+    """(diamond, sx, sy, sr) per local subdomain, enumeration order diamond, x, y, r."""
+    S, Sr = 1 << lateral_sdr, 1 << radial_sdr
+    out = [(d, x, y, r) for d in range(10) for x in range(S) for y in range(S) for r in range(Sr)]
+    return np.array(out, dtype=np.int64)
+
+
+def node_ids(coords, lateral_level, radial_level, lateral_sdr, radial_sdr, nr):
+    """Canonical node id for every node copy, shape (nsd, nx, nx, nr)."""
+    nsd, nx = coords.shape[0], coords.shape[1]
+    E, Lr = 1 << lateral_level, 1 << radial_level
+    info = subdomain_info(lateral_sdr, radial_sdr)
+    assert info.shape[0] == nsd
+    base = E // (1 << lateral_sdr)
+    d = np.broadcast_to(info[:, 0, None, None], (nsd, nx, nx))
+    i = info[:, 1, None, None] * base + np.arange(nx)[None, :, None]
+    j = info[:, 2, None, None] * base + np.arange(nx)[None, None, :]
+    i = np.broadcast_to(i, (nsd, nx, nx))
+    j = np.broadcast_to(j, (nsd, nx, nx))
+    key = np.ascontiguousarray(coords.reshape(-1, 3)).view(np.uint64)
+    _, inv = np.unique(key, axis=0, return_inverse=True)
+    inv = inv.ravel()
+    nlat = inv.max() + 1
+    lat_id = (d.ravel() * (E + 1) + i.ravel()) * (E + 1) + j.ravel()
+    # owner lateral id: copy with the lowest diamond (ties share (i, j))
+    order = np.lexsort((d.ravel(), inv))
+    first = np.ones(order.size, dtype=bool)
+    first[1:] = inv[order][1:] != inv[order][:-1]
+    owner_lat = np.empty(nlat, dtype=np.int64)
+    owner_lat[inv[order][first]] = lat_id[order][first]
+    lat_canon = owner_lat[inv].reshape(nsd, nx, nx)
+    sr = info[:, 3]
+    g = sr[:, None] * (nr - 1) + np.arange(nr)[None, :]  # (nsd, nr)
+    return (lat_canon[..., None] * (Lr + 1) + g[:, None, None, :]).astype(np.uint64)
+
+
+def _stream(seed, comp, ids):
+    s = splitmix64(np.uint64(seed) * np.uint64(256) + np.uint64(comp))
+    return splitmix64(s ^ ids)
+
+
+def uniform(seed, comp, ids):
+    h = _stream(seed, comp, ids)
+    return (h >> np.uint64(11)).astype(np.float64) * (2.0 ** -53) * 2.0 - 1.0
+
+
+def normal(seed, comp, ids):
+    h1 = _stream(seed, comp, ids)
+    h2 = splitmix64(h1)
+    u1 = ((h1 >> np.uint64(11)).astype(np.float64) + 1.0) * (2.0 ** -53)
+    u2 = (h2 >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+
+
+class Workload:
+    """Node ids and radii of one mesh, for generating fields."""
+
+    def __init__(self, coords, radii, lateral_level, radial_level, lateral_sdr=0, radial_sdr=0):
+        nr = radii.shape[1]
+        self.ids = node_ids(coords, lateral_level, radial_level, lateral_sdr, radial_sdr, nr)
+        self.shape = self.ids.shape
+        self.r = np.broadcast_to(radii[:, None, None, :], self.shape)
+
+    def velocity(self, seed=101):
+        return np.stack([uniform(seed, c, self.ids) for c in range(3)])
+
+    def eta_lognormal(self, seed=102, sigma=1.0):
+        return np.exp(sigma * normal(seed, 0, self.ids))
+
+    def eta_layered(self, seed=102, jump=1e3, r_jump=0.85, sigma=0.5):
+        """Layered viscosity: jump below r_jump times log-normal(sigma) noise."""
+        return np.where(self.r < r_jump, jump, 1.0) * np.exp(sigma * normal(seed, 0, self.ids))
