@@ -37,6 +37,9 @@ def main():
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="budget of the CPU oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only the plain vector-copy kernel on the workload's vector size "
+                         "(reference point for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -54,6 +57,17 @@ def main():
     print(f"[bench] building level-{L} shell", file=sys.stderr, flush=True)
     dom = tng.DistributedDomain.create_uniform(L, L, 0.55, 1.0)
     ctx = dom.ctx
+    if args.calibrate_copy:
+        a = tng.VectorQ1Vec(dom, dt)
+        b = tng.VectorQ1Vec(dom, dt)
+        a.fill(1.0)
+        for _ in range(args.warmup + args.steps):
+            b.assign(a)
+        ctx.synchronize()
+        print(json.dumps({"calibration": "copy_kernel", "bytes_read_per_dispatch": dom.num_alloc * esize,
+                          "bytes_written_per_dispatch": dom.num_alloc * esize,
+                          "dispatches": 3 * (args.warmup + args.steps)}))
+        return
     w = inputs.Workload(dom.coords(), dom.radii(), L, L)
     k = w.eta_layered(102, jump=1e3, r_jump=0.85, sigma=0.5)
     u = w.velocity(101)

@@ -140,6 +140,43 @@ tng_status tng_op_coefficient_updated(tng_op op);
 tng_status tng_op_apply(tng_op op, tng_vec src, tng_vec dst);
 tng_status tng_op_destroy(tng_op op);
 
+/* fe/wedge/operators/shell/divergence_kerngen.hpp:121-257 DivergenceKerngen
+ * (fine velocity mesh -> coarse pressure mesh, one refinement apart): B u,
+ * Dirichlet test rows zeroed, free-slip trial projection, Replace +
+ * CommunicateAdditively by default (tng_op_set_modes applies). */
+tng_status tng_divergence_create(tng_mesh fine, tng_mesh coarse, tng_bc bc_cmb, tng_bc bc_surface, tng_op *out);
+/* fe/wedge/operators/shell/gradient_kerngen.hpp:126-258 GradientKerngen
+ * (coarse pressure -> fine velocity): B^T p with the same boundary handling. */
+tng_status tng_gradient_create(tng_mesh fine, tng_mesh coarse, tng_bc bc_cmb, tng_bc bc_surface, tng_op *out);
+/* fe/wedge/operators/shell/prolongation_constant.hpp:14-200
+ * ProlongationConstant / ProlongationVecConstant (coarse -> fine, 1 or 3
+ * components chosen by the vectors); no communication. */
+tng_status tng_prolongation_create(tng_mesh coarse, tng_mesh fine, tng_op *out);
+/* fe/wedge/operators/shell/restriction_constant.hpp:18-230
+ * RestrictionConstant / RestrictionVecConstant (fine -> coarse): transpose
+ * of the prolongation over OWNED fine nodes, additive exchange. */
+tng_status tng_restriction_create(tng_mesh fine, tng_mesh coarse, tng_op *out);
+/* fe/wedge/operators/shell/kmass.hpp:44-190 KMass with lumped_diagonal =
+ * true (the Schur complement approximation of the Stokes preconditioner):
+ * dst = diag(row sums of the k-weighted mass matrix) src, additive exchange. */
+tng_status tng_kmass_lumped_create(tng_mesh mesh, tng_vec k, tng_op *out);
+
+/* ------------------------------------------------- vector linear algebra */
+
+/* linalg/vector.hpp:72-95 lincomb: y = sum_i c[i] x[i] + c0, 0 <= n <= 3;
+ * x[i] may alias y. */
+tng_status tng_vec_lincomb(tng_vec y, int n, const double *c, const tng_vec *x, double c0);
+/* linalg/vector.hpp:137-144 scale_in_place: y *= x entrywise */
+tng_status tng_vec_scale_with_vector(tng_vec y, tng_vec x);
+/* linalg/vector.hpp:127-135 invert_entries: y = 1 / y entrywise */
+tng_status tng_vec_invert(tng_vec y);
+/* linalg/vector_q1.hpp:96-102, 245-251 dot over OWNED nodes (synchronises) */
+tng_status tng_vec_dot(tng_vec x, tng_vec y, double *out);
+/* linalg/vector_q1.hpp:121, 270 max_abs_entry over all copies (synchronises) */
+tng_status tng_vec_max_abs(tng_vec x, double *out);
+/* linalg/vector.hpp:189-197 has_nan_or_inf (synchronises) */
+tng_status tng_vec_has_nan_or_inf(tng_vec x, int *out);
+
 /* ---------------------------------------------------------------- timing */
 
 /* Kernel timing with HIP events on the context stream (util/timer.hpp:22-389

@@ -51,6 +51,11 @@ def lib():
         L.orc_epsdivdiv_apply_slow.argtypes = [vp, dp, ip, ip, ip, ip, ip, C.c_double, dp, dp]
         L.orc_epsdivdiv_apply_fast.argtypes = [vp, dp, ip, ip, ip, ip, ip, C.c_double, dp, dp, dp]
         L.orc_penalty_modes.argtypes = [vp, dp]
+        L.orc_divergence_apply.argtypes = [vp, vp, ip, ip, ip, ip, dp, dp]
+        L.orc_gradient_apply.argtypes = [vp, vp, ip, ip, ip, ip, dp, dp]
+        L.orc_prolongate.argtypes = [vp, vp, ip, ip, dp, dp]
+        L.orc_restrict.argtypes = [vp, vp, ip, ip, dp, dp]
+        L.orc_kmass_lumped_apply.argtypes = [vp, dp, dp, dp]
         L.orc_num_threads.restype = ip
         _lib = L
     return _lib
@@ -138,7 +143,7 @@ class Mesh:
         """EpsilonDivDivKerngen::apply_impl restated (slow or fast path)."""
         k = np.ascontiguousarray(k, dtype=np.float64)
         src = np.ascontiguousarray(src, dtype=np.float64)
-        out = np.zeros((3,) + self.shape) if dst is None else np.array(dst, dtype=np.float64, copy=True)
+        out = np.zeros((3,) + self.shape) if dst is None else np.array(dst, dtype=np.float64, order="C", copy=True)
         if path == "slow":
             lib().orc_epsdivdiv_apply_slow(self.h, _dp(k), bc_cmb, bc_surf, int(diagonal), apply_mode, comm,
                                            penalty_eps, _dp(src), _dp(out))
@@ -147,6 +152,52 @@ class Mesh:
             lib().orc_epsdivdiv_apply_fast(self.h, _dp(k), bc_cmb, bc_surf, int(diagonal), apply_mode, comm,
                                            penalty_eps, _dp(src), _dp(out), pm)
         return out
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def divergence(fine, coarse, u, bc_cmb, bc_surf, apply_mode=REPLACE, comm=COMMUNICATE, dst=None):
+    """DivergenceKerngen::apply_impl (divergence_kerngen.hpp:206-257): fine velocity -> coarse pressure."""
+    u = _c(u)
+    out = np.zeros(coarse.shape) if dst is None else np.array(dst, dtype=np.float64, order="C", copy=True)
+    lib().orc_divergence_apply(fine.h, coarse.h, bc_cmb, bc_surf, apply_mode, comm, _dp(u), _dp(out))
+    return out
+
+
+def gradient(fine, coarse, p, bc_cmb, bc_surf, apply_mode=REPLACE, comm=COMMUNICATE, dst=None):
+    """GradientKerngen::apply_impl: coarse pressure -> fine velocity."""
+    p = _c(p)
+    out = np.zeros((3,) + fine.shape) if dst is None else np.array(dst, dtype=np.float64, order="C", copy=True)
+    lib().orc_gradient_apply(fine.h, coarse.h, bc_cmb, bc_surf, apply_mode, comm, _dp(p), _dp(out))
+    return out
+
+
+def prolongate(coarse, fine, x, apply_mode=REPLACE, dst=None):
+    """ProlongationVecConstant / scalar constant prolongation (21-point stencil)."""
+    x = _c(x)
+    nc = x.size // coarse.nalloc
+    out = np.zeros(((nc,) if nc > 1 else ()) + fine.shape) if dst is None else np.array(dst, dtype=np.float64, order="C", copy=True)
+    lib().orc_prolongate(coarse.h, fine.h, nc, apply_mode, _dp(x), _dp(out))
+    return out
+
+
+def restrict(fine, coarse, x, apply_mode=REPLACE, dst=None):
+    """RestrictionVecConstant: owned-masked transpose stencil + additive exchange."""
+    x = _c(x)
+    nc = x.size // fine.nalloc
+    out = np.zeros(((nc,) if nc > 1 else ()) + coarse.shape) if dst is None else np.array(dst, dtype=np.float64, order="C", copy=True)
+    lib().orc_restrict(fine.h, coarse.h, nc, apply_mode, _dp(x), _dp(out))
+    return out
+
+
+def kmass_lumped(mesh, k, x):
+    """KMass with set_lumped_diagonal(true): dst = diag(rowsum M_k) x, additive exchange."""
+    k, x = _c(k), _c(x)
+    out = np.zeros(mesh.shape)
+    lib().orc_kmass_lumped_apply(mesh.h, _dp(k), _dp(x), _dp(out))
+    return out
 
 
 def uniform_shell_radii(r_min, r_max, n):

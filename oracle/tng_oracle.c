@@ -962,3 +962,349 @@ int orc_num_threads(void)
     return 1;
 #endif
 }
+
+/* ------------------------------------------------------------------------- */
+/* Divergence B (fine velocity -> coarse pressure) and gradient B^T          */
+/* divergence_kerngen.hpp:373-518 (eval_cell), gradient_kerngen.hpp:339-496. */
+/* Fine wedge w of fine cell (x, y, r) contributes to coarse wedge w of the  */
+/* coarse cell (x/2, y/2, r/2), with the coarse shape functions of the       */
+/* fine wedge's position (fine_lateral_wedge_idx, kernel_helpers.hpp:600-608)*/
+/* ------------------------------------------------------------------------- */
+/* integrands.hpp:266-393 shape_rad_coarse / shape_lat_coarse */
+static double shape_rad_coarse(int i, int rad, double zeta)
+{
+    if (i / 3 == 0) return rad == 0 ? 0.25 * (3 - zeta) : 0.25 * (1 - zeta);
+    return rad == 0 ? 0.25 * (1 + zeta) : 0.25 * (3 + zeta);
+}
+static double shape_lat_coarse(int i, int lat, double xi, double eta)
+{
+    switch (i % 3) {
+    case 0:
+        if (lat == 0) return -0.5 * eta - 0.5 * xi + 1.0;
+        if (lat == 3) return 0.5 * eta + 0.5 * xi;
+        return -0.5 * eta - 0.5 * xi + 0.5;
+    case 1:
+        if (lat == 1) return 0.5 * xi + 0.5;
+        if (lat == 3) return 0.5 - 0.5 * xi;
+        return 0.5 * xi;
+    default:
+        if (lat == 2) return 0.5 * eta + 0.5;
+        if (lat == 3) return 0.5 - 0.5 * eta;
+        return 0.5 * eta;
+    }
+}
+static int fine_lateral_wedge_idx(int x, int y, int w)
+{
+    static const int idx[2][2][2] = {{{0, 1}, {2, 3}}, {{3, 2}, {1, 0}}};
+    return idx[w][y % 2][x % 2];
+}
+
+/* local 6x18 divergence matrix A(i, d*6+j) = -|det J| N^c_i (J^{-T} grad N_j)_d
+ * of fine wedge w of fine cell (sd, x, y, r); returns it dimension-wise. */
+static void div_local(const orc_mesh *m, int sd, int xc, int yc, int rc, int w, double A[6][18])
+{
+    const int nx = m->nx;
+    vec3 q[2][2];
+    for (int x = 0; x < 2; ++x)
+        for (int y = 0; y < 2; ++y) {
+            const double *c = m->coords + 3 * (((size_t)sd * nx + xc + x) * nx + yc + y);
+            q[x][y] = v3(c[0], c[1], c[2]);
+        }
+    vec3 P[3];
+    if (w == 0) { P[0] = q[0][0]; P[1] = q[1][0]; P[2] = q[0][1]; }
+    else { P[0] = q[1][1]; P[1] = q[0][1]; P[2] = q[1][0]; }
+    const double r_1 = m->radii[(size_t)sd * m->nr + rc], r_2 = m->radii[(size_t)sd * m->nr + rc + 1];
+    const double xi = 1.0 / 3.0, eta = 1.0 / 3.0, zeta = 0.0, qw = 1.0;
+    mat3 J = jac_w(P[0], P[1], P[2], r_1, r_2, xi, eta, zeta);
+    const double det = fabs(det3(&J));
+    mat3 JiT = inv_transposed3(&J, det3(&J));
+    const int rad = rc % 2, lat = fine_lateral_wedge_idx(xc, yc, w);
+    memset(A, 0, sizeof(double) * 6 * 18);
+    for (int i = 0; i < 6; i++) {
+        const double si = shape_lat_coarse(i, lat, xi, eta) * shape_rad_coarse(i, rad, zeta);
+        for (int j = 0; j < 6; j++) {
+            vec3 gr = grad_shape_w(j, xi, eta, zeta);
+            for (int d = 0; d < 3; d++) {
+                const double g = JiT.a[d][0] * gr.v[0] + JiT.a[d][1] * gr.v[1] + JiT.a[d][2] * gr.v[2];
+                A[i][d * 6 + j] += qw * (-(g)*si * det);
+            }
+        }
+    }
+}
+
+/* boundary handling shared by B and B^T: at a Dirichlet layer the fine
+ * velocity dofs of the boundary nodes are removed; at a free-slip layer the
+ * velocity is rotated to (n, t1, t2) and the normal component removed. */
+static void boundary_setup(const orc_mesh *m, int bc_cmb, int bc_surf, int sd, int xc, int yc, int rc, int *kind,
+                           int *at_surface, double R[2][18][18])
+{
+    const int at_cmb = has_flag(m->bmask[lin(m, sd, xc, yc, rc)], SBF_CMB);
+    *at_surface = has_flag(m->bmask[lin(m, sd, xc, yc, rc + 1)], SBF_SURFACE);
+    *kind = BC_NEUMANN;
+    if (!(at_cmb || *at_surface)) return;
+    const int bcf = at_cmb ? bc_cmb : bc_surf;
+    if (at_cmb) *at_surface = 0;
+    *kind = bcf;
+    if (bcf != BC_FREESLIP) return;
+    static const int offx[2][3] = {{0, 1, 0}, {1, 0, 1}}, offy[2][3] = {{0, 0, 1}, {1, 1, 0}};
+    for (int w = 0; w < 2; ++w) {
+        memset(R[w], 0, sizeof(R[w]));
+        for (int i = 0; i < 18; ++i) R[w][i][i] = 1.0;
+        for (int b = 0; b < 3; ++b) {
+            const int x = xc + offx[w][b], y = yc + offy[w][b], r = rc + (at_cmb ? 0 : 1);
+            const double *c = m->coords + 3 * (((size_t)sd * m->nx + x) * m->nx + y);
+            const double rad = m->radii[(size_t)sd * m->nr + r];
+            mat3 Ri = trafo_nt(v3(c[0] * rad, c[1] * rad, c[2] * rad));
+            const int off = at_cmb ? 0 : 9;
+            for (int a = 0; a < 3; ++a)
+                for (int bb = 0; bb < 3; ++bb) R[w][off + b * 3 + a][off + b * 3 + bb] = Ri.a[a][bb];
+        }
+    }
+}
+
+/* coarse wedge node (x, y, r) offsets: extract_local_wedge_scalar_coefficients */
+static int64_t coarse_idx(const orc_mesh *mc, int sd, int xc, int yc, int rc, int w, int n)
+{
+    return lin(mc, sd, xc + WOFF[w][n][0], yc + WOFF[w][n][1], rc + WOFF[w][n][2]);
+}
+
+void orc_divergence_apply(const orc_mesh *mf, const orc_mesh *mc, int bc_cmb, int bc_surf, int apply_mode, int comm,
+                          const double *src, double *dst)
+{
+    const int64_t N = mf->nalloc;
+    if (apply_mode == 0) memset(dst, 0, sizeof(double) * mc->nalloc);
+    for (int sd = 0; sd < mf->nsd; ++sd)
+        for (int xc = 0; xc < mf->nx - 1; ++xc)
+            for (int yc = 0; yc < mf->nx - 1; ++yc)
+                for (int rc = 0; rc < mf->nr - 1; ++rc) {
+                    int kind, at_surface;
+                    double R[2][18][18];
+                    boundary_setup(mf, bc_cmb, bc_surf, sd, xc, yc, rc, &kind, &at_surface, R);
+                    for (int w = 0; w < 2; ++w) {
+                        double A[6][18];
+                        div_local(mf, sd, xc, yc, rc, w, A);
+                        double s[18];
+                        for (int d = 0; d < 3; ++d)
+                            for (int n = 0; n < 6; ++n)
+                                s[d * 6 + n] = src[(size_t)d * N + lin(mf, sd, xc + WOFF[w][n][0], yc + WOFF[w][n][1],
+                                                                       rc + WOFF[w][n][2])];
+                        double bm[6][18];
+                        for (int i = 0; i < 6; ++i)
+                            for (int j = 0; j < 18; ++j) bm[i][j] = 1.0;
+                        if (kind == BC_DIRICHLET) {
+                            for (int d = 0; d < 3; ++d)
+                                for (int i = 0; i < 6; i++)
+                                    for (int j = 0; j < 6; j++)
+                                        if ((!at_surface && j < 3) || (at_surface && j >= 3)) bm[i][d * 6 + j] = 0.0;
+                        } else if (kind == BC_FREESLIP) {
+                            double At[6][18], sn[18];
+                            for (int i = 0; i < 6; ++i)
+                                for (int d = 0; d < 3; ++d)
+                                    for (int j = 0; j < 6; ++j) At[i][j * 3 + d] = A[i][j + d * 6];
+                            for (int n = 0; n < 6; ++n)
+                                for (int d = 0; d < 3; ++d) sn[n * 3 + d] = s[d * 6 + n];
+                            for (int i = 0; i < 6; ++i)
+                                for (int j = 0; j < 18; ++j) {
+                                    double acc = 0.0; /* A_tmp * R^T */
+                                    for (int k = 0; k < 18; ++k) acc += At[i][k] * R[w][j][k];
+                                    A[i][j] = acc;
+                                }
+                            for (int i = 0; i < 18; ++i) {
+                                double acc = 0.0;
+                                for (int j = 0; j < 18; ++j) acc += R[w][i][j] * sn[j];
+                                s[i] = acc;
+                            }
+                            const int ns = at_surface ? 3 : 0, ne = at_surface ? 6 : 3;
+                            for (int nd = ns; nd < ne; ++nd)
+                                for (int k = 0; k < 6; ++k) bm[k][nd * 3] = 0.0;
+                        }
+                        for (int i = 0; i < 6; ++i) {
+                            double acc = 0.0;
+                            for (int j = 0; j < 18; ++j) acc += A[i][j] * bm[i][j] * s[j];
+                            dst[coarse_idx(mc, sd, xc / 2, yc / 2, rc / 2, w, i)] += acc;
+                        }
+                    }
+                }
+    if (comm) orc_halo_sum(mc, dst, 1);
+}
+
+void orc_gradient_apply(const orc_mesh *mf, const orc_mesh *mc, int bc_cmb, int bc_surf, int apply_mode, int comm,
+                        const double *src, double *dst)
+{
+    const int64_t N = mf->nalloc;
+    if (apply_mode == 0) memset(dst, 0, sizeof(double) * 3 * N);
+    for (int sd = 0; sd < mf->nsd; ++sd)
+        for (int xc = 0; xc < mf->nx - 1; ++xc)
+            for (int yc = 0; yc < mf->nx - 1; ++yc)
+                for (int rc = 0; rc < mf->nr - 1; ++rc) {
+                    int kind, at_surface;
+                    double R[2][18][18];
+                    boundary_setup(mf, bc_cmb, bc_surf, sd, xc, yc, rc, &kind, &at_surface, R);
+                    for (int w = 0; w < 2; ++w) {
+                        double Ad[6][18], A[18][6];
+                        div_local(mf, sd, xc, yc, rc, w, Ad); /* the gradient matrix is the transpose */
+                        for (int i = 0; i < 18; ++i)
+                            for (int j = 0; j < 6; ++j) A[i][j] = Ad[j][i];
+                        double p[6];
+                        for (int n = 0; n < 6; ++n) p[n] = src[coarse_idx(mc, sd, xc / 2, yc / 2, rc / 2, w, n)];
+                        double bm[18][6];
+                        for (int i = 0; i < 18; ++i)
+                            for (int j = 0; j < 6; ++j) bm[i][j] = 1.0;
+                        int fs = 0;
+                        if (kind == BC_DIRICHLET) {
+                            for (int d = 0; d < 3; ++d)
+                                for (int i = 0; i < 6; i++)
+                                    for (int j = 0; j < 6; j++)
+                                        if ((!at_surface && i < 3) || (at_surface && i >= 3)) bm[d * 6 + i][j] = 0.0;
+                        } else if (kind == BC_FREESLIP) {
+                            fs = 1;
+                            double At[18][6];
+                            for (int i = 0; i < 6; ++i)
+                                for (int d = 0; d < 3; ++d)
+                                    for (int j = 0; j < 6; ++j) At[i * 3 + d][j] = A[i + d * 6][j];
+                            for (int i = 0; i < 18; ++i)
+                                for (int j = 0; j < 6; ++j) {
+                                    double acc = 0.0;
+                                    for (int k = 0; k < 18; ++k) acc += R[w][i][k] * At[k][j];
+                                    A[i][j] = acc;
+                                }
+                            const int ns = at_surface ? 3 : 0, ne = at_surface ? 6 : 3;
+                            for (int nd = ns; nd < ne; ++nd)
+                                for (int k = 0; k < 6; ++k) bm[nd * 3][k] = 0.0;
+                        }
+                        double o[18];
+                        for (int i = 0; i < 18; ++i) {
+                            double acc = 0.0;
+                            for (int j = 0; j < 6; ++j) acc += A[i][j] * bm[i][j] * p[j];
+                            o[i] = acc;
+                        }
+                        if (fs) {
+                            double t[18];
+                            for (int i = 0; i < 18; ++i) {
+                                double acc = 0.0;
+                                for (int j = 0; j < 18; ++j) acc += R[w][j][i] * o[j];
+                                t[i] = acc;
+                            }
+                            for (int n = 0; n < 6; ++n)
+                                for (int d = 0; d < 3; ++d) o[d * 6 + n] = t[n * 3 + d];
+                        }
+                        for (int d = 0; d < 3; ++d)
+                            for (int n = 0; n < 6; ++n)
+                                dst[(size_t)d * N + lin(mf, sd, xc + WOFF[w][n][0], yc + WOFF[w][n][1],
+                                                        rc + WOFF[w][n][2])] += o[d * 6 + n];
+                    }
+                }
+    if (comm) orc_halo_sum(mf, dst, 3);
+}
+
+/* ------------------------------------------------------------------------- */
+/* constant-weight transfers: fe/wedge/shell/grid_transfer_constant.hpp:9-90,*/
+/* prolongation_constant.hpp:102-191, restriction_constant.hpp:133-250       */
+/* ------------------------------------------------------------------------- */
+static double transfer_weight(int xf, int yf, int rf, int xc, int yc, int rc)
+{
+    const int dx = xf - 2 * xc, dy = yf - 2 * yc, dr = rf - 2 * rc;
+    double lat = 0.0;
+    if (dx == 0 && dy == 0) lat = 1.0;
+    else if (dy == 0 && (dx == 1 || dx == -1)) lat = 0.5;
+    else if (dx == 0 && (dy == 1 || dy == -1)) lat = 0.5;
+    else if (dx == -1 && dy == 1) lat = 0.5;
+    else if (dx == 1 && dy == -1) lat = 0.5;
+    if (dr == 0) return lat;
+    if (dr == 1 || dr == -1) return 0.5 * lat;
+    return 0.0;
+}
+static const int STENCIL[7][2] = {{-1, 0}, {-1, 1}, {0, 1}, {1, 0}, {1, -1}, {0, -1}, {0, 0}};
+
+void orc_prolongate(const orc_mesh *mc, const orc_mesh *mf, int ncomp, int apply_mode, const double *src, double *dst)
+{
+    if (apply_mode == 0) memset(dst, 0, sizeof(double) * ncomp * mf->nalloc);
+    for (int sd = 0; sd < mc->nsd; ++sd)
+        for (int x = 0; x < mc->nx; ++x)
+            for (int y = 0; y < mc->nx; ++y)
+                for (int r = 0; r < mc->nr; ++r)
+                    for (int dr = -1; dr <= 1; ++dr)
+                        for (int s = 0; s < 7; ++s) {
+                            const int xf = 2 * x + STENCIL[s][0], yf = 2 * y + STENCIL[s][1], rf = 2 * r + dr;
+                            if (xf < 0 || xf >= mf->nx || yf < 0 || yf >= mf->nx || rf < 0 || rf >= mf->nr) continue;
+                            const double wgt = transfer_weight(xf, yf, rf, x, y, r);
+                            for (int d = 0; d < ncomp; ++d)
+                                dst[(size_t)d * mf->nalloc + lin(mf, sd, xf, yf, rf)] +=
+                                    wgt * src[(size_t)d * mc->nalloc + lin(mc, sd, x, y, r)];
+                        }
+}
+
+void orc_restrict(const orc_mesh *mf, const orc_mesh *mc, int ncomp, int apply_mode, const double *src, double *dst)
+{
+    if (apply_mode == 0) memset(dst, 0, sizeof(double) * ncomp * mc->nalloc);
+    for (int sd = 0; sd < mc->nsd; ++sd)
+        for (int x = 0; x < mc->nx; ++x)
+            for (int y = 0; y < mc->nx; ++y)
+                for (int r = 0; r < mc->nr; ++r)
+                    for (int dr = -1; dr <= 1; ++dr)
+                        for (int s = 0; s < 7; ++s) {
+                            const int xf = 2 * x + STENCIL[s][0], yf = 2 * y + STENCIL[s][1], rf = 2 * r + dr;
+                            if (xf < 0 || xf >= mf->nx || yf < 0 || yf >= mf->nx || rf < 0 || rf >= mf->nr) continue;
+                            const int64_t fi = lin(mf, sd, xf, yf, rf);
+                            const double wgt = transfer_weight(xf, yf, rf, x, y, r) * (mf->omask[fi] == OWN_OWNED ? 1.0 : 0.0);
+                            for (int d = 0; d < ncomp; ++d)
+                                dst[(size_t)d * mc->nalloc + lin(mc, sd, x, y, r)] += wgt * src[(size_t)d * mf->nalloc + fi];
+                        }
+    orc_halo_sum(mc, dst, ncomp);
+}
+
+/* ------------------------------------------------------------------------- */
+/* lumped k-weighted mass (kmass.hpp:16-181 with set_lumped_diagonal(true)),  */
+/* 6-point Felippa rule (quadrature.hpp:66-101): dst = diag(rowsum(M_k)) src  */
+/* ------------------------------------------------------------------------- */
+void orc_kmass_lumped_apply(const orc_mesh *m, const double *k, const double *src, double *dst)
+{
+    static const double QP[6][3] = {{0.6666666666666666, 0.1666666666666667, -0.5773502691896257},
+                                    {0.1666666666666667, 0.6666666666666666, -0.5773502691896257},
+                                    {0.1666666666666667, 0.1666666666666667, -0.5773502691896257},
+                                    {0.6666666666666666, 0.1666666666666667, 0.5773502691896257},
+                                    {0.1666666666666667, 0.6666666666666666, 0.5773502691896257},
+                                    {0.1666666666666667, 0.1666666666666667, 0.5773502691896257}};
+    static const double QW = 0.1666666666666667;
+    memset(dst, 0, sizeof(double) * m->nalloc);
+    const int nx = m->nx;
+    for (int sd = 0; sd < m->nsd; ++sd)
+        for (int xc = 0; xc < nx - 1; ++xc)
+            for (int yc = 0; yc < nx - 1; ++yc)
+                for (int rc = 0; rc < m->nr - 1; ++rc) {
+                    const double r_1 = m->radii[(size_t)sd * m->nr + rc], r_2 = m->radii[(size_t)sd * m->nr + rc + 1];
+                    const double grad_r = 0.5 * (r_2 - r_1);
+                    for (int w = 0; w < 2; ++w) {
+                        vec3 P[3];
+                        int64_t idx[6];
+                        for (int n = 0; n < 6; ++n) idx[n] = lin(m, sd, xc + WOFF[w][n][0], yc + WOFF[w][n][1], rc + WOFF[w][n][2]);
+                        for (int n = 0; n < 3; ++n) {
+                            const double *c = m->coords + 3 * (((size_t)sd * nx + xc + WOFF[w][n][0]) * nx + yc + WOFF[w][n][1]);
+                            P[n] = v3(c[0], c[1], c[2]);
+                        }
+                        double A[6][6];
+                        memset(A, 0, sizeof(A));
+                        for (int q = 0; q < 6; ++q) {
+                            /* jac_lat: columns p2-p1, p3-p1, forward_map_lat (integrands.hpp:596-611) */
+                            vec3 c0 = vsub(P[1], P[0]), c1 = vsub(P[2], P[0]);
+                            const double xi = QP[q][0], eta = QP[q][1], zeta = QP[q][2];
+                            vec3 c2 = vadd(vadd(vscale(P[0], (1 - xi - eta)), vscale(P[1], xi)), vscale(P[2], eta));
+                            mat3 Jl;
+                            for (int i = 0; i < 3; ++i) { Jl.a[i][0] = c0.v[i]; Jl.a[i][1] = c1.v[i]; Jl.a[i][2] = c2.v[i]; }
+                            const double detl = fabs(det3(&Jl));
+                            const double r = r_1 + 0.5 * (r_2 - r_1) * (1 + zeta);
+                            double k_eval = 0.0;
+                            for (int j = 0; j < 6; j++) k_eval += shape_w(j, xi, eta, zeta) * k[idx[j]];
+                            for (int i = 0; i < 6; i++)
+                                for (int j = 0; j < 6; j++)
+                                    A[i][j] += QW * k_eval * (shape_w(i, xi, eta, zeta) * shape_w(j, xi, eta, zeta) * r * r * grad_r * detl);
+                        }
+                        for (int i = 0; i < 6; ++i) {
+                            double rs = 0.0;
+                            for (int j = 0; j < 6; ++j) rs += A[i][j] * 1.0;
+                            dst[idx[i]] += rs * src[idx[i]];
+                        }
+                    }
+                }
+    orc_halo_sum(m, dst, 1);
+}

@@ -354,13 +354,8 @@ tng_status tng_op_apply(tng_op op, tng_vec src, tng_vec dst)
     if (!op || !src || !dst) return TNG_INVALID_ARGUMENT;
     tng_context *ctx = op->mesh->ctx;
     return guarded(ctx, [&] {
-        TNG_REQUIRE(src != dst, "apply: src and dst must be different vectors");
-        TNG_REQUIRE(src->mesh == op->mesh && dst->mesh == op->mesh, "apply: vectors must live on the operator's mesh");
-        TNG_REQUIRE(src->ncomp == 3 && dst->ncomp == 3, "apply: EpsilonDivDiv acts on 3-component vectors");
-        TNG_REQUIRE(src->dtype == op->k->dtype && dst->dtype == op->k->dtype,
-                    "apply: src, dst and coefficient must share one dtype");
         set_device(ctx);
-        epsdivdiv_apply(*op, *src, *dst, ctx->stream);
+        op_apply(*op, *src, *dst);
     });
 }
 
@@ -384,3 +379,162 @@ hipEvent_t *tng_context::prof_next_set()
     }
     return &prof_events[4 * prof_used++];
 }
+
+namespace tng {
+
+void op_apply(tng_operator &op, const tng_vector &src, tng_vector &dst)
+{
+    TNG_REQUIRE(&src != &dst, "apply: src and dst must be different vectors");
+    TNG_REQUIRE(src.dtype == dst.dtype, "apply: src and dst must share one dtype");
+    tng_domain *in = nullptr, *out = nullptr;
+    int nin = 3, nout = 3;
+    switch (op.kind) {
+    case OP_EPSDIVDIV: in = out = op.mesh; break;
+    case OP_DIVERGENCE: in = op.mesh; out = op.mesh2; nout = 1; break;
+    case OP_GRADIENT: in = op.mesh2; out = op.mesh; nin = 1; break;
+    case OP_PROLONGATION: in = op.mesh2; out = op.mesh; nin = nout = src.ncomp; break;
+    case OP_RESTRICTION: in = op.mesh; out = op.mesh2; nin = nout = src.ncomp; break;
+    case OP_KMASS_LUMPED: in = out = op.mesh; nin = nout = 1; break;
+    }
+    TNG_REQUIRE(src.mesh == in && dst.mesh == out, "apply: vectors do not live on the operator's meshes");
+    TNG_REQUIRE(src.ncomp == nin && dst.ncomp == nout, "apply: wrong number of vector components");
+    if (op.k) TNG_REQUIRE(op.k->dtype == src.dtype, "apply: coefficient and vectors must share one dtype");
+    hipStream_t s = op.mesh->ctx->stream;
+    if (op.kind == OP_EPSDIVDIV) epsdivdiv_apply(op, src, dst, s);
+    else stokes_block_apply(op, src, dst, s);
+}
+
+}  // namespace tng
+
+extern "C" {
+
+static tng_status make_two_level(tng_mesh fine, tng_mesh coarse, OpKind kind, tng_bc bc_cmb, tng_bc bc_surf,
+                                 tng_op *out)
+{
+    if (!fine || !coarse || !out || !fine->ctx || fine->ctx != coarse->ctx) return TNG_INVALID_ARGUMENT;
+    *out = nullptr;
+    auto *op = new (std::nothrow) tng_operator();
+    if (!op) return TNG_OUT_OF_MEMORY;
+    tng_status st = guarded(fine->ctx, [&] {
+        TNG_REQUIRE(fine->nsd == coarse->nsd && fine->nx - 1 == 2 * (coarse->nx - 1) &&
+                        fine->nr - 1 == 2 * (coarse->nr - 1) && fine->lat_sdr == coarse->lat_sdr &&
+                        fine->rad_sdr == coarse->rad_sdr,
+                    "fine and coarse meshes must be one uniform refinement apart with the same subdomains");
+        TNG_REQUIRE(bc_cmb >= TNG_NEUMANN && bc_cmb <= TNG_FREESLIP && bc_surf >= TNG_NEUMANN && bc_surf <= TNG_FREESLIP,
+                    "invalid boundary condition");
+        op->kind = kind;
+        op->mesh = fine;
+        op->mesh2 = coarse;
+        op->bc_cmb = bc_cmb;
+        op->bc_surf = bc_surf;
+    });
+    if (st != TNG_OK) {
+        delete op;
+        return st;
+    }
+    *out = op;
+    return TNG_OK;
+}
+
+tng_status tng_divergence_create(tng_mesh fine, tng_mesh coarse, tng_bc bc_cmb, tng_bc bc_surface, tng_op *out)
+{
+    return make_two_level(fine, coarse, OP_DIVERGENCE, bc_cmb, bc_surface, out);
+}
+
+tng_status tng_gradient_create(tng_mesh fine, tng_mesh coarse, tng_bc bc_cmb, tng_bc bc_surface, tng_op *out)
+{
+    return make_two_level(fine, coarse, OP_GRADIENT, bc_cmb, bc_surface, out);
+}
+
+tng_status tng_prolongation_create(tng_mesh coarse, tng_mesh fine, tng_op *out)
+{
+    tng_status st = make_two_level(fine, coarse, OP_PROLONGATION, TNG_NEUMANN, TNG_NEUMANN, out);
+    if (st == TNG_OK) (*out)->comm_mode = TNG_SKIP_COMMUNICATION;
+    return st;
+}
+
+tng_status tng_restriction_create(tng_mesh fine, tng_mesh coarse, tng_op *out)
+{
+    return make_two_level(fine, coarse, OP_RESTRICTION, TNG_NEUMANN, TNG_NEUMANN, out);
+}
+
+tng_status tng_kmass_lumped_create(tng_mesh mesh, tng_vec k, tng_op *out)
+{
+    if (!mesh || !k || !out || !mesh->ctx) return TNG_INVALID_ARGUMENT;
+    *out = nullptr;
+    auto *op = new (std::nothrow) tng_operator();
+    if (!op) return TNG_OUT_OF_MEMORY;
+    tng_status st = guarded(mesh->ctx, [&] {
+        TNG_REQUIRE(k->mesh == mesh && k->ncomp == 1, "coefficient must be a scalar field on the operator's mesh");
+        op->kind = OP_KMASS_LUMPED;
+        op->mesh = mesh;
+        op->k = k;
+    });
+    if (st != TNG_OK) {
+        delete op;
+        return st;
+    }
+    *out = op;
+    return TNG_OK;
+}
+
+tng_status tng_vec_lincomb(tng_vec y, int n, const double *c, const tng_vec *x, double c0)
+{
+    if (!y || n < 0 || n > 3 || (n > 0 && (!c || !x))) return TNG_INVALID_ARGUMENT;
+    return guarded(y->mesh->ctx, [&] {
+        set_device(y->mesh->ctx);
+        const tng_vector *xs[3] = {nullptr, nullptr, nullptr};
+        for (int i = 0; i < n; ++i) {
+            TNG_REQUIRE(x[i] != nullptr, "lincomb: null input vector");
+            xs[i] = x[i];
+        }
+        launch_lincomb(*y, n, c, xs, c0, y->mesh->ctx->stream);
+    });
+}
+
+tng_status tng_vec_scale_with_vector(tng_vec y, tng_vec x)
+{
+    if (!y || !x) return TNG_INVALID_ARGUMENT;
+    return guarded(y->mesh->ctx, [&] {
+        set_device(y->mesh->ctx);
+        launch_mult(*y, *x, y->mesh->ctx->stream);
+    });
+}
+
+tng_status tng_vec_invert(tng_vec y)
+{
+    if (!y) return TNG_INVALID_ARGUMENT;
+    return guarded(y->mesh->ctx, [&] {
+        set_device(y->mesh->ctx);
+        launch_invert(*y, y->mesh->ctx->stream);
+    });
+}
+
+tng_status tng_vec_dot(tng_vec x, tng_vec y, double *out)
+{
+    if (!x || !y || !out) return TNG_INVALID_ARGUMENT;
+    return guarded(x->mesh->ctx, [&] {
+        set_device(x->mesh->ctx);
+        *out = reduce_to_host(REDUCE_DOT, *x, y, x->mesh->ctx->stream);
+    });
+}
+
+tng_status tng_vec_max_abs(tng_vec x, double *out)
+{
+    if (!x || !out) return TNG_INVALID_ARGUMENT;
+    return guarded(x->mesh->ctx, [&] {
+        set_device(x->mesh->ctx);
+        *out = reduce_to_host(REDUCE_MAXABS, *x, nullptr, x->mesh->ctx->stream);
+    });
+}
+
+tng_status tng_vec_has_nan_or_inf(tng_vec x, int *out)
+{
+    if (!x || !out) return TNG_INVALID_ARGUMENT;
+    return guarded(x->mesh->ctx, [&] {
+        set_device(x->mesh->ctx);
+        *out = reduce_to_host(REDUCE_NONFINITE, *x, nullptr, x->mesh->ctx->stream) > 0 ? 1 : 0;
+    });
+}
+
+}  // extern "C"

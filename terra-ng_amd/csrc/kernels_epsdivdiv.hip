@@ -26,9 +26,14 @@
 // a workgroup cover the cell columns XG-1 .. XG+VX-2; virtual column v
 // outputs node column XG+v.  The contributions of a cell to its left corners
 // belong to the neighbour on the left and are passed through LDS once per
-// y-step; radial neighbours are combined with a lane shuffle.  Every dst
-// entry is written exactly once with a plain store: no atomics, no zeroing
-// pass, deterministic results.
+// y-step, already combined radially; radial neighbours are combined with a
+// DPP wave shift.  The next row of node columns is prefetched one step ahead.
+// Boundary rows (Dirichlet, free-slip) are handled with per-lane factors, not
+// branches.  Every dst entry is written exactly once with a plain store: no
+// atomics, no zeroing pass, deterministic results.
+#include <algorithm>
+#include <cstdlib>
+
 #include "tng_internal.hpp"
 
 namespace tng {
@@ -41,8 +46,8 @@ template <typename T> struct ApplyArgs {
     const T *k;
     const T *radii;     // (nsd, nr)
     const T *latgeom;   // (nsd, nc, nc, 2, 10)
-    const T *coords;    // (nsd, nx, nx, 3)
-    const T *btab;      // (nsd, nx, nx, 2, 3) boundary rows, or nullptr
+    const T *coords;    // (nsd, nx, nx, 3) unit vectors (the radial normal)
+    const T *btab;      // (nsd, nx, nx, 2, 3) boundary rows
     const double *pen;  // 3 doubles (penalty axis b), or nullptr
     const uint8_t *sd_cmb, *sd_surf;
     int nsd, nx, nr, ncells;
@@ -60,59 +65,91 @@ template <typename T> __device__ __forceinline__ void normal_of(const T *c, T n[
     n[2] = c[2] * inv;
 }
 
-// Per-lane data of one node column for the cells of the lane.
+// DPP wave_shr:1 -- lane i receives lane i-1 (lane 0 receives 0).
+__device__ __forceinline__ double wave_shr1(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float wave_shr1(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+
+// Raw node values of one node column at the K+1 levels spanned by a lane.
+template <typename T, int K> struct RawCol {
+    T u[K + 1][3];
+    T k[K + 1];
+};
+
+// Processed node column: per cell of the lane sums and differences of the
+// bottom and top values (after the boundary modification of the trial side).
 template <typename T, int K> struct NodeCol {
-    T su[K][3];  // u(bottom) + u(top)
-    T du[K][3];  // u(top) - u(bottom)
-    T kk[K];     // k(bottom) + k(top)
+    T su[K][3];
+    T du[K][3];
+    T kk[K];
 };
 
 template <typename T, int K>
-__device__ __forceinline__ void load_col(const ApplyArgs<T> &a, int sd, int X, int Y, int rl, bool has_cmb,
-                                         bool has_surf, NodeCol<T, K> &o)
+__device__ __forceinline__ void load_raw(const ApplyArgs<T> &a, int64_t base, int lvl0, RawCol<T, K> &o)
 {
-    const bool ok = X >= 0 && X < a.nx && Y >= 0 && Y < a.nx;
-    const int64_t base = ((((int64_t)sd * a.nx + X) * a.nx) + Y) * a.nr;
-    T nrm[3] = {T(0), T(0), T(0)};
-    const bool need_n = ok && ((has_cmb && a.bc_cmb == TNG_FREESLIP) || (has_surf && a.bc_surf == TNG_FREESLIP));
-    if (need_n) normal_of(a.coords + 3 * (((int64_t)sd * a.nx + X) * a.nx + Y), nrm);
 #pragma unroll
-    for (int p = 0; p < K; ++p) {
-        const int l = rl * K + p;
-        T lo[3] = {T(0), T(0), T(0)}, hi[3] = {T(0), T(0), T(0)}, klo = T(0), khi = T(0);
-        if (ok && l < a.ncells) {
-            lo[0] = a.src0[base + l]; hi[0] = a.src0[base + l + 1];
-            lo[1] = a.src1[base + l]; hi[1] = a.src1[base + l + 1];
-            lo[2] = a.src2[base + l]; hi[2] = a.src2[base + l + 1];
-            klo = a.k[base + l]; khi = a.k[base + l + 1];
-            if (l == 0 && has_cmb) {
-                if (a.bc_cmb == TNG_DIRICHLET) {
-                    lo[0] = lo[1] = lo[2] = T(0);
-                } else if (a.bc_cmb == TNG_FREESLIP) {
-                    const T d = nrm[0] * lo[0] + nrm[1] * lo[1] + nrm[2] * lo[2];
-                    lo[0] -= d * nrm[0]; lo[1] -= d * nrm[1]; lo[2] -= d * nrm[2];
-                }
-            }
-            if (l + 1 == a.nr - 1 && has_surf) {
-                if (a.bc_surf == TNG_DIRICHLET) {
-                    hi[0] = hi[1] = hi[2] = T(0);
-                } else if (a.bc_surf == TNG_FREESLIP) {
-                    const T d = nrm[0] * hi[0] + nrm[1] * hi[1] + nrm[2] * hi[2];
-                    hi[0] -= d * nrm[0]; hi[1] -= d * nrm[1]; hi[2] -= d * nrm[2];
-                }
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            o.su[p][c] = lo[c] + hi[c];
-            o.du[p][c] = hi[c] - lo[c];
-        }
-        o.kk[p] = klo + khi;
+    for (int q = 0; q <= K; ++q) {
+        const int64_t i = base + lvl0 + q;
+        o.u[q][0] = a.src0[i];
+        o.u[q][1] = a.src1[i];
+        o.u[q][2] = a.src2[i];
+        o.k[q] = a.k[i];
     }
 }
 
-// One wedge: corners (P0, P1, P2) in wedge order; writes the lateral part
-// (per corner) and the radial part (common to the three corners).
+// Per-lane boundary factors of the trial side: the bottom value of the
+// lane's first cell (level 0 at the CMB) and the top value of its last cell
+// (level nr-1 at the surface).  Dirichlet: multiply by z (0 at the boundary),
+// free-slip: remove f * (n.u) n (f = 1 at the boundary).
+template <typename T> struct LaneBC {
+    T z_lo, f_lo, z_hi, f_hi;
+};
+
+template <typename T, int K>
+__device__ __forceinline__ void process(const RawCol<T, K> &r, const T n[3], const LaneBC<T> &bc, bool any_fs,
+                                        bool any_dir, NodeCol<T, K> &o)
+{
+    T lo[3] = {r.u[0][0], r.u[0][1], r.u[0][2]};
+    T hi[3] = {r.u[K][0], r.u[K][1], r.u[K][2]};
+    if (any_fs) {
+        const T dl = bc.f_lo * (n[0] * lo[0] + n[1] * lo[1] + n[2] * lo[2]);
+        const T dh = bc.f_hi * (n[0] * hi[0] + n[1] * hi[1] + n[2] * hi[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            lo[c] -= dl * n[c];
+            hi[c] -= dh * n[c];
+        }
+    }
+    if (any_dir) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            lo[c] *= bc.z_lo;
+            hi[c] *= bc.z_hi;
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const T b = p == 0 ? lo[c] : r.u[p][c];
+            const T t = p == K - 1 ? hi[c] : r.u[p + 1][c];
+            o.su[p][c] = b + t;
+            o.du[p][c] = t - b;
+        }
+        o.kk[p] = r.k[p] + r.k[p + 1];
+    }
+}
+
+// One wedge: corners (P0, P1, P2) in wedge order.  Returns the lateral parts
+// A0 (corner P1), A1 (corner P2), -(A0+A1) (corner P0) and the radial part Cc
+// (subtracted at the bottom node, added at the top node of every corner).
 template <typename T>
 __device__ __forceinline__ void wedge_stress(const T *g /*M0,M1,M2,detL*/, const T su0[3], const T su1[3],
                                              const T su2[3], const T du0[3], const T du1[3], const T du2[3],
@@ -143,7 +180,7 @@ __device__ __forceinline__ void wedge_stress(const T *g /*M0,M1,M2,detL*/, const
     tau[0][1] = tau[1][0] = f * (G[0][1] + G[1][0]);
     tau[0][2] = tau[2][0] = f * (G[0][2] + G[2][0]);
     tau[1][2] = tau[2][1] = f * (G[1][2] + G[2][1]);
-    T qM2[3] = {q * M2[0], q * M2[1], q * M2[2]};
+    const T qM2[3] = {q * M2[0], q * M2[1], q * M2[2]};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         A0[d] = tau[d][0] * M0[0] + tau[d][1] * M0[1] + tau[d][2] * M0[2];
@@ -153,51 +190,92 @@ __device__ __forceinline__ void wedge_stress(const T *g /*M0,M1,M2,detL*/, const
 }
 
 template <typename T, int R, int K, int NW>
-__global__ __launch_bounds__(NW * 64, 2) void epsdivdiv_apply_kernel(const ApplyArgs<T> a)
+__global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArgs<T> a)
 {
     constexpr int VPW = 64 / R;
     constexpr int VX = NW * VPW;
     constexpr int KL = K + 1;
-    // [buf][v][corner 00|01][level][comp][lane]
-    __shared__ T lds[2][VX][2][KL][3][R];
+    // left-corner node values of rows yc and yc+1, radially combined, per
+    // virtual column; plus the top level of the last cell (overflow)
+    __shared__ T xch[2][VX][2][K][3][R];
+    __shared__ T xovf[2][VX][2][3];
 
     const int tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
     const int vw = lane / R, rl = lane % R;
     const int v = wave * VPW + vw;
 
-    int b = blockIdx.x;
+    // Blocks are dispatched round-robin over the 8 XCDs; give every XCD a
+    // contiguous range of logical blocks so that neighbouring strips and
+    // segments (which share node columns) meet in one L2.
+    int b;
+    {
+        const int nb = gridDim.x, x = blockIdx.x & 7, i = blockIdx.x >> 3, q = nb >> 3, r = nb & 7;
+        b = x * q + min(x, r) + i;
+    }
     const int seg = b % a.segs;
     b /= a.segs;
     const int strip = b % a.strips;
     const int sd = b / a.strips;
-    const int nx = a.nx, nc = nx - 1;
+    const int nx = a.nx, nc = nx - 1, nr = a.nr, ncells = a.ncells;
     const int XG = strip * (VX - 1);
     const int xc = XG - 1 + v;
     const int X = xc + 1;
     const int Y0 = seg * a.seg_len;
     const int Yend = min(Y0 + a.seg_len, nx);
     const bool has_cmb = a.sd_cmb[sd] != 0, has_surf = a.sd_surf[sd] != 0;
+    const bool any_fs = (has_cmb && a.bc_cmb == TNG_FREESLIP) || (has_surf && a.bc_surf == TNG_FREESLIP);
+    const bool any_dir = (has_cmb && a.bc_cmb == TNG_DIRICHLET) || (has_surf && a.bc_surf == TNG_DIRICHLET);
 
-    // per-lane radial constants
+    // lanes: cells rl*K .. rl*K+K-1 (ncells is either <= R with K = 1, or R*K)
+    const bool lane_has = rl * K < ncells;
+    const bool ovf_lane = rl * K + K == ncells;  // lane holding the last cell
+    const int lvl0 = lane_has ? rl * K : ncells - K;  // clamped first level (loads stay in bounds)
+    LaneBC<T> lbc;
+    lbc.z_lo = (has_cmb && rl == 0 && a.bc_cmb == TNG_DIRICHLET) ? T(0) : T(1);
+    lbc.f_lo = (has_cmb && rl == 0 && a.bc_cmb == TNG_FREESLIP) ? T(1) : T(0);
+    lbc.z_hi = (has_surf && ovf_lane && a.bc_surf == TNG_DIRICHLET) ? T(0) : T(1);
+    lbc.f_hi = (has_surf && ovf_lane && a.bc_surf == TNG_FREESLIP) ? T(1) : T(0);
+
     T hs[K], tt[K], qq[K], cv[K];
 #pragma unroll
     for (int p = 0; p < K; ++p) {
-        const int l = rl * K + p;
-        hs[p] = tt[p] = qq[p] = cv[p] = T(0);
-        if (l < a.ncells) {
-            const T r0 = a.radii[(int64_t)sd * a.nr + l], r1 = a.radii[(int64_t)sd * a.nr + l + 1];
-            const T rmid = T(0.5) * (r0 + r1), h = T(0.5) * (r1 - r0);
-            hs[p] = T(0.5) / rmid;
-            tt[p] = T(1) / (T(6) * h);
-            qq[p] = tt[p] / hs[p];
-            cv[p] = hs[p] * rmid * rmid * h * T(1.0 / 6.0);
-        }
+        const int l = lvl0 + p;
+        const T r0 = a.radii[(int64_t)sd * nr + l], r1 = a.radii[(int64_t)sd * nr + l + 1];
+        const T rmid = T(0.5) * (r0 + r1), h = T(0.5) * (r1 - r0);
+        hs[p] = T(0.5) / rmid;
+        tt[p] = T(1) / (T(6) * h);
+        qq[p] = tt[p] / hs[p];
+        cv[p] = lane_has ? hs[p] * rmid * rmid * h * T(1.0 / 6.0) : T(0);
     }
 
+    auto colbase = [&](int Xq, int Yq) -> int64_t {
+        const int xq = min(max(Xq, 0), nx - 1), yq = min(max(Yq, 0), nx - 1);
+        return (((int64_t)sd * nx + xq) * nx + yq);
+    };
+    auto load_normal = [&](int Xq, int Yq, T n[3]) {
+        const T *c = a.coords + 3 * colbase(Xq, Yq);
+        n[0] = c[0];
+        n[1] = c[1];
+        n[2] = c[2];
+    };
+
     NodeCol<T, K> B0, B1, T0, T1;
-    load_col<T, K>(a, sd, xc, Y0 - 1, rl, has_cmb, has_surf, B0);
-    load_col<T, K>(a, sd, xc + 1, Y0 - 1, rl, has_cmb, has_surf, B1);
+    RawCol<T, K> P0, P1;
+    {
+        T n0[3] = {T(0), T(0), T(0)}, n1[3] = {T(0), T(0), T(0)};
+        load_raw<T, K>(a, colbase(xc, Y0 - 1) * nr, lvl0, P0);
+        load_raw<T, K>(a, colbase(xc + 1, Y0 - 1) * nr, lvl0, P1);
+        if (any_fs) {
+            load_normal(xc, Y0 - 1, n0);
+            load_normal(xc + 1, Y0 - 1, n1);
+        }
+        process<T, K>(P0, n0, lbc, any_fs, any_dir, B0);
+        process<T, K>(P1, n1, lbc, any_fs, any_dir, B1);
+        load_raw<T, K>(a, colbase(xc, Y0) * nr, lvl0, P0);
+        load_raw<T, K>(a, colbase(xc + 1, Y0) * nr, lvl0, P1);
+    }
 
     T acc_cur[KL][3], acc_next[KL][3];
 #pragma unroll
@@ -205,10 +283,22 @@ __global__ __launch_bounds__(NW * 64, 2) void epsdivdiv_apply_kernel(const Apply
 #pragma unroll
         for (int c = 0; c < 3; ++c) acc_cur[i][c] = acc_next[i][c] = T(0);
 
-    int step = 0;
-    for (int yc = Y0 - 1; yc < Yend; ++yc, ++step) {
-        load_col<T, K>(a, sd, xc, yc + 1, rl, has_cmb, has_surf, T0);
-        load_col<T, K>(a, sd, xc + 1, yc + 1, rl, has_cmb, has_surf, T1);
+    const int nsteps = Yend - (Y0 - 1);
+    for (int step = 0; step < nsteps; ++step) {
+        const int yc = Y0 - 1 + step;
+        // the prefetched row yc+1 becomes the top row; prefetch row yc+2
+        {
+            T n0[3] = {T(0), T(0), T(0)}, n1[3] = {T(0), T(0), T(0)};
+            if (any_fs) {
+                load_normal(xc, yc + 1, n0);
+                load_normal(xc + 1, yc + 1, n1);
+            }
+            process<T, K>(P0, n0, lbc, any_fs, any_dir, T0);
+            process<T, K>(P1, n1, lbc, any_fs, any_dir, T1);
+        }
+        load_raw<T, K>(a, colbase(xc, yc + 2) * nr, lvl0, P0);
+        load_raw<T, K>(a, colbase(xc + 1, yc + 2) * nr, lvl0, P1);
+
         T s00[KL][3], s01[KL][3];
 #pragma unroll
         for (int i = 0; i < KL; ++i)
@@ -216,8 +306,9 @@ __global__ __launch_bounds__(NW * 64, 2) void epsdivdiv_apply_kernel(const Apply
             for (int c = 0; c < 3; ++c) s00[i][c] = s01[i][c] = T(0);
 
         const bool cell_ok = xc >= 0 && xc < nc && yc >= 0 && yc < nc;
-        if (cell_ok) {
-            const T *gw = a.latgeom + ((((int64_t)sd * nc + xc) * nc + yc) * 2) * 10;
+        {
+            const int xcc = min(max(xc, 0), nc - 1), ycc = min(max(yc, 0), nc - 1);
+            const T *gw = a.latgeom + ((((int64_t)sd * nc + xcc) * nc + ycc) * 2) * 10;
             T g0[10], g1[10];
 #pragma unroll
             for (int i = 0; i < 10; ++i) {
@@ -226,11 +317,11 @@ __global__ __launch_bounds__(NW * 64, 2) void epsdivdiv_apply_kernel(const Apply
             }
 #pragma unroll
             for (int p = 0; p < K; ++p) {
-                if (rl * K + p >= a.ncells) continue;
+                const T cvp = cell_ok ? cv[p] : T(0);
                 T A0[3], A1[3], Cc[3];
                 // wedge 0: (c00, c10, c01)
                 wedge_stress<T>(g0, B0.su[p], B1.su[p], T0.su[p], B0.du[p], B1.du[p], T0.du[p],
-                                B0.kk[p] + B1.kk[p] + T0.kk[p], hs[p], tt[p], qq[p], cv[p], A0, A1, Cc);
+                                B0.kk[p] + B1.kk[p] + T0.kk[p], hs[p], tt[p], qq[p], cvp, A0, A1, Cc);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const T x00 = -(A0[c] + A1[c]);
@@ -243,7 +334,7 @@ __global__ __launch_bounds__(NW * 64, 2) void epsdivdiv_apply_kernel(const Apply
                 }
                 // wedge 1: (c11, c01, c10)
                 wedge_stress<T>(g1, T1.su[p], T0.su[p], B1.su[p], T1.du[p], T0.du[p], B1.du[p],
-                                T1.kk[p] + T0.kk[p] + B1.kk[p], hs[p], tt[p], qq[p], cv[p], A0, A1, Cc);
+                                T1.kk[p] + T0.kk[p] + B1.kk[p], hs[p], tt[p], qq[p], cvp, A0, A1, Cc);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const T x11 = -(A0[c] + A1[c]);
@@ -257,74 +348,88 @@ __global__ __launch_bounds__(NW * 64, 2) void epsdivdiv_apply_kernel(const Apply
             }
         }
 
-        // left-corner contributions -> neighbour v-1
+        // left corners -> neighbour v-1, combined radially
         const int buf = step & 1;
 #pragma unroll
-        for (int i = 0; i < KL; ++i)
+        for (int c = 0; c < 3; ++c) {
+            const T b00 = wave_shr1(s00[K][c]), b01 = wave_shr1(s01[K][c]);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                lds[buf][v][0][i][c][rl] = s00[i][c];
-                lds[buf][v][1][i][c][rl] = s01[i][c];
+            for (int p = 0; p < K; ++p) {
+                xch[buf][v][0][p][c][rl] = s00[p][c] + (p == 0 && rl > 0 ? b00 : T(0));
+                xch[buf][v][1][p][c][rl] = s01[p][c] + (p == 0 && rl > 0 ? b01 : T(0));
             }
+            if (ovf_lane) {
+                xovf[buf][v][0][c] = s00[K][c];
+                xovf[buf][v][1][c] = s01[K][c];
+            }
+        }
         __syncthreads();
         if (v + 1 < VX) {
 #pragma unroll
-            for (int i = 0; i < KL; ++i)
+            for (int c = 0; c < 3; ++c) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    acc_cur[i][c] += lds[buf][v + 1][0][i][c][rl];
-                    acc_next[i][c] += lds[buf][v + 1][1][i][c][rl];
+                for (int p = 0; p < K; ++p) {
+                    acc_cur[p][c] += xch[buf][v + 1][0][p][c][rl];
+                    acc_next[p][c] += xch[buf][v + 1][1][p][c][rl];
                 }
+                if (ovf_lane) {
+                    acc_cur[K][c] += xovf[buf][v + 1][0][c];
+                    acc_next[K][c] += xovf[buf][v + 1][1][c];
+                }
+            }
         }
 
         // node row yc is complete: combine radially and store
         T below[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const T up = __shfl_up(acc_cur[K][c], 1, R);
+            const T up = wave_shr1(acc_cur[K][c]);
             below[c] = rl > 0 ? up : T(0);
         }
         const int Y = yc;
-        if (Y >= Y0 && v < VX - 1 && X < nx && rl * K < a.ncells) {
+        if (Y >= Y0 && v < VX - 1 && X < nx && lane_has) {
             const int64_t lcol = ((int64_t)sd * nx + X) * nx + Y;
-            const int64_t base = lcol * a.nr;
+            const int64_t base = lcol * nr;
+            T pb[3] = {T(0), T(0), T(0)}, cc[3] = {T(0), T(0), T(0)};
+            if (a.pen) {
+                pb[0] = (T)a.pen[0];
+                pb[1] = (T)a.pen[1];
+                pb[2] = (T)a.pen[2];
+                cc[0] = a.coords[3 * lcol];
+                cc[1] = a.coords[3 * lcol + 1];
+                cc[2] = a.coords[3 * lcol + 2];
+            }
 #pragma unroll
             for (int p = 0; p < KL; ++p) {
+                if (p == K && !ovf_lane) continue;
                 const int l = rl * K + p;
-                if (l > a.ncells) continue;
-                if (p == K && l != a.ncells) continue;
                 T val[3];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) val[c] = acc_cur[p][c] + (p == 0 ? below[c] : T(0));
-                const bool at_cmb = (l == 0) && has_cmb;
-                const bool at_surf = (l == a.nr - 1) && has_surf;
+                const bool at_cmb = p == 0 && rl == 0 && has_cmb && a.bc_cmb != TNG_NEUMANN;
+                const bool at_surf = p == K && has_surf && a.bc_surf != TNG_NEUMANN;
                 if (at_cmb || at_surf) {
-                    const int bc = at_cmb ? a.bc_cmb : a.bc_surf;
-                    if (bc != TNG_NEUMANN) {
-                        const T *bt = a.btab + (lcol * 2 + (at_cmb ? 0 : 1)) * 3;
-                        const T u0 = a.src0[base + l], u1 = a.src1[base + l], u2 = a.src2[base + l];
-                        if (bc == TNG_DIRICHLET) {
-                            val[0] = bt[0] * u0;
-                            val[1] = bt[1] * u1;
-                            val[2] = bt[2] * u2;
-                        } else {
-                            T n[3];
-                            normal_of(a.coords + 3 * lcol, n);
-                            const T dv = n[0] * val[0] + n[1] * val[1] + n[2] * val[2];
-                            const T un = bt[0] * (n[0] * u0 + n[1] * u1 + n[2] * u2);
+                    const int bcv = at_cmb ? a.bc_cmb : a.bc_surf;
+                    const T *bt = a.btab + (lcol * 2 + (at_cmb ? 0 : 1)) * 3;
+                    const T u0 = a.src0[base + l], u1 = a.src1[base + l], u2 = a.src2[base + l];
+                    if (bcv == TNG_DIRICHLET) {
+                        val[0] = bt[0] * u0;
+                        val[1] = bt[1] * u1;
+                        val[2] = bt[2] * u2;
+                    } else {
+                        const T *n = a.coords + 3 * lcol;
+                        const T dv = n[0] * val[0] + n[1] * val[1] + n[2] * val[2];
+                        const T un = bt[0] * (n[0] * u0 + n[1] * u1 + n[2] * u2);
 #pragma unroll
-                            for (int c = 0; c < 3; ++c) val[c] = val[c] - dv * n[c] + un * n[c];
-                        }
+                        for (int c = 0; c < 3; ++c) val[c] = val[c] - dv * n[c] + un * n[c];
                     }
                 }
                 if (a.pen) {
-                    const T rad = a.radii[(int64_t)sd * a.nr + l];
-                    const T *cc = a.coords + 3 * lcol;
+                    const T rad = a.radii[(int64_t)sd * nr + l];
                     const T x0 = cc[0] * rad, x1 = cc[1] * rad, x2 = cc[2] * rad;
-                    const T b0 = (T)a.pen[0], b1 = (T)a.pen[1], b2 = (T)a.pen[2];
-                    val[0] += b1 * x2 - b2 * x1;
-                    val[1] += b2 * x0 - b0 * x2;
-                    val[2] += b0 * x1 - b1 * x0;
+                    val[0] += pb[1] * x2 - pb[2] * x1;
+                    val[1] += pb[2] * x0 - pb[0] * x2;
+                    val[2] += pb[0] * x1 - pb[1] * x0;
                 }
                 if (a.add) {
                     val[0] += a.dst0[base + l];
@@ -521,7 +626,7 @@ template <typename T> struct PenArgs {
     double *out_b;
     double eps;
     double ginv[9];
-    int nx, nr;
+    int nsd, nx, nr;
     int64_t n;
 };
 
@@ -536,21 +641,26 @@ template <typename T> __global__ __launch_bounds__(256) void penalty_moment_kern
     __shared__ double red[4][3];
     __shared__ bool last;
     double m0 = 0, m1 = 0, m2 = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * blockDim.x) {
-        if (a.omask[i] != OWNED) continue;
-        const int l = (int)(i % a.nr);
-        const int64_t lcol = i / a.nr;
-        const int sd = (int)(lcol / ((int64_t)a.nx * a.nx));
-        const double rad = (double)a.radii[(int64_t)sd * a.nr + l];
-        const double x0 = (double)a.coords[3 * lcol] * rad, x1 = (double)a.coords[3 * lcol + 1] * rad,
-                     x2 = (double)a.coords[3 * lcol + 2] * rad;
-        const double u0 = a.src0[i], u1 = a.src1[i], u2 = a.src2[i];
-        m0 += x1 * u2 - x2 * u1;
-        m1 += x2 * u0 - x0 * u2;
-        m2 += x0 * u1 - x1 * u0;
+    // one wave per lateral column, lanes along r (coalesced, no per-element index arithmetic)
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int64_t ncol = (int64_t)a.nsd * a.nx * a.nx;
+    const int64_t nxx = (int64_t)a.nx * a.nx;
+    for (int64_t col = (int64_t)blockIdx.x * 4 + wave; col < ncol; col += (int64_t)gridDim.x * 4) {
+        const int sd = (int)(col / nxx);
+        const double c0 = (double)a.coords[3 * col], c1 = (double)a.coords[3 * col + 1],
+                     c2 = (double)a.coords[3 * col + 2];
+        const int64_t base = col * a.nr;
+        for (int l = lane; l < a.nr; l += 64) {
+            const double w = a.omask[base + l] == OWNED ? 1.0 : 0.0;
+            const double rad = (double)a.radii[(int64_t)sd * a.nr + l] * w;
+            const double x0 = c0 * rad, x1 = c1 * rad, x2 = c2 * rad;
+            const double u0 = a.src0[base + l], u1 = a.src1[base + l], u2 = a.src2[base + l];
+            m0 += x1 * u2 - x2 * u1;
+            m1 += x2 * u0 - x0 * u2;
+            m2 += x0 * u1 - x1 * u0;
+        }
     }
     m0 = wave_sum(m0); m1 = wave_sum(m1); m2 = wave_sum(m2);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (lane == 0) { red[wave][0] = m0; red[wave][1] = m1; red[wave][2] = m2; }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -563,19 +673,30 @@ template <typename T> __global__ __launch_bounds__(256) void penalty_moment_kern
         last = (ticket == gridDim.x - 1);
     }
     __syncthreads();
-    if (last && threadIdx.x == 0) {
+    if (last) {
+        // the last block sums the per-block partials in a fixed order: thread t
+        // takes partials t, t+256, ..., then a fixed wave/block tree
         __threadfence();
-        double m[3] = {0, 0, 0};
-        for (unsigned int bb = 0; bb < gridDim.x; ++bb)
+        double s[3] = {0, 0, 0};
+        for (unsigned int bb = threadIdx.x; bb < gridDim.x; bb += blockDim.x)
             for (int c = 0; c < 3; ++c) {
                 const unsigned long long bits = __hip_atomic_load(
                     reinterpret_cast<unsigned long long *>(&a.partials[3 * bb + c]), __ATOMIC_RELAXED,
                     __HIP_MEMORY_SCOPE_AGENT);
-                m[c] += __longlong_as_double((long long)bits);
+                s[c] += __longlong_as_double((long long)bits);
             }
-        for (int r = 0; r < 3; ++r)
-            a.out_b[r] = a.eps * (a.ginv[3 * r] * m[0] + a.ginv[3 * r + 1] * m[1] + a.ginv[3 * r + 2] * m[2]);
-        atomicExch(a.counter, 0u);
+        for (int c = 0; c < 3; ++c) s[c] = wave_sum(s[c]);
+        __syncthreads();
+        if (lane == 0) { red[wave][0] = s[0]; red[wave][1] = s[1]; red[wave][2] = s[2]; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double m[3] = {0, 0, 0};
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+                for (int c = 0; c < 3; ++c) m[c] += red[w][c];
+            for (int r = 0; r < 3; ++r)
+                a.out_b[r] = a.eps * (a.ginv[3 * r] * m[0] + a.ginv[3 * r + 1] * m[1] + a.ginv[3 * r + 2] * m[2]);
+            atomicExch(a.counter, 0u);
+        }
     }
 }
 
@@ -599,9 +720,15 @@ template <typename T, int R, int K, int NW> void launch_apply_rk(const ApplyArgs
     hipLaunchKernelGGL((epsdivdiv_apply_kernel<T, R, K, NW>), dim3(blocks), dim3(NW * 64), 0, s, a);
 }
 
+int env_int(const char *name, int dflt)
+{
+    const char *v = std::getenv(name);
+    return v ? std::atoi(v) : dflt;
+}
+
 template <typename T> void launch_apply(ApplyArgs<T> a, hipStream_t s)
 {
-    constexpr int NW = 4;
+    TNG_REQUIRE(a.ncells >= 1 && (a.ncells & (a.ncells - 1)) == 0, "radial layers per subdomain must be a power of 2");
     int R, K;
     if (a.ncells <= 64) {
         K = 1;
@@ -609,31 +736,37 @@ template <typename T> void launch_apply(ApplyArgs<T> a, hipStream_t s)
         while (R < a.ncells) R *= 2;
     } else {
         R = 64;
-        K = (a.ncells + 63) / 64;
-        if (K == 3) K = 4;
+        K = a.ncells / 64;
         TNG_REQUIRE(K <= 4, "more than 256 radial layers per subdomain is not supported");
     }
+    const int NW = K == 1 ? 8 : 4;
     const int VX = NW * (64 / R);
     a.strips = (a.nx + (VX - 1) - 1) / (VX - 1);
-    // segment length: aim for enough workgroups to fill the chip
+    // y-segment length: fewer, longer segments recompute fewer halo rows;
+    // enough workgroups are needed to fill the 256 CUs
     const int64_t lateral_groups = (int64_t)a.nsd * a.strips;
-    int seg = a.nx;
-    while (seg > 8 && lateral_groups * ((a.nx + seg - 1) / seg) < 2048) seg = std::max(8, (seg + 1) / 2);
+    int seg = env_int("TNG_APPLY_SEG", 0);
+    if (seg <= 0) {
+        const int64_t target = env_int("TNG_APPLY_BLOCKS", 480);
+        int nseg = (int)std::max<int64_t>(1, (target + lateral_groups - 1) / lateral_groups);
+        nseg = std::min(nseg, std::max(1, a.nx / 8));
+        seg = (a.nx + nseg - 1) / nseg;
+    }
     a.seg_len = seg;
     a.segs = (a.nx + seg - 1) / seg;
     const int blocks = (int)(lateral_groups * a.segs);
     if (K == 1) {
         switch (R) {
-        case 4: launch_apply_rk<T, 4, 1, NW>(a, blocks, s); break;
-        case 8: launch_apply_rk<T, 8, 1, NW>(a, blocks, s); break;
-        case 16: launch_apply_rk<T, 16, 1, NW>(a, blocks, s); break;
-        case 32: launch_apply_rk<T, 32, 1, NW>(a, blocks, s); break;
-        default: launch_apply_rk<T, 64, 1, NW>(a, blocks, s); break;
+        case 4: launch_apply_rk<T, 4, 1, 8>(a, blocks, s); break;
+        case 8: launch_apply_rk<T, 8, 1, 8>(a, blocks, s); break;
+        case 16: launch_apply_rk<T, 16, 1, 8>(a, blocks, s); break;
+        case 32: launch_apply_rk<T, 32, 1, 8>(a, blocks, s); break;
+        default: launch_apply_rk<T, 64, 1, 8>(a, blocks, s); break;
         }
     } else if (K == 2) {
-        launch_apply_rk<T, 64, 2, NW>(a, blocks, s);
+        launch_apply_rk<T, 64, 2, 4>(a, blocks, s);
     } else {
-        launch_apply_rk<T, 64, 4, NW>(a, blocks, s);
+        launch_apply_rk<T, 64, 4, 4>(a, blocks, s);
     }
     TNG_HIP_CHECK(hipGetLastError());
 }
@@ -686,7 +819,7 @@ template <typename T> void apply_impl(tng_operator &op, const tng_vector &src, t
             p.out_b = op.d_penalty_b.as<double>();
             p.eps = op.penalty_eps;
             for (int i = 0; i < 9; ++i) p.ginv[i] = m.rot_gram_inv[i];
-            p.nx = m.nx; p.nr = m.nr; p.n = m.nalloc;
+            p.nsd = m.nsd; p.nx = m.nx; p.nr = m.nr; p.n = m.nalloc;
             hipLaunchKernelGGL(penalty_moment_kernel<T>, dim3(kPenBlocks), dim3(256), 0, s, p);
             TNG_HIP_CHECK(hipGetLastError());
         }

@@ -1,0 +1,501 @@
+// Divergence B (fine velocity -> coarse pressure), gradient B^T (coarse
+// pressure -> fine velocity), constant-weight prolongation / restriction and
+// the lumped k-weighted pressure mass.
+//
+// All kernels map consecutive threads to consecutive radial levels (the
+// fastest index of the node layout) and gather per output node, so every
+// output entry is written once, without atomics, deterministically.
+//
+// Divergence / gradient (reference: divergence_kerngen.hpp:373-518,
+// gradient_kerngen.hpp:339-496).  One-point rule at the fine wedge centroid:
+//   B:   coarse node i of coarse wedge w  +=  -|det J| (div u) N^c_i
+//   B^T: fine node j of the fine wedge     +=  -|det J| g_j p(centroid)
+// where fine wedge w of every fine cell belongs to coarse wedge w of the
+// coarse cell (x/2, y/2, r/2) and N^c_i are the coarse shape functions of the
+// fine wedge's position in its coarse triangle (fine_lateral_wedge_idx,
+// kernel_helpers.hpp:600-608; shape_coarse, integrands.hpp:266-393).  At a
+// Dirichlet layer the boundary velocity dofs are removed, at a free-slip
+// layer the velocity is projected onto the tangent plane (trial side for B,
+// test side for B^T).
+#include "tng_internal.hpp"
+
+namespace tng {
+
+namespace {
+
+__host__ __device__ __forceinline__ int fine_lat_idx(int x, int y, int w)
+{
+    // [w][y%2][x%2]
+    return w == 0 ? ((y & 1) ? ((x & 1) ? 3 : 2) : ((x & 1) ? 1 : 0))
+                  : ((y & 1) ? ((x & 1) ? 0 : 1) : ((x & 1) ? 2 : 3));
+}
+
+// Coarse shape function i (0..5) at the fine centroid (1/3, 1/3, 0).
+template <typename T> __device__ __forceinline__ T shape_coarse_c(int i, int lat, int rad)
+{
+    const T third = T(1.0 / 3.0);
+    T sl;
+    switch (i % 3) {
+    case 0: sl = lat == 0 ? T(-0.5) * third - T(0.5) * third + T(1) : (lat == 3 ? T(0.5) * third + T(0.5) * third
+                                                                                 : T(-0.5) * third - T(0.5) * third + T(0.5)); break;
+    case 1: sl = lat == 1 ? T(0.5) * third + T(0.5) : (lat == 3 ? T(0.5) - T(0.5) * third : T(0.5) * third); break;
+    default: sl = lat == 2 ? T(0.5) * third + T(0.5) : (lat == 3 ? T(0.5) - T(0.5) * third : T(0.5) * third); break;
+    }
+    const T sr = (i / 3 == 0) ? (rad == 0 ? T(0.75) : T(0.25)) : (rad == 0 ? T(0.25) : T(0.75));
+    return sl * sr;
+}
+
+// wedge corner (dx, dy) per wedge and local lateral index
+__constant__ int kWC[2][3][2] = {{{0, 0}, {1, 0}, {0, 1}}, {{1, 1}, {0, 1}, {1, 0}}};
+
+template <typename T> struct DivArgs {
+    const T *u0, *u1, *u2;       // fine velocity
+    const T *coords, *radii;     // fine mesh
+    const T *latgeom;            // fine mesh (nsd, nc, nc, 2, 10)
+    const uint8_t *sd_cmb, *sd_surf;
+    T *partial;                  // (nsd, ncc, ncc, nrc-1, 2, 6) coarse-wedge partials
+    int nsd, nx, nr;             // fine
+    int nxc, nrc;                // coarse
+    int bc_cmb, bc_surf;
+};
+
+// Boundary kind of fine cell layer r (boundary_setup of the reference's
+// eval_cell): a cell whose bottom lies on the CMB is a CMB cell, otherwise a
+// cell whose top lies on the surface is a surface cell.  Returns the BC of the
+// cell (TNG_NEUMANN for interior cells) and the wedge node layer (0 bottom,
+// 1 top) that the BC acts on.
+__device__ __forceinline__ int cell_bc(int r, int nr, bool has_cmb, bool has_surf, int bc_cmb, int bc_surf, int &layer)
+{
+    if (r == 0 && has_cmb) {
+        layer = 0;
+        return bc_cmb;
+    }
+    if (r + 1 == nr - 1 && has_surf) {
+        layer = 1;
+        return bc_surf;
+    }
+    layer = -1;
+    return TNG_NEUMANN;
+}
+
+// Trial-side boundary modification of a fine velocity value of column lcol:
+// Dirichlet removes it, free-slip removes its normal component.
+template <typename T> __device__ __forceinline__ void bc_project(const T *coords, int64_t lcol, int bc, T u[3])
+{
+    if (bc == TNG_DIRICHLET) {
+        u[0] = u[1] = u[2] = T(0);
+    } else if (bc == TNG_FREESLIP) {
+        const T *n = coords + 3 * lcol;
+        const T d = n[0] * u[0] + n[1] * u[1] + n[2] * u[2];
+        u[0] -= d * n[0]; u[1] -= d * n[1]; u[2] -= d * n[2];
+    }
+}
+
+// Phase 1 of B: per coarse cell and coarse wedge, the 6 nodal sums over the
+// 8 fine wedges it owns.  One thread per (coarse cell), r fastest.
+template <typename T> __global__ __launch_bounds__(256) void div_partial_kernel(const DivArgs<T> a)
+{
+    const int ncc = a.nxc - 1, nrcc = a.nrc - 1;
+    const int64_t n = (int64_t)a.nsd * ncc * ncc * nrcc;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n) return;
+    const int rc = (int)(gid % nrcc);
+    int64_t t = gid / nrcc;
+    const int yc = (int)(t % ncc); t /= ncc;
+    const int xc = (int)(t % ncc);
+    const int sd = (int)(t / ncc);
+    const int nx = a.nx, nr = a.nr, nc = nx - 1;
+    const bool has_cmb = a.sd_cmb[sd] != 0, has_surf = a.sd_surf[sd] != 0;
+    T acc[2][6];
+    for (int w = 0; w < 2; ++w)
+        for (int i = 0; i < 6; ++i) acc[w][i] = T(0);
+    for (int fx = 0; fx < 2; ++fx)
+        for (int fy = 0; fy < 2; ++fy)
+            for (int fr = 0; fr < 2; ++fr) {
+                const int x = 2 * xc + fx, y = 2 * yc + fy, r = 2 * rc + fr;
+                const T r0 = a.radii[(int64_t)sd * nr + r], r1 = a.radii[(int64_t)sd * nr + r + 1];
+                const T rmid = T(0.5) * (r0 + r1), h = T(0.5) * (r1 - r0);
+                const T s = T(1) / rmid, tz = T(1) / (T(6) * h);
+                const T vol = rmid * rmid * h;
+                int blayer;
+                const int cbc = cell_bc(r, nr, has_cmb, has_surf, a.bc_cmb, a.bc_surf, blayer);
+                for (int w = 0; w < 2; ++w) {
+                    const T *g = a.latgeom + ((((int64_t)sd * nc + x) * nc + y) * 2 + w) * 10;
+                    // div u = sum_j g_j . u_j over the 6 wedge nodes
+                    T div = T(0);
+                    for (int k = 0; k < 3; ++k) {
+                        const int X = x + kWC[w][k][0], Y = y + kWC[w][k][1];
+                        const int64_t lcol = ((int64_t)sd * nx + X) * nx + Y;
+                        const T l0 = k == 0 ? T(-0.5) : (k == 1 ? T(0.5) : T(0));
+                        const T l1 = k == 0 ? T(-0.5) : (k == 1 ? T(0) : T(0.5));
+                        for (int e = 0; e < 2; ++e) {
+                            const int l = r + e;
+                            const int64_t i = lcol * nr + l;
+                            T u[3] = {a.u0[i], a.u1[i], a.u2[i]};
+                            if (e == blayer) bc_project(a.coords, lcol, cbc, u);
+                            const T dz = e == 0 ? -tz : tz;
+                            for (int d = 0; d < 3; ++d) div += (s * (l0 * g[d] + l1 * g[3 + d]) + dz * g[6 + d]) * u[d];
+                        }
+                    }
+                    const T cdiv = -(g[9] * vol) * div;
+                    const int lat = fine_lat_idx(x, y, w);
+                    for (int i = 0; i < 6; ++i) acc[w][i] += cdiv * shape_coarse_c<T>(i, lat, fr);
+                }
+            }
+    T *o = a.partial + gid * 12;
+    for (int w = 0; w < 2; ++w)
+        for (int i = 0; i < 6; ++i) o[w * 6 + i] = acc[w][i];
+}
+
+// Phase 2 of B: coarse node = sum of the coarse-wedge partials it belongs to.
+template <typename T>
+__global__ __launch_bounds__(256) void wedge_gather_kernel(const T *partial, T *dst, int add, int nsd, int nx, int nr)
+{
+    const int64_t n = (int64_t)nsd * nx * nx * nr;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n) return;
+    const int l = (int)(gid % nr);
+    int64_t t = gid / nr;
+    const int Y = (int)(t % nx); t /= nx;
+    const int X = (int)(t % nx);
+    const int sd = (int)(t / nx);
+    const int nc = nx - 1, ncr = nr - 1;
+    // lateral corner index of (dx, dy) in wedge w (-1: not a corner)
+    const int corner_of[2][2][2] = {{{0, 2}, {1, -1}}, {{-1, 1}, {2, 0}}};
+    T s = T(0);
+    for (int dxc = -1; dxc <= 0; ++dxc) {
+        const int xc = X + dxc;
+        if (xc < 0 || xc >= nc) continue;
+        for (int dyc = -1; dyc <= 0; ++dyc) {
+            const int yc = Y + dyc;
+            if (yc < 0 || yc >= nc) continue;
+            for (int w = 0; w < 2; ++w) {
+                const int k = corner_of[w][X - xc][Y - yc];
+                if (k < 0) continue;
+                for (int rc = l - 1; rc <= l; ++rc) {
+                    if (rc < 0 || rc >= ncr) continue;
+                    const int node = k + (rc == l ? 0 : 3);
+                    s += partial[((((int64_t)sd * nc + xc) * nc + yc) * ncr + rc) * 12 + w * 6 + node];
+                }
+            }
+        }
+    }
+    dst[gid] = add ? dst[gid] + s : s;
+}
+
+template <typename T> struct GradArgs {
+    const T *p;                   // coarse pressure
+    T *d0, *d1, *d2;              // fine velocity
+    const T *coords, *radii, *latgeom;
+    const uint8_t *sd_cmb, *sd_surf;
+    int nsd, nx, nr, nxc, nrc;
+    int bc_cmb, bc_surf, add;
+};
+
+// B^T per fine node: sum over the <= 12 fine wedges containing the node.
+template <typename T> __global__ __launch_bounds__(256) void gradient_kernel(const GradArgs<T> a)
+{
+    const int nx = a.nx, nr = a.nr, nc = nx - 1;
+    const int64_t n = (int64_t)a.nsd * nx * nx * nr;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n) return;
+    const int l = (int)(gid % nr);
+    int64_t t = gid / nr;
+    const int Y = (int)(t % nx); t /= nx;
+    const int X = (int)(t % nx);
+    const int sd = (int)(t / nx);
+    const int64_t lcol = t * nx + Y;
+    const int corner_of[2][2][2] = {{{0, 2}, {1, -1}}, {{-1, 1}, {2, 0}}};
+    const bool has_cmb = a.sd_cmb[sd] != 0, has_surf = a.sd_surf[sd] != 0;
+    // o: interior contributions; ob: contributions of boundary cells in which
+    // this node lies on the boundary layer (free-slip: projected at the end)
+    T o[3] = {T(0), T(0), T(0)}, ob[3] = {T(0), T(0), T(0)};
+    int bbc = TNG_NEUMANN;
+    for (int dxc = -1; dxc <= 0; ++dxc) {
+        const int xc = X + dxc;
+        if (xc < 0 || xc >= nc) continue;
+        for (int dyc = -1; dyc <= 0; ++dyc) {
+            const int yc = Y + dyc;
+            if (yc < 0 || yc >= nc) continue;
+            for (int w = 0; w < 2; ++w) {
+                const int k = corner_of[w][X - xc][Y - yc];
+                if (k < 0) continue;
+                const T *g = a.latgeom + ((((int64_t)sd * nc + xc) * nc + yc) * 2 + w) * 10;
+                const T l0 = k == 0 ? T(-0.5) : (k == 1 ? T(0.5) : T(0));
+                const T l1 = k == 0 ? T(-0.5) : (k == 1 ? T(0) : T(0.5));
+                const int lat = fine_lat_idx(xc, yc, w);
+                for (int rc = l - 1; rc <= l; ++rc) {
+                    if (rc < 0 || rc >= nr - 1) continue;
+                    int blayer;
+                    const int cbc = cell_bc(rc, nr, has_cmb, has_surf, a.bc_cmb, a.bc_surf, blayer);
+                    const bool on_bd = blayer == (rc == l ? 0 : 1);
+                    if (on_bd && cbc == TNG_DIRICHLET) continue;
+                    const T r0 = a.radii[(int64_t)sd * nr + rc], r1 = a.radii[(int64_t)sd * nr + rc + 1];
+                    const T rmid = T(0.5) * (r0 + r1), h = T(0.5) * (r1 - r0);
+                    const T s = T(1) / rmid, dz = (rc == l ? T(-1) : T(1)) / (T(6) * h);
+                    // pressure interpolated at the centroid from coarse wedge w of cell (xc/2, yc/2, rc/2)
+                    const int cx = xc / 2, cy = yc / 2, cr = rc / 2;
+                    T pi = T(0);
+                    for (int j = 0; j < 6; ++j) {
+                        const int jx = cx + kWC[w][j % 3][0], jy = cy + kWC[w][j % 3][1], jr = cr + j / 3;
+                        pi += shape_coarse_c<T>(j, lat, rc & 1) *
+                              a.p[(((int64_t)sd * a.nxc + jx) * a.nxc + jy) * a.nrc + jr];
+                    }
+                    const T f = -(g[9] * rmid * rmid * h) * pi;
+                    T *acc = o;
+                    if (on_bd && cbc == TNG_FREESLIP) {
+                        acc = ob;
+                        bbc = TNG_FREESLIP;
+                    }
+                    for (int d = 0; d < 3; ++d) acc[d] += f * (s * (l0 * g[d] + l1 * g[3 + d]) + dz * g[6 + d]);
+                }
+            }
+        }
+    }
+    if (bbc == TNG_FREESLIP) {
+        bc_project(a.coords, lcol, TNG_FREESLIP, ob);
+        o[0] += ob[0]; o[1] += ob[1]; o[2] += ob[2];
+    }
+    if (a.add) {
+        o[0] += a.d0[gid]; o[1] += a.d1[gid]; o[2] += a.d2[gid];
+    }
+    a.d0[gid] = o[0]; a.d1[gid] = o[1]; a.d2[gid] = o[2];
+}
+
+// Constant prolongation (fe/wedge/shell/grid_transfer_constant.hpp:9-90):
+// P1 interpolation per fine node from the coarse nodes of its position.
+template <typename T>
+__global__ __launch_bounds__(256) void prolongate_kernel(const T *src, T *dst, int64_t sstride, int64_t dstride,
+                                                         int ncomp, int add, int nsd, int nxf, int nrf, int nxc, int nrc)
+{
+    const int64_t n = (int64_t)nsd * nxf * nxf * nrf;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n) return;
+    const int rf = (int)(gid % nrf);
+    int64_t t = gid / nrf;
+    const int yf = (int)(t % nxf); t /= nxf;
+    const int xf = (int)(t % nxf);
+    const int sd = (int)(t / nxf);
+    // lateral parents (up to 2) with weights
+    int px[2], py[2], np = 0;
+    T pw;
+    if (!(xf & 1) && !(yf & 1)) { px[0] = xf / 2; py[0] = yf / 2; np = 1; pw = T(1); }
+    else if ((xf & 1) && !(yf & 1)) { px[0] = xf / 2; py[0] = yf / 2; px[1] = xf / 2 + 1; py[1] = yf / 2; np = 2; pw = T(0.5); }
+    else if (!(xf & 1) && (yf & 1)) { px[0] = xf / 2; py[0] = yf / 2; px[1] = xf / 2; py[1] = yf / 2 + 1; np = 2; pw = T(0.5); }
+    else { px[0] = (xf + 1) / 2; py[0] = (yf - 1) / 2; px[1] = (xf - 1) / 2; py[1] = (yf + 1) / 2; np = 2; pw = T(0.5); }
+    int pr[2], nrp;
+    T rw;
+    if (!(rf & 1)) { pr[0] = rf / 2; nrp = 1; rw = T(1); }
+    else { pr[0] = rf / 2; pr[1] = rf / 2 + 1; nrp = 2; rw = T(0.5); }
+    for (int c = 0; c < ncomp; ++c) {
+        T s = T(0);
+        for (int i = 0; i < np; ++i)
+            for (int j = 0; j < nrp; ++j)
+                s += (pw * rw) * src[c * sstride + (((int64_t)sd * nxc + px[i]) * nxc + py[i]) * nrc + pr[j]];
+        T *o = dst + c * dstride + gid;
+        *o = add ? *o + s : s;
+    }
+}
+
+// Constant restriction (restriction_constant.hpp:133-250): owned-masked
+// transpose stencil; the additive exchange runs afterwards.
+template <typename T>
+__global__ __launch_bounds__(256) void restrict_kernel(const T *src, T *dst, int64_t sstride, int64_t dstride, int ncomp,
+                                                       int add, const uint8_t *omask_f, int nsd, int nxf, int nrf,
+                                                       int nxc, int nrc)
+{
+    const int64_t n = (int64_t)nsd * nxc * nxc * nrc;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n) return;
+    const int rc = (int)(gid % nrc);
+    int64_t t = gid / nrc;
+    const int yc = (int)(t % nxc); t /= nxc;
+    const int xc = (int)(t % nxc);
+    const int sd = (int)(t / nxc);
+    const int st[7][2] = {{-1, 0}, {-1, 1}, {0, 1}, {1, 0}, {1, -1}, {0, -1}, {0, 0}};
+    T s[3] = {T(0), T(0), T(0)};
+    for (int dr = -1; dr <= 1; ++dr) {
+        const int rf = 2 * rc + dr;
+        if (rf < 0 || rf >= nrf) continue;
+        const T rw = dr == 0 ? T(1) : T(0.5);
+        for (int k = 0; k < 7; ++k) {
+            const int xf = 2 * xc + st[k][0], yf = 2 * yc + st[k][1];
+            if (xf < 0 || xf >= nxf || yf < 0 || yf >= nxf) continue;
+            const int64_t fi = (((int64_t)sd * nxf + xf) * nxf + yf) * nrf + rf;
+            if (omask_f[fi] != OWNED) continue;
+            const T w = rw * (k == 6 ? T(1) : T(0.5));
+            for (int c = 0; c < ncomp; ++c) s[c] += w * src[c * sstride + fi];
+        }
+    }
+    for (int c = 0; c < ncomp; ++c) {
+        T *o = dst + c * dstride + gid;
+        *o = add ? *o + s[c] : s[c];
+    }
+}
+
+// Lumped k-weighted mass (kmass.hpp:96-180, 6-point rule, row sums), per node.
+template <typename T>
+__global__ __launch_bounds__(256) void kmass_lumped_kernel(const T *k, const T *src, T *dst, const T *radii,
+                                                           const T *latgeom, int nsd, int nx, int nr)
+{
+    const int64_t n = (int64_t)nsd * nx * nx * nr;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n) return;
+    const int l = (int)(gid % nr);
+    int64_t t = gid / nr;
+    const int Y = (int)(t % nx); t /= nx;
+    const int X = (int)(t % nx);
+    const int sd = (int)(t / nx);
+    const int nc = nx - 1;
+    const int corner_of[2][2][2] = {{{0, 2}, {1, -1}}, {{-1, 1}, {2, 0}}};
+    // quadrature.hpp:66-101 (Felippa 3x2)
+    const T qx[3] = {T(0.6666666666666666), T(0.1666666666666667), T(0.1666666666666667)};
+    const T qy[3] = {T(0.1666666666666667), T(0.6666666666666666), T(0.1666666666666667)};
+    const T qz[2] = {T(-0.5773502691896257), T(0.5773502691896257)};
+    const T qw = T(0.1666666666666667);
+    T acc = T(0);
+    for (int dxc = -1; dxc <= 0; ++dxc) {
+        const int xc = X + dxc;
+        if (xc < 0 || xc >= nc) continue;
+        for (int dyc = -1; dyc <= 0; ++dyc) {
+            const int yc = Y + dyc;
+            if (yc < 0 || yc >= nc) continue;
+            for (int w = 0; w < 2; ++w) {
+                const int kc = corner_of[w][X - xc][Y - yc];
+                if (kc < 0) continue;
+                const T detl = latgeom[((((int64_t)sd * nc + xc) * nc + yc) * 2 + w) * 10 + 9];
+                for (int rc = l - 1; rc <= l; ++rc) {
+                    if (rc < 0 || rc >= nr - 1) continue;
+                    const int top = rc == l ? 0 : 1;
+                    const T r1 = radii[(int64_t)sd * nr + rc], r2 = radii[(int64_t)sd * nr + rc + 1];
+                    const T gr = T(0.5) * (r2 - r1);
+                    T kv[6];
+                    for (int j = 0; j < 6; ++j) {
+                        const int jx = xc + kWC[w][j % 3][0], jy = yc + kWC[w][j % 3][1];
+                        kv[j] = k[(((int64_t)sd * nx + jx) * nx + jy) * nr + rc + j / 3];
+                    }
+                    for (int qq = 0; qq < 6; ++qq) {
+                        const T xi = qx[qq % 3], eta = qy[qq % 3], zeta = qz[qq / 3];
+                        const T Nl[3] = {T(1) - xi - eta, xi, eta};
+                        const T Nr[2] = {T(0.5) * (T(1) - zeta), T(0.5) * (T(1) + zeta)};
+                        T ke = T(0);
+                        for (int j = 0; j < 6; ++j) ke += Nl[j % 3] * Nr[j / 3] * kv[j];
+                        const T r = r1 + T(0.5) * (r2 - r1) * (T(1) + zeta);
+                        acc += qw * ke * (Nl[kc] * Nr[top] * r * r * gr * detl);
+                    }
+                }
+            }
+        }
+    }
+    dst[gid] = acc * src[gid];
+}
+
+unsigned blocks_for(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+template <typename T> struct MeshP {
+    const T *coords, *radii, *latgeom;
+};
+template <typename T> MeshP<T> mp(const tng_domain &m);
+template <> MeshP<double> mp<double>(const tng_domain &m)
+{
+    return {m.d_coords.as<double>(), m.d_radii.as<double>(), m.d_latgeom.as<double>()};
+}
+template <> MeshP<float> mp<float>(const tng_domain &m)
+{
+    return {m.d_coords_f.as<float>(), m.d_radii_f.as<float>(), m.d_latgeom_f.as<float>()};
+}
+
+template <typename T> void div_impl(tng_operator &op, const tng_vector &src, tng_vector &dst, hipStream_t s)
+{
+    const tng_domain &f = *op.mesh, &c = *op.mesh2;
+    const int64_t ncell_c = (int64_t)c.nsd * (c.nx - 1) * (c.nx - 1) * (c.nr - 1);
+    const size_t bytes = (size_t)ncell_c * 12 * sizeof(T);
+    if (op.scratch.bytes < bytes) op.scratch.alloc(bytes);
+    DivArgs<T> a{};
+    const MeshP<T> m = mp<T>(f);
+    a.u0 = (const T *)src.comp(0); a.u1 = (const T *)src.comp(1); a.u2 = (const T *)src.comp(2);
+    a.coords = m.coords; a.radii = m.radii; a.latgeom = m.latgeom;
+    a.sd_cmb = f.d_sd_cmb.as<uint8_t>(); a.sd_surf = f.d_sd_surf.as<uint8_t>();
+    a.partial = op.scratch.as<T>();
+    a.nsd = f.nsd; a.nx = f.nx; a.nr = f.nr; a.nxc = c.nx; a.nrc = c.nr;
+    a.bc_cmb = op.bc_cmb; a.bc_surf = op.bc_surf;
+    hipLaunchKernelGGL(div_partial_kernel<T>, dim3(blocks_for(ncell_c)), dim3(256), 0, s, a);
+    TNG_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(wedge_gather_kernel<T>, dim3(blocks_for(c.nalloc)), dim3(256), 0, s,
+                       (const T *)op.scratch.p, (T *)dst.comp(0), op.apply_mode == TNG_ADD ? 1 : 0, c.nsd, c.nx, c.nr);
+    TNG_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T> void grad_impl(tng_operator &op, const tng_vector &src, tng_vector &dst, hipStream_t s)
+{
+    const tng_domain &f = *op.mesh, &c = *op.mesh2;
+    GradArgs<T> a{};
+    const MeshP<T> m = mp<T>(f);
+    a.p = (const T *)src.comp(0);
+    a.d0 = (T *)dst.comp(0); a.d1 = (T *)dst.comp(1); a.d2 = (T *)dst.comp(2);
+    a.coords = m.coords; a.radii = m.radii; a.latgeom = m.latgeom;
+    a.sd_cmb = f.d_sd_cmb.as<uint8_t>(); a.sd_surf = f.d_sd_surf.as<uint8_t>();
+    a.nsd = f.nsd; a.nx = f.nx; a.nr = f.nr; a.nxc = c.nx; a.nrc = c.nr;
+    a.bc_cmb = op.bc_cmb; a.bc_surf = op.bc_surf; a.add = op.apply_mode == TNG_ADD ? 1 : 0;
+    hipLaunchKernelGGL(gradient_kernel<T>, dim3(blocks_for(f.nalloc)), dim3(256), 0, s, a);
+    TNG_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T> void prolong_impl(tng_operator &op, const tng_vector &src, tng_vector &dst, hipStream_t s)
+{
+    const tng_domain &c = *op.mesh2, &f = *op.mesh;  // mesh = fine (dst), mesh2 = coarse (src)
+    hipLaunchKernelGGL(prolongate_kernel<T>, dim3(blocks_for(f.nalloc)), dim3(256), 0, s, (const T *)src.buf.p,
+                       (T *)dst.buf.p, src.comp_stride, dst.comp_stride, src.ncomp, op.apply_mode == TNG_ADD ? 1 : 0,
+                       f.nsd, f.nx, f.nr, c.nx, c.nr);
+    TNG_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T> void restrict_impl(tng_operator &op, const tng_vector &src, tng_vector &dst, hipStream_t s)
+{
+    const tng_domain &f = *op.mesh, &c = *op.mesh2;  // mesh = fine (src), mesh2 = coarse (dst)
+    hipLaunchKernelGGL(restrict_kernel<T>, dim3(blocks_for(c.nalloc)), dim3(256), 0, s, (const T *)src.buf.p,
+                       (T *)dst.buf.p, src.comp_stride, dst.comp_stride, src.ncomp, op.apply_mode == TNG_ADD ? 1 : 0,
+                       f.d_omask.as<uint8_t>(), f.nsd, f.nx, f.nr, c.nx, c.nr);
+    TNG_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T> void kmass_impl(tng_operator &op, const tng_vector &src, tng_vector &dst, hipStream_t s)
+{
+    const tng_domain &m = *op.mesh;
+    const MeshP<T> g = mp<T>(m);
+    hipLaunchKernelGGL(kmass_lumped_kernel<T>, dim3(blocks_for(m.nalloc)), dim3(256), 0, s, (const T *)op.k->comp(0),
+                       (const T *)src.comp(0), (T *)dst.comp(0), g.radii, g.latgeom, m.nsd, m.nx, m.nr);
+    TNG_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+void stokes_block_apply(tng_operator &op, const tng_vector &src, tng_vector &dst, hipStream_t s)
+{
+    const bool f64 = src.dtype == TNG_F64;
+    switch (op.kind) {
+    case OP_DIVERGENCE:
+        f64 ? div_impl<double>(op, src, dst, s) : div_impl<float>(op, src, dst, s);
+        if (op.comm_mode == TNG_COMMUNICATE_ADDITIVELY) launch_halo_sum(*op.mesh2, dst, s);
+        break;
+    case OP_GRADIENT:
+        f64 ? grad_impl<double>(op, src, dst, s) : grad_impl<float>(op, src, dst, s);
+        if (op.comm_mode == TNG_COMMUNICATE_ADDITIVELY) launch_halo_sum(*op.mesh, dst, s);
+        break;
+    case OP_PROLONGATION:
+        f64 ? prolong_impl<double>(op, src, dst, s) : prolong_impl<float>(op, src, dst, s);
+        break;
+    case OP_RESTRICTION:
+        f64 ? restrict_impl<double>(op, src, dst, s) : restrict_impl<float>(op, src, dst, s);
+        launch_halo_sum(*op.mesh2, dst, s);
+        break;
+    case OP_KMASS_LUMPED:
+        f64 ? kmass_impl<double>(op, src, dst, s) : kmass_impl<float>(op, src, dst, s);
+        if (op.comm_mode == TNG_COMMUNICATE_ADDITIVELY) launch_halo_sum(*op.mesh, dst, s);
+        break;
+    default:
+        throw Error(TNG_INVALID_ARGUMENT, "stokes_block_apply: wrong operator kind");
+    }
+}
+
+}  // namespace tng

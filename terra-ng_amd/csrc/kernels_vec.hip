@@ -102,3 +102,206 @@ void launch_halo_sum(const tng_domain &m, tng_vector &v, hipStream_t s)
 }
 
 }  // namespace tng
+
+// ----------------------------------------------------------------------------
+// BLAS-1 over all components (grid_operations.hpp:11-273) and reductions over
+// OWNED node copies (grid_operations.hpp:275-733).  Reductions are two-stage
+// and deterministic: per-block partials in fixed order, the block that draws
+// the last ticket sums them and resets the ticket.
+// ----------------------------------------------------------------------------
+namespace tng {
+
+namespace {
+
+template <typename T>
+__global__ __launch_bounds__(256) void lincomb_kernel(T *y, int64_t ystride, const T *x0, const T *x1, const T *x2,
+                                                      int64_t xstride, int nx, double c0, double c1, double c2,
+                                                      double c3, int ncomp, int64_t n)
+{
+    const T a0 = (T)c0, a1 = (T)c1, a2 = (T)c2, a3 = (T)c3;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        for (int c = 0; c < ncomp; ++c) {
+            T v = a0;
+            if (nx > 0) v += a1 * x0[c * xstride + i];
+            if (nx > 1) v += a2 * x1[c * xstride + i];
+            if (nx > 2) v += a3 * x2[c * xstride + i];
+            y[c * ystride + i] = v;
+        }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void mult_kernel(T *y, const T *x, int64_t ys, int64_t xs, int ncomp, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        for (int c = 0; c < ncomp; ++c) y[c * ys + i] *= x[c * xs + i];
+}
+
+template <typename T> __global__ __launch_bounds__(256) void invert_kernel(T *y, int64_t ys, int ncomp, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        for (int c = 0; c < ncomp; ++c) y[c * ys + i] = T(1) / y[c * ys + i];
+}
+
+__device__ __forceinline__ double wsum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wmax(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+enum RedOp { RED_DOT = 0, RED_MAXABS = 1, RED_NONFINITE = 2 };
+
+// op RED_DOT: sum over OWNED copies of sum_c x_c y_c (y may alias x)
+// op RED_MAXABS: max |x| over all copies; RED_NONFINITE: count of non-finite entries
+template <typename T, int OP>
+__global__ __launch_bounds__(256) void reduce_kernel(const T *x, const T *y, int64_t xs, int64_t ys, int ncomp,
+                                                     const uint8_t *omask, int64_t n, double *partials,
+                                                     unsigned int *counter, double *out)
+{
+    __shared__ double red[4];
+    __shared__ bool last;
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (OP == RED_DOT) {
+            if (omask[i] != OWNED) continue;
+            for (int c = 0; c < ncomp; ++c) acc += (double)x[c * xs + i] * (double)y[c * ys + i];
+        } else if (OP == RED_MAXABS) {
+            for (int c = 0; c < ncomp; ++c) acc = fmax(acc, fabs((double)x[c * xs + i]));
+        } else {
+            for (int c = 0; c < ncomp; ++c) acc += isfinite((double)x[c * xs + i]) ? 0.0 : 1.0;
+        }
+    }
+    acc = OP == RED_MAXABS ? wmax(acc) : wsum(acc);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) red[wave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = red[0];
+        for (int w = 1; w < 4; ++w) s = OP == RED_MAXABS ? fmax(s, red[w]) : s + red[w];
+        partials[blockIdx.x] = s;
+        __threadfence();
+        last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last) {
+        __threadfence();
+        double s = 0.0;
+        for (unsigned int b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+            const unsigned long long bits = __hip_atomic_load(reinterpret_cast<unsigned long long *>(&partials[b]),
+                                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const double v = __longlong_as_double((long long)bits);
+            s = OP == RED_MAXABS ? fmax(s, v) : s + v;
+        }
+        s = OP == RED_MAXABS ? wmax(s) : wsum(s);
+        __syncthreads();
+        if (lane == 0) red[wave] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = red[0];
+            for (int w = 1; w < 4; ++w) t = OP == RED_MAXABS ? fmax(t, red[w]) : t + red[w];
+            *out = t;
+            atomicExch(counter, 0u);
+        }
+    }
+}
+
+constexpr int kRedBlocks = 1024;
+
+unsigned gs(int64_t n)
+{
+    int64_t g = (n + 255) / 256;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 16384));
+}
+
+}  // namespace
+
+void launch_lincomb(tng_vector &y, int nx, const double *c, const tng_vector *const *x, double c0, hipStream_t s)
+{
+    for (int i = 0; i < nx; ++i)
+        TNG_REQUIRE(x[i]->mesh == y.mesh && x[i]->ncomp == y.ncomp && x[i]->dtype == y.dtype,
+                    "lincomb: vectors must share mesh, component count and dtype");
+    TNG_REQUIRE(nx >= 0 && nx <= 3, "lincomb: at most 3 input vectors");
+    const int64_t n = y.mesh->nalloc;
+    const double cc[3] = {nx > 0 ? c[0] : 0, nx > 1 ? c[1] : 0, nx > 2 ? c[2] : 0};
+    if (y.dtype == TNG_F64)
+        hipLaunchKernelGGL(lincomb_kernel<double>, dim3(gs(n)), dim3(256), 0, s, y.buf.as<double>(), y.comp_stride,
+                           nx > 0 ? x[0]->buf.as<double>() : nullptr, nx > 1 ? x[1]->buf.as<double>() : nullptr,
+                           nx > 2 ? x[2]->buf.as<double>() : nullptr, y.comp_stride, nx, c0, cc[0], cc[1], cc[2],
+                           y.ncomp, n);
+    else
+        hipLaunchKernelGGL(lincomb_kernel<float>, dim3(gs(n)), dim3(256), 0, s, y.buf.as<float>(), y.comp_stride,
+                           nx > 0 ? x[0]->buf.as<float>() : nullptr, nx > 1 ? x[1]->buf.as<float>() : nullptr,
+                           nx > 2 ? x[2]->buf.as<float>() : nullptr, y.comp_stride, nx, c0, cc[0], cc[1], cc[2],
+                           y.ncomp, n);
+    TNG_HIP_CHECK(hipGetLastError());
+}
+
+void launch_mult(tng_vector &y, const tng_vector &x, hipStream_t s)
+{
+    TNG_REQUIRE(x.mesh == y.mesh && x.dtype == y.dtype && (x.ncomp == y.ncomp), "scale: mismatched vectors");
+    const int64_t n = y.mesh->nalloc;
+    if (y.dtype == TNG_F64)
+        hipLaunchKernelGGL(mult_kernel<double>, dim3(gs(n)), dim3(256), 0, s, y.buf.as<double>(), x.buf.as<double>(),
+                           y.comp_stride, x.comp_stride, y.ncomp, n);
+    else
+        hipLaunchKernelGGL(mult_kernel<float>, dim3(gs(n)), dim3(256), 0, s, y.buf.as<float>(), x.buf.as<float>(),
+                           y.comp_stride, x.comp_stride, y.ncomp, n);
+    TNG_HIP_CHECK(hipGetLastError());
+}
+
+void launch_invert(tng_vector &y, hipStream_t s)
+{
+    const int64_t n = y.mesh->nalloc;
+    if (y.dtype == TNG_F64)
+        hipLaunchKernelGGL(invert_kernel<double>, dim3(gs(n)), dim3(256), 0, s, y.buf.as<double>(), y.comp_stride,
+                           y.ncomp, n);
+    else
+        hipLaunchKernelGGL(invert_kernel<float>, dim3(gs(n)), dim3(256), 0, s, y.buf.as<float>(), y.comp_stride,
+                           y.ncomp, n);
+    TNG_HIP_CHECK(hipGetLastError());
+}
+
+// Reduction into ctx.red_result[slot] on the device.
+void launch_reduce(int op, const tng_vector &x, const tng_vector *y, int slot, hipStream_t s)
+{
+    tng_context &ctx = *x.mesh->ctx;
+    if (y) TNG_REQUIRE(y->mesh == x.mesh && y->ncomp == x.ncomp && y->dtype == x.dtype, "dot: mismatched vectors");
+    const int64_t n = x.mesh->nalloc;
+    const unsigned g = std::min<unsigned>(kRedBlocks, gs(n));
+    double *out = ctx.red_result.as<double>() + slot;
+    const uint8_t *om = x.mesh->d_omask.as<uint8_t>();
+    double *part = ctx.red_partials.as<double>();
+    unsigned int *cnt = ctx.red_counter.as<unsigned int>();
+    const void *yp = y ? y->buf.p : x.buf.p;
+    const int64_t ys = y ? y->comp_stride : x.comp_stride;
+#define TNG_RED(T, OPV)                                                                                            \
+    hipLaunchKernelGGL((reduce_kernel<T, OPV>), dim3(g), dim3(256), 0, s, (const T *)x.buf.p, (const T *)yp,      \
+                       x.comp_stride, ys, x.ncomp, om, n, part, cnt, out)
+    if (x.dtype == TNG_F64) {
+        if (op == RED_DOT) TNG_RED(double, RED_DOT);
+        else if (op == RED_MAXABS) TNG_RED(double, RED_MAXABS);
+        else TNG_RED(double, RED_NONFINITE);
+    } else {
+        if (op == RED_DOT) TNG_RED(float, RED_DOT);
+        else if (op == RED_MAXABS) TNG_RED(float, RED_MAXABS);
+        else TNG_RED(float, RED_NONFINITE);
+    }
+#undef TNG_RED
+    TNG_HIP_CHECK(hipGetLastError());
+}
+
+double reduce_to_host(int op, const tng_vector &x, const tng_vector *y, hipStream_t s)
+{
+    tng_context &ctx = *x.mesh->ctx;
+    launch_reduce(op, x, y, 0, s);
+    double v = 0;
+    TNG_HIP_CHECK(hipMemcpyAsync(&v, ctx.red_result.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    TNG_HIP_CHECK(hipStreamSynchronize(s));
+    return v;
+}
+
+}  // namespace tng

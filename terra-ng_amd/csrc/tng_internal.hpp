@@ -126,11 +126,13 @@ struct tng_vector {
     void *comp(int c) const { return static_cast<char *>(buf.p) + (size_t)c * comp_stride * elem_size(); }
 };
 
-enum OpKind { OP_EPSDIVDIV = 1 };
+enum OpKind { OP_EPSDIVDIV = 1, OP_DIVERGENCE, OP_GRADIENT, OP_PROLONGATION, OP_RESTRICTION, OP_KMASS_LUMPED };
 
 struct tng_operator {
     OpKind kind = OP_EPSDIVDIV;
-    tng_domain *mesh = nullptr;
+    tng_domain *mesh = nullptr;   // fine / own mesh
+    tng_domain *mesh2 = nullptr;  // coarse mesh of the two-level operators
+    tng::DevBuf scratch;          // per-operator work space (coarse-wedge partials)
     tng_vector *k = nullptr;
     int bc_cmb = TNG_NEUMANN, bc_surf = TNG_NEUMANN;
     bool diagonal = false;
@@ -155,5 +157,17 @@ void launch_copy(tng_vector &dst, const tng_vector &src, hipStream_t s);
 void launch_halo_sum(const tng_domain &m, tng_vector &v, hipStream_t s);
 void epsdivdiv_update_boundary_table(tng_operator &op, hipStream_t s);
 void epsdivdiv_apply(tng_operator &op, const tng_vector &src, tng_vector &dst, hipStream_t s);
+void stokes_block_apply(tng_operator &op, const tng_vector &src, tng_vector &dst, hipStream_t s);
+
+// BLAS-1 and reductions (kernels_vec.hip)
+void launch_lincomb(tng_vector &y, int nx, const double *c, const tng_vector *const *x, double c0, hipStream_t s);
+void launch_mult(tng_vector &y, const tng_vector &x, hipStream_t s);
+void launch_invert(tng_vector &y, hipStream_t s);
+enum { REDUCE_DOT = 0, REDUCE_MAXABS = 1, REDUCE_NONFINITE = 2 };
+void launch_reduce(int op, const tng_vector &x, const tng_vector *y, int slot, hipStream_t s);
+double reduce_to_host(int op, const tng_vector &x, const tng_vector *y, hipStream_t s);
+
+// generic operator apply (dispatch on kind)
+void op_apply(tng_operator &op, const tng_vector &src, tng_vector &dst);
 
 }  // namespace tng

@@ -79,6 +79,17 @@ EXPORTS = [
     ("tng_op_apply", _i, [_vp, _vp, _vp]),
     ("tng_op_destroy", _i, [_vp]),
     ("tng_ctx_profile_read", _i, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    ("tng_divergence_create", _i, [_vp, _vp, _i, _i, _pv]),
+    ("tng_gradient_create", _i, [_vp, _vp, _i, _i, _pv]),
+    ("tng_prolongation_create", _i, [_vp, _vp, _pv]),
+    ("tng_restriction_create", _i, [_vp, _vp, _pv]),
+    ("tng_kmass_lumped_create", _i, [_vp, _vp, _pv]),
+    ("tng_vec_lincomb", _i, [_vp, _i, C.POINTER(C.c_double), C.POINTER(C.c_void_p), _d]),
+    ("tng_vec_scale_with_vector", _i, [_vp, _vp]),
+    ("tng_vec_invert", _i, [_vp]),
+    ("tng_vec_dot", _i, [_vp, _vp, C.POINTER(C.c_double)]),
+    ("tng_vec_max_abs", _i, [_vp, C.POINTER(C.c_double)]),
+    ("tng_vec_has_nan_or_inf", _i, [_vp, C.POINTER(C.c_int)]),
 ]
 
 _lib = None
@@ -251,18 +262,40 @@ class VectorQ1Vec(_Vector):
     ncomp = 3
 
 
-class EpsilonDivDivKerngen:
-    """fe/wedge/operators/shell/epsilon_divdiv_kerngen.hpp:70-2393 (the HIP path)."""
+class _Operator:
+    """Common part of the operators: modes, apply, lifetime (linalg/operator.hpp:29-159)."""
 
-    def __init__(self, domain, k, bcs=(DIRICHLET, DIRICHLET), diagonal=False):
-        self.domain, self.k = domain, k
-        h = C.c_void_p()
-        domain.ctx.check(load_library().tng_epsdivdiv_create(domain.h, k.h, int(bcs[0]), int(bcs[1]),
-                                                             int(diagonal), C.byref(h)))
-        self.h = h
+    def __init__(self, domain, h):
+        self.domain, self.h = domain, h
 
     def set_operator_apply_and_communication_modes(self, apply_mode, comm_mode):
         self.domain.ctx.check(load_library().tng_op_set_modes(self.h, apply_mode, comm_mode))
+
+    def apply(self, src, dst):
+        self.domain.ctx.check(load_library().tng_op_apply(self.h, src.h, dst.h))
+
+    def __del__(self):
+        try:
+            if self.h:
+                load_library().tng_op_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def _create(domain, fn, *args):
+    h = C.c_void_p()
+    domain.ctx.check(getattr(load_library(), fn)(*args, C.byref(h)))
+    return h
+
+
+class EpsilonDivDivKerngen(_Operator):
+    """fe/wedge/operators/shell/epsilon_divdiv_kerngen.hpp:70-2393 (the HIP path)."""
+
+    def __init__(self, domain, k, bcs=(DIRICHLET, DIRICHLET), diagonal=False):
+        super().__init__(domain, _create(domain, "tng_epsdivdiv_create", domain.h, k.h, int(bcs[0]), int(bcs[1]),
+                                         int(diagonal)))
+        self.k = k
 
     def set_diagonal(self, v):
         self.domain.ctx.check(load_library().tng_op_set_diagonal(self.h, int(v)))
@@ -276,16 +309,97 @@ class EpsilonDivDivKerngen:
     def coefficient_updated(self):
         self.domain.ctx.check(load_library().tng_op_coefficient_updated(self.h))
 
-    def apply(self, src, dst):
-        self.domain.ctx.check(load_library().tng_op_apply(self.h, src.h, dst.h))
 
-    def __del__(self):
-        try:
-            if self.h:
-                load_library().tng_op_destroy(self.h)
-                self.h = None
-        except Exception:
-            pass
+class DivergenceKerngen(_Operator):
+    """fe/wedge/operators/shell/divergence_kerngen.hpp:121-257: fine velocity -> coarse pressure."""
+
+    def __init__(self, domain_fine, domain_coarse, bcs=(DIRICHLET, DIRICHLET)):
+        super().__init__(domain_fine, _create(domain_fine, "tng_divergence_create", domain_fine.h, domain_coarse.h,
+                                              int(bcs[0]), int(bcs[1])))
+        self.domain_coarse = domain_coarse
+
+
+class GradientKerngen(_Operator):
+    """fe/wedge/operators/shell/gradient_kerngen.hpp:126-258: coarse pressure -> fine velocity."""
+
+    def __init__(self, domain_fine, domain_coarse, bcs=(DIRICHLET, DIRICHLET)):
+        super().__init__(domain_fine, _create(domain_fine, "tng_gradient_create", domain_fine.h, domain_coarse.h,
+                                              int(bcs[0]), int(bcs[1])))
+        self.domain_coarse = domain_coarse
+
+
+class ProlongationConstant(_Operator):
+    """fe/wedge/operators/shell/prolongation_constant.hpp:14-200 (scalar or vector by the vectors)."""
+
+    def __init__(self, domain_coarse, domain_fine):
+        super().__init__(domain_fine, _create(domain_fine, "tng_prolongation_create", domain_coarse.h,
+                                              domain_fine.h))
+        self.domain_coarse = domain_coarse
+
+
+ProlongationVecConstant = ProlongationConstant
+
+
+class RestrictionConstant(_Operator):
+    """fe/wedge/operators/shell/restriction_constant.hpp:18-230 (scalar or vector by the vectors)."""
+
+    def __init__(self, domain_fine, domain_coarse):
+        super().__init__(domain_fine, _create(domain_fine, "tng_restriction_create", domain_fine.h,
+                                              domain_coarse.h))
+        self.domain_coarse = domain_coarse
+
+
+RestrictionVecConstant = RestrictionConstant
+
+
+class KMassLumped(_Operator):
+    """fe/wedge/operators/shell/kmass.hpp:44-190 with lumped_diagonal = true."""
+
+    def __init__(self, domain, k):
+        super().__init__(domain, _create(domain, "tng_kmass_lumped_create", domain.h, k.h))
+        self.k = k
+
+
+# ------------------------------------------------- linalg (linalg/vector.hpp)
+
+def lincomb(y, c, x, c0=0.0):
+    """y = sum c[i] x[i] + c0 (linalg/vector.hpp:72-95)."""
+    n = len(c)
+    if n != len(x):
+        raise ValueError("lincomb: c and x must have the same size")
+    ca = (C.c_double * max(n, 1))(*[float(v) for v in c])
+    xa = (C.c_void_p * max(n, 1))(*[v.h.value if isinstance(v.h, C.c_void_p) else v.h for v in x])
+    y.domain.ctx.check(load_library().tng_vec_lincomb(y.h, n, ca, xa, float(c0)))
+
+
+def scale_in_place(y, x):
+    y.domain.ctx.check(load_library().tng_vec_scale_with_vector(y.h, x.h))
+
+
+def invert_entries(y):
+    y.domain.ctx.check(load_library().tng_vec_invert(y.h))
+
+
+def dot(x, y):
+    out = C.c_double()
+    x.domain.ctx.check(load_library().tng_vec_dot(x.h, y.h, C.byref(out)))
+    return out.value
+
+
+def norm_2(x):
+    return float(np.sqrt(dot(x, x)))
+
+
+def norm_inf(x):
+    out = C.c_double()
+    x.domain.ctx.check(load_library().tng_vec_max_abs(x.h, C.byref(out)))
+    return out.value
+
+
+def has_nan_or_inf(x):
+    out = C.c_int()
+    x.domain.ctx.check(load_library().tng_vec_has_nan_or_inf(x.h, C.byref(out)))
+    return bool(out.value)
 
 
 def apply(op, src, dst):

@@ -23,7 +23,7 @@ def consistent(mesh, field):
     flat = f.reshape((-1, mesh.nalloc))
     val = np.zeros((flat.shape[0], mesh.nunique))
     val[:, cid[own]] = flat[:, own]
-    return val[:, cid].reshape(lead + mesh.shape)
+    return np.ascontiguousarray(val[:, cid].reshape(lead + mesh.shape))
 
 
 def rel_err(a, b, mask=None):
