@@ -102,6 +102,12 @@ tng_status tng_mesh_copy_ownership_mask(tng_mesh mesh, uint8_t *host_out);
 /* linalg/vector_q1.hpp:19-155 VectorQ1Scalar (ncomp = 1) and :166-304
  * VectorQ1Vec (ncomp = 3); storage grid/grid_types.hpp:44-113. */
 tng_status tng_vec_create(tng_mesh mesh, tng_dtype dtype, int ncomp, tng_vec *out);
+/* Non-owning view of caller device memory: ncomp component arrays, each
+ * num_alloc contiguous elements in (sd, x, y, r) order -- the layout of a
+ * Kokkos-HIP Grid4DDataVec component (grid/grid_types.hpp:44-113), so a
+ * TERRA-NG vector can be passed without copies.  The memory must outlive the
+ * view; destroying the view does not free it. */
+tng_status tng_vec_wrap(tng_mesh mesh, tng_dtype dtype, int ncomp, void *const *component_ptrs, tng_vec *out);
 tng_status tng_vec_destroy(tng_vec v);
 /* Host <-> device in the reference order (comp, sd, x, y, r); dtype of v. */
 tng_status tng_vec_upload(tng_vec v, const void *host);
@@ -176,6 +182,102 @@ tng_status tng_vec_dot(tng_vec x, tng_vec y, double *out);
 tng_status tng_vec_max_abs(tng_vec x, double *out);
 /* linalg/vector.hpp:189-197 has_nan_or_inf (synchronises) */
 tng_status tng_vec_has_nan_or_inf(tng_vec x, int *out);
+
+/* ---------------------------------------------------------------- solvers */
+
+/* linalg/solvers/pcg.hpp:28-188 PCG with the identity preconditioner
+ * (identity_solver.hpp:20-40) on any square operator (src and dst on the same
+ * mesh).  x holds the initial guess.  Returns the iterations done and the
+ * last relative residual ||r_k|| / ||r_0||. */
+tng_status tng_pcg_solve(tng_op op, tng_vec x, tng_vec b, int max_iterations, double relative_tolerance,
+                         double absolute_tolerance, int *iterations, double *relative_residual);
+/* linalg/solvers/power_iteration.hpp:13-58 on D^{-1} A (D^{-1} = inv_diag),
+ * seeded uniform start vector. */
+tng_status tng_power_iteration(tng_op op, tng_vec inv_diag, int iterations, unsigned long long seed,
+                               double *estimate);
+/* linalg/solvers/chebyshev.hpp:102-165 with a given estimate of the largest
+ * eigenvalue of D^{-1} A (lambda_max = 1.5 est, lambda_min = 0.1 est);
+ * `iterations` sweeps of order `order` on A x = b. */
+tng_status tng_chebyshev_smooth(tng_op op, tng_vec inv_diag, int order, int iterations, double lambda_max_estimate,
+                                tng_vec x, tng_vec b);
+
+/* apps/mantlecirculation/src/parameters.hpp:155-185 StokesSolverParameters
+ * (same field names and defaults) plus the multigrid precision, the coarse
+ * PCG parameters (stokes_solver.hpp:469-470) and the power-iteration seed. */
+typedef struct {
+    int krylov_restart;                         /* 10 */
+    int krylov_max_iterations;                  /* 10 */
+    double krylov_relative_tolerance;           /* 1e-6 */
+    double krylov_absolute_tolerance;           /* 1e-12 */
+    int viscous_pc_num_vcycles;                 /* 1 */
+    int viscous_pc_chebyshev_order;             /* 2 */
+    int viscous_pc_num_smoothing_steps_prepost; /* 2 */
+    int viscous_pc_num_power_iterations;        /* 10 */
+    int gca;                                    /* 0 (re-discretised coarse operators) */
+    int viscous_pc_dtype;                       /* TNG_F64, or TNG_F32 for an FP32 V-cycle */
+    int coarse_max_iterations;                  /* 50 */
+    double coarse_relative_tolerance;           /* 1e-6 */
+    double coarse_absolute_tolerance;           /* 1e-16 */
+    unsigned long long seed;                    /* 12345 */
+} tng_stokes_params;
+
+/* Outcome of a solve.  history (caller memory, may be NULL) receives the
+ * FGMRES residual estimates, history_len counts all of them. */
+typedef struct {
+    int iterations;
+    int restarts;
+    int converged;
+    int breakdowns;               /* Arnoldi breakdown / Givens stagnation restarts */
+    int nonfinite_preconditioner; /* preconditioner outputs replaced by their input */
+    int nonfinite_residual;
+    long long coarse_iterations;  /* PCG iterations on the coarsest level */
+    double initial_residual;
+    double final_residual;        /* true residual ||f - K x|| at the last restart */
+    double relative_residual;
+    double *history;
+    int history_capacity;
+    int history_len;
+} tng_solve_report;
+
+typedef struct tng_stokes_solver_s *tng_stokes_solver;
+
+void tng_stokes_params_default(tng_stokes_params *params);
+
+/* apps/mantlecirculation/src/stokes_solver.hpp:143-642 StokesContext set-up
+ * for one rank: meshes of lateral levels min_level..max_level (radial level =
+ * lateral level + radial_extra_levels), velocity on max_level, pressure on
+ * max_level - 1, K = EpsDivDivStokes (epsilon_divdiv_stokes.hpp:18-133),
+ * multigrid (Chebyshev smoothers, constant P/R, coarse PCG), lumped-KMass
+ * Schur approximation, block-triangular preconditioner and FGMRES.  The
+ * viscosity starts at 1 on every level. */
+tng_status tng_stokes_solver_create(tng_ctx ctx, int min_level, int max_level, int radial_extra_levels, double r_min,
+                                    double r_max, int lateral_sdr, int radial_sdr, tng_bc bc_cmb, tng_bc bc_surface,
+                                    const tng_stokes_params *params, tng_stokes_solver *out);
+tng_status tng_stokes_solver_destroy(tng_stokes_solver solver);
+tng_status tng_stokes_solver_num_levels(tng_stokes_solver solver, int *num_levels);
+/* Mesh of multigrid level `level` (0 = coarsest; num_levels - 1 = velocity,
+ * num_levels - 2 = pressure).  The solver keeps ownership: do not destroy. */
+tng_status tng_stokes_solver_mesh(tng_stokes_solver solver, int level, tng_mesh *mesh);
+/* Viscosity on the velocity mesh; coarse levels receive its injection.
+ * Rebuilds the inverse diagonals, the Schur scaling (k = 1/eta on the
+ * pressure level) and re-arms the Chebyshev eigenvalue estimates. */
+tng_status tng_stokes_solver_set_viscosity(tng_stokes_solver solver, tng_vec eta);
+/* K [u; p] (epsilon_divdiv_stokes.hpp:55-84) */
+tng_status tng_stokes_apply(tng_stokes_solver solver, tng_vec u, tng_vec p, tng_vec out_u, tng_vec out_p);
+/* The velocity preconditioner alone: num_vcycles V-cycles on A x = b, x as given. */
+tng_status tng_stokes_precondition_velocity(tng_stokes_solver solver, tng_vec x, tng_vec b);
+/* RHS enforcement of StokesContext::solve (stokes_solver.hpp:664-674) on a
+ * velocity RHS: zero at the Dirichlet boundary, normal component removed at
+ * the free-slip boundary; as in the reference, only the first boundary (CMB,
+ * then surface) carrying each condition is treated
+ * (grid/shell/bit_masks.hpp:68-76). */
+tng_status tng_stokes_enforce_rhs(tng_stokes_solver solver, tng_vec f_u);
+/* FGMRES solve of K [u; p] = [f_u; f_p] from the initial guess in (u, p),
+ * then the pressure mean over unique nodes is removed
+ * (stokes_solver.hpp:676-689).  Returns TNG_OK also when the iteration limit
+ * is reached (see report->converged), as the reference logs and continues. */
+tng_status tng_stokes_solve(tng_stokes_solver solver, tng_vec u, tng_vec p, tng_vec f_u, tng_vec f_p,
+                            tng_solve_report *report);
 
 /* ---------------------------------------------------------------- timing */
 

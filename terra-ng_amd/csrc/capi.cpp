@@ -1,6 +1,7 @@
 // C ABI entry points (include/tng.h).  Every call converts library errors
 // into a tng_status and records the message on the context.
 #include <cstring>
+#include <memory>
 #include <new>
 
 #include "tng_internal.hpp"
@@ -122,20 +123,7 @@ tng_status tng_mesh_create_uniform(tng_ctx ctx, int lateral_level, int radial_le
 {
     if (!out) return TNG_INVALID_ARGUMENT;
     *out = nullptr;
-    auto *m = new (std::nothrow) tng_domain();
-    if (!m) return TNG_OUT_OF_MEMORY;
-    m->ctx = ctx;
-    tng_status st = guarded(ctx, [&] {
-        if (ctx) set_device(ctx);
-        mesh_build(*m, lateral_level, radial_level, r_min, r_max, lateral_sdr, radial_sdr, ctx != nullptr);
-        if (ctx) mesh_upload(*m);
-    });
-    if (st != TNG_OK) {
-        delete m;
-        return st;
-    }
-    *out = m;
-    return TNG_OK;
+    return guarded(ctx, [&] { *out = new_mesh(ctx, lateral_level, radial_level, r_min, r_max, lateral_sdr, radial_sdr); });
 }
 
 tng_status tng_mesh_destroy(tng_mesh mesh)
@@ -196,23 +184,25 @@ tng_status tng_vec_create(tng_mesh mesh, tng_dtype dtype, int ncomp, tng_vec *ou
         return TNG_INVALID_ARGUMENT;
     if (!mesh->ctx) return TNG_INVALID_ARGUMENT;  // host-only mesh
     *out = nullptr;
+    return guarded(mesh->ctx, [&] { *out = new_vector(mesh, dtype, ncomp); });
+}
+
+tng_status tng_vec_wrap(tng_mesh mesh, tng_dtype dtype, int ncomp, void *const *component_ptrs, tng_vec *out)
+{
+    if (!mesh || !out || !component_ptrs || (ncomp != 1 && ncomp != 3) || (dtype != TNG_F64 && dtype != TNG_F32))
+        return TNG_INVALID_ARGUMENT;
+    if (!mesh->ctx) return TNG_INVALID_ARGUMENT;
+    *out = nullptr;
+    for (int c = 0; c < ncomp; ++c) {
+        if (!component_ptrs[c]) return TNG_INVALID_ARGUMENT;
+        if (reinterpret_cast<uintptr_t>(component_ptrs[c]) % dtype_size(dtype)) return TNG_INVALID_ARGUMENT;
+    }
     auto *v = new (std::nothrow) tng_vector();
     if (!v) return TNG_OUT_OF_MEMORY;
-    tng_status st = guarded(mesh->ctx, [&] {
-        set_device(mesh->ctx);
-        v->mesh = mesh;
-        v->dtype = dtype;
-        v->ncomp = ncomp;
-        // components start on 256-byte boundaries
-        const int64_t align = 256 / (int64_t)dtype_size(dtype);
-        v->comp_stride = (mesh->nalloc + align - 1) / align * align;
-        v->buf.alloc((size_t)v->comp_stride * ncomp * dtype_size(dtype));
-        launch_fill(*v, 0.0, mesh->ctx->stream);
-    });
-    if (st != TNG_OK) {
-        delete v;
-        return st;
-    }
+    v->mesh = mesh;
+    v->dtype = dtype;
+    v->ncomp = ncomp;
+    for (int c = 0; c < ncomp; ++c) v->cp[c] = component_ptrs[c];
     *out = v;
     return TNG_OK;
 }
@@ -281,31 +271,7 @@ tng_status tng_epsdivdiv_create(tng_mesh mesh, tng_vec k, tng_bc bc_cmb, tng_bc 
 {
     if (!mesh || !k || !out || !mesh->ctx) return TNG_INVALID_ARGUMENT;
     *out = nullptr;
-    auto *op = new (std::nothrow) tng_operator();
-    if (!op) return TNG_OUT_OF_MEMORY;
-    tng_status st = guarded(mesh->ctx, [&] {
-        TNG_REQUIRE(k->mesh == mesh && k->ncomp == 1, "coefficient must be a scalar field on the operator's mesh");
-        TNG_REQUIRE(bc_cmb >= TNG_NEUMANN && bc_cmb <= TNG_FREESLIP && bc_surface >= TNG_NEUMANN &&
-                        bc_surface <= TNG_FREESLIP,
-                    "invalid boundary condition");
-        set_device(mesh->ctx);
-        op->kind = OP_EPSDIVDIV;
-        op->mesh = mesh;
-        op->k = k;
-        op->bc_cmb = bc_cmb;
-        op->bc_surf = bc_surface;
-        op->diagonal = diagonal != 0;
-        op->penalty_active = bc_cmb == TNG_FREESLIP && bc_surface == TNG_FREESLIP;
-        op->d_penalty_b.alloc(4 * sizeof(double));
-        TNG_HIP_CHECK(hipMemset(op->d_penalty_b.p, 0, 4 * sizeof(double)));
-        epsdivdiv_update_boundary_table(*op, mesh->ctx->stream);
-    });
-    if (st != TNG_OK) {
-        delete op;
-        return st;
-    }
-    *out = op;
-    return TNG_OK;
+    return guarded(mesh->ctx, [&] { *out = new_epsdivdiv(mesh, k, bc_cmb, bc_surface, diagonal != 0); });
 }
 
 tng_status tng_op_set_modes(tng_op op, tng_apply_mode apply_mode, tng_comm_mode comm_mode)
@@ -382,6 +348,88 @@ hipEvent_t *tng_context::prof_next_set()
 
 namespace tng {
 
+tng_domain *new_mesh(tng_context *ctx, int lat_level, int rad_level, double r_min, double r_max, int lat_sdr,
+                     int rad_sdr)
+{
+    std::unique_ptr<tng_domain> m(new tng_domain());
+    m->ctx = ctx;
+    if (ctx) set_device(ctx);
+    mesh_build(*m, lat_level, rad_level, r_min, r_max, lat_sdr, rad_sdr, ctx != nullptr);
+    if (ctx) mesh_upload(*m);
+    return m.release();
+}
+
+tng_vector *new_vector(tng_domain *mesh, tng_dtype dtype, int ncomp)
+{
+    TNG_REQUIRE(mesh && mesh->ctx, "vectors need a mesh with a device context");
+    TNG_REQUIRE((ncomp == 1 || ncomp == 3) && (dtype == TNG_F64 || dtype == TNG_F32), "invalid vector shape");
+    set_device(mesh->ctx);
+    std::unique_ptr<tng_vector> v(new tng_vector());
+    v->mesh = mesh;
+    v->dtype = dtype;
+    v->ncomp = ncomp;
+    // components start on 256-byte boundaries
+    const int64_t align = 256 / (int64_t)dtype_size(dtype);
+    const int64_t stride = (mesh->nalloc + align - 1) / align * align;
+    v->buf.alloc((size_t)stride * ncomp * dtype_size(dtype));
+    for (int c = 0; c < ncomp; ++c) v->cp[c] = static_cast<char *>(v->buf.p) + (size_t)c * stride * dtype_size(dtype);
+    launch_fill(*v, 0.0, mesh->ctx->stream);
+    return v.release();
+}
+
+static void require_bc(int bc_cmb, int bc_surf)
+{
+    TNG_REQUIRE(bc_cmb >= TNG_NEUMANN && bc_cmb <= TNG_FREESLIP && bc_surf >= TNG_NEUMANN && bc_surf <= TNG_FREESLIP,
+                "invalid boundary condition");
+}
+
+tng_operator *new_epsdivdiv(tng_domain *mesh, tng_vector *k, int bc_cmb, int bc_surf, bool diagonal)
+{
+    TNG_REQUIRE(k->mesh == mesh && k->ncomp == 1, "coefficient must be a scalar field on the operator's mesh");
+    require_bc(bc_cmb, bc_surf);
+    set_device(mesh->ctx);
+    std::unique_ptr<tng_operator> op(new tng_operator());
+    op->kind = OP_EPSDIVDIV;
+    op->mesh = mesh;
+    op->k = k;
+    op->bc_cmb = bc_cmb;
+    op->bc_surf = bc_surf;
+    op->diagonal = diagonal;
+    op->penalty_active = bc_cmb == TNG_FREESLIP && bc_surf == TNG_FREESLIP;
+    op->d_penalty_b.alloc(4 * sizeof(double));
+    TNG_HIP_CHECK(hipMemsetAsync(op->d_penalty_b.p, 0, 4 * sizeof(double), mesh->ctx->stream));
+    epsdivdiv_update_boundary_table(*op, mesh->ctx->stream);
+    return op.release();
+}
+
+tng_operator *new_two_level(OpKind kind, tng_domain *fine, tng_domain *coarse, int bc_cmb, int bc_surf)
+{
+    TNG_REQUIRE(fine->ctx && fine->ctx == coarse->ctx, "meshes must share one context");
+    TNG_REQUIRE(fine->nsd == coarse->nsd && fine->nx - 1 == 2 * (coarse->nx - 1) &&
+                    fine->nr - 1 == 2 * (coarse->nr - 1) && fine->lat_sdr == coarse->lat_sdr &&
+                    fine->rad_sdr == coarse->rad_sdr,
+                "fine and coarse meshes must be one uniform refinement apart with the same subdomains");
+    require_bc(bc_cmb, bc_surf);
+    std::unique_ptr<tng_operator> op(new tng_operator());
+    op->kind = kind;
+    op->mesh = fine;
+    op->mesh2 = coarse;
+    op->bc_cmb = bc_cmb;
+    op->bc_surf = bc_surf;
+    if (kind == OP_PROLONGATION) op->comm_mode = TNG_SKIP_COMMUNICATION;
+    return op.release();
+}
+
+tng_operator *new_kmass(tng_domain *mesh, tng_vector *k)
+{
+    TNG_REQUIRE(k->mesh == mesh && k->ncomp == 1, "coefficient must be a scalar field on the operator's mesh");
+    std::unique_ptr<tng_operator> op(new tng_operator());
+    op->kind = OP_KMASS_LUMPED;
+    op->mesh = mesh;
+    op->k = k;
+    return op.release();
+}
+
 void op_apply(tng_operator &op, const tng_vector &src, tng_vector &dst)
 {
     TNG_REQUIRE(&src != &dst, "apply: src and dst must be different vectors");
@@ -408,74 +456,39 @@ void op_apply(tng_operator &op, const tng_vector &src, tng_vector &dst)
 
 extern "C" {
 
-static tng_status make_two_level(tng_mesh fine, tng_mesh coarse, OpKind kind, tng_bc bc_cmb, tng_bc bc_surf,
-                                 tng_op *out)
-{
-    if (!fine || !coarse || !out || !fine->ctx || fine->ctx != coarse->ctx) return TNG_INVALID_ARGUMENT;
-    *out = nullptr;
-    auto *op = new (std::nothrow) tng_operator();
-    if (!op) return TNG_OUT_OF_MEMORY;
-    tng_status st = guarded(fine->ctx, [&] {
-        TNG_REQUIRE(fine->nsd == coarse->nsd && fine->nx - 1 == 2 * (coarse->nx - 1) &&
-                        fine->nr - 1 == 2 * (coarse->nr - 1) && fine->lat_sdr == coarse->lat_sdr &&
-                        fine->rad_sdr == coarse->rad_sdr,
-                    "fine and coarse meshes must be one uniform refinement apart with the same subdomains");
-        TNG_REQUIRE(bc_cmb >= TNG_NEUMANN && bc_cmb <= TNG_FREESLIP && bc_surf >= TNG_NEUMANN && bc_surf <= TNG_FREESLIP,
-                    "invalid boundary condition");
-        op->kind = kind;
-        op->mesh = fine;
-        op->mesh2 = coarse;
-        op->bc_cmb = bc_cmb;
-        op->bc_surf = bc_surf;
-    });
-    if (st != TNG_OK) {
-        delete op;
-        return st;
-    }
-    *out = op;
-    return TNG_OK;
-}
-
 tng_status tng_divergence_create(tng_mesh fine, tng_mesh coarse, tng_bc bc_cmb, tng_bc bc_surface, tng_op *out)
 {
-    return make_two_level(fine, coarse, OP_DIVERGENCE, bc_cmb, bc_surface, out);
+    if (!fine || !coarse || !out || !fine->ctx) return TNG_INVALID_ARGUMENT;
+    *out = nullptr;
+    return guarded(fine->ctx, [&] { *out = new_two_level(OP_DIVERGENCE, fine, coarse, bc_cmb, bc_surface); });
 }
 
 tng_status tng_gradient_create(tng_mesh fine, tng_mesh coarse, tng_bc bc_cmb, tng_bc bc_surface, tng_op *out)
 {
-    return make_two_level(fine, coarse, OP_GRADIENT, bc_cmb, bc_surface, out);
+    if (!fine || !coarse || !out || !fine->ctx) return TNG_INVALID_ARGUMENT;
+    *out = nullptr;
+    return guarded(fine->ctx, [&] { *out = new_two_level(OP_GRADIENT, fine, coarse, bc_cmb, bc_surface); });
 }
 
 tng_status tng_prolongation_create(tng_mesh coarse, tng_mesh fine, tng_op *out)
 {
-    tng_status st = make_two_level(fine, coarse, OP_PROLONGATION, TNG_NEUMANN, TNG_NEUMANN, out);
-    if (st == TNG_OK) (*out)->comm_mode = TNG_SKIP_COMMUNICATION;
-    return st;
+    if (!fine || !coarse || !out || !fine->ctx) return TNG_INVALID_ARGUMENT;
+    *out = nullptr;
+    return guarded(fine->ctx, [&] { *out = new_two_level(OP_PROLONGATION, fine, coarse, TNG_NEUMANN, TNG_NEUMANN); });
 }
 
 tng_status tng_restriction_create(tng_mesh fine, tng_mesh coarse, tng_op *out)
 {
-    return make_two_level(fine, coarse, OP_RESTRICTION, TNG_NEUMANN, TNG_NEUMANN, out);
+    if (!fine || !coarse || !out || !fine->ctx) return TNG_INVALID_ARGUMENT;
+    *out = nullptr;
+    return guarded(fine->ctx, [&] { *out = new_two_level(OP_RESTRICTION, fine, coarse, TNG_NEUMANN, TNG_NEUMANN); });
 }
 
 tng_status tng_kmass_lumped_create(tng_mesh mesh, tng_vec k, tng_op *out)
 {
     if (!mesh || !k || !out || !mesh->ctx) return TNG_INVALID_ARGUMENT;
     *out = nullptr;
-    auto *op = new (std::nothrow) tng_operator();
-    if (!op) return TNG_OUT_OF_MEMORY;
-    tng_status st = guarded(mesh->ctx, [&] {
-        TNG_REQUIRE(k->mesh == mesh && k->ncomp == 1, "coefficient must be a scalar field on the operator's mesh");
-        op->kind = OP_KMASS_LUMPED;
-        op->mesh = mesh;
-        op->k = k;
-    });
-    if (st != TNG_OK) {
-        delete op;
-        return st;
-    }
-    *out = op;
-    return TNG_OK;
+    return guarded(mesh->ctx, [&] { *out = new_kmass(mesh, k); });
 }
 
 tng_status tng_vec_lincomb(tng_vec y, int n, const double *c, const tng_vec *x, double c0)

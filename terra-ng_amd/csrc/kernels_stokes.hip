@@ -265,8 +265,8 @@ template <typename T> __global__ __launch_bounds__(256) void gradient_kernel(con
 // Constant prolongation (fe/wedge/shell/grid_transfer_constant.hpp:9-90):
 // P1 interpolation per fine node from the coarse nodes of its position.
 template <typename T>
-__global__ __launch_bounds__(256) void prolongate_kernel(const T *src, T *dst, int64_t sstride, int64_t dstride,
-                                                         int ncomp, int add, int nsd, int nxf, int nrf, int nxc, int nrc)
+__global__ __launch_bounds__(256) void prolongate_kernel(const Comp3<T> src, const Comp3<T> dst, int ncomp, int add,
+                                                         int nsd, int nxf, int nrf, int nxc, int nrc)
 {
     const int64_t n = (int64_t)nsd * nxf * nxf * nrf;
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -291,8 +291,8 @@ __global__ __launch_bounds__(256) void prolongate_kernel(const T *src, T *dst, i
         T s = T(0);
         for (int i = 0; i < np; ++i)
             for (int j = 0; j < nrp; ++j)
-                s += (pw * rw) * src[c * sstride + (((int64_t)sd * nxc + px[i]) * nxc + py[i]) * nrc + pr[j]];
-        T *o = dst + c * dstride + gid;
+                s += (pw * rw) * src.p[c][(((int64_t)sd * nxc + px[i]) * nxc + py[i]) * nrc + pr[j]];
+        T *o = dst.p[c] + gid;
         *o = add ? *o + s : s;
     }
 }
@@ -300,9 +300,9 @@ __global__ __launch_bounds__(256) void prolongate_kernel(const T *src, T *dst, i
 // Constant restriction (restriction_constant.hpp:133-250): owned-masked
 // transpose stencil; the additive exchange runs afterwards.
 template <typename T>
-__global__ __launch_bounds__(256) void restrict_kernel(const T *src, T *dst, int64_t sstride, int64_t dstride, int ncomp,
-                                                       int add, const uint8_t *omask_f, int nsd, int nxf, int nrf,
-                                                       int nxc, int nrc)
+__global__ __launch_bounds__(256) void restrict_kernel(const Comp3<T> src, const Comp3<T> dst, int ncomp, int add,
+                                                       const uint8_t *omask_f, int nsd, int nxf, int nrf, int nxc,
+                                                       int nrc)
 {
     const int64_t n = (int64_t)nsd * nxc * nxc * nrc;
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -324,11 +324,11 @@ __global__ __launch_bounds__(256) void restrict_kernel(const T *src, T *dst, int
             const int64_t fi = (((int64_t)sd * nxf + xf) * nxf + yf) * nrf + rf;
             if (omask_f[fi] != OWNED) continue;
             const T w = rw * (k == 6 ? T(1) : T(0.5));
-            for (int c = 0; c < ncomp; ++c) s[c] += w * src[c * sstride + fi];
+            for (int c = 0; c < ncomp; ++c) s[c] += w * src.p[c][fi];
         }
     }
     for (int c = 0; c < ncomp; ++c) {
-        T *o = dst + c * dstride + gid;
+        T *o = dst.p[c] + gid;
         *o = add ? *o + s[c] : s[c];
     }
 }
@@ -444,18 +444,17 @@ template <typename T> void grad_impl(tng_operator &op, const tng_vector &src, tn
 template <typename T> void prolong_impl(tng_operator &op, const tng_vector &src, tng_vector &dst, hipStream_t s)
 {
     const tng_domain &c = *op.mesh2, &f = *op.mesh;  // mesh = fine (dst), mesh2 = coarse (src)
-    hipLaunchKernelGGL(prolongate_kernel<T>, dim3(blocks_for(f.nalloc)), dim3(256), 0, s, (const T *)src.buf.p,
-                       (T *)dst.buf.p, src.comp_stride, dst.comp_stride, src.ncomp, op.apply_mode == TNG_ADD ? 1 : 0,
-                       f.nsd, f.nx, f.nr, c.nx, c.nr);
+    hipLaunchKernelGGL(prolongate_kernel<T>, dim3(blocks_for(f.nalloc)), dim3(256), 0, s, comps<T>(src),
+                       comps<T>(dst), src.ncomp, op.apply_mode == TNG_ADD ? 1 : 0, f.nsd, f.nx, f.nr, c.nx, c.nr);
     TNG_HIP_CHECK(hipGetLastError());
 }
 
 template <typename T> void restrict_impl(tng_operator &op, const tng_vector &src, tng_vector &dst, hipStream_t s)
 {
     const tng_domain &f = *op.mesh, &c = *op.mesh2;  // mesh = fine (src), mesh2 = coarse (dst)
-    hipLaunchKernelGGL(restrict_kernel<T>, dim3(blocks_for(c.nalloc)), dim3(256), 0, s, (const T *)src.buf.p,
-                       (T *)dst.buf.p, src.comp_stride, dst.comp_stride, src.ncomp, op.apply_mode == TNG_ADD ? 1 : 0,
-                       f.d_omask.as<uint8_t>(), f.nsd, f.nx, f.nr, c.nx, c.nr);
+    hipLaunchKernelGGL(restrict_kernel<T>, dim3(blocks_for(c.nalloc)), dim3(256), 0, s, comps<T>(src),
+                       comps<T>(dst), src.ncomp, op.apply_mode == TNG_ADD ? 1 : 0, f.d_omask.as<uint8_t>(), f.nsd,
+                       f.nx, f.nr, c.nx, c.nr);
     TNG_HIP_CHECK(hipGetLastError());
 }
 
@@ -496,6 +495,99 @@ void stokes_block_apply(tng_operator &op, const tng_vector &src, tng_vector &dst
     default:
         throw Error(TNG_INVALID_ARGUMENT, "stokes_block_apply: wrong operator kind");
     }
+}
+
+}  // namespace tng
+
+// ----------------------------------------------------------------------------
+// Injection of a node field onto the next coarser mesh (coarse node (x, y, r)
+// is fine node (2x, 2y, 2r) of the same subdomain) and the strong RHS
+// boundary enforcement of a velocity vector
+// (fe/strong_algebraic_dirichlet_enforcement.hpp:114-124,
+// fe/strong_algebraic_freeslip_enforcement.hpp:22-38).
+// ----------------------------------------------------------------------------
+namespace tng {
+
+namespace {
+
+template <typename T>
+__global__ __launch_bounds__(256) void inject_kernel(const T *f, T *c, int nsd, int nxf, int nrf, int nxc, int nrc)
+{
+    const int64_t n = (int64_t)nsd * nxc * nxc * nrc;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n) return;
+    const int r = (int)(gid % nrc);
+    int64_t t = gid / nrc;
+    const int y = (int)(t % nxc); t /= nxc;
+    const int x = (int)(t % nxc);
+    const int sd = (int)(t / nxc);
+    c[gid] = f[(((int64_t)sd * nxf + 2 * x) * nxf + 2 * y) * nrf + 2 * r];
+}
+
+// One thread per boundary-layer node: layer 0 of CMB subdomains and layer
+// nr-1 of surface subdomains.
+template <typename T>
+__global__ __launch_bounds__(256) void rhs_enforce_kernel(const Comp3<T> u, const T *coords, const uint8_t *sd_cmb,
+                                                          const uint8_t *sd_surf, int nsd, int nx, int nr, int zero_cmb,
+                                                          int zero_surf, int fs_cmb, int fs_surf)
+{
+    const int64_t n = (int64_t)nsd * nx * nx * 2;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n) return;
+    const int side = (int)(gid & 1);
+    const int64_t lcol = gid >> 1;
+    const int sd = (int)(lcol / ((int64_t)nx * nx));
+    if (side == 0 ? !sd_cmb[sd] : !sd_surf[sd]) return;
+    const bool zero = side == 0 ? zero_cmb : zero_surf;
+    const bool fs = side == 0 ? fs_cmb : fs_surf;
+    if (!zero && !fs) return;
+    const int64_t i = lcol * nr + (side == 0 ? 0 : nr - 1);
+    if (zero) {
+        u.p[0][i] = T(0); u.p[1][i] = T(0); u.p[2][i] = T(0);
+        return;
+    }
+    const T *c = coords + 3 * lcol;
+    const T inv = T(1) / sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    const T n0 = c[0] * inv, n1 = c[1] * inv, n2 = c[2] * inv;
+    const T d = n0 * u.p[0][i] + n1 * u.p[1][i] + n2 * u.p[2][i];
+    u.p[0][i] -= d * n0; u.p[1][i] -= d * n1; u.p[2][i] -= d * n2;
+}
+
+}  // namespace
+
+void launch_inject(tng_vector &coarse, const tng_vector &fine, hipStream_t s)
+{
+    const tng_domain &c = *coarse.mesh, &f = *fine.mesh;
+    TNG_REQUIRE(coarse.dtype == fine.dtype && coarse.ncomp == fine.ncomp, "inject: mismatched vectors");
+    TNG_REQUIRE(f.nsd == c.nsd && f.nx - 1 == 2 * (c.nx - 1) && f.nr - 1 == 2 * (c.nr - 1),
+                "inject: meshes must be one refinement apart");
+    for (int k = 0; k < coarse.ncomp; ++k) {
+        if (coarse.dtype == TNG_F64)
+            hipLaunchKernelGGL(inject_kernel<double>, dim3(blocks_for(c.nalloc)), dim3(256), 0, s,
+                               (const double *)fine.comp(k), (double *)coarse.comp(k), c.nsd, f.nx, f.nr, c.nx, c.nr);
+        else
+            hipLaunchKernelGGL(inject_kernel<float>, dim3(blocks_for(c.nalloc)), dim3(256), 0, s,
+                               (const float *)fine.comp(k), (float *)coarse.comp(k), c.nsd, f.nx, f.nr, c.nx, c.nr);
+        TNG_HIP_CHECK(hipGetLastError());
+    }
+}
+
+void launch_rhs_enforce(tng_vector &u, uint8_t dirichlet_flag, uint8_t freeslip_flag, hipStream_t s)
+{
+    TNG_REQUIRE(u.ncomp == 3, "boundary enforcement needs a velocity vector");
+    const tng_domain &m = *u.mesh;
+    const int zc = dirichlet_flag == SBF_CMB, zs = dirichlet_flag == SBF_SURFACE;
+    const int fc = freeslip_flag == SBF_CMB, fsf = freeslip_flag == SBF_SURFACE;
+    const int64_t n = (int64_t)m.nsd * m.nx * m.nx * 2;
+    if (u.dtype == TNG_F64)
+        hipLaunchKernelGGL(rhs_enforce_kernel<double>, dim3(blocks_for(n)), dim3(256), 0, s, comps<double>(u),
+                           m.d_coords.as<double>(), m.d_sd_cmb.as<uint8_t>(), m.d_sd_surf.as<uint8_t>(), m.nsd, m.nx,
+                           m.nr, zc, zs, fc, fsf);
+    else
+        hipLaunchKernelGGL(rhs_enforce_kernel<float>, dim3(blocks_for(n)), dim3(256), 0, s, comps<float>(u),
+                           m.d_coords_f.as<float>(), m.d_sd_cmb.as<uint8_t>(), m.d_sd_surf.as<uint8_t>(), m.nsd, m.nx,
+                           m.nr, zc, zs, fc, fsf);
+    TNG_HIP_CHECK(hipGetLastError());
 }
 
 }  // namespace tng

@@ -55,12 +55,15 @@ unsigned grid_for(int64_t n, int bs = 256)
 
 void launch_fill(tng_vector &v, double value, hipStream_t s)
 {
-    const int64_t n = v.comp_stride * v.ncomp;
-    if (v.dtype == TNG_F64)
-        hipLaunchKernelGGL(fill_kernel<double>, dim3(grid_for(n)), dim3(256), 0, s, v.buf.as<double>(), n, value);
-    else
-        hipLaunchKernelGGL(fill_kernel<float>, dim3(grid_for(n)), dim3(256), 0, s, v.buf.as<float>(), n, (float)value);
-    TNG_HIP_CHECK(hipGetLastError());
+    const int64_t n = v.mesh->nalloc;
+    for (int c = 0; c < v.ncomp; ++c) {
+        if (v.dtype == TNG_F64)
+            hipLaunchKernelGGL(fill_kernel<double>, dim3(grid_for(n)), dim3(256), 0, s, (double *)v.comp(c), n, value);
+        else
+            hipLaunchKernelGGL(fill_kernel<float>, dim3(grid_for(n)), dim3(256), 0, s, (float *)v.comp(c), n,
+                               (float)value);
+        TNG_HIP_CHECK(hipGetLastError());
+    }
 }
 
 void launch_copy(tng_vector &dst, const tng_vector &src, hipStream_t s)
@@ -114,32 +117,31 @@ namespace tng {
 namespace {
 
 template <typename T>
-__global__ __launch_bounds__(256) void lincomb_kernel(T *y, int64_t ystride, const T *x0, const T *x1, const T *x2,
-                                                      int64_t xstride, int nx, double c0, double c1, double c2,
+__global__ __launch_bounds__(256) void lincomb_kernel(const Comp3<T> y, const Comp3<T> x0, const Comp3<T> x1,
+                                                      const Comp3<T> x2, int nx, double c0, double c1, double c2,
                                                       double c3, int ncomp, int64_t n)
 {
     const T a0 = (T)c0, a1 = (T)c1, a2 = (T)c2, a3 = (T)c3;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         for (int c = 0; c < ncomp; ++c) {
             T v = a0;
-            if (nx > 0) v += a1 * x0[c * xstride + i];
-            if (nx > 1) v += a2 * x1[c * xstride + i];
-            if (nx > 2) v += a3 * x2[c * xstride + i];
-            y[c * ystride + i] = v;
+            if (nx > 0) v += a1 * x0.p[c][i];
+            if (nx > 1) v += a2 * x1.p[c][i];
+            if (nx > 2) v += a3 * x2.p[c][i];
+            y.p[c][i] = v;
         }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void mult_kernel(T *y, const T *x, int64_t ys, int64_t xs, int ncomp, int64_t n)
+template <typename T> __global__ __launch_bounds__(256) void mult_kernel(const Comp3<T> y, const Comp3<T> x, int ncomp, int64_t n)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        for (int c = 0; c < ncomp; ++c) y[c * ys + i] *= x[c * xs + i];
+        for (int c = 0; c < ncomp; ++c) y.p[c][i] *= x.p[c][i];
 }
 
-template <typename T> __global__ __launch_bounds__(256) void invert_kernel(T *y, int64_t ys, int ncomp, int64_t n)
+template <typename T> __global__ __launch_bounds__(256) void invert_kernel(const Comp3<T> y, int ncomp, int64_t n)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        for (int c = 0; c < ncomp; ++c) y[c * ys + i] = T(1) / y[c * ys + i];
+        for (int c = 0; c < ncomp; ++c) y.p[c][i] = T(1) / y.p[c][i];
 }
 
 __device__ __forceinline__ double wsum(double v)
@@ -158,7 +160,7 @@ enum RedOp { RED_DOT = 0, RED_MAXABS = 1, RED_NONFINITE = 2 };
 // op RED_DOT: sum over OWNED copies of sum_c x_c y_c (y may alias x)
 // op RED_MAXABS: max |x| over all copies; RED_NONFINITE: count of non-finite entries
 template <typename T, int OP>
-__global__ __launch_bounds__(256) void reduce_kernel(const T *x, const T *y, int64_t xs, int64_t ys, int ncomp,
+__global__ __launch_bounds__(256) void reduce_kernel(const Comp3<T> x, const Comp3<T> y, int ncomp,
                                                      const uint8_t *omask, int64_t n, double *partials,
                                                      unsigned int *counter, double *out)
 {
@@ -168,11 +170,11 @@ __global__ __launch_bounds__(256) void reduce_kernel(const T *x, const T *y, int
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         if (OP == RED_DOT) {
             if (omask[i] != OWNED) continue;
-            for (int c = 0; c < ncomp; ++c) acc += (double)x[c * xs + i] * (double)y[c * ys + i];
+            for (int c = 0; c < ncomp; ++c) acc += (double)x.p[c][i] * (double)y.p[c][i];
         } else if (OP == RED_MAXABS) {
-            for (int c = 0; c < ncomp; ++c) acc = fmax(acc, fabs((double)x[c * xs + i]));
+            for (int c = 0; c < ncomp; ++c) acc = fmax(acc, fabs((double)x.p[c][i]));
         } else {
-            for (int c = 0; c < ncomp; ++c) acc += isfinite((double)x[c * xs + i]) ? 0.0 : 1.0;
+            for (int c = 0; c < ncomp; ++c) acc += isfinite((double)x.p[c][i]) ? 0.0 : 1.0;
         }
     }
     acc = OP == RED_MAXABS ? wmax(acc) : wsum(acc);
@@ -227,16 +229,13 @@ void launch_lincomb(tng_vector &y, int nx, const double *c, const tng_vector *co
     TNG_REQUIRE(nx >= 0 && nx <= 3, "lincomb: at most 3 input vectors");
     const int64_t n = y.mesh->nalloc;
     const double cc[3] = {nx > 0 ? c[0] : 0, nx > 1 ? c[1] : 0, nx > 2 ? c[2] : 0};
+    const tng_vector &x0 = nx > 0 ? *x[0] : y, &x1 = nx > 1 ? *x[1] : y, &x2 = nx > 2 ? *x[2] : y;
     if (y.dtype == TNG_F64)
-        hipLaunchKernelGGL(lincomb_kernel<double>, dim3(gs(n)), dim3(256), 0, s, y.buf.as<double>(), y.comp_stride,
-                           nx > 0 ? x[0]->buf.as<double>() : nullptr, nx > 1 ? x[1]->buf.as<double>() : nullptr,
-                           nx > 2 ? x[2]->buf.as<double>() : nullptr, y.comp_stride, nx, c0, cc[0], cc[1], cc[2],
-                           y.ncomp, n);
+        hipLaunchKernelGGL(lincomb_kernel<double>, dim3(gs(n)), dim3(256), 0, s, comps<double>(y), comps<double>(x0),
+                           comps<double>(x1), comps<double>(x2), nx, c0, cc[0], cc[1], cc[2], y.ncomp, n);
     else
-        hipLaunchKernelGGL(lincomb_kernel<float>, dim3(gs(n)), dim3(256), 0, s, y.buf.as<float>(), y.comp_stride,
-                           nx > 0 ? x[0]->buf.as<float>() : nullptr, nx > 1 ? x[1]->buf.as<float>() : nullptr,
-                           nx > 2 ? x[2]->buf.as<float>() : nullptr, y.comp_stride, nx, c0, cc[0], cc[1], cc[2],
-                           y.ncomp, n);
+        hipLaunchKernelGGL(lincomb_kernel<float>, dim3(gs(n)), dim3(256), 0, s, comps<float>(y), comps<float>(x0),
+                           comps<float>(x1), comps<float>(x2), nx, c0, cc[0], cc[1], cc[2], y.ncomp, n);
     TNG_HIP_CHECK(hipGetLastError());
 }
 
@@ -245,11 +244,11 @@ void launch_mult(tng_vector &y, const tng_vector &x, hipStream_t s)
     TNG_REQUIRE(x.mesh == y.mesh && x.dtype == y.dtype && (x.ncomp == y.ncomp), "scale: mismatched vectors");
     const int64_t n = y.mesh->nalloc;
     if (y.dtype == TNG_F64)
-        hipLaunchKernelGGL(mult_kernel<double>, dim3(gs(n)), dim3(256), 0, s, y.buf.as<double>(), x.buf.as<double>(),
-                           y.comp_stride, x.comp_stride, y.ncomp, n);
+        hipLaunchKernelGGL(mult_kernel<double>, dim3(gs(n)), dim3(256), 0, s, comps<double>(y), comps<double>(x),
+                           y.ncomp, n);
     else
-        hipLaunchKernelGGL(mult_kernel<float>, dim3(gs(n)), dim3(256), 0, s, y.buf.as<float>(), x.buf.as<float>(),
-                           y.comp_stride, x.comp_stride, y.ncomp, n);
+        hipLaunchKernelGGL(mult_kernel<float>, dim3(gs(n)), dim3(256), 0, s, comps<float>(y), comps<float>(x), y.ncomp,
+                           n);
     TNG_HIP_CHECK(hipGetLastError());
 }
 
@@ -257,11 +256,9 @@ void launch_invert(tng_vector &y, hipStream_t s)
 {
     const int64_t n = y.mesh->nalloc;
     if (y.dtype == TNG_F64)
-        hipLaunchKernelGGL(invert_kernel<double>, dim3(gs(n)), dim3(256), 0, s, y.buf.as<double>(), y.comp_stride,
-                           y.ncomp, n);
+        hipLaunchKernelGGL(invert_kernel<double>, dim3(gs(n)), dim3(256), 0, s, comps<double>(y), y.ncomp, n);
     else
-        hipLaunchKernelGGL(invert_kernel<float>, dim3(gs(n)), dim3(256), 0, s, y.buf.as<float>(), y.comp_stride,
-                           y.ncomp, n);
+        hipLaunchKernelGGL(invert_kernel<float>, dim3(gs(n)), dim3(256), 0, s, comps<float>(y), y.ncomp, n);
     TNG_HIP_CHECK(hipGetLastError());
 }
 
@@ -276,11 +273,10 @@ void launch_reduce(int op, const tng_vector &x, const tng_vector *y, int slot, h
     const uint8_t *om = x.mesh->d_omask.as<uint8_t>();
     double *part = ctx.red_partials.as<double>();
     unsigned int *cnt = ctx.red_counter.as<unsigned int>();
-    const void *yp = y ? y->buf.p : x.buf.p;
-    const int64_t ys = y ? y->comp_stride : x.comp_stride;
+    const tng_vector &yy = y ? *y : x;
 #define TNG_RED(T, OPV)                                                                                            \
-    hipLaunchKernelGGL((reduce_kernel<T, OPV>), dim3(g), dim3(256), 0, s, (const T *)x.buf.p, (const T *)yp,      \
-                       x.comp_stride, ys, x.ncomp, om, n, part, cnt, out)
+    hipLaunchKernelGGL((reduce_kernel<T, OPV>), dim3(g), dim3(256), 0, s, comps<T>(x), comps<T>(yy), x.ncomp, om, n, \
+                       part, cnt, out)
     if (x.dtype == TNG_F64) {
         if (op == RED_DOT) TNG_RED(double, RED_DOT);
         else if (op == RED_MAXABS) TNG_RED(double, RED_MAXABS);
@@ -302,6 +298,47 @@ double reduce_to_host(int op, const tng_vector &x, const tng_vector *y, hipStrea
     TNG_HIP_CHECK(hipMemcpyAsync(&v, ctx.red_result.p, sizeof(double), hipMemcpyDeviceToHost, s));
     TNG_HIP_CHECK(hipStreamSynchronize(s));
     return v;
+}
+
+}  // namespace tng
+
+// ----------------------------------------------------------------------------
+// Seeded uniform [0, 1) fill (the power iteration's random start; the
+// reference draws from a Kokkos XorShift64 pool, grid_operations.hpp:748-773).
+// Every entry is SplitMix64 of (seed, component, linear index), so the fill is
+// reproducible and independent of the launch shape.
+// ----------------------------------------------------------------------------
+namespace tng {
+
+namespace {
+
+__device__ __forceinline__ double splitmix_unit(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * 0x1.0p-53;
+}
+
+template <typename T> __global__ __launch_bounds__(256) void randomize_kernel(T *p, int64_t n, uint64_t base)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = (T)splitmix_unit(base + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull);
+}
+
+}  // namespace
+
+void launch_randomize(tng_vector &v, uint64_t seed, hipStream_t s)
+{
+    const int64_t n = v.mesh->nalloc;
+    for (int c = 0; c < v.ncomp; ++c) {
+        const uint64_t base = seed * 0xD1B54A32D192ED03ull + (uint64_t)c * 0x8CB92BA72F3D8DD7ull;
+        if (v.dtype == TNG_F64)
+            hipLaunchKernelGGL(randomize_kernel<double>, dim3(grid_for(n)), dim3(256), 0, s, (double *)v.comp(c), n, base);
+        else
+            hipLaunchKernelGGL(randomize_kernel<float>, dim3(grid_for(n)), dim3(256), 0, s, (float *)v.comp(c), n, base);
+        TNG_HIP_CHECK(hipGetLastError());
+    }
 }
 
 }  // namespace tng

@@ -1,0 +1,777 @@
+// Solver layer of the Stokes path, host C++ driving the library's kernels on
+// the context stream:
+//   - Chebyshev-accelerated Jacobi smoother + power iteration
+//     (linalg/solvers/chebyshev.hpp:51-185, power_iteration.hpp:13-58)
+//   - PCG with the identity preconditioner (linalg/solvers/pcg.hpp:28-188)
+//   - geometric multigrid V-cycle (linalg/solvers/multigrid.hpp:216-432)
+//   - the Stokes block operator K = [A B^T; B 0]
+//     (fe/wedge/operators/shell/epsilon_divdiv_stokes.hpp:55-84)
+//   - the block-triangular preconditioner
+//     (linalg/solvers/block_preconditioner_2x2.hpp:205-221)
+//   - FGMRES(m) with modified Gram-Schmidt and Givens rotations
+//     (linalg/solvers/fgmres.hpp:109-373)
+//   - the solver set-up of the mantle-circulation app
+//     (apps/mantlecirculation/src/stokes_solver.hpp:143-691).
+// The multigrid levels can run in FP32 (vectors, operators, smoothers and
+// transfers) while FGMRES, the Stokes operator, the Schur scaling and the
+// coarsest-level PCG stay in FP64.
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <vector>
+
+#include "tng_internal.hpp"
+#include "tng_solvers.hpp"
+
+namespace tng {
+
+namespace {
+
+using VecPtr = std::unique_ptr<tng_vector>;
+using OpPtr = std::unique_ptr<tng_operator>;
+using MeshPtr = std::unique_ptr<tng_domain>;
+
+hipStream_t stream_of(const tng_vector &v) { return v.mesh->ctx->stream; }
+
+// -- vector helpers (linalg/vector.hpp) --------------------------------------
+void lincomb(tng_vector &y, std::initializer_list<double> c, std::initializer_list<const tng_vector *> x,
+             double c0 = 0.0)
+{
+    const tng_vector *xs[3] = {nullptr, nullptr, nullptr};
+    double cs[3] = {0, 0, 0};
+    int n = 0;
+    for (const tng_vector *v : x) xs[n++] = v;
+    n = 0;
+    for (double v : c) cs[n++] = v;
+    launch_lincomb(y, (int)x.size(), cs, xs, c0, stream_of(y));
+}
+void assign(tng_vector &y, double c) { launch_fill(y, c, stream_of(y)); }
+void assign(tng_vector &y, const tng_vector &x) { launch_copy(y, x, stream_of(y)); }
+void scale_in_place(tng_vector &y, const tng_vector &x) { launch_mult(y, x, stream_of(y)); }
+double dot(const tng_vector &x, const tng_vector &y) { return reduce_to_host(REDUCE_DOT, x, &y, stream_of(x)); }
+double norm_2(const tng_vector &x) { return std::sqrt(dot(x, x)); }
+bool has_nan_or_inf(const tng_vector &x) { return reduce_to_host(REDUCE_NONFINITE, x, nullptr, stream_of(x)) > 0; }
+void apply(tng_operator &op, const tng_vector &src, tng_vector &dst) { op_apply(op, src, dst); }
+
+tng_vector *vec(std::vector<VecPtr> &pool, tng_domain *m, tng_dtype dt, int ncomp)
+{
+    pool.emplace_back(new_vector(m, dt, ncomp));
+    return pool.back().get();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// power_iteration (power_iteration.hpp:13-58) on D^{-1} A
+// ---------------------------------------------------------------------------
+double power_iteration(tng_operator &A, const tng_vector &inv_diag, tng_vector &it, tng_vector &aux, int iterations,
+                       uint64_t seed)
+{
+    launch_randomize(it, seed, stream_of(it));
+    double norm = norm_2(it);
+    if (norm < std::numeric_limits<double>::min()) return 1.0;
+    lincomb(it, {1.0 / norm}, {&it});
+    apply(A, it, aux);
+    scale_in_place(aux, inv_diag);
+    double radius = 0.0;
+    for (int i = 0; i < iterations; ++i) {
+        norm = norm_2(aux);
+        if (norm < std::numeric_limits<double>::min()) return radius > 0.0 ? radius : 1.0;
+        lincomb(it, {1.0 / norm}, {&aux});
+        apply(A, it, aux);
+        scale_in_place(aux, inv_diag);
+        radius = dot(it, aux);
+    }
+    return radius;
+}
+
+// ---------------------------------------------------------------------------
+// Chebyshev smoother (chebyshev.hpp:102-165)
+// ---------------------------------------------------------------------------
+void Chebyshev::solve(tng_operator &A, tng_vector &x, const tng_vector &b)
+{
+    if (need_estimate) {
+        lmax_estimate = power_iteration(A, *inv_diag, *d, *z, power_iterations, seed);
+        need_estimate = false;
+    }
+    if (!std::isfinite(lmax_estimate) || lmax_estimate <= 0.0) lmax_estimate = 1.0;
+    const double lambda_max = 1.5 * lmax_estimate, lambda_min = 0.1 * lmax_estimate;
+    const double theta = 0.5 * (lambda_max + lambda_min), delta = 0.5 * (lambda_max - lambda_min);
+    const double sigma = theta / delta;
+    double rho = 1.0 / sigma;  // carried across iterations, as in the reference
+    for (int iter = 0; iter < iterations; ++iter) {
+        apply(A, x, *z);
+        lincomb(*z, {1.0, -1.0}, {&b, z});
+        scale_in_place(*z, *inv_diag);
+        lincomb(*d, {1.0 / theta}, {z});
+        lincomb(x, {1.0, 1.0}, {&x, d});
+        for (int i = 2; i <= order; ++i) {
+            const double rho_new = 1.0 / (2.0 * sigma - rho);
+            const double beta = rho_new * rho, alpha = 2.0 * rho_new / delta;
+            apply(A, x, *z);
+            lincomb(*z, {1.0, -1.0}, {&b, z});
+            scale_in_place(*z, *inv_diag);
+            lincomb(*d, {alpha, beta}, {z, d});
+            lincomb(x, {1.0, 1.0}, {&x, d});
+            rho = rho_new;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PCG, identity preconditioner (pcg.hpp:81-170)
+// ---------------------------------------------------------------------------
+int pcg_solve(tng_operator &A, tng_vector &x, const tng_vector &b, tng_vector &r, tng_vector &p, tng_vector &ap,
+              tng_vector &z, int max_iterations, double rtol, double atol, double *final_rel)
+{
+    apply(A, x, r);
+    lincomb(r, {1.0, -1.0}, {&b, &r});
+    assign(z, r);  // identity preconditioner
+    assign(p, z);
+    const double initial = std::sqrt(dot(r, r));
+    if (final_rel) *final_rel = 1.0;
+    if (initial < atol) return 0;
+    for (int it = 1; it <= max_iterations; ++it) {
+        const double alpha_num = dot(z, r);
+        apply(A, p, ap);
+        const double alpha_den = dot(ap, p);
+        if (std::abs(alpha_den) < std::numeric_limits<double>::min()) return it;
+        const double alpha = alpha_num / alpha_den;
+        lincomb(x, {1.0, alpha}, {&x, &p});
+        lincomb(r, {1.0, -alpha}, {&r, &ap});
+        const double abs_res = std::sqrt(dot(r, r));
+        const double rel = abs_res / initial;
+        if (final_rel) *final_rel = rel;
+        if (rel <= rtol || abs_res < atol) return it;
+        assign(z, r);
+        const double beta_num = dot(z, r);
+        if (std::abs(alpha_num) < std::numeric_limits<double>::min()) return it;
+        lincomb(p, {1.0, beta_num / alpha_num}, {&z, &p});
+    }
+    return max_iterations;
+}
+
+// ---------------------------------------------------------------------------
+// Multigrid V-cycle (multigrid.hpp:216-432)
+// ---------------------------------------------------------------------------
+void Multigrid::coarse_solve(tng_vector &e0, const tng_vector &r0)
+{
+    // The coarsest solve runs in FP64 whatever the precision of the levels.
+    tng_vector &x = e0.dtype == TNG_F64 ? e0 : *cg_x;
+    const tng_vector *b = &r0;
+    if (e0.dtype != TNG_F64) {
+        assign(*cg_x, e0);
+        assign(*cg_b, r0);
+        b = cg_b;
+    }
+    coarse_iterations += pcg_solve(*coarse_op, x, *b, *cg_tmp[0], *cg_tmp[1], *cg_tmp[2], *cg_tmp[3], coarse_max_it,
+                                   coarse_rtol, coarse_atol, nullptr);
+    if (e0.dtype != TNG_F64) assign(e0, *cg_x);
+}
+
+void Multigrid::recurse(int level, tng_vector &x, const tng_vector &b)
+{
+    if (level == 0) {
+        coarse_solve(x, b);
+        return;
+    }
+    Level &L = levels[level];
+    pre[level].solve(*L.A, x, b);
+    apply(*L.A, x, *L.tmp);
+    lincomb(*L.tmp, {1.0, -1.0}, {&b, L.tmp});
+    Level &C = levels[level - 1];
+    apply(*L.R, *L.tmp, *C.r);
+    assign(*C.e, 0.0);
+    recurse(level - 1, *C.e, *C.r);
+    apply(*L.P, *C.e, x);  // Add mode
+    post[level].solve(*L.A, x, b);
+}
+
+void Multigrid::solve(tng_vector &x, const tng_vector &b)
+{
+    for (int c = 0; c < num_cycles; ++c) recurse((int)levels.size() - 1, x, b);
+}
+
+// ---------------------------------------------------------------------------
+// Stokes solver (stokes_solver.hpp:143-691)
+// ---------------------------------------------------------------------------
+StokesSolver::StokesSolver(tng_context *ctx, int min_level, int max_level, int radial_extra, double r_min,
+                           double r_max, int lat_sdr, int rad_sdr, int bc_cmb_, int bc_surf_,
+                           const tng_stokes_params &prm)
+    : params(prm), bc_cmb(bc_cmb_), bc_surf(bc_surf_)
+{
+    TNG_REQUIRE(min_level >= 1 && max_level >= min_level + 1, "need at least two levels (velocity and pressure)");
+    TNG_REQUIRE(prm.krylov_restart >= 1 && prm.krylov_max_iterations >= 0, "invalid Krylov parameters");
+    TNG_REQUIRE(prm.viscous_pc_chebyshev_order >= 1, "Chebyshev order must be at least 1");
+    TNG_REQUIRE(prm.viscous_pc_dtype == TNG_F64 || prm.viscous_pc_dtype == TNG_F32, "invalid multigrid precision");
+    TNG_REQUIRE(prm.gca == 0, "only re-discretised coarse operators (gca = 0) are available");
+    for (int l = min_level; l <= max_level; ++l)
+        meshes.emplace_back(new_mesh(ctx, l, l + radial_extra, r_min, r_max, lat_sdr, rad_sdr));
+    const int nl = (int)meshes.size();
+    vel = meshes[nl - 1].get();
+    pre_mesh = meshes[nl - 2].get();
+    const tng_dtype mgt = (tng_dtype)prm.viscous_pc_dtype;
+
+    // viscosity on every level (FP64), plus the multigrid-precision copies
+    for (int l = 0; l < nl; ++l) eta.push_back(vec(pool, meshes[l].get(), TNG_F64, 1));
+    for (int l = 0; l < nl; ++l) eta_mg.push_back(mgt == TNG_F64 ? eta[l] : vec(pool, meshes[l].get(), mgt, 1));
+    for (int l = 0; l < nl; ++l) assign(*eta[l], 1.0);
+
+    // Stokes operator blocks (FP64)
+    A.reset(new_epsdivdiv(vel, eta[nl - 1], bc_cmb, bc_surf, false));
+    Bt.reset(new_two_level(OP_GRADIENT, vel, pre_mesh, bc_cmb, bc_surf));
+    B.reset(new_two_level(OP_DIVERGENCE, vel, pre_mesh, bc_cmb, bc_surf));
+
+    // multigrid hierarchy in the multigrid precision
+    mg.num_cycles = prm.viscous_pc_num_vcycles;
+    mg.levels.resize(nl);
+    for (int l = 0; l < nl; ++l) {
+        Multigrid::Level &L = mg.levels[l];
+        tng_domain *m = meshes[l].get();
+        if (l == nl - 1 && mgt == TNG_F64) {
+            L.A = A.get();
+        } else {
+            mg_ops.emplace_back(new_epsdivdiv(m, eta_mg[l], bc_cmb, bc_surf, false));
+            L.A = mg_ops.back().get();
+        }
+        L.tmp = vec(pool, m, mgt, 3);
+        L.inv_diag = vec(pool, m, mgt, 3);
+        if (l < nl - 1) {
+            L.r = vec(pool, m, mgt, 3);
+            L.e = vec(pool, m, mgt, 3);
+        }
+        if (l > 0) {
+            mg_ops.emplace_back(new_two_level(OP_PROLONGATION, m, meshes[l - 1].get(), TNG_NEUMANN, TNG_NEUMANN));
+            L.P = mg_ops.back().get();
+            L.P->apply_mode = TNG_ADD;
+            mg_ops.emplace_back(new_two_level(OP_RESTRICTION, m, meshes[l - 1].get(), TNG_NEUMANN, TNG_NEUMANN));
+            L.R = mg_ops.back().get();
+        }
+    }
+    mg.pre.resize(nl);
+    mg.post.resize(nl);
+    for (int l = 0; l < nl; ++l)
+        for (Chebyshev *s : {&mg.pre[l], &mg.post[l]}) {
+            s->order = prm.viscous_pc_chebyshev_order;
+            s->iterations = prm.viscous_pc_num_smoothing_steps_prepost;
+            s->power_iterations = prm.viscous_pc_num_power_iterations;
+            s->inv_diag = mg.levels[l].inv_diag;
+            s->d = vec(pool, meshes[l].get(), mgt, 3);
+            s->z = vec(pool, meshes[l].get(), mgt, 3);
+            s->seed = prm.seed + 1000 * (uint64_t)l + (s == &mg.post[l] ? 1 : 0);
+        }
+    // FP64 coarsest solve
+    mg.coarse_max_it = prm.coarse_max_iterations;
+    mg.coarse_rtol = prm.coarse_relative_tolerance;
+    mg.coarse_atol = prm.coarse_absolute_tolerance;
+    if (mgt == TNG_F64) {
+        mg.coarse_op = mg.levels[0].A;
+    } else {
+        coarse_op64.reset(new_epsdivdiv(meshes[0].get(), eta[0], bc_cmb, bc_surf, false));
+        mg.coarse_op = coarse_op64.get();
+        mg.cg_x = vec(pool, meshes[0].get(), TNG_F64, 3);
+        mg.cg_b = vec(pool, meshes[0].get(), TNG_F64, 3);
+    }
+    for (int i = 0; i < 4; ++i) mg.cg_tmp[i] = vec(pool, meshes[0].get(), TNG_F64, 3);
+
+    // Schur complement approximation: lumped (1/eta)-weighted pressure mass
+    k_pm = vec(pool, pre_mesh, TNG_F64, 1);
+    pmass.reset(new_kmass(pre_mesh, k_pm));
+    inv_lumped_pmass = vec(pool, pre_mesh, TNG_F64, 1);
+    ones_p = vec(pool, pre_mesh, TNG_F64, 1);
+    assign(*ones_p, 1.0);
+
+    // block-triangular preconditioner and FGMRES work space (FP64)
+    prec_tmp_u = vec(pool, vel, TNG_F64, 3);
+    if (mgt != TNG_F64) {
+        mg_x = vec(pool, vel, mgt, 3);
+        mg_b = vec(pool, vel, mgt, 3);
+    }
+    const int nt = 2 * prm.krylov_restart + 4;
+    for (int i = 0; i < nt; ++i) fg.push_back({vec(pool, vel, TNG_F64, 3), vec(pool, pre_mesh, TNG_F64, 1)});
+    ones_v = nullptr;
+    num_dofs_pressure = pre_mesh->num_owned;
+    viscosity_updated();
+}
+
+void StokesSolver::set_viscosity(const tng_vector &eta_fine)
+{
+    TNG_REQUIRE(eta_fine.mesh == vel && eta_fine.ncomp == 1, "viscosity must be a scalar field on the velocity mesh");
+    const int nl = (int)meshes.size();
+    assign(*eta[nl - 1], eta_fine);
+    // coarse levels: injection of the fine viscosity (nested nodes)
+    for (int l = nl - 2; l >= 0; --l) launch_inject(*eta[l], *eta[l + 1], stream_of(*eta[l]));
+    viscosity_updated();
+}
+
+void StokesSolver::viscosity_updated()
+{
+    const int nl = (int)meshes.size();
+    for (int l = 0; l < nl; ++l)
+        if (eta_mg[l] != eta[l]) assign(*eta_mg[l], *eta[l]);
+    A->btab_valid = false;
+    if (coarse_op64) coarse_op64->btab_valid = false;
+    for (auto &op : mg_ops)
+        if (op->kind == OP_EPSDIVDIV) op->btab_valid = false;
+    // inverse diagonals (stokes_solver.hpp:390-417)
+    for (int l = 0; l < nl; ++l) {
+        Multigrid::Level &L = mg.levels[l];
+        tng_vector &ones = *L.tmp;
+        assign(ones, 1.0);
+        L.A->diagonal = true;
+        apply(*L.A, ones, *L.inv_diag);
+        L.A->diagonal = false;
+        launch_invert(*L.inv_diag, stream_of(*L.inv_diag));
+    }
+    for (int l = 0; l < nl; ++l) {
+        mg.pre[l].need_estimate = true;
+        mg.post[l].need_estimate = true;
+    }
+    // k_pm = 1 / eta at the pressure level; lumped diagonal, inverted (DiagonalSolver)
+    assign(*k_pm, *eta[nl - 2]);
+    launch_invert(*k_pm, stream_of(*k_pm));
+    apply(*pmass, *ones_p, *inv_lumped_pmass);
+    launch_invert(*inv_lumped_pmass, stream_of(*inv_lumped_pmass));
+}
+
+// K = [A B^T; B 0]: A (Replace, Skip) -> B^T (Add, Communicate) -> B (Replace, Communicate)
+void StokesSolver::apply_stokes(const SVec &src, SVec &dst)
+{
+    A->apply_mode = TNG_REPLACE;
+    A->comm_mode = TNG_SKIP_COMMUNICATION;
+    Bt->apply_mode = TNG_ADD;
+    Bt->comm_mode = TNG_COMMUNICATE_ADDITIVELY;
+    B->apply_mode = TNG_REPLACE;
+    B->comm_mode = TNG_COMMUNICATE_ADDITIVELY;
+    apply(*A, *src.u, *dst.u);
+    apply(*Bt, *src.p, *dst.u);
+    apply(*B, *src.u, *dst.p);
+    A->comm_mode = TNG_COMMUNICATE_ADDITIVELY;
+    Bt->apply_mode = TNG_REPLACE;
+}
+
+// velocity block preconditioner: V-cycle(s) on A x = b, x as given
+void StokesSolver::precondition_velocity(tng_vector &x, const tng_vector &b)
+{
+    if (!mg_x) {
+        mg.solve(x, b);
+        return;
+    }
+    assign(*mg_x, x);
+    assign(*mg_b, b);
+    mg.solve(*mg_x, *mg_b);
+    assign(x, *mg_x);
+}
+
+// block-triangular preconditioner (block_preconditioner_2x2.hpp:205-221)
+void StokesSolver::precondition(SVec &x, const SVec &b)
+{
+    assign(*x.p, *b.p);
+    scale_in_place(*x.p, *inv_lumped_pmass);
+    apply(*Bt, *x.p, *prec_tmp_u);  // Replace, Communicate
+    lincomb(*prec_tmp_u, {1.0, 1.0}, {prec_tmp_u, b.u});
+    lincomb(*x.p, {-1.0}, {x.p});
+    precondition_velocity(*x.u, *prec_tmp_u);
+}
+
+namespace {
+void s_lincomb(StokesSolver::SVec &y, std::initializer_list<double> c,
+               std::initializer_list<const StokesSolver::SVec *> x, double c0 = 0.0)
+{
+    const tng_vector *xu[3], *xp[3];
+    double cs[3] = {0, 0, 0};
+    int n = 0;
+    for (const StokesSolver::SVec *v : x) {
+        xu[n] = v->u;
+        xp[n] = v->p;
+        ++n;
+    }
+    n = 0;
+    for (double v : c) cs[n++] = v;
+    launch_lincomb(*y.u, (int)x.size(), cs, xu, c0, stream_of(*y.u));
+    launch_lincomb(*y.p, (int)x.size(), cs, xp, c0, stream_of(*y.p));
+}
+double s_dot(const StokesSolver::SVec &a, const StokesSolver::SVec &b) { return dot(*a.u, *b.u) + dot(*a.p, *b.p); }
+void s_assign(StokesSolver::SVec &y, double c)
+{
+    assign(*y.u, c);
+    assign(*y.p, c);
+}
+void s_assign(StokesSolver::SVec &y, const StokesSolver::SVec &x)
+{
+    assign(*y.u, *x.u);
+    assign(*y.p, *x.p);
+}
+bool s_nonfinite(const StokesSolver::SVec &x) { return has_nan_or_inf(*x.u) || has_nan_or_inf(*x.p); }
+}  // namespace
+
+// FGMRES(m) (fgmres.hpp:109-373)
+void StokesSolver::fgmres(SVec &x, const SVec &b, tng_solve_report &rep)
+{
+    const int m = params.krylov_restart;
+    const double rtol = params.krylov_relative_tolerance, atol = params.krylov_absolute_tolerance;
+    SVec &r = fg[0];
+    apply_stokes(x, r);
+    s_lincomb(r, {1.0, -1.0}, {&b, &r});
+    double beta0 = std::sqrt(s_dot(r, r));
+    const double initial = beta0;
+    rep.initial_residual = initial;
+    rep.final_residual = beta0;
+    rep.relative_residual = 1.0;
+    record(rep, beta0);
+    if (beta0 <= atol) {
+        rep.converged = 1;
+        return;
+    }
+    const int offV = 1, offZ = offV + m + 1, idxW = offZ + m, idxAcc = idxW + 1;
+    std::vector<double> H((size_t)(m + 1) * m), g(m + 1), cs(m + 1), sn(m + 1), y;
+    auto h = [&](int i, int j) -> double & { return H[(size_t)i * m + j]; };
+    int total = 0;
+    while (total < params.krylov_max_iterations) {
+        SVec &V0 = fg[offV];
+        s_lincomb(V0, {1.0 / beta0}, {&r});
+        std::fill(H.begin(), H.end(), 0.0);
+        std::fill(g.begin(), g.end(), 0.0);
+        std::fill(cs.begin(), cs.end(), 0.0);
+        std::fill(sn.begin(), sn.end(), 0.0);
+        g[0] = beta0;
+        int inner = 0;
+        for (int j = 0; j < m && total < params.krylov_max_iterations; ++j, ++total) {
+            SVec &vj = fg[offV + j], &zj = fg[offZ + j];
+            rep.iterations += 1;  // preconditioned Arnoldi steps
+            s_assign(zj, 0.0);
+            precondition(zj, vj);
+            if (s_nonfinite(zj)) {
+                rep.nonfinite_preconditioner += 1;
+                s_assign(zj, vj);  // skip the preconditioner in this iteration
+            }
+            SVec &w = fg[idxW];
+            apply_stokes(zj, w);
+            for (int i = 0; i <= j; ++i) {
+                SVec &vi = fg[offV + i];
+                const double hij = s_dot(w, vi);
+                h(i, j) = hij;
+                s_lincomb(w, {1.0, -hij}, {&w, &vi});
+            }
+            const double hj1 = std::sqrt(s_dot(w, w));
+            if (hj1 < std::numeric_limits<double>::epsilon() * initial) {
+                h(j + 1, j) = 0;
+                inner = j + 1;
+                rep.breakdowns += 1;
+                break;
+            }
+            h(j + 1, j) = hj1;
+            if (hj1 > 0) s_lincomb(fg[offV + j + 1], {1.0 / hj1}, {&w});
+            for (int i = 0; i < j; ++i) {
+                const double t = cs[i] * h(i, j) + sn[i] * h(i + 1, j);
+                h(i + 1, j) = -sn[i] * h(i, j) + cs[i] * h(i + 1, j);
+                h(i, j) = t;
+            }
+            {
+                const double a = h(j, j), bb = h(j + 1, j);
+                const double rab = std::hypot(a, bb);
+                cs[j] = rab == 0.0 ? 1.0 : a / rab;
+                sn[j] = rab == 0.0 ? 0.0 : bb / rab;
+                h(j, j) = rab;
+                h(j + 1, j) = 0;
+                if (std::abs(h(j, j)) < std::numeric_limits<double>::epsilon()) {
+                    inner = j;
+                    rep.breakdowns += 1;
+                    break;
+                }
+                const double gj = g[j], gj1 = g[j + 1];
+                g[j] = cs[j] * gj + sn[j] * gj1;
+                g[j + 1] = -sn[j] * gj + cs[j] * gj1;
+            }
+            const double abs_res = std::abs(g[j + 1]);
+            const double rel = abs_res / initial;
+            record(rep, abs_res);
+            if (!std::isfinite(abs_res)) {
+                inner = j;
+                ++total;
+                rep.nonfinite_residual += 1;
+                break;
+            }
+            inner = j + 1;
+            if (rel <= rtol || abs_res < atol) break;
+        }
+        // back substitution and update x += sum y_i z_i
+        y.assign(inner, 0.0);
+        for (int row = inner - 1; row >= 0; --row) {
+            double s = g[row];
+            for (int col = row + 1; col < inner; ++col) s -= h(row, col) * y[col];
+            y[row] = s / h(row, row);
+        }
+        SVec &acc = fg[idxAcc];
+        s_assign(acc, 0.0);
+        for (int i = 0; i < inner; ++i) s_lincomb(acc, {1.0, y[i]}, {&acc, &fg[offZ + i]});
+        s_lincomb(x, {1.0, 1.0}, {&x, &acc});
+        rep.restarts += 1;
+        apply_stokes(x, r);
+        s_lincomb(r, {1.0, -1.0}, {&b, &r});
+        beta0 = std::sqrt(s_dot(r, r));
+        rep.final_residual = beta0;
+        rep.relative_residual = beta0 / initial;
+        if (!std::isfinite(beta0)) {
+            rep.nonfinite_residual += 1;
+            return;
+        }
+        if (beta0 <= atol || beta0 / initial <= rtol) {
+            rep.converged = 1;
+            return;
+        }
+    }
+}
+
+void StokesSolver::record(tng_solve_report &rep, double abs_res)
+{
+    if (rep.history && rep.history_len < rep.history_capacity) rep.history[rep.history_len] = abs_res;
+    rep.history_len += 1;
+}
+
+void StokesSolver::solve(SVec &x, const SVec &b, tng_solve_report &rep)
+{
+    TNG_REQUIRE(x.u->mesh == vel && b.u->mesh == vel && x.p->mesh == pre_mesh && b.p->mesh == pre_mesh,
+                "solve: vectors must live on the solver's velocity and pressure meshes");
+    TNG_REQUIRE(x.u->dtype == TNG_F64 && x.p->dtype == TNG_F64 && b.u->dtype == TNG_F64 && b.p->dtype == TNG_F64,
+                "solve: the Stokes vectors are FP64");
+    double *hist = rep.history;
+    const int cap = rep.history_capacity;
+    std::memset(&rep, 0, sizeof(rep));
+    rep.history = hist;
+    rep.history_capacity = cap;
+    mg.coarse_iterations = 0;
+    fgmres(x, b, rep);
+    rep.coarse_iterations = mg.coarse_iterations;
+    // subtract the pressure mean over the unique pressure nodes (stokes_solver.hpp:680-689)
+    const double avg = dot(*x.p, *ones_p) / (double)num_dofs_pressure;
+    lincomb(*x.p, {1.0}, {x.p}, -avg);
+}
+
+// RHS boundary enforcement as the app applies it (stokes_solver.hpp:664-674):
+// the Dirichlet and free-slip flags are those of the FIRST boundary carrying
+// that condition (grid/shell/bit_masks.hpp:68-76), CMB before surface.
+void StokesSolver::enforce_rhs(tng_vector &f_u)
+{
+    auto first = [&](int bc) -> uint8_t {
+        if (bc_cmb == bc) return SBF_CMB;
+        if (bc_surf == bc) return SBF_SURFACE;
+        return 0;
+    };
+    launch_rhs_enforce(f_u, first(TNG_DIRICHLET), first(TNG_FREESLIP), stream_of(f_u));
+}
+
+}  // namespace tng
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+namespace {
+
+template <class F> tng_status sguard(tng_context *ctx, F &&f)
+{
+    try {
+        TNG_HIP_CHECK(hipSetDevice(ctx->device));
+        f();
+        return TNG_OK;
+    } catch (const tng::Error &e) {
+        ctx->last_error = e.what();
+        return e.status;
+    } catch (const std::bad_alloc &) {
+        ctx->last_error = "host allocation failed";
+        return TNG_OUT_OF_MEMORY;
+    } catch (const std::exception &e) {
+        ctx->last_error = e.what();
+        return TNG_DEVICE_ERROR;
+    }
+}
+
+struct Scratch {
+    std::vector<std::unique_ptr<tng_vector>> v;
+    tng_vector *like(const tng_vector &x)
+    {
+        v.emplace_back(tng::new_vector(x.mesh, x.dtype, x.ncomp));
+        return v.back().get();
+    }
+};
+
+void check_square(const tng_operator &op, const tng_vector &x, const tng_vector &b)
+{
+    TNG_REQUIRE(op.kind == OP_EPSDIVDIV || op.kind == OP_KMASS_LUMPED, "the solver needs a square operator");
+    TNG_REQUIRE(x.mesh == op.mesh && b.mesh == op.mesh && x.ncomp == b.ncomp && x.dtype == b.dtype,
+                "vectors must live on the operator's mesh with one shape");
+}
+
+}  // namespace
+
+extern "C" {
+
+tng_status tng_pcg_solve(tng_op op, tng_vec x, tng_vec b, int max_iterations, double rtol, double atol,
+                         int *iterations, double *relative_residual)
+{
+    if (!op || !x || !b || max_iterations < 0) return TNG_INVALID_ARGUMENT;
+    return sguard(op->mesh->ctx, [&] {
+        check_square(*op, *x, *b);
+        Scratch s;
+        double rel = 0;
+        const int it = tng::pcg_solve(*op, *x, *b, *s.like(*x), *s.like(*x), *s.like(*x), *s.like(*x), max_iterations,
+                                      rtol, atol, &rel);
+        if (iterations) *iterations = it;
+        if (relative_residual) *relative_residual = rel;
+    });
+}
+
+tng_status tng_power_iteration(tng_op op, tng_vec inv_diag, int iterations, unsigned long long seed, double *estimate)
+{
+    if (!op || !inv_diag || !estimate || iterations < 0) return TNG_INVALID_ARGUMENT;
+    return sguard(op->mesh->ctx, [&] {
+        check_square(*op, *inv_diag, *inv_diag);
+        Scratch s;
+        *estimate = tng::power_iteration(*op, *inv_diag, *s.like(*inv_diag), *s.like(*inv_diag), iterations, seed);
+    });
+}
+
+tng_status tng_chebyshev_smooth(tng_op op, tng_vec inv_diag, int order, int iterations, double lambda_max_estimate,
+                                tng_vec x, tng_vec b)
+{
+    if (!op || !inv_diag || !x || !b || order < 1 || iterations < 0) return TNG_INVALID_ARGUMENT;
+    return sguard(op->mesh->ctx, [&] {
+        check_square(*op, *x, *b);
+        TNG_REQUIRE(inv_diag->mesh == x->mesh && inv_diag->ncomp == x->ncomp && inv_diag->dtype == x->dtype,
+                    "inverse diagonal must match the vectors");
+        Scratch s;
+        tng::Chebyshev c;
+        c.order = order;
+        c.iterations = iterations;
+        c.inv_diag = inv_diag;
+        c.d = s.like(*x);
+        c.z = s.like(*x);
+        c.need_estimate = false;
+        c.lmax_estimate = lambda_max_estimate;
+        c.solve(*op, *x, *b);
+    });
+}
+
+void tng_stokes_params_default(tng_stokes_params *p)
+{
+    if (!p) return;
+    p->krylov_restart = 10;
+    p->krylov_max_iterations = 10;
+    p->krylov_relative_tolerance = 1e-6;
+    p->krylov_absolute_tolerance = 1e-12;
+    p->viscous_pc_num_vcycles = 1;
+    p->viscous_pc_chebyshev_order = 2;
+    p->viscous_pc_num_smoothing_steps_prepost = 2;
+    p->viscous_pc_num_power_iterations = 10;
+    p->gca = 0;
+    p->viscous_pc_dtype = TNG_F64;
+    p->coarse_max_iterations = 50;
+    p->coarse_relative_tolerance = 1e-6;
+    p->coarse_absolute_tolerance = 1e-16;
+    p->seed = 12345;
+}
+
+tng_status tng_stokes_solver_create(tng_ctx ctx, int min_level, int max_level, int radial_extra_levels, double r_min,
+                                    double r_max, int lateral_sdr, int radial_sdr, tng_bc bc_cmb, tng_bc bc_surface,
+                                    const tng_stokes_params *params, tng_stokes_solver *out)
+{
+    if (!ctx || !out) return TNG_INVALID_ARGUMENT;
+    *out = nullptr;
+    tng_stokes_params prm;
+    tng_stokes_params_default(&prm);
+    if (params) prm = *params;
+    return sguard(ctx, [&] {
+        TNG_REQUIRE(bc_cmb >= TNG_NEUMANN && bc_cmb <= TNG_FREESLIP && bc_surface >= TNG_NEUMANN &&
+                        bc_surface <= TNG_FREESLIP,
+                    "invalid boundary condition");
+        std::unique_ptr<tng_stokes_solver_s> s(new tng_stokes_solver_s());
+        s->s.reset(new tng::StokesSolver(ctx, min_level, max_level, radial_extra_levels, r_min, r_max, lateral_sdr,
+                                         radial_sdr, bc_cmb, bc_surface, prm));
+        TNG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        *out = s.release();
+    });
+}
+
+tng_status tng_stokes_solver_destroy(tng_stokes_solver solver)
+{
+    if (!solver) return TNG_INVALID_ARGUMENT;
+    tng_context *ctx = solver->s->vel->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete solver;
+    return TNG_OK;
+}
+
+tng_status tng_stokes_solver_num_levels(tng_stokes_solver solver, int *n)
+{
+    if (!solver || !n) return TNG_INVALID_ARGUMENT;
+    *n = (int)solver->s->meshes.size();
+    return TNG_OK;
+}
+
+tng_status tng_stokes_solver_mesh(tng_stokes_solver solver, int level, tng_mesh *mesh)
+{
+    if (!solver || !mesh || level < 0 || level >= (int)solver->s->meshes.size()) return TNG_INVALID_ARGUMENT;
+    *mesh = solver->s->meshes[level].get();
+    return TNG_OK;
+}
+
+tng_status tng_stokes_solver_set_viscosity(tng_stokes_solver solver, tng_vec eta)
+{
+    if (!solver || !eta) return TNG_INVALID_ARGUMENT;
+    return sguard(solver->s->vel->ctx, [&] { solver->s->set_viscosity(*eta); });
+}
+
+tng_status tng_stokes_apply(tng_stokes_solver solver, tng_vec u, tng_vec p, tng_vec out_u, tng_vec out_p)
+{
+    if (!solver || !u || !p || !out_u || !out_p) return TNG_INVALID_ARGUMENT;
+    tng::StokesSolver &S = *solver->s;
+    return sguard(S.vel->ctx, [&] {
+        TNG_REQUIRE(u->mesh == S.vel && out_u->mesh == S.vel && p->mesh == S.pre_mesh && out_p->mesh == S.pre_mesh &&
+                        u->ncomp == 3 && out_u->ncomp == 3 && p->ncomp == 1 && out_p->ncomp == 1,
+                    "Stokes vectors must be (velocity, pressure) on the solver's meshes");
+        TNG_REQUIRE(u != out_u && p != out_p, "apply: src and dst must differ");
+        tng::StokesSolver::SVec src{u, p}, dst{out_u, out_p};
+        S.apply_stokes(src, dst);
+    });
+}
+
+tng_status tng_stokes_precondition_velocity(tng_stokes_solver solver, tng_vec x, tng_vec b)
+{
+    if (!solver || !x || !b) return TNG_INVALID_ARGUMENT;
+    tng::StokesSolver &S = *solver->s;
+    return sguard(S.vel->ctx, [&] {
+        TNG_REQUIRE(x->mesh == S.vel && b->mesh == S.vel && x->ncomp == 3 && b->ncomp == 3 &&
+                        x->dtype == TNG_F64 && b->dtype == TNG_F64,
+                    "velocity vectors (FP64) on the solver's velocity mesh expected");
+        S.precondition_velocity(*x, *b);
+    });
+}
+
+tng_status tng_stokes_enforce_rhs(tng_stokes_solver solver, tng_vec f_u)
+{
+    if (!solver || !f_u) return TNG_INVALID_ARGUMENT;
+    tng::StokesSolver &S = *solver->s;
+    return sguard(S.vel->ctx, [&] {
+        TNG_REQUIRE(f_u->mesh == S.vel && f_u->ncomp == 3, "velocity vector on the solver's velocity mesh expected");
+        S.enforce_rhs(*f_u);
+    });
+}
+
+tng_status tng_stokes_solve(tng_stokes_solver solver, tng_vec u, tng_vec p, tng_vec f_u, tng_vec f_p,
+                            tng_solve_report *report)
+{
+    if (!solver || !u || !p || !f_u || !f_p) return TNG_INVALID_ARGUMENT;
+    tng::StokesSolver &S = *solver->s;
+    tng_solve_report local{};
+    tng_solve_report &rep = report ? *report : local;
+    return sguard(S.vel->ctx, [&] {
+        TNG_REQUIRE(u->ncomp == 3 && f_u->ncomp == 3 && p->ncomp == 1 && f_p->ncomp == 1,
+                    "Stokes vectors must be (velocity, pressure)");
+        tng::StokesSolver::SVec x{u, p}, b{f_u, f_p};
+        S.solve(x, b, rep);
+    });
+}
+
+}  // extern "C"

@@ -116,15 +116,29 @@ struct tng_domain {
     int64_t lin(int sd, int x, int y, int r) const { return (((int64_t)sd * nx + x) * nx + y) * nr + r; }
 };
 
+// A node field of 1 or 3 components, each a separate (sd, x, y, r) array.
+// Owned vectors keep their components in one allocation, each starting on a
+// 256-byte boundary; wrapped vectors point at caller memory.
 struct tng_vector {
     tng_domain *mesh = nullptr;
     tng_dtype dtype = TNG_F64;
     int ncomp = 1;
-    int64_t comp_stride = 0;  // elements between components
-    tng::DevBuf buf;
+    void *cp[3] = {nullptr, nullptr, nullptr};  // component base addresses
+    tng::DevBuf buf;                            // storage of an owned vector (empty when wrapped)
     size_t elem_size() const { return dtype == TNG_F64 ? 8 : 4; }
-    void *comp(int c) const { return static_cast<char *>(buf.p) + (size_t)c * comp_stride * elem_size(); }
+    void *comp(int c) const { return cp[c]; }
 };
+
+namespace tng {
+// Component pointers passed to kernels by value.
+template <typename T> struct Comp3 {
+    T *p[3];
+};
+template <typename T> inline Comp3<T> comps(const tng_vector &v)
+{
+    return {{static_cast<T *>(v.cp[0]), static_cast<T *>(v.cp[1]), static_cast<T *>(v.cp[2])}};
+}
+}  // namespace tng
 
 enum OpKind { OP_EPSDIVDIV = 1, OP_DIVERGENCE, OP_GRADIENT, OP_PROLONGATION, OP_RESTRICTION, OP_KMASS_LUMPED };
 
@@ -169,5 +183,21 @@ double reduce_to_host(int op, const tng_vector &x, const tng_vector *y, hipStrea
 
 // generic operator apply (dispatch on kind)
 void op_apply(tng_operator &op, const tng_vector &src, tng_vector &dst);
+
+// object factories (throw tng::Error; the caller owns the result)
+tng_domain *new_mesh(tng_context *ctx, int lat_level, int rad_level, double r_min, double r_max, int lat_sdr,
+                     int rad_sdr);
+tng_vector *new_vector(tng_domain *mesh, tng_dtype dtype, int ncomp);
+tng_operator *new_epsdivdiv(tng_domain *mesh, tng_vector *k, int bc_cmb, int bc_surf, bool diagonal);
+tng_operator *new_two_level(OpKind kind, tng_domain *fine, tng_domain *coarse, int bc_cmb, int bc_surf);
+tng_operator *new_kmass(tng_domain *mesh, tng_vector *k);
+
+// solver support kernels
+void launch_randomize(tng_vector &v, uint64_t seed, hipStream_t s);
+void launch_inject(tng_vector &coarse, const tng_vector &fine, hipStream_t s);
+// RHS boundary enforcement of a velocity vector: zero at the nodes of
+// boundary `dirichlet_flag`, normal component removed at the nodes of
+// boundary `freeslip_flag` (flags: 0 none, SBF_CMB or SBF_SURFACE)
+void launch_rhs_enforce(tng_vector &u, uint8_t dirichlet_flag, uint8_t freeslip_flag, hipStream_t s);
 
 }  // namespace tng

@@ -1,0 +1,86 @@
+// Solver objects of the Stokes path (host side; see solvers.cpp).
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "tng_internal.hpp"
+
+namespace tng {
+
+double power_iteration(tng_operator &A, const tng_vector &inv_diag, tng_vector &it, tng_vector &aux, int iterations,
+                       uint64_t seed);
+
+int pcg_solve(tng_operator &A, tng_vector &x, const tng_vector &b, tng_vector &r, tng_vector &p, tng_vector &ap,
+              tng_vector &z, int max_iterations, double rtol, double atol, double *final_rel);
+
+// Chebyshev-accelerated Jacobi smoother on D^{-1} A.
+struct Chebyshev {
+    int order = 2, iterations = 1, power_iterations = 10;
+    tng_vector *inv_diag = nullptr;
+    tng_vector *d = nullptr, *z = nullptr;  // work vectors
+    bool need_estimate = true;
+    double lmax_estimate = 0.0;  // estimate of the largest eigenvalue of D^{-1} A
+    uint64_t seed = 12345;
+    void solve(tng_operator &A, tng_vector &x, const tng_vector &b);
+};
+
+// Geometric multigrid V-cycle; level 0 is the coarsest.
+struct Multigrid {
+    struct Level {
+        tng_operator *A = nullptr;
+        tng_operator *P = nullptr;  // level-1 -> level (Add mode); null on level 0
+        tng_operator *R = nullptr;  // level -> level-1; null on level 0
+        tng_vector *tmp = nullptr, *r = nullptr, *e = nullptr, *inv_diag = nullptr;
+    };
+    std::vector<Level> levels;
+    std::vector<Chebyshev> pre, post;
+    int num_cycles = 1;
+    // FP64 PCG on the coarsest level
+    tng_operator *coarse_op = nullptr;
+    tng_vector *cg_x = nullptr, *cg_b = nullptr, *cg_tmp[4] = {nullptr, nullptr, nullptr, nullptr};
+    int coarse_max_it = 50;
+    double coarse_rtol = 1e-6, coarse_atol = 1e-16;
+    long coarse_iterations = 0;
+
+    void solve(tng_vector &x, const tng_vector &b);
+    void recurse(int level, tng_vector &x, const tng_vector &b);
+    void coarse_solve(tng_vector &e0, const tng_vector &r0);
+};
+
+struct StokesSolver {
+    struct SVec {
+        tng_vector *u, *p;
+    };
+    tng_stokes_params params;
+    int bc_cmb, bc_surf;
+    std::vector<std::unique_ptr<tng_domain>> meshes;  // coarsest .. velocity level
+    tng_domain *vel = nullptr, *pre_mesh = nullptr;
+    std::vector<std::unique_ptr<tng_vector>> pool;   // every vector the solver owns
+    std::vector<tng_vector *> eta, eta_mg;           // per level: FP64 viscosity, multigrid-precision copy
+    std::unique_ptr<tng_operator> A, Bt, B, coarse_op64, pmass;
+    std::vector<std::unique_ptr<tng_operator>> mg_ops;
+    Multigrid mg;
+    tng_vector *k_pm = nullptr, *inv_lumped_pmass = nullptr, *ones_p = nullptr, *ones_v = nullptr;
+    tng_vector *prec_tmp_u = nullptr, *mg_x = nullptr, *mg_b = nullptr;
+    std::vector<SVec> fg;  // FGMRES work space: 2m + 4 Stokes vectors
+    long long num_dofs_pressure = 0;
+
+    StokesSolver(tng_context *ctx, int min_level, int max_level, int radial_extra, double r_min, double r_max,
+                 int lat_sdr, int rad_sdr, int bc_cmb, int bc_surf, const tng_stokes_params &prm);
+    void set_viscosity(const tng_vector &eta_fine);
+    void viscosity_updated();
+    void apply_stokes(const SVec &src, SVec &dst);
+    void precondition_velocity(tng_vector &x, const tng_vector &b);
+    void precondition(SVec &x, const SVec &b);
+    void fgmres(SVec &x, const SVec &b, tng_solve_report &rep);
+    void solve(SVec &x, const SVec &b, tng_solve_report &rep);
+    void enforce_rhs(tng_vector &f_u);
+    static void record(tng_solve_report &rep, double abs_res);
+};
+
+}  // namespace tng
+
+struct tng_stokes_solver_s {
+    std::unique_ptr<tng::StokesSolver> s;
+};

@@ -64,6 +64,7 @@ EXPORTS = [
     ("tng_mesh_copy_boundary_mask", _i, [_vp, _vp]),
     ("tng_mesh_copy_ownership_mask", _i, [_vp, _vp]),
     ("tng_vec_create", _i, [_vp, _i, _i, _pv]),
+    ("tng_vec_wrap", _i, [_vp, _i, _i, C.POINTER(C.c_void_p), _pv]),
     ("tng_vec_destroy", _i, [_vp]),
     ("tng_vec_upload", _i, [_vp, _vp]),
     ("tng_vec_download", _i, [_vp, _vp]),
@@ -90,7 +91,47 @@ EXPORTS = [
     ("tng_vec_dot", _i, [_vp, _vp, C.POINTER(C.c_double)]),
     ("tng_vec_max_abs", _i, [_vp, C.POINTER(C.c_double)]),
     ("tng_vec_has_nan_or_inf", _i, [_vp, C.POINTER(C.c_int)]),
+    ("tng_pcg_solve", _i, [_vp, _vp, _vp, _i, _d, _d, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    ("tng_power_iteration", _i, [_vp, _vp, _i, C.c_ulonglong, C.POINTER(C.c_double)]),
+    ("tng_chebyshev_smooth", _i, [_vp, _vp, _i, _i, _d, _vp, _vp]),
+    ("tng_stokes_params_default", None, [_vp]),
+    ("tng_stokes_solver_create", _i, [_vp, _i, _i, _i, _d, _d, _i, _i, _i, _i, _vp, _pv]),
+    ("tng_stokes_solver_destroy", _i, [_vp]),
+    ("tng_stokes_solver_num_levels", _i, [_vp, C.POINTER(C.c_int)]),
+    ("tng_stokes_solver_mesh", _i, [_vp, _i, _pv]),
+    ("tng_stokes_solver_set_viscosity", _i, [_vp, _vp]),
+    ("tng_stokes_apply", _i, [_vp, _vp, _vp, _vp, _vp]),
+    ("tng_stokes_precondition_velocity", _i, [_vp, _vp, _vp]),
+    ("tng_stokes_enforce_rhs", _i, [_vp, _vp]),
+    ("tng_stokes_solve", _i, [_vp, _vp, _vp, _vp, _vp, _vp]),
 ]
+
+class StokesParams(C.Structure):
+    """apps/mantlecirculation/src/parameters.hpp:155-185 StokesSolverParameters (+ precision, coarse PCG, seed)."""
+    _fields_ = [("krylov_restart", C.c_int), ("krylov_max_iterations", C.c_int),
+                ("krylov_relative_tolerance", C.c_double), ("krylov_absolute_tolerance", C.c_double),
+                ("viscous_pc_num_vcycles", C.c_int), ("viscous_pc_chebyshev_order", C.c_int),
+                ("viscous_pc_num_smoothing_steps_prepost", C.c_int), ("viscous_pc_num_power_iterations", C.c_int),
+                ("gca", C.c_int), ("viscous_pc_dtype", C.c_int), ("coarse_max_iterations", C.c_int),
+                ("coarse_relative_tolerance", C.c_double), ("coarse_absolute_tolerance", C.c_double),
+                ("seed", C.c_ulonglong)]
+
+    @classmethod
+    def defaults(cls, **kw):
+        p = cls()
+        load_library().tng_stokes_params_default(C.byref(p))
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+
+class SolveReport(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("restarts", C.c_int), ("converged", C.c_int), ("breakdowns", C.c_int),
+                ("nonfinite_preconditioner", C.c_int), ("nonfinite_residual", C.c_int),
+                ("coarse_iterations", C.c_longlong), ("initial_residual", C.c_double),
+                ("final_residual", C.c_double), ("relative_residual", C.c_double),
+                ("history", C.POINTER(C.c_double)), ("history_capacity", C.c_int), ("history_len", C.c_int)]
+
 
 _lib = None
 
@@ -208,11 +249,21 @@ class DistributedDomain:
 class _Vector:
     ncomp = 1
 
-    def __init__(self, domain, dtype=F64):
+    def __init__(self, domain, dtype=F64, component_ptrs=None, keepalive=None):
         self.domain, self.dtype = domain, dtype
         h = C.c_void_p()
-        domain.ctx.check(load_library().tng_vec_create(domain.h, dtype, self.ncomp, C.byref(h)))
+        if component_ptrs is None:
+            domain.ctx.check(load_library().tng_vec_create(domain.h, dtype, self.ncomp, C.byref(h)))
+        else:
+            arr = (C.c_void_p * self.ncomp)(*[int(p) for p in component_ptrs])
+            domain.ctx.check(load_library().tng_vec_wrap(domain.h, dtype, self.ncomp, arr, C.byref(h)))
+        self._keepalive = keepalive
         self.h = h
+
+    @classmethod
+    def wrap(cls, domain, dtype, component_ptrs, keepalive=None):
+        """Non-owning view of existing device arrays (one per component)."""
+        return cls(domain, dtype, component_ptrs, keepalive)
 
     @property
     def np_dtype(self):
@@ -394,6 +445,95 @@ def norm_inf(x):
     out = C.c_double()
     x.domain.ctx.check(load_library().tng_vec_max_abs(x.h, C.byref(out)))
     return out.value
+
+
+def pcg_solve(op, x, b, max_iterations=50, rtol=1e-6, atol=1e-16):
+    """PCG with the identity preconditioner (linalg/solvers/pcg.hpp); returns (iterations, relative residual)."""
+    it, rel = C.c_int(), C.c_double()
+    op.domain.ctx.check(load_library().tng_pcg_solve(op.h, x.h, b.h, max_iterations, rtol, atol, C.byref(it),
+                                                     C.byref(rel)))
+    return it.value, rel.value
+
+
+def power_iteration(op, inv_diag, iterations=10, seed=12345):
+    """Largest-eigenvalue estimate of D^{-1} A (linalg/solvers/power_iteration.hpp)."""
+    out = C.c_double()
+    op.domain.ctx.check(load_library().tng_power_iteration(op.h, inv_diag.h, iterations, seed, C.byref(out)))
+    return out.value
+
+
+def chebyshev_smooth(op, inv_diag, x, b, order=2, iterations=1, lambda_max_estimate=1.0):
+    """Chebyshev sweeps with a given eigenvalue estimate (linalg/solvers/chebyshev.hpp)."""
+    op.domain.ctx.check(load_library().tng_chebyshev_smooth(op.h, inv_diag.h, order, iterations,
+                                                            float(lambda_max_estimate), x.h, b.h))
+
+
+class _BorrowedDomain(DistributedDomain):
+    """A mesh owned by a solver (not destroyed from Python)."""
+
+    def __init__(self, ctx, h, owner):
+        super().__init__(ctx, h)
+        self._owner = owner
+
+    def __del__(self):
+        pass
+
+
+class StokesSolver:
+    """StokesContext (apps/mantlecirculation/src/stokes_solver.hpp:143-691) for one GPU."""
+
+    def __init__(self, min_level, max_level, r_min=0.55, r_max=1.0, bcs=(DIRICHLET, DIRICHLET), params=None,
+                 radial_extra_levels=0, lateral_sdr=0, radial_sdr=0, ctx=None):
+        self.ctx = ctx or default_context()
+        self.params = params if params is not None else StokesParams.defaults()
+        h = C.c_void_p()
+        self.ctx.check(load_library().tng_stokes_solver_create(
+            self.ctx.h, min_level, max_level, radial_extra_levels, r_min, r_max, lateral_sdr, radial_sdr,
+            int(bcs[0]), int(bcs[1]), C.byref(self.params), C.byref(h)))
+        self.h = h
+        n = C.c_int()
+        self.ctx.check(load_library().tng_stokes_solver_num_levels(h, C.byref(n)))
+        self.num_levels = n.value
+        self.domains = []
+        for lv in range(self.num_levels):
+            mh = C.c_void_p()
+            self.ctx.check(load_library().tng_stokes_solver_mesh(h, lv, C.byref(mh)))
+            self.domains.append(_BorrowedDomain(self.ctx, mh, self))
+        self.domain_velocity = self.domains[-1]
+        self.domain_pressure = self.domains[-2]
+
+    def set_viscosity(self, eta):
+        self.ctx.check(load_library().tng_stokes_solver_set_viscosity(self.h, eta.h))
+
+    def apply(self, u, p, out_u, out_p):
+        self.ctx.check(load_library().tng_stokes_apply(self.h, u.h, p.h, out_u.h, out_p.h))
+
+    def precondition_velocity(self, x, b):
+        self.ctx.check(load_library().tng_stokes_precondition_velocity(self.h, x.h, b.h))
+
+    def enforce_rhs(self, f_u):
+        self.ctx.check(load_library().tng_stokes_enforce_rhs(self.h, f_u.h))
+
+    def solve(self, u, p, f_u, f_p, history_capacity=1000):
+        rep = SolveReport()
+        hist = (C.c_double * history_capacity)()
+        rep.history = C.cast(hist, C.POINTER(C.c_double))
+        rep.history_capacity = history_capacity
+        self.ctx.check(load_library().tng_stokes_solve(self.h, u.h, p.h, f_u.h, f_p.h, C.byref(rep)))
+        out = {name: getattr(rep, name) for name, _ in SolveReport._fields_ if name not in ("history",)}
+        out["history"] = [hist[i] for i in range(min(rep.history_len, history_capacity))]
+        return out
+
+    def close(self):
+        if self.h:
+            load_library().tng_stokes_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def has_nan_or_inf(x):

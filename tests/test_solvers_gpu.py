@@ -1,0 +1,299 @@
+"""Solver layer (through the C ABI) against dense linear algebra built from the
+oracle's operator applies at small levels.
+
+- power iteration (linalg/solvers/power_iteration.hpp:13-58) vs the dense
+  largest eigenvalue of D^{-1} A;
+- one Chebyshev sweep (linalg/solvers/chebyshev.hpp:102-165) vs the closed-form
+  Chebyshev error polynomial T_p((theta - M)/delta) / T_p(theta/delta);
+- PCG (linalg/solvers/pcg.hpp:81-170) vs textbook CG on the dense matrix;
+- the Stokes operator (epsilon_divdiv_stokes.hpp:55-84) vs the oracle blocks;
+- V-cycle contraction, and the FGMRES Stokes solve (fgmres.hpp:109-373 with the
+  block-triangular preconditioner) vs a dense direct solve, in FP64 and with
+  the FP32 V-cycle.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import tng
+from conftest import consistent, rel_err
+
+pytestmark = pytest.mark.gpu
+
+N, D, F = tng.NEUMANN, tng.DIRICHLET, tng.FREESLIP
+
+
+def unique_index(om):
+    """For every unique node id, the linear index of its OWNED copy."""
+    cid = om.canonical_ids.ravel()
+    own = np.nonzero(om.ownership_mask.ravel() == 1)[0]
+    idx = np.empty(om.nunique, dtype=np.int64)
+    idx[cid[own]] = own
+    return idx
+
+
+def to_unique(om, f):
+    f = np.asarray(f).reshape(-1, om.nalloc)
+    return f[:, unique_index(om)].ravel()
+
+
+def from_unique(om, v, ncomp):
+    cid = om.canonical_ids.ravel()
+    v = np.asarray(v).reshape(ncomp, om.nunique)
+    out = v[:, cid]
+    return out.reshape(((ncomp,) if ncomp > 1 else ()) + om.shape)
+
+
+def dense(om, fn, ncomp_in, om_in=None, ncomp_out=None):
+    """Dense matrix on unique dofs of a linear map given on copy arrays."""
+    om_in = om_in or om
+    ncomp_out = ncomp_out or ncomp_in
+    n_in = ncomp_in * om_in.nunique
+    cols = []
+    for j in range(n_in):
+        e = np.zeros(n_in)
+        e[j] = 1.0
+        cols.append(to_unique(om, fn(from_unique(om_in, e, ncomp_in))))
+    return np.array(cols).T
+
+
+def field(om, ncomp, seed):
+    r = np.random.default_rng(seed)
+    return consistent(om, r.uniform(-1, 1, ((ncomp,) if ncomp > 1 else ()) + om.shape))
+
+
+def vec(dom, ncomp, data=None, dtype=tng.F64):
+    v = (tng.VectorQ1Vec if ncomp == 3 else tng.VectorQ1Scalar)(dom, dtype)
+    if data is not None:
+        v.upload(data)
+    return v
+
+
+@pytest.fixture(scope="module")
+def level1():
+    om = O.Mesh(1)
+    dom = tng.DistributedDomain.create_uniform(1, 1, 0.55, 1.0)
+    k = np.exp(field(om, 1, 5))
+    A = dense(om, lambda x: om.epsdivdiv(k, x, D, D, path="fast"), 3)
+    diag = to_unique(om, om.epsdivdiv(k, np.ones((3,) + om.shape), D, D, diagonal=True, path="fast"))
+    return om, dom, k, A, diag
+
+
+def gpu_op(dom, k, bcs=(D, D)):
+    kv = vec(dom, 1, k)
+    return tng.EpsilonDivDivKerngen(dom, kv, bcs), kv
+
+
+def inv_diag_vec(dom, op):
+    ones, d = vec(dom, 3), vec(dom, 3)
+    ones.fill(1.0)
+    op.set_diagonal(True)
+    op.apply(ones, d)
+    op.set_diagonal(False)
+    tng.invert_entries(d)
+    return d
+
+
+def test_power_iteration(level1, gpu_ctx):
+    om, dom, k, A, diag = level1
+    M = A / diag[:, None]
+    lmax = np.max(np.abs(np.linalg.eigvals(M)))
+    op, kv = gpu_op(dom, k)
+    invd = inv_diag_vec(dom, op)
+    assert rel_err(to_unique(om, invd.download()), 1.0 / diag) < 1e-13
+    est10 = tng.power_iteration(op, invd, 10)
+    est400 = tng.power_iteration(op, invd, 400)
+    assert 0.3 * lmax < est10 < 1.2 * lmax
+    assert abs(est400 - lmax) < 2e-3 * lmax
+    # the seeded start makes the estimate reproducible
+    assert tng.power_iteration(op, invd, 10) == est10
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_chebyshev_closed_form(order, level1, gpu_ctx):
+    """One sweep from x = 0 on A x = A x*: the error is the Chebyshev
+    polynomial T_p((theta I - M)/delta) / T_p(theta/delta) applied to x*."""
+    om, dom, k, A, diag = level1
+    M = A / diag[:, None]
+    est = 0.9 * np.max(np.abs(np.linalg.eigvals(M)))
+    lmax, lmin = 1.5 * est, 0.1 * est
+    theta, delta = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
+    xs = field(om, 3, 7)
+    xu = to_unique(om, xs)
+    X = lambda v: (theta * v - M @ v) / delta  # noqa: E731
+    t_prev, t = xu, X(xu)
+    s_prev, s = 1.0, theta / delta
+    for _ in range(order - 1):
+        t_prev, t = t, 2 * X(t) - t_prev
+        s_prev, s = s, 2 * (theta / delta) * s - s_prev
+    err_exp = (t if order > 1 else X(xu)) / (s if order > 1 else theta / delta)
+    op, kv = gpu_op(dom, k)
+    invd = inv_diag_vec(dom, op)
+    b = vec(dom, 3)
+    op.apply(vec(dom, 3, xs), b)
+    x = vec(dom, 3)
+    tng.chebyshev_smooth(op, invd, x, b, order=order, iterations=1, lambda_max_estimate=est)
+    err = xu - to_unique(om, x.download())
+    assert rel_err(err, err_exp) < 1e-11
+
+
+def test_pcg_matches_textbook_cg(level1, gpu_ctx):
+    om, dom, k, A, diag = level1
+    xs = field(om, 3, 8)
+    bu = A @ to_unique(om, xs)
+    op, kv = gpu_op(dom, k)
+    b = vec(dom, 3)
+    op.apply(vec(dom, 3, xs), b)
+    for kmax in (5, 15):
+        x = vec(dom, 3)
+        it, rel = tng.pcg_solve(op, x, b, max_iterations=kmax, rtol=0.0, atol=0.0)
+        # textbook CG, x0 = 0
+        xr = np.zeros_like(bu)
+        r = bu.copy()
+        p = r.copy()
+        for _ in range(kmax):
+            ap = A @ p
+            a = (r @ r) / (p @ ap)
+            xr += a * p
+            rn = r - a * ap
+            p = rn + (rn @ rn) / (r @ r) * p
+            r = rn
+        assert it == kmax
+        assert rel_err(to_unique(om, x.download()), xr) < 1e-9
+        assert abs(rel - np.linalg.norm(r) / np.linalg.norm(bu)) < 1e-8
+    x = vec(dom, 3)
+    it, rel = tng.pcg_solve(op, x, b, max_iterations=500, rtol=1e-12, atol=0.0)
+    assert rel <= 1e-12
+    assert rel_err(to_unique(om, x.download()), to_unique(om, xs)) < 1e-9
+
+
+# ------------------------------------------------------------------ Stokes
+def stokes_solver(minl, maxl, bcs, **kw):
+    return tng.StokesSolver(minl, maxl, 0.55, 1.0, bcs, tng.StokesParams.defaults(**kw))
+
+
+@pytest.mark.parametrize("bcs", [(D, D), (F, F), (F, D)])
+def test_stokes_apply(bcs, gpu_ctx):
+    S = stokes_solver(1, 3, bcs)
+    of, oc = O.Mesh(3), O.Mesh(2)
+    u, p = field(of, 3, 9), field(oc, 1, 10)
+    ou, op_ = vec(S.domain_velocity, 3), vec(S.domain_pressure, 1)
+    S.apply(vec(S.domain_velocity, 3, u), vec(S.domain_pressure, 1, p), ou, op_)
+    k = np.ones(of.shape)
+    ru = of.epsdivdiv(k, u, *bcs, apply_mode=O.REPLACE, comm=O.SKIP, path="fast")
+    ru = O.gradient(of, oc, p, *bcs, apply_mode=O.ADD, comm=O.COMMUNICATE, dst=ru)
+    rp = O.divergence(of, oc, u, *bcs)
+    assert rel_err(ou.download(), ru) < 1e-12
+    assert rel_err(op_.download(), rp) < 1e-12
+
+
+def test_vcycle_contracts(gpu_ctx):
+    """Repeated V-cycles (FP64 and FP32 levels) on A x = b reduce the error
+    by a level-independent factor."""
+    of = O.Mesh(4)
+    rates = {}
+    for dt in (tng.F64, tng.F32):
+        S = stokes_solver(2, 4, (D, D), viscous_pc_dtype=dt)
+        dom = S.domain_velocity
+        xs = field(of, 3, 11)
+        kv = vec(dom, 1)
+        kv.fill(1.0)
+        A = tng.EpsilonDivDivKerngen(dom, kv, (D, D))
+        b = vec(dom, 3)
+        A.apply(vec(dom, 3, xs), b)
+        x = vec(dom, 3)
+        errs = []
+        for _ in range(6):
+            S.precondition_velocity(x, b)
+            errs.append(np.linalg.norm(x.download() - xs))
+        rate = (errs[-1] / errs[1]) ** (1 / 4)
+        rates[dt] = rate
+        assert rate < 0.5, (dt, errs)
+    assert abs(rates[tng.F64] - rates[tng.F32]) < 0.05
+
+
+@pytest.fixture(scope="module")
+def stokes_dense():
+    """Dense K on unique dofs: velocity level 2, pressure level 1, Dirichlet."""
+    of, oc = O.Mesh(2), O.Mesh(1)
+    eta = np.exp(0.5 * field(of, 1, 12))
+    nu, npr = 3 * of.nunique, oc.nunique
+
+    def K_u(u):
+        out = of.epsdivdiv(eta, u, D, D, comm=O.SKIP, path="fast")
+        return of.halo_sum(out), O.divergence(of, oc, u, D, D)
+
+    Kuu = dense(of, lambda u: K_u(u)[0], 3)
+    Kpu = dense(oc, lambda u: K_u(u)[1], 3, om_in=of, ncomp_out=1)
+    Kup = dense(of, lambda p: O.gradient(of, oc, p, D, D), 1, om_in=oc, ncomp_out=3)
+    K = np.block([[Kuu, Kup], [Kpu, np.zeros((npr, npr))]])
+    return of, oc, eta, K, nu, npr
+
+
+@pytest.mark.parametrize("dt", [tng.F64, tng.F32])
+def test_stokes_solve_vs_dense(dt, stokes_dense, gpu_ctx):
+    of, oc, eta, K, nu, npr = stokes_dense
+    S = stokes_solver(1, 2, (D, D), krylov_restart=30, krylov_max_iterations=300, krylov_relative_tolerance=1e-11,
+                      krylov_absolute_tolerance=1e-30, viscous_pc_dtype=dt)
+    S.set_viscosity(vec(S.domain_velocity, 1, eta))
+    xs_u, xs_p = field(of, 3, 13), field(oc, 1, 14)
+    fu, fp = vec(S.domain_velocity, 3), vec(S.domain_pressure, 1)
+    S.apply(vec(S.domain_velocity, 3, xs_u), vec(S.domain_pressure, 1, xs_p), fu, fp)
+    # dense reference: minimum-norm solution, then the pressure mean removed
+    f = np.concatenate([to_unique(of, fu.download()), to_unique(oc, fp.download())])
+    sol = np.linalg.lstsq(K, f, rcond=None)[0]
+    u_ref, p_ref = sol[:nu], sol[nu:] - sol[nu:].mean()
+    u, p = vec(S.domain_velocity, 3), vec(S.domain_pressure, 1)
+    rep = S.solve(u, p, fu, fp)
+    assert rep["converged"] == 1, rep
+    assert rep["relative_residual"] <= 1e-11
+    assert rel_err(to_unique(of, u.download()), u_ref) < 1e-9
+    assert rel_err(to_unique(oc, p.download()), p_ref) < 1e-8
+    # the velocity of the manufactured solution is recovered (pressure up to a constant)
+    assert rel_err(to_unique(of, u.download()), to_unique(of, xs_u)) < 1e-9
+
+
+def test_stokes_solve_fp32_vcycle_matches_fp64(gpu_ctx):
+    """Velocity level 4 / pressure level 3, free slip on both surfaces, viscosity
+    jump: the FP32 V-cycle preconditioner reaches the same 1e-8 FP64 residual
+    and the same solution as the FP64 one."""
+    of, oc = O.Mesh(4), O.Mesh(3)
+    r = np.broadcast_to(of.radii[:, None, None, :], of.shape)
+    eta = np.where(r < 0.85, 100.0, 1.0) * np.exp(0.5 * field(of, 1, 15))
+    sols = {}
+    for dt in (tng.F64, tng.F32):
+        S = stokes_solver(2, 4, (F, F), krylov_restart=30, krylov_max_iterations=200, krylov_relative_tolerance=1e-8,
+                          krylov_absolute_tolerance=1e-30, viscous_pc_dtype=dt)
+        S.set_viscosity(vec(S.domain_velocity, 1, eta))
+        fu = vec(S.domain_velocity, 3, field(of, 3, 16))
+        S.enforce_rhs(fu)
+        fp = vec(S.domain_pressure, 1)
+        u, p = vec(S.domain_velocity, 3), vec(S.domain_pressure, 1)
+        rep = S.solve(u, p, fu, fp)
+        assert rep["converged"] == 1 and rep["relative_residual"] <= 1e-8, rep
+        sols[dt] = (u.download(), p.download(), rep)
+    (u64, p64, r64), (u32, p32, r32) = sols[tng.F64], sols[tng.F32]
+    own = of.ownership_mask == 1
+    assert rel_err(u32[:, own], u64[:, own]) < 1e-6
+    assert r32["iterations"] <= r64["iterations"] + 3
+
+
+def test_enforce_rhs_first_boundary_only(gpu_ctx):
+    """Dirichlet on both surfaces: only the CMB rows are zeroed, as the
+    reference's get_shell_boundary_flag returns the first match."""
+    S = stokes_solver(1, 2, (D, D))
+    of = O.Mesh(2)
+    f = field(of, 3, 17)
+    fu = vec(S.domain_velocity, 3, f)
+    S.enforce_rhs(fu)
+    g = fu.download()
+    assert np.all(g[..., 0] == 0)
+    assert np.array_equal(g[..., 1:], f[..., 1:])
+    S2 = stokes_solver(1, 2, (D, F))
+    fu = vec(S2.domain_velocity, 3, f)
+    S2.enforce_rhs(fu)
+    g = fu.download()
+    n = of.coords / np.linalg.norm(of.coords, axis=-1, keepdims=True)
+    assert np.all(g[..., 0] == 0)
+    assert np.abs(np.einsum("csxy,sxyc->sxy", g[..., -1], n)).max() < 1e-15
+    assert np.array_equal(g[..., 1:-1], f[..., 1:-1])

@@ -226,3 +226,27 @@ def test_operator_argument_checks(gpu_ctx):
     d2, _, _, _ = levels(4)
     with pytest.raises(tng.TngError):
         tng.DivergenceKerngen(d2, dc, (D, D))  # not one refinement apart
+
+
+def test_wrapped_vectors(gpu_ctx):
+    """tng_vec_wrap views of existing device arrays behave like owned vectors
+    (separate component arrays, as a Kokkos Grid4DDataVec holds them)."""
+    df, dc, of, oc = levels(4)
+    u, p = field(of, 3, 27), field(oc, 1, 28)
+    k = np.exp(field(of, 1, 29))
+    comps = [vec(df, 1, u[c]) for c in range(3)]  # three separate allocations
+    outs = [vec(df, 1) for _ in range(3)]
+    src = tng.VectorQ1Vec.wrap(df, tng.F64, [v.device_ptr() for v in comps], keepalive=comps)
+    dst = tng.VectorQ1Vec.wrap(df, tng.F64, [v.device_ptr() for v in outs], keepalive=outs)
+    kv = vec(df, 1, k)
+    kw = tng.VectorQ1Scalar.wrap(df, tng.F64, [kv.device_ptr()], keepalive=kv)
+    tng.EpsilonDivDivKerngen(df, kw, (D, F)).apply(src, dst)
+    got = np.stack([o.download() for o in outs])
+    assert rel_err(got, of.epsdivdiv(k, u, D, F, path="fast")) < 1e-12
+    # BLAS and transfers on wrapped vectors
+    tng.lincomb(dst, [2.0, -1.0], [src, dst], 0.5)
+    assert rel_err(np.stack([o.download() for o in outs]), 2 * u - got + 0.5) < 1e-15
+    pw = vec(dc, 1, p)
+    gw = tng.VectorQ1Vec.wrap(df, tng.F64, [v.device_ptr() for v in outs], keepalive=outs)
+    tng.GradientKerngen(df, dc, (D, F)).apply(tng.VectorQ1Scalar.wrap(dc, tng.F64, [pw.device_ptr()], keepalive=pw), gw)
+    assert rel_err(np.stack([o.download() for o in outs]), O.gradient(of, oc, p, D, F)) < 1e-12
