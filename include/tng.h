@@ -54,6 +54,11 @@ typedef struct tng_operator *tng_op;
 
 /* util/init.hpp:75-80 terra_initialize (device selection + stream) */
 tng_status tng_ctx_create(int device_ordinal, tng_ctx *out);
+/* Same, on the caller's HIP stream (a hipStream_t passed as void*), e.g. the
+ * stream of a Kokkos::HIP execution space instance, so library work orders
+ * with the caller's kernels without extra fences.  The stream is not
+ * destroyed with the context. */
+tng_status tng_ctx_create_with_stream(int device_ordinal, void *hip_stream, tng_ctx *out);
 tng_status tng_ctx_destroy(tng_ctx ctx);
 /* Kokkos::fence() */
 tng_status tng_ctx_synchronize(tng_ctx ctx);
@@ -145,6 +150,27 @@ tng_status tng_op_coefficient_updated(tng_op op);
  * (epsilon_divdiv_kerngen.hpp:577-792). */
 tng_status tng_op_apply(tng_op op, tng_vec src, tng_vec dst);
 tng_status tng_op_destroy(tng_op op);
+
+/* Stored local matrices (epsilon_divdiv_kerngen.hpp:503-575
+ * set_stored_matrix_mode(Full) + set_local_matrix / get_local_matrix, storage
+ * linalg/solvers/gca/local_matrix_storage.hpp:28-243).  Layout of the host
+ * arrays: (sd, x_cell, y_cell, r_cell, wedge, 18, 18) doubles, dof index
+ * node + 6 * dimension.  With stored matrices the operator applies the slow
+ * path (epsilon_divdiv_kerngen.hpp:1013-1242) on them; the mesh needs at
+ * least two radial layers.  get returns the stored matrices, or the Neumann
+ * matrices assembled from k (assemble_local_matrix, :2315-2392) when none
+ * are stored. */
+tng_status tng_op_get_local_matrices(tng_op op, double *host_out);
+tng_status tng_op_set_local_matrices(tng_op op, const double *host_in);
+tng_status tng_op_clear_local_matrices(tng_op op);
+/* Copy (and convert to dst's precision) the stored FP64 matrices of src. */
+tng_status tng_op_copy_local_matrices(tng_op dst, tng_op src);
+/* linalg/solvers/gca/gca.hpp:41-531 TwoGridGCA(fine_op, coarse_op) with
+ * constant interpolation on all elements (gca = 1): the coarse operator (FP64)
+ * receives, per coarse wedge, the sum of P^T A_fine P over its fine wedges.
+ * A_fine is the fine operator's stored matrices, or its Neumann local
+ * matrices assembled from its (FP64) k when it stores none. */
+tng_status tng_gca_assemble(tng_op fine, tng_op coarse);
 
 /* fe/wedge/operators/shell/divergence_kerngen.hpp:121-257 DivergenceKerngen
  * (fine velocity mesh -> coarse pressure mesh, one refinement apart): B u,

@@ -56,6 +56,9 @@ def lib():
         L.orc_prolongate.argtypes = [vp, vp, ip, ip, dp, dp]
         L.orc_restrict.argtypes = [vp, vp, ip, ip, dp, dp]
         L.orc_kmass_lumped_apply.argtypes = [vp, dp, dp, dp]
+        L.orc_epsdivdiv_apply_stored.argtypes = [vp, dp, ip, ip, ip, ip, ip, C.c_double, dp, dp]
+        L.orc_epsdivdiv_local_matrices.argtypes = [vp, dp, dp]
+        L.orc_gca.argtypes = [vp, vp, dp, dp, ip]
         L.orc_num_threads.restype = ip
         _lib = L
     return _lib
@@ -197,6 +200,32 @@ def kmass_lumped(mesh, k, x):
     k, x = _c(k), _c(x)
     out = np.zeros(mesh.shape)
     lib().orc_kmass_lumped_apply(mesh.h, _dp(k), _dp(x), _dp(out))
+    return out
+
+
+def local_matrices(mesh, k):
+    """Neumann 18x18 local matrices of every wedge, storage layout (sd, xc, yc, rc, wedge, 18, 18)."""
+    k = _c(k)
+    out = np.zeros((mesh.nsd, mesh.nx - 1, mesh.nx - 1, mesh.nr - 1, 2, 18, 18))
+    lib().orc_epsdivdiv_local_matrices(mesh.h, _dp(k), _dp(out))
+    return out
+
+
+def gca(fine, coarse, lmat_fine, exact_top=False):
+    """TwoGridGCA (linalg/solvers/gca/gca.hpp:41-531): coarse stored matrices from fine ones."""
+    lf = _c(lmat_fine)
+    out = np.zeros((coarse.nsd, coarse.nx - 1, coarse.nx - 1, coarse.nr - 1, 2, 18, 18))
+    lib().orc_gca(fine.h, coarse.h, _dp(lf), _dp(out), int(exact_top))
+    return out
+
+
+def epsdivdiv_stored(mesh, lmat, src, bc_cmb, bc_surf, diagonal=False, apply_mode=REPLACE, comm=COMMUNICATE,
+                     penalty_eps=1e-7, dst=None):
+    """Slow path on stored local matrices (epsilon_divdiv_kerngen.hpp:1013-1242, stored mode Full)."""
+    lm, src = _c(lmat), _c(src)
+    out = np.zeros((3,) + mesh.shape) if dst is None else np.array(dst, dtype=np.float64, order="C", copy=True)
+    lib().orc_epsdivdiv_apply_stored(mesh.h, _dp(lm), bc_cmb, bc_surf, int(diagonal), apply_mode, comm, penalty_eps,
+                                     _dp(src), _dp(out))
     return out
 
 

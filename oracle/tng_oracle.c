@@ -461,6 +461,7 @@ typedef struct {
     const orc_mesh *m;
     const double *k;
     int bc_cmb, bc_surf;
+    const double *lmat; /* stored local matrices, or NULL: assembled on the fly */
 } op_ctx;
 
 static int bc_of(const op_ctx *o, int sbf) { return sbf == SBF_CMB ? o->bc_cmb : o->bc_surf; }
@@ -519,7 +520,7 @@ static void assemble_local_matrix(const op_ctx *o, int sd, int xc, int yc, int r
 void orc_epsdivdiv_local_matrix(const orc_mesh *m, const double *k, int sd, int xc, int yc, int rc, int w,
                                 double *A_out)
 {
-    op_ctx o = {m, k, 0, 0};
+    op_ctx o = {m, k, 0, 0, NULL};
     double A[18][18];
     memset(A, 0, sizeof(A));
     assemble_local_matrix(&o, sd, xc, yc, rc, w, A);
@@ -537,6 +538,14 @@ static void matmul18(const double A[18][18], const double B[18][18], double C[18
 }
 
 /* epsilon_divdiv_kerngen.hpp:1013-1242 operator_slow_path, one hex cell */
+/* stored local matrices (OperatorStoredMatrixMode::Full): (sd, xc, yc, rc, wedge, 18, 18),
+ * local_matrix_storage.hpp:103-108 */
+static const double *stored_matrix(const orc_mesh *m, const double *lmat, int sd, int xc, int yc, int rc, int w)
+{
+    const int nc = m->nx - 1, ncr = m->nr - 1;
+    return lmat + (((((size_t)sd * nc + xc) * nc + yc) * ncr + rc) * 2 + w) * 324;
+}
+
 static void slow_cell(const op_ctx *o, int sd, int xc, int yc, int rc, int diagonal, const double *src, double *dst)
 {
     const orc_mesh *m = o->m;
@@ -545,8 +554,13 @@ static void slow_cell(const op_ctx *o, int sd, int xc, int yc, int rc, int diago
     const int at_surface = has_flag(m->bmask[lin(m, sd, xc, yc, rc + 1)], SBF_SURFACE);
     double A[2][18][18];
     memset(A, 0, sizeof(A));
-    assemble_local_matrix(o, sd, xc, yc, rc, 0, A[0]);
-    assemble_local_matrix(o, sd, xc, yc, rc, 1, A[1]);
+    if (o->lmat) {
+        memcpy(A[0], stored_matrix(m, o->lmat, sd, xc, yc, rc, 0), sizeof(A[0]));
+        memcpy(A[1], stored_matrix(m, o->lmat, sd, xc, yc, rc, 1), sizeof(A[1]));
+    } else {
+        assemble_local_matrix(o, sd, xc, yc, rc, 0, A[0]);
+        assemble_local_matrix(o, sd, xc, yc, rc, 1, A[1]);
+    }
     double s[2][18];
     int64_t idx[2][6];
     for (int w = 0; w < 2; ++w)
@@ -730,7 +744,7 @@ void orc_epsdivdiv_apply_slow(const orc_mesh *m, const double *k, int bc_cmb, in
                               int apply_mode, int comm, double penalty_eps, const double *src, double *dst)
 {
     const int64_t N = m->nalloc;
-    op_ctx o = {m, k, bc_cmb, bc_surf};
+    op_ctx o = {m, k, bc_cmb, bc_surf, NULL};
     if (apply_mode == 0) memset(dst, 0, sizeof(double) * 3 * N);
     for (int sd = 0; sd < m->nsd; ++sd)
         for (int xc = 0; xc < m->nx - 1; ++xc)
@@ -743,6 +757,131 @@ void orc_epsdivdiv_apply_slow(const orc_mesh *m, const double *k, int bc_cmb, in
         free(nm);
     }
     if (comm) orc_halo_sum(m, dst, 3);
+}
+
+/* Slow path on stored local matrices (OperatorStoredMatrixMode::Full):
+ * epsilon_divdiv_kerngen.hpp:1013-1242 with A read from the storage. */
+void orc_epsdivdiv_apply_stored(const orc_mesh *m, const double *lmat, int bc_cmb, int bc_surf, int diagonal,
+                                int apply_mode, int comm, double penalty_eps, const double *src, double *dst)
+{
+    const int64_t N = m->nalloc;
+    op_ctx o = {m, NULL, bc_cmb, bc_surf, lmat};
+    if (apply_mode == 0) memset(dst, 0, sizeof(double) * 3 * N);
+    for (int sd = 0; sd < m->nsd; ++sd)
+        for (int xc = 0; xc < m->nx - 1; ++xc)
+            for (int yc = 0; yc < m->nx - 1; ++yc)
+                for (int rc = 0; rc < m->nr - 1; ++rc) slow_cell(&o, sd, xc, yc, rc, diagonal, src, dst);
+    if (bc_cmb == BC_FREESLIP && bc_surf == BC_FREESLIP && !diagonal) {
+        double *nm = (double *)malloc(sizeof(double) * 9 * N);
+        penalty_modes(m, nm);
+        penalty_apply(m, nm, penalty_eps, src, dst);
+        free(nm);
+    }
+    if (comm) orc_halo_sum(m, dst, 3);
+}
+
+/* Neumann local matrices of every wedge (get_local_matrix of an operator
+ * without storage -> assemble_local_matrix), in the storage layout. */
+void orc_epsdivdiv_local_matrices(const orc_mesh *m, const double *k, double *out)
+{
+    op_ctx o = {m, k, 0, 0, NULL};
+    const int nc = m->nx - 1, ncr = m->nr - 1;
+    for (int sd = 0; sd < m->nsd; ++sd)
+        for (int xc = 0; xc < nc; ++xc)
+            for (int yc = 0; yc < nc; ++yc)
+                for (int rc = 0; rc < ncr; ++rc)
+                    for (int w = 0; w < 2; ++w) {
+                        double A[18][18];
+                        memset(A, 0, sizeof(A));
+                        assemble_local_matrix(&o, sd, xc, yc, rc, w, A);
+                        memcpy((double *)stored_matrix(m, out, sd, xc, yc, rc, w), A, sizeof(A));
+                    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* TwoGridGCA (linalg/solvers/gca/gca.hpp:41-531), constant interpolation,   */
+/* all elements (gca = 1), no boundary treatment (the block is disabled,     */
+/* gca.hpp:404).  Per coarse cell: sum over its 8 fine hexes x 2 wedges of   */
+/* P^T A_fine P with P (6x6 per dimension) the constant-weight interpolation */
+/* from the parent coarse wedge's nodes.                                     */
+/* exact_top != 0 places the weights of fine nodes on the top face of an     */
+/* interior coarse cell on the coarse top nodes; 0 keeps the reference's     */
+/* r_idx_coarse_bot = r_fine / 2 (gca.hpp:206-208), which puts them on the   */
+/* bottom nodes of the coarse wedge.                                         */
+/* ------------------------------------------------------------------------- */
+static double transfer_weight(int xf, int yf, int rf, int xc, int yc, int rc);
+
+void orc_gca(const orc_mesh *mf, const orc_mesh *mc, const double *lf, double *lc, int exact_top)
+{
+    static const int SH[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+    const int ncc = mc->nx - 1, ncrc = mc->nr - 1, nrf = mf->nr;
+    for (int sd = 0; sd < mc->nsd; ++sd)
+        for (int X = 0; X < ncc; ++X)
+            for (int Y = 0; Y < ncc; ++Y)
+                for (int Rr = 0; Rr < ncrc; ++Rr) {
+                    double Ac[2][18][18];
+                    memset(Ac, 0, sizeof(Ac));
+                    for (int h = 0; h < 8; ++h) {
+                        const int fx = 2 * X + SH[h][0], fy = 2 * Y + SH[h][1], fr = 2 * Rr + SH[h][2];
+                        for (int w = 0; w < 2; ++w) {
+                            double P[6][6];
+                            memset(P, 0, sizeof(P));
+                            for (int n = 0; n < 6; ++n) {
+                                const int xf = fx + WOFF[w][n][0], yf = fy + WOFF[w][n][1], rf = fr + WOFF[w][n][2];
+                                if (xf % 2 == 0 && yf % 2 == 0 && rf % 2 == 0) {
+                                    P[n][n] = 1.0;
+                                    continue;
+                                }
+                                int rb = rf < nrf - 1 ? rf / 2 : rf / 2 - 1;
+                                if (exact_top && rf == 2 * Rr + 2) rb = Rr;
+                                if (xf % 2 == 0 && yf % 2 == 0) {
+                                    P[n][n % 3] = 0.5;
+                                    P[n][n % 3 + 3] = 0.5;
+                                    continue;
+                                }
+                                int x0, y0, x1, y1, l[4];
+                                if (xf % 2 == 0) {
+                                    x0 = xf / 2; x1 = xf / 2; y0 = yf / 2; y1 = yf / 2 + 1;
+                                    l[0] = 0; l[1] = 2; l[2] = 3; l[3] = 5;
+                                } else if (yf % 2 == 0) {
+                                    x0 = xf / 2; x1 = xf / 2 + 1; y0 = yf / 2; y1 = yf / 2;
+                                    l[0] = 0; l[1] = 1; l[2] = 3; l[3] = 4;
+                                } else {
+                                    x0 = xf / 2 + 1; x1 = xf / 2; y0 = yf / 2; y1 = yf / 2 + 1;
+                                    l[0] = 1; l[1] = 2; l[2] = 4; l[3] = 5;
+                                }
+                                P[n][l[0]] = transfer_weight(xf, yf, rf, x0, y0, rb);
+                                P[n][l[1]] = transfer_weight(xf, yf, rf, x1, y1, rb);
+                                P[n][l[2]] = transfer_weight(xf, yf, rf, x0, y0, rb + 1);
+                                P[n][l[3]] = transfer_weight(xf, yf, rf, x1, y1, rb + 1);
+                            }
+                            const double *Af = stored_matrix(mf, lf, sd, fx, fy, fr, w);
+                            /* P^T A P per dimension block: (di, i), (dj, j) */
+                            const int tgt = ((w == 0 && (h == 0 || h == 1 || h == 2 || h == 4 || h == 5 || h == 6)) ||
+                                             (w == 1 && (h == 0 || h == 4)))
+                                                ? 0
+                                                : 1;
+                            for (int di = 0; di < 3; ++di)
+                                for (int dj = 0; dj < 3; ++dj) {
+                                    double AP[6][6];
+                                    for (int a = 0; a < 6; ++a)
+                                        for (int j = 0; j < 6; ++j) {
+                                            double acc = 0.0;
+                                            for (int b = 0; b < 6; ++b) acc += Af[(a + 6 * di) * 18 + b + 6 * dj] * P[b][j];
+                                            AP[a][j] = acc;
+                                        }
+                                    for (int i = 0; i < 6; ++i)
+                                        for (int j = 0; j < 6; ++j) {
+                                            double acc = 0.0;
+                                            for (int a = 0; a < 6; ++a) acc += P[a][i] * AP[a][j];
+                                            Ac[tgt][i + 6 * di][j + 6 * dj] += acc;
+                                        }
+                                }
+                        }
+                    }
+                    memcpy((double *)stored_matrix(mc, lc, sd, X, Y, Rr, 0), Ac[0], sizeof(Ac[0]));
+                    memcpy((double *)stored_matrix(mc, lc, sd, X, Y, Rr, 1), Ac[1], sizeof(Ac[1]));
+                }
 }
 
 /* ------------------------------------------------------------------------- */
@@ -923,7 +1062,7 @@ void orc_epsdivdiv_apply_fast(const orc_mesh *m, const double *k, int bc_cmb, in
                               const double *penalty_modes_in)
 {
     const int64_t N = m->nalloc;
-    op_ctx o = {m, k, bc_cmb, bc_surf};
+    op_ctx o = {m, k, bc_cmb, bc_surf, NULL};
     if (apply_mode == 0) {
 #pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < 3 * N; ++i) dst[i] = 0.0;

@@ -70,6 +70,34 @@ tng_status tng_ctx_create(int device_ordinal, tng_ctx *out)
     return TNG_OK;
 }
 
+tng_status tng_ctx_create_with_stream(int device_ordinal, void *hip_stream, tng_ctx *out)
+{
+    if (!out || !hip_stream) return TNG_INVALID_ARGUMENT;
+    *out = nullptr;
+    auto *ctx = new (std::nothrow) tng_context();
+    if (!ctx) return TNG_OUT_OF_MEMORY;
+    tng_status st = guarded(nullptr, [&] {
+        int n = 0;
+        TNG_HIP_CHECK(hipGetDeviceCount(&n));
+        TNG_REQUIRE(device_ordinal >= 0 && device_ordinal < n, "device ordinal out of range");
+        ctx->device = device_ordinal;
+        set_device(ctx);
+        ctx->stream = static_cast<hipStream_t>(hip_stream);
+        ctx->owns_stream = false;
+        ctx->red_partials.alloc(4096 * 3 * sizeof(double));
+        ctx->red_counter.alloc(64);
+        TNG_HIP_CHECK(hipMemsetAsync(ctx->red_counter.p, 0, 64, ctx->stream));
+        ctx->red_result.alloc(4096);
+        TNG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+    if (st != TNG_OK) {
+        delete ctx;
+        return st;
+    }
+    *out = ctx;
+    return TNG_OK;
+}
+
 tng_status tng_ctx_destroy(tng_ctx ctx)
 {
     if (!ctx) return TNG_INVALID_ARGUMENT;
@@ -77,7 +105,7 @@ tng_status tng_ctx_destroy(tng_ctx ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &e : ctx->prof_events)
         if (e) (void)hipEventDestroy(e);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream && ctx->owns_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return TNG_OK;
 }
@@ -128,7 +156,7 @@ tng_status tng_mesh_create_uniform(tng_ctx ctx, int lateral_level, int radial_le
 
 tng_status tng_mesh_destroy(tng_mesh mesh)
 {
-    if (!mesh) return TNG_INVALID_ARGUMENT;
+    if (!mesh || mesh->solver_owned) return TNG_INVALID_ARGUMENT;
     if (mesh->ctx) {
         (void)hipSetDevice(mesh->ctx->device);
         (void)hipStreamSynchronize(mesh->ctx->stream);
@@ -547,6 +575,89 @@ tng_status tng_vec_has_nan_or_inf(tng_vec x, int *out)
     return guarded(x->mesh->ctx, [&] {
         set_device(x->mesh->ctx);
         *out = reduce_to_host(REDUCE_NONFINITE, *x, nullptr, x->mesh->ctx->stream) > 0 ? 1 : 0;
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// stored local matrices and GCA
+// ---------------------------------------------------------------------------
+extern "C" {
+
+tng_status tng_op_get_local_matrices(tng_op op, double *host_out)
+{
+    if (!op || !host_out || op->kind != OP_EPSDIVDIV) return TNG_INVALID_ARGUMENT;
+    return guarded(op->mesh->ctx, [&] {
+        set_device(op->mesh->ctx);
+        hipStream_t s = op->mesh->ctx->stream;
+        const size_t n = stored_matrix_count(*op->mesh) * 324;
+        if (op->stored && op->lmat_dtype == TNG_F64) {
+            TNG_HIP_CHECK(hipMemcpyAsync(host_out, op->d_lmat.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        } else if (op->stored) {
+            std::vector<float> tmp(n);
+            TNG_HIP_CHECK(hipMemcpyAsync(tmp.data(), op->d_lmat.p, n * sizeof(float), hipMemcpyDeviceToHost, s));
+            TNG_HIP_CHECK(hipStreamSynchronize(s));
+            for (size_t i = 0; i < n; ++i) host_out[i] = tmp[i];
+        } else {
+            DevBuf tmp;
+            epsdivdiv_local_matrices(*op, tmp, s);
+            TNG_HIP_CHECK(hipMemcpyAsync(host_out, tmp.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+            TNG_HIP_CHECK(hipStreamSynchronize(s));
+        }
+        TNG_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+tng_status tng_op_set_local_matrices(tng_op op, const double *host_in)
+{
+    if (!op || !host_in || op->kind != OP_EPSDIVDIV) return TNG_INVALID_ARGUMENT;
+    return guarded(op->mesh->ctx, [&] {
+        set_device(op->mesh->ctx);
+        hipStream_t s = op->mesh->ctx->stream;
+        TNG_REQUIRE(op->mesh->nr >= 3, "stored matrices need at least two radial layers");
+        const size_t n = stored_matrix_count(*op->mesh) * 324;
+        if (op->k->dtype == TNG_F64) {
+            if (op->d_lmat.bytes != n * sizeof(double)) op->d_lmat.alloc(n * sizeof(double));
+            TNG_HIP_CHECK(hipMemcpyAsync(op->d_lmat.p, host_in, n * sizeof(double), hipMemcpyHostToDevice, s));
+            TNG_HIP_CHECK(hipStreamSynchronize(s));
+        } else {
+            std::vector<float> tmp(host_in, host_in + n);
+            if (op->d_lmat.bytes != n * sizeof(float)) op->d_lmat.alloc(n * sizeof(float));
+            TNG_HIP_CHECK(hipMemcpyAsync(op->d_lmat.p, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice, s));
+            TNG_HIP_CHECK(hipStreamSynchronize(s));
+        }
+        op->stored = true;
+        op->lmat_dtype = op->k->dtype;
+    });
+}
+
+tng_status tng_op_clear_local_matrices(tng_op op)
+{
+    if (!op || op->kind != OP_EPSDIVDIV) return TNG_INVALID_ARGUMENT;
+    return guarded(op->mesh->ctx, [&] {
+        TNG_HIP_CHECK(hipStreamSynchronize(op->mesh->ctx->stream));
+        op->stored = false;
+        op->d_lmat.release();
+    });
+}
+
+tng_status tng_gca_assemble(tng_op fine, tng_op coarse)
+{
+    if (!fine || !coarse || fine->kind != OP_EPSDIVDIV || coarse->kind != OP_EPSDIVDIV) return TNG_INVALID_ARGUMENT;
+    return guarded(coarse->mesh->ctx, [&] {
+        set_device(coarse->mesh->ctx);
+        TNG_REQUIRE(coarse->k->dtype == TNG_F64, "GCA assembles FP64 matrices: the coarse operator must be FP64");
+        gca_assemble(*fine, *coarse, coarse->mesh->ctx->stream);
+    });
+}
+
+tng_status tng_op_copy_local_matrices(tng_op dst, tng_op src)
+{
+    if (!dst || !src || dst->kind != OP_EPSDIVDIV || src->kind != OP_EPSDIVDIV) return TNG_INVALID_ARGUMENT;
+    return guarded(dst->mesh->ctx, [&] {
+        set_device(dst->mesh->ctx);
+        copy_stored_matrices(*dst, *src, dst->k->dtype, dst->mesh->ctx->stream);
     });
 }
 

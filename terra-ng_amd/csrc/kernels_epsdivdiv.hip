@@ -150,17 +150,22 @@ __device__ __forceinline__ void process(const RawCol<T, K> &r, const T n[3], con
 // One wedge: corners (P0, P1, P2) in wedge order.  Returns the lateral parts
 // A0 (corner P1), A1 (corner P2), -(A0+A1) (corner P0) and the radial part Cc
 // (subtracted at the bottom node, added at the top node of every corner).
+//
+// The trial gradient is formed without its lateral factor hs = 1/(2 r_mid):
+// G' = grad u / hs = (s1-s0) M0^T + (s2-s0) M1^T + q (d0+d1+d2) M2^T with
+// q = (1/(6h)) / hs, and hs is folded into the stress scale
+// (cvol = hs^2 * r_mid^2 h / 6 per lane).
 template <typename T>
 __device__ __forceinline__ void wedge_stress(const T *g /*M0,M1,M2,detL*/, const T su0[3], const T su1[3],
                                              const T su2[3], const T du0[3], const T du1[3], const T du2[3],
-                                             T kksum, T hs, T t, T q, T cvol, T A0[3], T A1[3], T Cc[3])
+                                             T kksum, T q, T cvol, T A0[3], T A1[3], T Cc[3])
 {
     T e1[3], e2[3], D[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        e1[d] = hs * (su1[d] - su0[d]);
-        e2[d] = hs * (su2[d] - su0[d]);
-        D[d] = t * (du0[d] + du1[d] + du2[d]);
+        e1[d] = su1[d] - su0[d];
+        e2[d] = su2[d] - su0[d];
+        D[d] = q * (du0[d] + du1[d] + du2[d]);
     }
     const T *M0 = g, *M1 = g + 3, *M2 = g + 6;
     T G[3][3];
@@ -169,7 +174,7 @@ __device__ __forceinline__ void wedge_stress(const T *g /*M0,M1,M2,detL*/, const
 #pragma unroll
         for (int e = 0; e < 3; ++e) G[d][e] = e1[d] * M0[e] + e2[d] * M1[e] + D[d] * M2[e];
     const T div = G[0][0] + G[1][1] + G[2][2];
-    // f = hs * k_eval * |det J| = hs * (kksum / 6) * |det L| * r_mid^2 h
+    // f = hs^2 * k_eval * |det J| = hs^2 * (kksum / 6) * |det L| * r_mid^2 h
     const T f = cvol * kksum * g[9];
     const T f2 = T(2) * f;
     const T fd = T(-2.0 / 3.0) * f * div;
@@ -238,16 +243,15 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
     lbc.z_hi = (has_surf && ovf_lane && a.bc_surf == TNG_DIRICHLET) ? T(0) : T(1);
     lbc.f_hi = (has_surf && ovf_lane && a.bc_surf == TNG_FREESLIP) ? T(1) : T(0);
 
-    T hs[K], tt[K], qq[K], cv[K];
+    T qq[K], cv[K];
 #pragma unroll
     for (int p = 0; p < K; ++p) {
         const int l = lvl0 + p;
         const T r0 = a.radii[(int64_t)sd * nr + l], r1 = a.radii[(int64_t)sd * nr + l + 1];
         const T rmid = T(0.5) * (r0 + r1), h = T(0.5) * (r1 - r0);
-        hs[p] = T(0.5) / rmid;
-        tt[p] = T(1) / (T(6) * h);
-        qq[p] = tt[p] / hs[p];
-        cv[p] = lane_has ? hs[p] * rmid * rmid * h * T(1.0 / 6.0) : T(0);
+        const T hs = T(0.5) / rmid, tt = T(1) / (T(6) * h);
+        qq[p] = tt / hs;
+        cv[p] = lane_has ? hs * hs * rmid * rmid * h * T(1.0 / 6.0) : T(0);
     }
 
     auto colbase = [&](int Xq, int Yq) -> int64_t {
@@ -308,12 +312,26 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
         const bool cell_ok = xc >= 0 && xc < nc && yc >= 0 && yc < nc;
         {
             const int xcc = min(max(xc, 0), nc - 1), ycc = min(max(yc, 0), nc - 1);
-            const T *gw = a.latgeom + ((((int64_t)sd * nc + xcc) * nc + ycc) * 2) * 10;
             T g0[10], g1[10];
+            if constexpr (R == 64) {
+                // one lateral cell per wave: the geometry is wave-uniform, so it
+                // is read through the scalar cache into SGPRs (the table is
+                // never written by a kernel)
+                using CT = const T __attribute__((address_space(4)));
+                const int gi = __builtin_amdgcn_readfirstlane(((sd * nc + xcc) * nc + ycc) * 20);
+                const CT *gw = (const CT *)a.latgeom + gi;
 #pragma unroll
-            for (int i = 0; i < 10; ++i) {
-                g0[i] = gw[i];
-                g1[i] = gw[10 + i];
+                for (int i = 0; i < 10; ++i) {
+                    g0[i] = gw[i];
+                    g1[i] = gw[10 + i];
+                }
+            } else {
+                const T *gw = a.latgeom + ((((int64_t)sd * nc + xcc) * nc + ycc) * 2) * 10;
+#pragma unroll
+                for (int i = 0; i < 10; ++i) {
+                    g0[i] = gw[i];
+                    g1[i] = gw[10 + i];
+                }
             }
 #pragma unroll
             for (int p = 0; p < K; ++p) {
@@ -321,7 +339,7 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
                 T A0[3], A1[3], Cc[3];
                 // wedge 0: (c00, c10, c01)
                 wedge_stress<T>(g0, B0.su[p], B1.su[p], T0.su[p], B0.du[p], B1.du[p], T0.du[p],
-                                B0.kk[p] + B1.kk[p] + T0.kk[p], hs[p], tt[p], qq[p], cvp, A0, A1, Cc);
+                                B0.kk[p] + B1.kk[p] + T0.kk[p], qq[p], cvp, A0, A1, Cc);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const T x00 = -(A0[c] + A1[c]);
@@ -334,7 +352,7 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
                 }
                 // wedge 1: (c11, c01, c10)
                 wedge_stress<T>(g1, T1.su[p], T0.su[p], B1.su[p], T1.du[p], T0.du[p], B1.du[p],
-                                T1.kk[p] + T0.kk[p] + B1.kk[p], hs[p], tt[p], qq[p], cvp, A0, A1, Cc);
+                                T1.kk[p] + T0.kk[p] + B1.kk[p], qq[p], cvp, A0, A1, Cc);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const T x11 = -(A0[c] + A1[c]);
@@ -802,27 +820,35 @@ template <typename T> void apply_impl(tng_operator &op, const tng_vector &src, t
     const bool add = op.apply_mode == TNG_ADD;
     hipEvent_t *ev = ctx.profiling ? ctx.prof_next_set() : nullptr;
     if (ev) TNG_HIP_CHECK(hipEventRecord(ev[0], s));
-    if (op.diagonal) {
+    auto penalty_moment = [&]() {
+        PenArgs<T> p{};
+        const MeshPtrs<T> mp = mesh_ptrs<T>(m);
+        p.src0 = (const T *)src.comp(0); p.src1 = (const T *)src.comp(1); p.src2 = (const T *)src.comp(2);
+        p.coords = mp.coords; p.radii = mp.radii; p.omask = m.d_omask.as<uint8_t>();
+        p.partials = ctx.red_partials.as<double>();
+        p.counter = ctx.red_counter.as<unsigned int>();
+        p.out_b = op.d_penalty_b.as<double>();
+        p.eps = op.penalty_eps;
+        for (int i = 0; i < 9; ++i) p.ginv[i] = m.rot_gram_inv[i];
+        p.nsd = m.nsd; p.nx = m.nx; p.nr = m.nr; p.n = m.nalloc;
+        hipLaunchKernelGGL(penalty_moment_kernel<T>, dim3(kPenBlocks), dim3(256), 0, s, p);
+        TNG_HIP_CHECK(hipGetLastError());
+    };
+    if (op.stored) {
+        // slow path on stored matrices (Dirichlet/free-slip handled per row)
+        const bool pen = op.penalty_active && !op.diagonal;
+        if (pen) penalty_moment();
+        if (ev) TNG_HIP_CHECK(hipEventRecord(ev[1], s));
+        stored_apply<T>(op, src, dst, pen ? op.d_penalty_b.as<double>() : nullptr, add, s);
+        if (ev) TNG_HIP_CHECK(hipEventRecord(ev[2], s));
+    } else if (op.diagonal) {
         if (ev) TNG_HIP_CHECK(hipEventRecord(ev[1], s));
         run_diag_kernel<T>(op, &src, &dst, 0, add, s);
         if (ev) TNG_HIP_CHECK(hipEventRecord(ev[2], s));
     } else {
         if (!op.btab_valid) epsdivdiv_update_boundary_table(op, s);
         const bool pen = op.penalty_active;
-        if (pen) {
-            PenArgs<T> p{};
-            const MeshPtrs<T> mp = mesh_ptrs<T>(m);
-            p.src0 = (const T *)src.comp(0); p.src1 = (const T *)src.comp(1); p.src2 = (const T *)src.comp(2);
-            p.coords = mp.coords; p.radii = mp.radii; p.omask = m.d_omask.as<uint8_t>();
-            p.partials = ctx.red_partials.as<double>();
-            p.counter = ctx.red_counter.as<unsigned int>();
-            p.out_b = op.d_penalty_b.as<double>();
-            p.eps = op.penalty_eps;
-            for (int i = 0; i < 9; ++i) p.ginv[i] = m.rot_gram_inv[i];
-            p.nsd = m.nsd; p.nx = m.nx; p.nr = m.nr; p.n = m.nalloc;
-            hipLaunchKernelGGL(penalty_moment_kernel<T>, dim3(kPenBlocks), dim3(256), 0, s, p);
-            TNG_HIP_CHECK(hipGetLastError());
-        }
+        if (pen) penalty_moment();
         ApplyArgs<T> a{};
         const MeshPtrs<T> mp = mesh_ptrs<T>(m);
         a.src0 = (const T *)src.comp(0); a.src1 = (const T *)src.comp(1); a.src2 = (const T *)src.comp(2);

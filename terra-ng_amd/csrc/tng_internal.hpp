@@ -66,6 +66,7 @@ struct DevBuf {
 struct tng_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    bool owns_stream = true;  // false: the caller's stream (tng_ctx_create_with_stream)
     std::string last_error;
     bool profiling = false;
     // Scratch for two-stage reductions: partial sums + a ticket counter.
@@ -91,6 +92,7 @@ struct tng_domain {
     int64_t nalloc = 0;         // nlat * nr
     int64_t num_owned = 0, num_interface = 0;
     double r_min = 0, r_max = 0;
+    bool solver_owned = false;  // a Stokes solver's level mesh (not destroyable through the ABI)
 
     std::vector<int> sd_info;      // (diamond, sx, sy, sr) per local subdomain
     std::vector<double> coords;    // (sd, x, y, 3)
@@ -157,6 +159,10 @@ struct tng_operator {
     tng::DevBuf d_btab;
     bool btab_valid = false;
     tng::DevBuf d_penalty_b;  // 3 doubles: eps * G^{-1} m, written on device
+    // stored local matrices (OperatorStoredMatrixMode::Full): (sd, xc, yc, rc, wedge, 18, 18)
+    bool stored = false;
+    tng_dtype lmat_dtype = TNG_F64;
+    tng::DevBuf d_lmat;
 };
 
 namespace tng {
@@ -191,6 +197,14 @@ tng_vector *new_vector(tng_domain *mesh, tng_dtype dtype, int ncomp);
 tng_operator *new_epsdivdiv(tng_domain *mesh, tng_vector *k, int bc_cmb, int bc_surf, bool diagonal);
 tng_operator *new_two_level(OpKind kind, tng_domain *fine, tng_domain *coarse, int bc_cmb, int bc_surf);
 tng_operator *new_kmass(tng_domain *mesh, tng_vector *k);
+
+// stored local matrices and GCA (kernels_gca.hip)
+size_t stored_matrix_count(const tng_domain &m);  // wedges = matrices
+void epsdivdiv_local_matrices(const tng_operator &op, DevBuf &out, hipStream_t s);  // FP64, from k
+void gca_assemble(const tng_operator &fine, tng_operator &coarse, hipStream_t s);
+void copy_stored_matrices(tng_operator &dst, const tng_operator &src, tng_dtype dtype, hipStream_t s);
+template <typename T>
+void stored_apply(tng_operator &op, const tng_vector &src, tng_vector &dst, const double *pen, bool add, hipStream_t s);
 
 // solver support kernels
 void launch_randomize(tng_vector &v, uint64_t seed, hipStream_t s);

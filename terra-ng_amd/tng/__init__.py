@@ -53,6 +53,7 @@ EXPORTS = [
     ("tng_version", C.c_char_p, []),
     ("tng_last_error", C.c_char_p, [_vp]),
     ("tng_ctx_create", _i, [_i, _pv]),
+    ("tng_ctx_create_with_stream", _i, [_i, _vp, _pv]),
     ("tng_ctx_destroy", _i, [_vp]),
     ("tng_ctx_synchronize", _i, [_vp]),
     ("tng_ctx_set_profiling", _i, [_vp, _i]),
@@ -91,6 +92,11 @@ EXPORTS = [
     ("tng_vec_dot", _i, [_vp, _vp, C.POINTER(C.c_double)]),
     ("tng_vec_max_abs", _i, [_vp, C.POINTER(C.c_double)]),
     ("tng_vec_has_nan_or_inf", _i, [_vp, C.POINTER(C.c_int)]),
+    ("tng_op_get_local_matrices", _i, [_vp, _vp]),
+    ("tng_op_set_local_matrices", _i, [_vp, _vp]),
+    ("tng_op_clear_local_matrices", _i, [_vp]),
+    ("tng_op_copy_local_matrices", _i, [_vp, _vp]),
+    ("tng_gca_assemble", _i, [_vp, _vp]),
     ("tng_pcg_solve", _i, [_vp, _vp, _vp, _i, _d, _d, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     ("tng_power_iteration", _i, [_vp, _vp, _i, C.c_ulonglong, C.POINTER(C.c_double)]),
     ("tng_chebyshev_smooth", _i, [_vp, _vp, _i, _i, _d, _vp, _vp]),
@@ -359,6 +365,31 @@ class EpsilonDivDivKerngen(_Operator):
 
     def coefficient_updated(self):
         self.domain.ctx.check(load_library().tng_op_coefficient_updated(self.h))
+
+    def _lmat_shape(self):
+        d = self.domain
+        return (d.nsd, d.nx - 1, d.nx - 1, d.nr - 1, 2, 18, 18)
+
+    def get_local_matrices(self):
+        """Stored matrices, or the Neumann local matrices assembled from k."""
+        out = np.empty(self._lmat_shape())
+        self.domain.ctx.check(load_library().tng_op_get_local_matrices(self.h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def set_local_matrices(self, lmat):
+        a = np.ascontiguousarray(lmat, dtype=np.float64).reshape(self._lmat_shape())
+        self.domain.ctx.check(load_library().tng_op_set_local_matrices(self.h, a.ctypes.data_as(C.c_void_p)))
+
+    def clear_local_matrices(self):
+        self.domain.ctx.check(load_library().tng_op_clear_local_matrices(self.h))
+
+    def copy_local_matrices(self, other):
+        self.domain.ctx.check(load_library().tng_op_copy_local_matrices(self.h, other.h))
+
+
+def gca_assemble(fine_op, coarse_op):
+    """TwoGridGCA (linalg/solvers/gca/gca.hpp): coarse stored matrices from the fine operator."""
+    coarse_op.domain.ctx.check(load_library().tng_gca_assemble(fine_op.h, coarse_op.h))
 
 
 class DivergenceKerngen(_Operator):

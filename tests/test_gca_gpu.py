@@ -1,0 +1,146 @@
+"""Stored local matrices, the stored-matrix slow path and TwoGridGCA on the GPU
+against the oracle (epsilon_divdiv_kerngen.hpp:503-575, 1013-1242, 2315-2392;
+linalg/solvers/gca/gca.hpp:41-531).  FP64 bar 1e-12 relative (matrices
+1e-13), FP32 operators 2e-6."""
+import numpy as np
+import pytest
+
+import oracle as O
+import tng
+from conftest import consistent, rel_err
+
+pytestmark = pytest.mark.gpu
+
+N, D, F = tng.NEUMANN, tng.DIRICHLET, tng.FREESLIP
+_cache = {}
+
+
+def setup(lat, rad=None, ls=0, rs=0):
+    rad = lat if rad is None else rad
+    key = (lat, rad, ls, rs)
+    if key not in _cache:
+        _cache[key] = (tng.DistributedDomain.create_uniform(lat, rad, 0.55, 1.0, ls, rs),
+                       O.Mesh(lat, rad, 0.55, 1.0, ls, rs))
+    return _cache[key]
+
+
+def field(om, ncomp, seed):
+    r = np.random.default_rng(seed)
+    return consistent(om, r.uniform(-1, 1, ((ncomp,) if ncomp > 1 else ()) + om.shape))
+
+
+def vec(dom, ncomp, data=None, dtype=tng.F64):
+    v = (tng.VectorQ1Vec if ncomp == 3 else tng.VectorQ1Scalar)(dom, dtype)
+    if data is not None:
+        v.upload(data)
+    return v
+
+
+def op_on(dom, k, bcs, dtype=tng.F64):
+    kv = vec(dom, 1, k, dtype)
+    return tng.EpsilonDivDivKerngen(dom, kv, bcs), kv
+
+
+@pytest.mark.parametrize("lat,ls,rs", [(2, 0, 0), (3, 1, 1)])
+def test_local_matrices_match_oracle(lat, ls, rs, gpu_ctx):
+    dom, om = setup(lat, lat, ls, rs)
+    k = np.exp(field(om, 1, 1))
+    op, kv = op_on(dom, k, (N, N))
+    got = op.get_local_matrices()
+    ref = O.local_matrices(om, k)
+    assert rel_err(got, ref) < 1e-13
+
+
+@pytest.mark.parametrize("bcs", [(N, N), (D, D), (F, F), (D, F), (F, N)])
+@pytest.mark.parametrize("diagonal", [False, True])
+def test_stored_apply_matches_slow_path(bcs, diagonal, gpu_ctx):
+    dom, om = setup(3, 3, 1, 0)
+    k = np.exp(field(om, 1, 2))
+    u = field(om, 3, 3)
+    lm = O.local_matrices(om, k) * (1.0 + 0.1 * np.random.default_rng(4).uniform(size=(1,) * 5 + (18, 18)))
+    op, kv = op_on(dom, k, bcs, tng.F64)
+    op.set_local_matrices(lm)
+    op.set_diagonal(diagonal)
+    out = vec(dom, 3)
+    op.apply(vec(dom, 3, u), out)
+    ref = O.epsdivdiv_stored(om, lm, u, *bcs, diagonal=diagonal)
+    assert rel_err(out.download(), ref) < 1e-12
+    # FP32 operator with converted matrices
+    op32, kv32 = op_on(dom, k, bcs, tng.F32)
+    op32.copy_local_matrices(op)
+    op32.set_diagonal(diagonal)
+    out32 = vec(dom, 3, dtype=tng.F32)
+    op32.apply(vec(dom, 3, u, tng.F32), out32)
+    assert rel_err(out32.download(), ref) < 2e-6
+
+
+def test_stored_apply_modes(gpu_ctx):
+    dom, om = setup(3, 3, 0, 1)
+    k = np.exp(field(om, 1, 5))
+    u, base = field(om, 3, 6), field(om, 3, 7)
+    lm = O.local_matrices(om, k)
+    op, kv = op_on(dom, k, (F, F))
+    op.set_local_matrices(lm)
+    op.set_penalty_epsilon(0.5)
+    for mode, comm in [(tng.ADD, tng.COMMUNICATE_ADDITIVELY), (tng.REPLACE, tng.SKIP_COMMUNICATION)]:
+        op.set_operator_apply_and_communication_modes(mode, comm)
+        out = vec(dom, 3, base)
+        op.apply(vec(dom, 3, u), out)
+        ref = O.epsdivdiv_stored(om, lm, u, F, F, apply_mode=mode, comm=comm, penalty_eps=0.5, dst=base)
+        assert rel_err(out.download(), ref) < 1e-12
+
+
+@pytest.mark.parametrize("lat,rad,ls", [(3, 3, 0), (4, 4, 0), (3, 2, 1)])
+def test_gca_matches_oracle(lat, rad, ls, gpu_ctx):
+    df, of = setup(lat, rad, ls, 0)
+    dc, oc = setup(lat - 1, rad - 1, ls, 0)
+    k = np.exp(field(of, 1, 8))
+    fine, kf = op_on(df, k, (D, D))
+    coarse, kc = op_on(dc, np.ones(oc.shape), (D, D))
+    tng.gca_assemble(fine, coarse)  # fine matrices assembled on the fly
+    got = coarse.get_local_matrices()
+    ref = O.gca(of, oc, O.local_matrices(of, k))
+    assert rel_err(got, ref) < 1e-13
+    # second step from stored matrices
+    dcc, occ = setup(lat - 2, rad - 2, ls, 0)
+    if occ.nr >= 3:
+        c2, kc2 = op_on(dcc, np.ones(occ.shape), (D, D))
+        tng.gca_assemble(coarse, c2)
+        assert rel_err(c2.get_local_matrices(), O.gca(oc, occ, ref)) < 1e-13
+
+
+def test_gca_operator_apply_freeslip(gpu_ctx):
+    df, of = setup(4)
+    dc, oc = setup(3)
+    k = np.exp(field(of, 1, 9))
+    fine, kf = op_on(df, k, (F, F))
+    coarse, kc = op_on(dc, np.ones(oc.shape), (F, F))
+    tng.gca_assemble(fine, coarse)
+    x = field(oc, 3, 10)
+    out = vec(dc, 3)
+    coarse.apply(vec(dc, 3, x), out)
+    ref = O.epsdivdiv_stored(oc, O.gca(of, oc, O.local_matrices(of, k)), x, F, F)
+    assert rel_err(out.download(), ref) < 1e-12
+
+
+@pytest.mark.parametrize("dt", [tng.F64, tng.F32])
+def test_stokes_solve_with_gca(dt, gpu_ctx):
+    """gca = 1 preconditioner reaches the same solution as gca = 0."""
+    of, oc = O.Mesh(4), O.Mesh(3)
+    r = np.broadcast_to(of.radii[:, None, None, :], of.shape)
+    eta = np.where(r < 0.85, 30.0, 1.0) * np.exp(0.5 * field(of, 1, 11))
+    f = field(of, 3, 12)
+    sols = {}
+    for gca in (0, 1):
+        S = tng.StokesSolver(2, 4, 0.55, 1.0, (D, F), tng.StokesParams.defaults(
+            gca=gca, viscous_pc_dtype=dt, krylov_restart=30, krylov_max_iterations=200,
+            krylov_relative_tolerance=1e-9, krylov_absolute_tolerance=1e-30))
+        S.set_viscosity(vec(S.domain_velocity, 1, eta))
+        fu = vec(S.domain_velocity, 3, f)
+        S.enforce_rhs(fu)
+        u, p = vec(S.domain_velocity, 3), vec(S.domain_pressure, 1)
+        rep = S.solve(u, p, fu, vec(S.domain_pressure, 1))
+        assert rep["converged"] == 1, (gca, rep)
+        sols[gca] = u.download()
+    own = of.ownership_mask == 1
+    assert rel_err(sols[1][:, own], sols[0][:, own]) < 1e-7
