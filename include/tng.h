@@ -305,6 +305,24 @@ tng_status tng_stokes_enforce_rhs(tng_stokes_solver solver, tng_vec f_u);
 tng_status tng_stokes_solve(tng_stokes_solver solver, tng_vec u, tng_vec p, tng_vec f_u, tng_vec f_p,
                             tng_solve_report *report);
 
+/* ---------------------------------------------------------------- adjoint */
+
+/* No counterpart in the reference (the viscosity adjoint is new to this
+ * path).  g[i] = d(lam^T A(k) u) / d log k_i for the viscous operator op
+ * (FP64), with A's Dirichlet / free-slip rows and without the fs/fs penalty
+ * (independent of k); summed over the copies of every node, so all copies
+ * hold the derivative with respect to the unique nodal value.  lam, u:
+ * velocity fields, g: scalar field, all on op's mesh. */
+tng_status tng_epsdivdiv_log_coefficient_gradient(tng_op op, tng_vec lam, tng_vec u, tng_vec g);
+/* Gradient of J(u) with respect to the nodal log-viscosity of the velocity
+ * level, u the solution of K [u; p] = f: solves K [lam; mu] = [dJ/du; 0]
+ * with the solver's FGMRES (K is symmetric; the fs/fs penalty makes it so
+ * only up to its multiplicity weighting, at the penalty's 1e-7 scale) and
+ * returns g = -d(lam^T A(eta) u)/d log eta.  dJdu must be consistent across
+ * copies (the derivative with respect to the unique nodal values). */
+tng_status tng_stokes_adjoint_gradient(tng_stokes_solver solver, tng_vec u, tng_vec dJdu, tng_vec g_log_eta,
+                                       tng_solve_report *report);
+
 /* ---------------------------------------------------------------- timing */
 
 /* Kernel timing with HIP events on the context stream (util/timer.hpp:22-389

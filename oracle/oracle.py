@@ -59,6 +59,7 @@ def lib():
         L.orc_epsdivdiv_apply_stored.argtypes = [vp, dp, ip, ip, ip, ip, ip, C.c_double, dp, dp]
         L.orc_epsdivdiv_local_matrices.argtypes = [vp, dp, dp]
         L.orc_gca.argtypes = [vp, vp, dp, dp, ip]
+        L.orc_epsdivdiv_log_k_gradient.argtypes = [vp, dp, ip, ip, dp, dp, dp]
         L.orc_num_threads.restype = ip
         _lib = L
     return _lib
@@ -135,6 +136,13 @@ class Mesh:
         A = np.zeros((18, 18))
         lib().orc_epsdivdiv_local_matrix(self.h, _dp(k), sd, xc, yc, rc, wedge, _dp(A))
         return A
+
+    def log_k_gradient(self, k, lam, u, bc_cmb, bc_surf):
+        """d(lam^T A(k) u) / d log k per node (summed over copies)."""
+        k, lam, u = (np.ascontiguousarray(a, dtype=np.float64) for a in (k, lam, u))
+        out = np.zeros(self.shape)
+        lib().orc_epsdivdiv_log_k_gradient(self.h, _dp(k), bc_cmb, bc_surf, _dp(lam), _dp(u), _dp(out))
+        return out
 
     def penalty_modes(self):
         out = np.zeros((3, 3) + self.shape)

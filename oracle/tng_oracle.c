@@ -546,7 +546,23 @@ static const double *stored_matrix(const orc_mesh *m, const double *lmat, int sd
     return lmat + (((((size_t)sd * nc + xc) * nc + yc) * ncr + rc) * 2 + w) * 324;
 }
 
+static void slow_cell_local(const op_ctx *o, int sd, int xc, int yc, int rc, int diagonal, const double *src,
+                            double d[2][18], int64_t idx[2][6]);
+
 static void slow_cell(const op_ctx *o, int sd, int xc, int yc, int rc, int diagonal, const double *src, double *dst)
+{
+    const int64_t N = o->m->nalloc;
+    double d[2][18];
+    int64_t idx[2][6];
+    slow_cell_local(o, sd, xc, yc, rc, diagonal, src, d, idx);
+    for (int w = 0; w < 2; ++w)
+        for (int dd = 0; dd < 3; ++dd)
+            for (int n = 0; n < 6; ++n) dst[(size_t)dd * N + idx[w][n]] += d[w][dd * 6 + n];
+}
+
+/* the slow path's local outputs of one hex cell: d[w][dim * 6 + node] */
+static void slow_cell_local(const op_ctx *o, int sd, int xc, int yc, int rc, int diagonal, const double *src,
+                            double d[2][18], int64_t idx[2][6])
 {
     const orc_mesh *m = o->m;
     const int64_t N = m->nalloc;
@@ -562,7 +578,6 @@ static void slow_cell(const op_ctx *o, int sd, int xc, int yc, int rc, int diago
         assemble_local_matrix(o, sd, xc, yc, rc, 1, A[1]);
     }
     double s[2][18];
-    int64_t idx[2][6];
     for (int w = 0; w < 2; ++w)
         for (int n = 0; n < 6; ++n) idx[w][n] = lin(m, sd, xc + WOFF[w][n][0], yc + WOFF[w][n][1], rc + WOFF[w][n][2]);
     for (int w = 0; w < 2; ++w)
@@ -641,7 +656,6 @@ static void slow_cell(const op_ctx *o, int sd, int xc, int yc, int rc, int diago
             for (int i = 0; i < 18; ++i)
                 for (int j = 0; j < 18; ++j)
                     if (i != j) A[w][i][j] = 0.0;
-    double d[2][18];
     for (int w = 0; w < 2; ++w)
         for (int i = 0; i < 18; ++i) {
             double acc = 0.0;
@@ -661,9 +675,37 @@ static void slow_cell(const op_ctx *o, int sd, int xc, int yc, int rc, int diago
                 for (int dd = 0; dd < 3; ++dd) d[w][dd * 6 + n] = t[n * 3 + dd];
         }
     }
-    for (int w = 0; w < 2; ++w)
-        for (int dd = 0; dd < 3; ++dd)
-            for (int n = 0; n < 6; ++n) dst[(size_t)dd * N + idx[w][n]] += d[w][dd * 6 + n];
+}
+
+/* Coefficient gradient of the bilinear form lam^T A(k) u (no penalty: it does
+ * not depend on k).  A is linear in the wedge coefficient k_w = mean of the
+ * six nodal k, so d(lam^T A u)/d log k_i = (k_i / 6) sum_{wedges w of copy i}
+ * lam_w^T A_w(k = 1) u_w, with A_w the slow path's local operator including its
+ * Dirichlet / free-slip rows; then summed over the copies of each node. */
+void orc_epsdivdiv_log_k_gradient(const orc_mesh *m, const double *k, int bc_cmb, int bc_surf, const double *lam,
+                                  const double *u, double *g)
+{
+    const int64_t N = m->nalloc;
+    double *ones = (double *)malloc(sizeof(double) * N);
+    for (int64_t i = 0; i < N; ++i) ones[i] = 1.0;
+    op_ctx o = {m, ones, bc_cmb, bc_surf, NULL};
+    memset(g, 0, sizeof(double) * N);
+    for (int sd = 0; sd < m->nsd; ++sd)
+        for (int xc = 0; xc < m->nx - 1; ++xc)
+            for (int yc = 0; yc < m->nx - 1; ++yc)
+                for (int rc = 0; rc < m->nr - 1; ++rc) {
+                    double d[2][18];
+                    int64_t idx[2][6];
+                    slow_cell_local(&o, sd, xc, yc, rc, 0, u, d, idx);
+                    for (int w = 0; w < 2; ++w) {
+                        double phi = 0.0;
+                        for (int dd = 0; dd < 3; ++dd)
+                            for (int n = 0; n < 6; ++n) phi += lam[(size_t)dd * N + idx[w][n]] * d[w][dd * 6 + n];
+                        for (int n = 0; n < 6; ++n) g[idx[w][n]] += phi * k[idx[w][n]] / 6.0;
+                    }
+                }
+    free(ones);
+    orc_halo_sum(m, g, 1);
 }
 
 /* epsilon_divdiv_kerngen.hpp:324-445 init_penalty: rotation modes e_i x x,

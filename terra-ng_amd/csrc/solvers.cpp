@@ -573,6 +573,25 @@ void StokesSolver::solve(SVec &x, const SVec &b, tng_solve_report &rep)
     lincomb(*x.p, {1.0}, {x.p}, -avg);
 }
 
+void StokesSolver::adjoint_gradient(const tng_vector &u, const tng_vector &dJdu, tng_vector &g, tng_solve_report &rep)
+{
+    TNG_REQUIRE(u.mesh == vel && dJdu.mesh == vel && g.mesh == vel && u.ncomp == 3 && dJdu.ncomp == 3 &&
+                    g.ncomp == 1 && u.dtype == TNG_F64 && dJdu.dtype == TNG_F64 && g.dtype == TNG_F64,
+                "adjoint: u, dJ/du (FP64 velocity) and g (FP64 scalar) on the velocity mesh");
+    if (!lam_u) {
+        lam_u = vec(pool, vel, TNG_F64, 3);
+        lam_p = vec(pool, pre_mesh, TNG_F64, 1);
+        zero_p = vec(pool, pre_mesh, TNG_F64, 1);
+    }
+    assign(*lam_u, 0.0);
+    assign(*lam_p, 0.0);
+    assign(*zero_p, 0.0);
+    SVec x{lam_u, lam_p}, b{const_cast<tng_vector *>(&dJdu), zero_p};
+    solve(x, b, rep);  // K is symmetric: K^T lam = K lam
+    epsdivdiv_log_k_gradient(*A, *lam_u, u, g, stream_of(g));
+    lincomb(g, {-1.0}, {&g});
+}
+
 // RHS boundary enforcement as the app applies it (stokes_solver.hpp:664-674):
 // the Dirichlet and free-slip flags are those of the FIRST boundary carrying
 // that condition (grid/shell/bit_masks.hpp:68-76), CMB before surface.
@@ -781,6 +800,22 @@ tng_status tng_stokes_enforce_rhs(tng_stokes_solver solver, tng_vec f_u)
         TNG_REQUIRE(f_u->mesh == S.vel && f_u->ncomp == 3, "velocity vector on the solver's velocity mesh expected");
         S.enforce_rhs(*f_u);
     });
+}
+
+tng_status tng_epsdivdiv_log_coefficient_gradient(tng_op op, tng_vec lam, tng_vec u, tng_vec g)
+{
+    if (!op || !lam || !u || !g) return TNG_INVALID_ARGUMENT;
+    return sguard(op->mesh->ctx, [&] { tng::epsdivdiv_log_k_gradient(*op, *lam, *u, *g, op->mesh->ctx->stream); });
+}
+
+tng_status tng_stokes_adjoint_gradient(tng_stokes_solver solver, tng_vec u, tng_vec dJdu, tng_vec g_log_eta,
+                                       tng_solve_report *report)
+{
+    if (!solver || !u || !dJdu || !g_log_eta) return TNG_INVALID_ARGUMENT;
+    tng::StokesSolver &S = *solver->s;
+    tng_solve_report local{};
+    tng_solve_report &rep = report ? *report : local;
+    return sguard(S.vel->ctx, [&] { S.adjoint_gradient(*u, *dJdu, *g_log_eta, rep); });
 }
 
 tng_status tng_stokes_solve(tng_stokes_solver solver, tng_vec u, tng_vec p, tng_vec f_u, tng_vec f_p,

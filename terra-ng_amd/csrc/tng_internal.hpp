@@ -206,6 +206,10 @@ void copy_stored_matrices(tng_operator &dst, const tng_operator &src, tng_dtype 
 template <typename T>
 void stored_apply(tng_operator &op, const tng_vector &src, tng_vector &dst, const double *pen, bool add, hipStream_t s);
 
+// viscosity adjoint kernel (kernels_adjoint.hip): g = d(lam^T A(k) u)/d log k
+void epsdivdiv_log_k_gradient(tng_operator &op, const tng_vector &lam, const tng_vector &u, tng_vector &g,
+                              hipStream_t s);
+
 // solver support kernels
 void launch_randomize(tng_vector &v, uint64_t seed, hipStream_t s);
 void launch_inject(tng_vector &coarse, const tng_vector &fine, hipStream_t s);

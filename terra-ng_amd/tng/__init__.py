@@ -97,6 +97,8 @@ EXPORTS = [
     ("tng_op_clear_local_matrices", _i, [_vp]),
     ("tng_op_copy_local_matrices", _i, [_vp, _vp]),
     ("tng_gca_assemble", _i, [_vp, _vp]),
+    ("tng_epsdivdiv_log_coefficient_gradient", _i, [_vp, _vp, _vp, _vp]),
+    ("tng_stokes_adjoint_gradient", _i, [_vp, _vp, _vp, _vp, _vp]),
     ("tng_pcg_solve", _i, [_vp, _vp, _vp, _i, _d, _d, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     ("tng_power_iteration", _i, [_vp, _vp, _i, C.c_ulonglong, C.POINTER(C.c_double)]),
     ("tng_chebyshev_smooth", _i, [_vp, _vp, _i, _i, _d, _vp, _vp]),
@@ -366,6 +368,10 @@ class EpsilonDivDivKerngen(_Operator):
     def coefficient_updated(self):
         self.domain.ctx.check(load_library().tng_op_coefficient_updated(self.h))
 
+    def log_coefficient_gradient(self, lam, u, g):
+        """g = d(lam^T A(k) u)/d log k per node (no reference counterpart)."""
+        self.domain.ctx.check(load_library().tng_epsdivdiv_log_coefficient_gradient(self.h, lam.h, u.h, g.h))
+
     def _lmat_shape(self):
         d = self.domain
         return (d.nsd, d.nx - 1, d.nx - 1, d.nr - 1, 2, 18, 18)
@@ -554,6 +560,15 @@ class StokesSolver:
         out = {name: getattr(rep, name) for name, _ in SolveReport._fields_ if name not in ("history",)}
         out["history"] = [hist[i] for i in range(min(rep.history_len, history_capacity))]
         return out
+
+    def adjoint_gradient(self, u, dJdu, g, history_capacity=1000):
+        """g = dJ/dlog(eta) on the velocity mesh for J(u), u the solution of the last solve."""
+        rep = SolveReport()
+        hist = (C.c_double * history_capacity)()
+        rep.history = C.cast(hist, C.POINTER(C.c_double))
+        rep.history_capacity = history_capacity
+        self.ctx.check(load_library().tng_stokes_adjoint_gradient(self.h, u.h, dJdu.h, g.h, C.byref(rep)))
+        return {name: getattr(rep, name) for name, _ in SolveReport._fields_ if name != "history"}
 
     def close(self):
         if self.h:
