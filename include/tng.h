@@ -193,6 +193,10 @@ tng_status tng_restriction_create(tng_mesh fine, tng_mesh coarse, tng_op *out);
  * dst = diag(row sums of the k-weighted mass matrix) src, additive exchange. */
 tng_status tng_kmass_lumped_create(tng_mesh mesh, tng_vec k, tng_op *out);
 
+/* fe/wedge/operators/shell/vector_mass.hpp:17-130 VectorMass (consistent,
+ * 6-point rule, per component; Replace + CommunicateAdditively). */
+tng_status tng_vector_mass_create(tng_mesh mesh, tng_op *out);
+
 /* ------------------------------------------------- vector linear algebra */
 
 /* linalg/vector.hpp:72-95 lincomb: y = sum_i c[i] x[i] + c0, 0 <= n <= 3;
@@ -298,6 +302,11 @@ tng_status tng_stokes_precondition_velocity(tng_stokes_solver solver, tng_vec x,
  * then surface) carrying each condition is treated
  * (grid/shell/bit_masks.hpp:68-76). */
 tng_status tng_stokes_enforce_rhs(tng_stokes_solver solver, tng_vec f_u);
+/* The RHS assembly of StokesContext::solve (stokes_solver.hpp:650-674):
+ * f_u = M (Ra T x/|x|) (interpolators.hpp:99-133, vector_mass.hpp), then the
+ * boundary enforcement of tng_stokes_enforce_rhs.  T: scalar field on the
+ * velocity mesh. */
+tng_status tng_stokes_buoyancy_rhs(tng_stokes_solver solver, tng_vec temperature, double rayleigh, tng_vec f_u);
 /* FGMRES solve of K [u; p] = [f_u; f_p] from the initial guess in (u, p),
  * then the pressure mean over unique nodes is removed
  * (stokes_solver.hpp:676-689).  Returns TNG_OK also when the iteration limit

@@ -60,6 +60,7 @@ def lib():
         L.orc_epsdivdiv_local_matrices.argtypes = [vp, dp, dp]
         L.orc_gca.argtypes = [vp, vp, dp, dp, ip]
         L.orc_epsdivdiv_log_k_gradient.argtypes = [vp, dp, ip, ip, dp, dp, dp]
+        L.orc_vector_mass_apply.argtypes = [vp, dp, dp]
         L.orc_num_threads.restype = ip
         _lib = L
     return _lib
@@ -234,6 +235,14 @@ def epsdivdiv_stored(mesh, lmat, src, bc_cmb, bc_surf, diagonal=False, apply_mod
     out = np.zeros((3,) + mesh.shape) if dst is None else np.array(dst, dtype=np.float64, order="C", copy=True)
     lib().orc_epsdivdiv_apply_stored(mesh.h, _dp(lm), bc_cmb, bc_surf, int(diagonal), apply_mode, comm, penalty_eps,
                                      _dp(src), _dp(out))
+    return out
+
+
+def vector_mass(mesh, u):
+    """VectorMass (vector_mass.hpp:66-130): consistent mass per component."""
+    u = _c(u)
+    out = np.zeros((3,) + mesh.shape)
+    lib().orc_vector_mass_apply(mesh.h, _dp(u), _dp(out))
     return out
 
 

@@ -1489,3 +1489,58 @@ void orc_kmass_lumped_apply(const orc_mesh *m, const double *k, const double *sr
                 }
     orc_halo_sum(m, dst, 1);
 }
+
+/* ------------------------------------------------------------------------- */
+/* VectorMass (fe/wedge/operators/shell/vector_mass.hpp:66-130): consistent   */
+/* mass per component, 6-point Felippa rule, additive exchange.               */
+/* ------------------------------------------------------------------------- */
+void orc_vector_mass_apply(const orc_mesh *m, const double *src, double *dst)
+{
+    static const double QP[6][3] = {{0.6666666666666666, 0.1666666666666667, -0.5773502691896257},
+                                    {0.1666666666666667, 0.6666666666666666, -0.5773502691896257},
+                                    {0.1666666666666667, 0.1666666666666667, -0.5773502691896257},
+                                    {0.6666666666666666, 0.1666666666666667, 0.5773502691896257},
+                                    {0.1666666666666667, 0.6666666666666666, 0.5773502691896257},
+                                    {0.1666666666666667, 0.1666666666666667, 0.5773502691896257}};
+    static const double QW = 0.1666666666666667;
+    const int64_t N = m->nalloc;
+    memset(dst, 0, sizeof(double) * 3 * N);
+    const int nx = m->nx;
+    for (int sd = 0; sd < m->nsd; ++sd)
+        for (int xc = 0; xc < nx - 1; ++xc)
+            for (int yc = 0; yc < nx - 1; ++yc)
+                for (int rc = 0; rc < m->nr - 1; ++rc) {
+                    const double r_1 = m->radii[(size_t)sd * m->nr + rc], r_2 = m->radii[(size_t)sd * m->nr + rc + 1];
+                    const double grad_r = 0.5 * (r_2 - r_1);
+                    for (int w = 0; w < 2; ++w) {
+                        vec3 P[3];
+                        int64_t idx[6];
+                        for (int n = 0; n < 6; ++n) idx[n] = lin(m, sd, xc + WOFF[w][n][0], yc + WOFF[w][n][1], rc + WOFF[w][n][2]);
+                        for (int n = 0; n < 3; ++n) {
+                            const double *c = m->coords + 3 * (((size_t)sd * nx + xc + WOFF[w][n][0]) * nx + yc + WOFF[w][n][1]);
+                            P[n] = v3(c[0], c[1], c[2]);
+                        }
+                        double A[6][6];
+                        memset(A, 0, sizeof(A));
+                        for (int q = 0; q < 6; ++q) {
+                            vec3 c0 = vsub(P[1], P[0]), c1 = vsub(P[2], P[0]);
+                            const double xi = QP[q][0], eta = QP[q][1], zeta = QP[q][2];
+                            vec3 c2 = vadd(vadd(vscale(P[0], (1 - xi - eta)), vscale(P[1], xi)), vscale(P[2], eta));
+                            mat3 Jl;
+                            for (int i = 0; i < 3; ++i) { Jl.a[i][0] = c0.v[i]; Jl.a[i][1] = c1.v[i]; Jl.a[i][2] = c2.v[i]; }
+                            const double detl = fabs(det3(&Jl));
+                            const double r = r_1 + 0.5 * (r_2 - r_1) * (1 + zeta);
+                            for (int i = 0; i < 6; i++)
+                                for (int j = 0; j < 6; j++)
+                                    A[i][j] += QW * (shape_w(i, xi, eta, zeta) * shape_w(j, xi, eta, zeta) * r * r * grad_r * detl);
+                        }
+                        for (int d = 0; d < 3; ++d)
+                            for (int i = 0; i < 6; ++i) {
+                                double acc = 0.0;
+                                for (int j = 0; j < 6; ++j) acc += A[i][j] * src[(size_t)d * N + idx[j]];
+                                dst[(size_t)d * N + idx[i]] += acc;
+                            }
+                    }
+                }
+    orc_halo_sum(m, dst, 3);
+}

@@ -471,6 +471,7 @@ void op_apply(tng_operator &op, const tng_vector &src, tng_vector &dst)
     case OP_PROLONGATION: in = op.mesh2; out = op.mesh; nin = nout = src.ncomp; break;
     case OP_RESTRICTION: in = op.mesh; out = op.mesh2; nin = nout = src.ncomp; break;
     case OP_KMASS_LUMPED: in = out = op.mesh; nin = nout = 1; break;
+    case OP_VECTOR_MASS: in = out = op.mesh; break;
     }
     TNG_REQUIRE(src.mesh == in && dst.mesh == out, "apply: vectors do not live on the operator's meshes");
     TNG_REQUIRE(src.ncomp == nin && dst.ncomp == nout, "apply: wrong number of vector components");
@@ -517,6 +518,18 @@ tng_status tng_kmass_lumped_create(tng_mesh mesh, tng_vec k, tng_op *out)
     if (!mesh || !k || !out || !mesh->ctx) return TNG_INVALID_ARGUMENT;
     *out = nullptr;
     return guarded(mesh->ctx, [&] { *out = new_kmass(mesh, k); });
+}
+
+tng_status tng_vector_mass_create(tng_mesh mesh, tng_op *out)
+{
+    if (!mesh || !out || !mesh->ctx) return TNG_INVALID_ARGUMENT;
+    *out = nullptr;
+    return guarded(mesh->ctx, [&] {
+        std::unique_ptr<tng_operator> op(new tng_operator());
+        op->kind = OP_VECTOR_MASS;
+        op->mesh = mesh;
+        *out = op.release();
+    });
 }
 
 tng_status tng_vec_lincomb(tng_vec y, int n, const double *c, const tng_vec *x, double c0)

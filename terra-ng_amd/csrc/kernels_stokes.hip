@@ -390,6 +390,82 @@ __global__ __launch_bounds__(256) void kmass_lumped_kernel(const T *k, const T *
     dst[gid] = acc * src[gid];
 }
 
+// Consistent vector mass (vector_mass.hpp:66-130, 6-point rule), per node and
+// component: sum over the <= 12 wedges around the node of
+// sum_q w_q N_i(q) u_h(q) r(q)^2 (dr/2) |det L|.
+template <typename T>
+__global__ __launch_bounds__(256) void vector_mass_kernel(const Comp3<const T> src, const Comp3<T> dst, const T *radii,
+                                                          const T *latgeom, int nsd, int nx, int nr)
+{
+    const int64_t n = (int64_t)nsd * nx * nx * nr;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n) return;
+    const int l = (int)(gid % nr);
+    int64_t t = gid / nr;
+    const int Y = (int)(t % nx); t /= nx;
+    const int X = (int)(t % nx);
+    const int sd = (int)(t / nx);
+    const int nc = nx - 1;
+    const int corner_of[2][2][2] = {{{0, 2}, {1, -1}}, {{-1, 1}, {2, 0}}};
+    const T qx[3] = {T(0.6666666666666666), T(0.1666666666666667), T(0.1666666666666667)};
+    const T qy[3] = {T(0.1666666666666667), T(0.6666666666666666), T(0.1666666666666667)};
+    const T qz[2] = {T(-0.5773502691896257), T(0.5773502691896257)};
+    const T qw = T(0.1666666666666667);
+    T acc[3] = {T(0), T(0), T(0)};
+    for (int dxc = -1; dxc <= 0; ++dxc) {
+        const int xc = X + dxc;
+        if (xc < 0 || xc >= nc) continue;
+        for (int dyc = -1; dyc <= 0; ++dyc) {
+            const int yc = Y + dyc;
+            if (yc < 0 || yc >= nc) continue;
+            for (int w = 0; w < 2; ++w) {
+                const int kc = corner_of[w][X - xc][Y - yc];
+                if (kc < 0) continue;
+                const T detl = latgeom[((((int64_t)sd * nc + xc) * nc + yc) * 2 + w) * 10 + 9];
+                for (int rc = l - 1; rc <= l; ++rc) {
+                    if (rc < 0 || rc >= nr - 1) continue;
+                    const int top = rc == l ? 0 : 1;
+                    const T r1 = radii[(int64_t)sd * nr + rc], r2 = radii[(int64_t)sd * nr + rc + 1];
+                    const T gr = T(0.5) * (r2 - r1);
+                    T uv[6][3];
+                    for (int j = 0; j < 6; ++j) {
+                        const int jx = xc + kWC[w][j % 3][0], jy = yc + kWC[w][j % 3][1];
+                        const int64_t ji = (((int64_t)sd * nx + jx) * nx + jy) * nr + rc + j / 3;
+                        for (int c = 0; c < 3; ++c) uv[j][c] = src.p[c][ji];
+                    }
+                    for (int q = 0; q < 6; ++q) {
+                        const T xi = qx[q % 3], eta = qy[q % 3], zeta = qz[q / 3];
+                        const T Nl[3] = {T(1) - xi - eta, xi, eta};
+                        const T Nr[2] = {T(0.5) * (T(1) - zeta), T(0.5) * (T(1) + zeta)};
+                        const T r = r1 + T(0.5) * (r2 - r1) * (T(1) + zeta);
+                        const T f = qw * Nl[kc] * Nr[top] * r * r * gr * detl;
+                        for (int c = 0; c < 3; ++c) {
+                            T uh = T(0);
+                            for (int j = 0; j < 6; ++j) uh += Nl[j % 3] * Nr[j / 3] * uv[j][c];
+                            acc[c] += f * uh;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    for (int c = 0; c < 3; ++c) dst.p[c][gid] = acc[c];
+}
+
+// Buoyancy of the app's RHS interpolator (apps/mantlecirculation/src/interpolators.hpp:99-133):
+// u = Ra * T * x/|x| per node.
+template <typename T>
+__global__ __launch_bounds__(256) void buoyancy_kernel(const T *temp, const Comp3<T> u, const T *coords, T ra,
+                                                       int64_t ncol, int nr)
+{
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= ncol * nr) return;
+    const T *c = coords + 3 * (gid / nr);
+    const T inv = T(1) / sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    const T v = ra * temp[gid];
+    for (int d = 0; d < 3; ++d) u.p[d][gid] = v * c[d] * inv;
+}
+
 unsigned blocks_for(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
 template <typename T> struct MeshP {
@@ -492,6 +568,22 @@ void stokes_block_apply(tng_operator &op, const tng_vector &src, tng_vector &dst
         f64 ? kmass_impl<double>(op, src, dst, s) : kmass_impl<float>(op, src, dst, s);
         if (op.comm_mode == TNG_COMMUNICATE_ADDITIVELY) launch_halo_sum(*op.mesh, dst, s);
         break;
+    case OP_VECTOR_MASS: {
+        const tng_domain &m = *op.mesh;
+        if (f64)
+            hipLaunchKernelGGL(vector_mass_kernel<double>, dim3(blocks_for(m.nalloc)), dim3(256), 0, s,
+                               Comp3<const double>{{(const double *)src.comp(0), (const double *)src.comp(1),
+                                                    (const double *)src.comp(2)}},
+                               comps<double>(dst), m.d_radii.as<double>(), m.d_latgeom.as<double>(), m.nsd, m.nx, m.nr);
+        else
+            hipLaunchKernelGGL(vector_mass_kernel<float>, dim3(blocks_for(m.nalloc)), dim3(256), 0, s,
+                               Comp3<const float>{{(const float *)src.comp(0), (const float *)src.comp(1),
+                                                   (const float *)src.comp(2)}},
+                               comps<float>(dst), m.d_radii_f.as<float>(), m.d_latgeom_f.as<float>(), m.nsd, m.nx, m.nr);
+        TNG_HIP_CHECK(hipGetLastError());
+        if (op.comm_mode == TNG_COMMUNICATE_ADDITIVELY) launch_halo_sum(*op.mesh, dst, s);
+        break;
+    }
     default:
         throw Error(TNG_INVALID_ARGUMENT, "stokes_block_apply: wrong operator kind");
     }
@@ -590,4 +682,22 @@ void launch_rhs_enforce(tng_vector &u, uint8_t dirichlet_flag, uint8_t freeslip_
     TNG_HIP_CHECK(hipGetLastError());
 }
 
+}  // namespace tng
+
+namespace tng {
+void launch_buoyancy(const tng_vector &temp, double rayleigh, tng_vector &u, hipStream_t s)
+{
+    const tng_domain &m = *u.mesh;
+    TNG_REQUIRE(temp.mesh == u.mesh && temp.ncomp == 1 && u.ncomp == 3 && temp.dtype == u.dtype,
+                "buoyancy: temperature (scalar) and velocity on one mesh");
+    if (u.dtype == TNG_F64)
+        hipLaunchKernelGGL(buoyancy_kernel<double>, dim3(blocks_for(m.nalloc)), dim3(256), 0, s,
+                           (const double *)temp.comp(0), comps<double>(u), m.d_coords.as<double>(), rayleigh, m.nlat,
+                           m.nr);
+    else
+        hipLaunchKernelGGL(buoyancy_kernel<float>, dim3(blocks_for(m.nalloc)), dim3(256), 0, s,
+                           (const float *)temp.comp(0), comps<float>(u), m.d_coords_f.as<float>(), (float)rayleigh,
+                           m.nlat, m.nr);
+    TNG_HIP_CHECK(hipGetLastError());
+}
 }  // namespace tng

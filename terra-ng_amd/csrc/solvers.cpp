@@ -592,6 +592,22 @@ void StokesSolver::adjoint_gradient(const tng_vector &u, const tng_vector &dJdu,
     lincomb(g, {-1.0}, {&g});
 }
 
+void StokesSolver::buoyancy_rhs(const tng_vector &temp, double rayleigh, tng_vector &f_u)
+{
+    TNG_REQUIRE(temp.mesh == vel && f_u.mesh == vel && temp.ncomp == 1 && f_u.ncomp == 3 && temp.dtype == TNG_F64 &&
+                    f_u.dtype == TNG_F64,
+                "buoyancy: FP64 temperature (scalar) and RHS (velocity) on the velocity mesh");
+    if (!vmass) {
+        vmass.reset(new tng_operator());
+        vmass->kind = OP_VECTOR_MASS;
+        vmass->mesh = vel;
+        rhs_tmp = vec(pool, vel, TNG_F64, 3);
+    }
+    launch_buoyancy(temp, rayleigh, *rhs_tmp, stream_of(f_u));
+    apply(*vmass, *rhs_tmp, f_u);
+    enforce_rhs(f_u);
+}
+
 // RHS boundary enforcement as the app applies it (stokes_solver.hpp:664-674):
 // the Dirichlet and free-slip flags are those of the FIRST boundary carrying
 // that condition (grid/shell/bit_masks.hpp:68-76), CMB before surface.
@@ -790,6 +806,13 @@ tng_status tng_stokes_precondition_velocity(tng_stokes_solver solver, tng_vec x,
                     "velocity vectors (FP64) on the solver's velocity mesh expected");
         S.precondition_velocity(*x, *b);
     });
+}
+
+tng_status tng_stokes_buoyancy_rhs(tng_stokes_solver solver, tng_vec temperature, double rayleigh, tng_vec f_u)
+{
+    if (!solver || !temperature || !f_u) return TNG_INVALID_ARGUMENT;
+    tng::StokesSolver &S = *solver->s;
+    return sguard(S.vel->ctx, [&] { S.buoyancy_rhs(*temperature, rayleigh, *f_u); });
 }
 
 tng_status tng_stokes_enforce_rhs(tng_stokes_solver solver, tng_vec f_u)

@@ -142,7 +142,15 @@ template <typename T> inline Comp3<T> comps(const tng_vector &v)
 }
 }  // namespace tng
 
-enum OpKind { OP_EPSDIVDIV = 1, OP_DIVERGENCE, OP_GRADIENT, OP_PROLONGATION, OP_RESTRICTION, OP_KMASS_LUMPED };
+enum OpKind {
+    OP_EPSDIVDIV = 1,
+    OP_DIVERGENCE,
+    OP_GRADIENT,
+    OP_PROLONGATION,
+    OP_RESTRICTION,
+    OP_KMASS_LUMPED,
+    OP_VECTOR_MASS
+};
 
 struct tng_operator {
     OpKind kind = OP_EPSDIVDIV;
@@ -209,6 +217,9 @@ void stored_apply(tng_operator &op, const tng_vector &src, tng_vector &dst, cons
 // viscosity adjoint kernel (kernels_adjoint.hip): g = d(lam^T A(k) u)/d log k
 void epsdivdiv_log_k_gradient(tng_operator &op, const tng_vector &lam, const tng_vector &u, tng_vector &g,
                               hipStream_t s);
+
+// u = Ra T x/|x| (the app's buoyancy interpolator)
+void launch_buoyancy(const tng_vector &temp, double rayleigh, tng_vector &u, hipStream_t s);
 
 // solver support kernels
 void launch_randomize(tng_vector &v, uint64_t seed, hipStream_t s);

@@ -77,6 +77,10 @@ struct StokesSolver {
     void fgmres(SVec &x, const SVec &b, tng_solve_report &rep);
     void solve(SVec &x, const SVec &b, tng_solve_report &rep);
     void enforce_rhs(tng_vector &f_u);
+    // f_u = M (Ra T x/|x|) with the consistent vector mass, then enforce_rhs (stokes_solver.hpp:650-674)
+    void buoyancy_rhs(const tng_vector &temp, double rayleigh, tng_vector &f_u);
+    std::unique_ptr<tng_operator> vmass;
+    tng_vector *rhs_tmp = nullptr;
     // dJ/dlog eta for J(u) with K [u; p] = f: lam solves K [lam_u; lam_p] = [dJdu; 0],
     // g = -d(lam_u^T A(eta) u)/dlog eta (B, B^T, the penalty and the preconditioner do not depend on eta)
     void adjoint_gradient(const tng_vector &u, const tng_vector &dJdu, tng_vector &g, tng_solve_report &rep);

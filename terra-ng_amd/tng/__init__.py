@@ -86,6 +86,8 @@ EXPORTS = [
     ("tng_prolongation_create", _i, [_vp, _vp, _pv]),
     ("tng_restriction_create", _i, [_vp, _vp, _pv]),
     ("tng_kmass_lumped_create", _i, [_vp, _vp, _pv]),
+    ("tng_vector_mass_create", _i, [_vp, _pv]),
+    ("tng_stokes_buoyancy_rhs", _i, [_vp, _vp, _d, _vp]),
     ("tng_vec_lincomb", _i, [_vp, _i, C.POINTER(C.c_double), C.POINTER(C.c_void_p), _d]),
     ("tng_vec_scale_with_vector", _i, [_vp, _vp]),
     ("tng_vec_invert", _i, [_vp]),
@@ -448,6 +450,13 @@ class KMassLumped(_Operator):
         self.k = k
 
 
+class VectorMass(_Operator):
+    """fe/wedge/operators/shell/vector_mass.hpp:17-130 (consistent, per component)."""
+
+    def __init__(self, domain):
+        super().__init__(domain, _create(domain, "tng_vector_mass_create", domain.h))
+
+
 # ------------------------------------------------- linalg (linalg/vector.hpp)
 
 def lincomb(y, c, x, c0=0.0):
@@ -547,6 +556,10 @@ class StokesSolver:
 
     def precondition_velocity(self, x, b):
         self.ctx.check(load_library().tng_stokes_precondition_velocity(self.h, x.h, b.h))
+
+    def buoyancy_rhs(self, temperature, rayleigh, f_u):
+        """f_u = M (Ra T x/|x|), boundary-enforced (StokesContext::solve's RHS)."""
+        self.ctx.check(load_library().tng_stokes_buoyancy_rhs(self.h, temperature.h, float(rayleigh), f_u.h))
 
     def enforce_rhs(self, f_u):
         self.ctx.check(load_library().tng_stokes_enforce_rhs(self.h, f_u.h))

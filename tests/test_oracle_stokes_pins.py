@@ -306,3 +306,40 @@ def test_divergence_apply_modes():
     # Add + exchange: the exchange sums the accumulated dst over the copies
     out = O.divergence(mf, mc, u, D, F, apply_mode=O.ADD, comm=O.COMMUNICATE, dst=base)
     assert np.abs(out - mc.halo_sum(base + loc)).max() < 1e-14
+
+
+# ------------------------------------------------------------------ VectorMass
+def test_vector_mass_row_sums_are_lumped_kmass():
+    """With k = 1 the row sums of the consistent mass are the lumped mass."""
+    m = O.Mesh(2, 2, 0.55, 1.0, 1, 0)
+    ones = np.ones((3,) + m.shape)
+    lumped = O.kmass_lumped(m, np.ones(m.shape), np.ones(m.shape))
+    mm = O.vector_mass(m, ones)
+    for d in range(3):
+        assert rel_err(mm[d], lumped) < 1e-14
+
+
+def test_vector_mass_exact_for_lateral_fields():
+    """For radially constant fields the 6-point rule is exact:
+    (M u)_i = sum_w |det P| sum_b u_b (1 + delta_ab)/24 * int N_l(r) r^2 dr."""
+    m = O.Mesh(2)
+    c, rad = m.coords, m.radii
+    ulat = np.stack([np.sin(2 * c[..., 0]) + c[..., 1], c[..., 2] ** 2, 1.0 + 0.3 * c[..., 0] * c[..., 1]])
+    u = np.broadcast_to(ulat[..., None], (3,) + m.shape).copy()
+    exp = np.zeros((3,) + m.shape)
+    for sd in range(m.nsd):
+        for _, corners in _lat_wedges(m.nx):
+            det = abs(np.linalg.det(np.array([c[sd, x, y] for x, y in corners])))
+            for r in range(m.nr - 1):
+                r1, r2 = rad[sd, r], rad[sd, r + 1]
+                i3, i4 = (r2 ** 3 - r1 ** 3) / 3, (r2 ** 4 - r1 ** 4) / 4
+                rint = ((r2 * i3 - i4) / (r2 - r1), (i4 - r1 * i3) / (r2 - r1))
+                for a, (x, y) in enumerate(corners):
+                    for d in range(3):
+                        ub = np.array([ulat[d, sd, xx, yy] for xx, yy in corners])
+                        lat = (ub.sum() + ub[a]) / 24.0
+                        for e in (0, 1):
+                            # radially constant u: int N_e(r) r^2 dr against the sum of both layers
+                            exp[d, sd, x, y, r + e] += det * lat * rint[e]
+    exp = m.halo_sum(exp)
+    assert rel_err(O.vector_mass(m, u), exp) < 1e-14

@@ -250,3 +250,34 @@ def test_wrapped_vectors(gpu_ctx):
     gw = tng.VectorQ1Vec.wrap(df, tng.F64, [v.device_ptr() for v in outs], keepalive=outs)
     tng.GradientKerngen(df, dc, (D, F)).apply(tng.VectorQ1Scalar.wrap(dc, tng.F64, [pw.device_ptr()], keepalive=pw), gw)
     assert rel_err(np.stack([o.download() for o in outs]), O.gradient(of, oc, p, D, F)) < 1e-12
+
+
+@pytest.mark.parametrize("L,ls,rs", [(3, 0, 0), (4, 1, 1)])
+def test_vector_mass(L, ls, rs, gpu_ctx):
+    df, dc, of, oc = levels(L, ls, rs)
+    u = field(of, 3, 30)
+    out = vec(df, 3)
+    tng.VectorMass(df).apply(vec(df, 3, u), out)
+    ref = O.vector_mass(of, u)
+    assert rel_err(out.download(), ref) < 1e-12
+    out32 = vec(df, 3, dtype=tng.F32)
+    tng.VectorMass(df).apply(vec(df, 3, u, tng.F32), out32)
+    assert rel_err(out32.download(), ref) < 2e-6
+
+
+def test_buoyancy_rhs(gpu_ctx):
+    """StokesContext::solve's RHS: f = M (Ra T x/|x|), CMB rows zeroed (Dirichlet,
+    first boundary only), free-slip surface rows projected."""
+    S = tng.StokesSolver(1, 3, 0.55, 1.0, (D, F))
+    of = O.Mesh(3)
+    T = field(of, 1, 31)
+    fu = vec(S.domain_velocity, 3)
+    S.buoyancy_rhs(vec(S.domain_velocity, 1, T), 1e5, fu)
+    n = of.coords / np.linalg.norm(of.coords, axis=-1, keepdims=True)
+    u = 1e5 * T[None] * np.moveaxis(n, -1, 0)[..., None]
+    ref = O.vector_mass(of, u)
+    ref[..., 0] = 0.0
+    nn = np.moveaxis(n, -1, 0)
+    d = (ref[..., -1] * nn).sum(0)
+    ref[..., -1] -= d * nn
+    assert rel_err(fu.download(), ref) < 1e-12
