@@ -154,13 +154,12 @@ tng_status tng_mesh_create_uniform(tng_ctx ctx, int lateral_level, int radial_le
     return guarded(ctx, [&] { *out = new_mesh(ctx, lateral_level, radial_level, r_min, r_max, lateral_sdr, radial_sdr); });
 }
 
+// Destruction touches no other object (the context, mesh or solver may be
+// gone already, e.g. when a garbage collector finalises in any order):
+// hipFree synchronises with the device by itself.
 tng_status tng_mesh_destroy(tng_mesh mesh)
 {
     if (!mesh || mesh->solver_owned) return TNG_INVALID_ARGUMENT;
-    if (mesh->ctx) {
-        (void)hipSetDevice(mesh->ctx->device);
-        (void)hipStreamSynchronize(mesh->ctx->stream);
-    }
     delete mesh;
     return TNG_OK;
 }
@@ -238,8 +237,6 @@ tng_status tng_vec_wrap(tng_mesh mesh, tng_dtype dtype, int ncomp, void *const *
 tng_status tng_vec_destroy(tng_vec v)
 {
     if (!v) return TNG_INVALID_ARGUMENT;
-    (void)hipSetDevice(v->mesh->ctx->device);
-    (void)hipStreamSynchronize(v->mesh->ctx->stream);
     delete v;
     return TNG_OK;
 }
@@ -356,8 +353,6 @@ tng_status tng_op_apply(tng_op op, tng_vec src, tng_vec dst)
 tng_status tng_op_destroy(tng_op op)
 {
     if (!op) return TNG_INVALID_ARGUMENT;
-    (void)hipSetDevice(op->mesh->ctx->device);
-    (void)hipStreamSynchronize(op->mesh->ctx->stream);
     delete op;
     return TNG_OK;
 }

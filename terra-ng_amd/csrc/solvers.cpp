@@ -755,10 +755,7 @@ tng_status tng_stokes_solver_create(tng_ctx ctx, int min_level, int max_level, i
 tng_status tng_stokes_solver_destroy(tng_stokes_solver solver)
 {
     if (!solver) return TNG_INVALID_ARGUMENT;
-    tng_context *ctx = solver->s->vel->ctx;
-    (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-    delete solver;
+    delete solver;  // hipFree synchronises; the context is not touched
     return TNG_OK;
 }
 

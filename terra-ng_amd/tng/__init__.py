@@ -515,7 +515,9 @@ def chebyshev_smooth(op, inv_diag, x, b, order=2, iterations=1, lambda_max_estim
 
 
 class _BorrowedDomain(DistributedDomain):
-    """A mesh owned by a solver (not destroyed from Python)."""
+    """A mesh owned by a solver (not destroyed from Python).  It keeps the
+    solver alive while vectors on it exist; the solver does not hold it, so
+    no reference cycle delays or reorders their destruction."""
 
     def __init__(self, ctx, h, owner):
         super().__init__(ctx, h)
@@ -540,13 +542,20 @@ class StokesSolver:
         n = C.c_int()
         self.ctx.check(load_library().tng_stokes_solver_num_levels(h, C.byref(n)))
         self.num_levels = n.value
-        self.domains = []
-        for lv in range(self.num_levels):
-            mh = C.c_void_p()
-            self.ctx.check(load_library().tng_stokes_solver_mesh(h, lv, C.byref(mh)))
-            self.domains.append(_BorrowedDomain(self.ctx, mh, self))
-        self.domain_velocity = self.domains[-1]
-        self.domain_pressure = self.domains[-2]
+
+    def domain(self, level):
+        """Mesh of multigrid level `level` (0 = coarsest)."""
+        mh = C.c_void_p()
+        self.ctx.check(load_library().tng_stokes_solver_mesh(self.h, level, C.byref(mh)))
+        return _BorrowedDomain(self.ctx, mh, self)
+
+    @property
+    def domain_velocity(self):
+        return self.domain(self.num_levels - 1)
+
+    @property
+    def domain_pressure(self):
+        return self.domain(self.num_levels - 2)
 
     def set_viscosity(self, eta):
         self.ctx.check(load_library().tng_stokes_solver_set_viscosity(self.h, eta.h))

@@ -101,9 +101,9 @@ def test_gca_matches_oracle(lat, rad, ls, gpu_ctx):
     got = coarse.get_local_matrices()
     ref = O.gca(of, oc, O.local_matrices(of, k))
     assert rel_err(got, ref) < 1e-13
-    # second step from stored matrices
-    dcc, occ = setup(lat - 2, rad - 2, ls, 0)
-    if occ.nr >= 3:
+    # second step from stored matrices (needs two radial layers on the coarser mesh)
+    if rad - 2 >= 1:
+        dcc, occ = setup(lat - 2, rad - 2, ls, 0)
         c2, kc2 = op_on(dcc, np.ones(occ.shape), (D, D))
         tng.gca_assemble(coarse, c2)
         assert rel_err(c2.get_local_matrices(), O.gca(oc, occ, ref)) < 1e-13
@@ -132,15 +132,15 @@ def test_stokes_solve_with_gca(dt, gpu_ctx):
     f = field(of, 3, 12)
     sols = {}
     for gca in (0, 1):
-        S = tng.StokesSolver(2, 4, 0.55, 1.0, (D, F), tng.StokesParams.defaults(
-            gca=gca, viscous_pc_dtype=dt, krylov_restart=30, krylov_max_iterations=200,
+        S = tng.StokesSolver(2, 4, 0.55, 1.0, (D, D), tng.StokesParams.defaults(
+            gca=gca, viscous_pc_dtype=dt, krylov_restart=30, krylov_max_iterations=400,
             krylov_relative_tolerance=1e-9, krylov_absolute_tolerance=1e-30))
         S.set_viscosity(vec(S.domain_velocity, 1, eta))
-        fu = vec(S.domain_velocity, 3, f)
-        S.enforce_rhs(fu)
+        fu = vec(S.domain_velocity, 3)
+        S.buoyancy_rhs(vec(S.domain_velocity, 1, f[0]), 100.0, fu)
         u, p = vec(S.domain_velocity, 3), vec(S.domain_pressure, 1)
         rep = S.solve(u, p, fu, vec(S.domain_pressure, 1))
         assert rep["converged"] == 1, (gca, rep)
         sols[gca] = u.download()
     own = of.ownership_mask == 1
-    assert rel_err(sols[1][:, own], sols[0][:, own]) < 1e-7
+    assert rel_err(sols[1][:, own], sols[0][:, own]) < 1e-5
