@@ -12,7 +12,14 @@ GDoF/s = 3 * unique nodes / seconds per apply (the reference's definition,
 apps/benchmarks/performance/benchmark_operators.cpp:213-214).
 
 Prints one JSON line.  Also reports the roofline of the apply kernel (HIP
-events on the library stream) and the CPU oracle timed on this host.
+events on the library stream), the CPU oracle timed on this host, and the
+mixed-precision Stokes solve ("solve"): Q1-iso-Q2/Q1 on the level-7 velocity
+/ level-6 pressure grids, no-slip both surfaces, multigrid levels 2..7,
+viscosity contrast 1e4 (radial profile) times log-normal(1) noise, buoyancy
+RHS from a degree <= 8 spherical-harmonic temperature, FGMRES(30) to 1e-8
+relative with the FP32 V-cycle preconditioner and again with the FP64 one;
+seconds per solve, iterations, and the FP32 solution's error against the FP64
+solve.
 """
 import argparse
 import hashlib
@@ -45,6 +52,54 @@ def measured_traffic(level, bcs, dtype):
     return rec["traffic_bytes"] if sha == rec["source_sha256"] else None
 
 
+def solve_leg(args, tng, inputs, np):
+    """Mixed-precision Stokes solve against the FP64-preconditioned solve, same inputs."""
+    lv = args.solve_level
+    out = {"workload": f"Stokes solve, velocity level {lv} / pressure level {lv - 1}, multigrid levels "
+                       f"{args.solve_min_level}..{lv}, no-slip both surfaces, eta = 1e4 radial contrast x "
+                       f"lognormal(1), buoyancy RHS (T: spherical harmonics l<=8), FGMRES({args.solve_restart}) "
+                       f"to {args.solve_rtol:g} rel, Chebyshev order 3, gca={args.solve_gca}"}
+    sols = {}
+    for name, dt in (("fp64_mg", tng.F64), ("fp32_mg", tng.F32)):
+        prm = tng.StokesParams.defaults(krylov_restart=args.solve_restart,
+                                        krylov_max_iterations=args.solve_max_iterations,
+                                        krylov_relative_tolerance=args.solve_rtol, krylov_absolute_tolerance=1e-30,
+                                        viscous_pc_chebyshev_order=3, viscous_pc_dtype=dt, gca=args.solve_gca)
+        print(f"[bench] solve {name}: building levels {args.solve_min_level}..{lv}", file=sys.stderr, flush=True)
+        c0 = time.perf_counter()
+        S = tng.StokesSolver(args.solve_min_level, lv, 0.55, 1.0, (tng.DIRICHLET, tng.DIRICHLET), prm)
+        dv, dp = S.domain_velocity, S.domain_pressure
+        if "eta" not in sols:
+            w = inputs.Workload(dv.coords(), dv.radii(), lv, lv)
+            sols["eta"] = w.eta_radial_contrast(103, 1e4, 1.0)
+            sols["T"] = w.temperature_harmonics(104, 8)
+            del w
+        S.set_viscosity(tng.VectorQ1Scalar(dv).upload(sols["eta"]))
+        fu = tng.VectorQ1Vec(dv)
+        S.buoyancy_rhs(tng.VectorQ1Scalar(dv).upload(sols["T"]), 1e5, fu)
+        fp = tng.VectorQ1Scalar(dp)
+        u, p = tng.VectorQ1Vec(dv), tng.VectorQ1Scalar(dp)
+        dv.ctx.synchronize()
+        setup_s = time.perf_counter() - c0
+        print(f"[bench] solve {name}: setup {setup_s:.2f} s; solving", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        rep = S.solve(u, p, fu, fp)
+        dv.ctx.synchronize()
+        sec = time.perf_counter() - t0
+        own = dv.ownership_mask() == 1
+        sols[name] = u.download()[:, own]
+        out[name] = {"seconds": round(sec, 4), "setup_seconds": round(setup_s, 3),
+                     "iterations": rep["iterations"], "converged": bool(rep["converged"]),
+                     "relative_residual": rep["relative_residual"], "coarse_iterations": rep["coarse_iterations"]}
+        print(f"[bench] solve {name}: {sec:.3f} s, {rep['iterations']} iterations, "
+              f"rel residual {rep['relative_residual']:.3e}", file=sys.stderr, flush=True)
+        del S, u, p, fu, fp
+    ref = sols["fp64_mg"]
+    out["fp32_mg_velocity_error_vs_fp64"] = float(np.linalg.norm(sols["fp32_mg"] - ref) / np.linalg.norm(ref))
+    out["speedup_fp32_mg"] = round(out["fp64_mg"]["seconds"] / out["fp32_mg"]["seconds"], 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -55,6 +110,13 @@ def main():
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="budget of the CPU oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true", help="skip the mixed-precision Stokes solve")
+    ap.add_argument("--solve-level", type=int, default=7, help="velocity level of the Stokes solve")
+    ap.add_argument("--solve-min-level", type=int, default=2)
+    ap.add_argument("--solve-gca", type=int, default=0)
+    ap.add_argument("--solve-restart", type=int, default=30)
+    ap.add_argument("--solve-max-iterations", type=int, default=300)
+    ap.add_argument("--solve-rtol", type=float, default=1e-8)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only the plain vector-copy kernel on the workload's vector size "
                          "(reference point for the HBM byte counters)")
@@ -189,6 +251,12 @@ def main():
             "sample": f"{n} FP64 applies of the same level-{L} workload with the oracle's restatement of the "
                       f"reference fast path (OpenMP), {csec:.4f} s/apply",
         }
+    if not args.no_solve:
+        del op, src, dst, kv, dom
+        try:
+            result["solve"] = solve_leg(args, tng, inputs, np)
+        except Exception as e:  # the apply measurement stands on its own
+            result["solve"] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(result))
 
 

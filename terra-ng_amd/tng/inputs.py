@@ -93,6 +93,8 @@ class Workload:
         self.ids = node_ids(coords, lateral_level, radial_level, lateral_sdr, radial_sdr, nr)
         self.shape = self.ids.shape
         self.r = np.broadcast_to(radii[:, None, None, :], self.shape)
+        self.coords = coords
+        self.r_min, self.r_max = float(radii.min()), float(radii.max())
 
     def velocity(self, seed=101):
         return np.stack([uniform(seed, c, self.ids) for c in range(3)])
@@ -103,3 +105,33 @@ class Workload:
     def eta_layered(self, seed=102, jump=1e3, r_jump=0.85, sigma=0.5):
         """Layered viscosity: jump below r_jump times log-normal(sigma) noise."""
         return np.where(self.r < r_jump, jump, 1.0) * np.exp(sigma * normal(seed, 0, self.ids))
+
+    def eta_radial_contrast(self, seed=103, contrast=1e4, sigma=1.0):
+        """Radial profile falling by `contrast` from the inner to the outer
+        surface (exponential in r) times log-normal(sigma) noise."""
+        t = (self.r_max - self.r) / (self.r_max - self.r_min)
+        return np.exp(np.log(contrast) * t) * np.exp(sigma * normal(seed, 0, self.ids))
+
+    def temperature_harmonics(self, seed=104, max_degree=8):
+        """sum_{l <= max_degree, |m| <= l} a_lm Y_lm(lat, lon) * sin(pi (r - r_min) / (r_max - r_min)),
+        real orthonormal spherical harmonics, a_lm ~ N(0, 1) from the seed (component l*l + l + m)."""
+        from scipy.special import sph_harm_y
+
+        x = self.coords.reshape(-1, 3)
+        theta = np.arccos(np.clip(x[:, 2] / np.linalg.norm(x, axis=1), -1.0, 1.0))
+        phi = np.arctan2(x[:, 1], x[:, 0])
+        lat = np.zeros(x.shape[0])
+        for l in range(max_degree + 1):
+            for m in range(-l, l + 1):
+                a = float(normal(seed, l * l + l + m, np.zeros(1, dtype=np.uint64))[0])
+                y = sph_harm_y(l, abs(m), theta, phi)
+                if m > 0:
+                    y = np.sqrt(2.0) * (-1) ** m * y.real
+                elif m < 0:
+                    y = np.sqrt(2.0) * (-1) ** m * y.imag
+                else:
+                    y = y.real
+                lat += a * y
+        lat = lat.reshape(self.coords.shape[:3])
+        radial = np.sin(np.pi * (self.r[0, 0, 0] - self.r_min) / (self.r_max - self.r_min))
+        return lat[..., None] * radial[None, None, None, :]
