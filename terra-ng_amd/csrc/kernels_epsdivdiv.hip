@@ -77,6 +77,31 @@ __device__ __forceinline__ float wave_shr1(float v)
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
 }
 
+// Value of lane j (a compile-time constant) as a wave-uniform scalar.
+__device__ __forceinline__ double lane_value(double v, int j)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), j);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), j);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float lane_value(float v, int j)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+
+// Lateral record of one y-step of a wave that owns one cell column (R = 64):
+// lane j holds element j, loaded one step ahead with a single vector load.
+enum : int {
+    REC_GEOM = 0,       // 20: geometry of cell (xc, yc), wedge 0 then wedge 1
+    REC_NT0 = 20,       // 3: unit normal of node (xc, yc+1)
+    REC_NT1 = 23,       // 3: unit normal of node (xc+1, yc+1)
+    REC_NOUT = 26,      // 3: unit normal of the output node (xc+1, yc)
+    REC_BTAB = 29,      // 6: boundary rows of the output node (CMB 3, surface 3)
+    REC_UCMB = 35,      // 3: src of the output node at the CMB level
+    REC_USURF = 38,     // 3: src of the output node at the surface level
+    REC_LEN = 41
+};
+
 // Raw node values of one node column at the K+1 levels spanned by a lane.
 template <typename T, int K> struct RawCol {
     T u[K + 1][3];
@@ -194,22 +219,32 @@ __device__ __forceinline__ void wedge_stress(const T *g /*M0,M1,M2,detL*/, const
     }
 }
 
-template <typename T, int R, int K, int NW>
+template <typename T, int R, int K, int NW, int RG>
 __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArgs<T> a)
 {
+    // RG > 1: the radial cells of a column are split into RG chunks of 64,
+    // one wave each (waves hc * VX + v); the node between two chunks is
+    // output by the upper chunk, which receives the lower chunk's partial
+    // sums through LDS.
+    static_assert(RG == 1 || (R == 64 && K == 1), "radial chunks need one cell per lane");
     constexpr int VPW = 64 / R;
-    constexpr int VX = NW * VPW;
+    constexpr int VX = NW / RG * VPW;
+    constexpr int NV = VX * RG;
     constexpr int KL = K + 1;
     // left-corner node values of rows yc and yc+1, radially combined, per
-    // virtual column; plus the top level of the last cell (overflow)
-    __shared__ T xch[2][VX][2][K][3][R];
-    __shared__ T xovf[2][VX][2][3];
+    // virtual column; the top level of the last cell (overflow); the top
+    // level of a lower chunk (its own cells' sum, before the lateral exchange)
+    __shared__ T xch[2][NV][2][K][3][R];
+    __shared__ T xovf[2][NV][2][3];
+    __shared__ T xtop[2][RG > 1 ? NV : 1][3];
 
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63;
     const int vw = lane / R, rl = lane % R;
-    const int v = wave * VPW + vw;
+    const int v = RG > 1 ? wave % VX : wave * VPW + vw;  // virtual column within the strip
+    const int hc = RG > 1 ? wave / VX : 0;               // radial chunk
+    const int vi = hc * VX + v;                          // LDS slot
 
     // Blocks are dispatched round-robin over the 8 XCDs; give every XCD a
     // contiguous range of logical blocks so that neighbouring strips and
@@ -233,15 +268,17 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
     const bool any_fs = (has_cmb && a.bc_cmb == TNG_FREESLIP) || (has_surf && a.bc_surf == TNG_FREESLIP);
     const bool any_dir = (has_cmb && a.bc_cmb == TNG_DIRICHLET) || (has_surf && a.bc_surf == TNG_DIRICHLET);
 
-    // lanes: cells rl*K .. rl*K+K-1 (ncells is either <= R with K = 1, or R*K)
-    const bool lane_has = rl * K < ncells;
-    const bool ovf_lane = rl * K + K == ncells;  // lane holding the last cell
-    const int lvl0 = lane_has ? rl * K : ncells - K;  // clamped first level (loads stay in bounds)
+    // lanes: cells hc*nch + rl*K .. +K-1 (nch is either <= R with K = 1, or R*K)
+    const int nch = ncells / RG;
+    const bool lane_has = rl * K < nch;
+    const bool ovf_lane = rl * K + K == nch;  // lane holding the last cell of the chunk
+    const bool bot_chunk = hc == 0, top_chunk = hc == RG - 1;
+    const int lvl0 = hc * nch + (lane_has ? rl * K : nch - K);  // clamped first level (loads stay in bounds)
     LaneBC<T> lbc;
-    lbc.z_lo = (has_cmb && rl == 0 && a.bc_cmb == TNG_DIRICHLET) ? T(0) : T(1);
-    lbc.f_lo = (has_cmb && rl == 0 && a.bc_cmb == TNG_FREESLIP) ? T(1) : T(0);
-    lbc.z_hi = (has_surf && ovf_lane && a.bc_surf == TNG_DIRICHLET) ? T(0) : T(1);
-    lbc.f_hi = (has_surf && ovf_lane && a.bc_surf == TNG_FREESLIP) ? T(1) : T(0);
+    lbc.z_lo = (has_cmb && bot_chunk && rl == 0 && a.bc_cmb == TNG_DIRICHLET) ? T(0) : T(1);
+    lbc.f_lo = (has_cmb && bot_chunk && rl == 0 && a.bc_cmb == TNG_FREESLIP) ? T(1) : T(0);
+    lbc.z_hi = (has_surf && top_chunk && ovf_lane && a.bc_surf == TNG_DIRICHLET) ? T(0) : T(1);
+    lbc.f_hi = (has_surf && top_chunk && ovf_lane && a.bc_surf == TNG_FREESLIP) ? T(1) : T(0);
 
     T qq[K], cv[K];
 #pragma unroll
@@ -265,8 +302,29 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
         n[2] = c[2];
     };
 
+    // R == 64: lateral data of the next step, one element per lane
+    auto load_record = [&](int yc) -> T {
+        // every lane computes a valid address and loads unconditionally (a
+        // branch-free load keeps the wait counts of the pipeline exact)
+        const int j = lane;
+        const int xcc = min(max(xc, 0), nc - 1), ycc = min(max(yc, 0), nc - 1);
+        const int64_t out_col = colbase(xc + 1, yc);
+        const int cs = (j - REC_UCMB) % 3;
+        const T *sc = cs == 0 ? a.src0 : (cs == 1 ? a.src1 : a.src2);
+        const T *ptr = a.latgeom + (((int64_t)sd * nc + xcc) * nc + ycc) * 20 + min(j, REC_NT0 - 1);
+        ptr = j >= REC_NT0 ? a.coords + 3 * colbase(xc, yc + 1) + (j - REC_NT0) : ptr;
+        ptr = j >= REC_NT1 ? a.coords + 3 * colbase(xc + 1, yc + 1) + (j - REC_NT1) : ptr;
+        ptr = j >= REC_NOUT ? a.coords + 3 * out_col + (j - REC_NOUT) : ptr;
+        ptr = j >= REC_BTAB && a.btab ? a.btab + 6 * out_col + (j - REC_BTAB) : ptr;
+        ptr = j >= REC_UCMB ? sc + out_col * nr + (j < REC_USURF ? 0 : nr - 1) : ptr;
+        ptr = j >= REC_LEN ? a.latgeom : ptr;
+        const T val = *ptr;
+        return j < REC_LEN ? val : T(0);
+    };
+
     NodeCol<T, K> B0, B1, T0, T1;
     RawCol<T, K> P0, P1;
+    T rec = T(0);
     {
         T n0[3] = {T(0), T(0), T(0)}, n1[3] = {T(0), T(0), T(0)};
         load_raw<T, K>(a, colbase(xc, Y0 - 1) * nr, lvl0, P0);
@@ -279,27 +337,52 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
         process<T, K>(P1, n1, lbc, any_fs, any_dir, B1);
         load_raw<T, K>(a, colbase(xc, Y0) * nr, lvl0, P0);
         load_raw<T, K>(a, colbase(xc + 1, Y0) * nr, lvl0, P1);
+        if constexpr (R == 64) rec = load_record(Y0 - 1);
     }
 
-    T acc_cur[KL][3], acc_next[KL][3];
+    // loop invariants of the output phase
+    T pb[3] = {T(0), T(0), T(0)};
+    if (a.pen) {
+        pb[0] = (T)a.pen[0];
+        pb[1] = (T)a.pen[1];
+        pb[2] = (T)a.pen[2];
+    }
+    T radl[KL];
+#pragma unroll
+    for (int p = 0; p < KL; ++p) radl[p] = a.radii[(int64_t)sd * nr + min(hc * nch + rl * K + p, nr - 1)];
+
+    T accA[KL][3], accB[KL][3];
 #pragma unroll
     for (int i = 0; i < KL; ++i)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) acc_cur[i][c] = acc_next[i][c] = T(0);
+        for (int c = 0; c < 3; ++c) accA[i][c] = accB[i][c] = T(0);
 
-    const int nsteps = Yend - (Y0 - 1);
-    for (int step = 0; step < nsteps; ++step) {
+    // One y-step: bottom rows B0/B1 (processed), the prefetched raw row yc+1
+    // becomes the top rows T0/T1; acc_cur collects node row yc, acc_next row
+    // yc+1.  The caller alternates the roles of the two row pairs and the two
+    // accumulators, so nothing is copied between steps.
+    auto do_step = [&](int step, NodeCol<T, K> &B0, NodeCol<T, K> &B1, NodeCol<T, K> &T0, NodeCol<T, K> &T1,
+                       T (&acc_cur)[KL][3], T (&acc_next)[KL][3], T &rec, T &rec_next) {
         const int yc = Y0 - 1 + step;
         // the prefetched row yc+1 becomes the top row; prefetch row yc+2
         {
             T n0[3] = {T(0), T(0), T(0)}, n1[3] = {T(0), T(0), T(0)};
             if (any_fs) {
-                load_normal(xc, yc + 1, n0);
-                load_normal(xc + 1, yc + 1, n1);
+                if constexpr (R == 64) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        n0[c] = lane_value(rec, REC_NT0 + c);
+                        n1[c] = lane_value(rec, REC_NT1 + c);
+                    }
+                } else {
+                    load_normal(xc, yc + 1, n0);
+                    load_normal(xc + 1, yc + 1, n1);
+                }
             }
             process<T, K>(P0, n0, lbc, any_fs, any_dir, T0);
             process<T, K>(P1, n1, lbc, any_fs, any_dir, T1);
         }
+        if constexpr (R == 64) rec_next = load_record(yc + 1);
         load_raw<T, K>(a, colbase(xc, yc + 2) * nr, lvl0, P0);
         load_raw<T, K>(a, colbase(xc + 1, yc + 2) * nr, lvl0, P1);
 
@@ -311,21 +394,15 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
 
         const bool cell_ok = xc >= 0 && xc < nc && yc >= 0 && yc < nc;
         {
-            const int xcc = min(max(xc, 0), nc - 1), ycc = min(max(yc, 0), nc - 1);
             T g0[10], g1[10];
             if constexpr (R == 64) {
-                // one lateral cell per wave: the geometry is wave-uniform, so it
-                // is read through the scalar cache into SGPRs (the table is
-                // never written by a kernel)
-                using CT = const T __attribute__((address_space(4)));
-                const int gi = __builtin_amdgcn_readfirstlane(((sd * nc + xcc) * nc + ycc) * 20);
-                const CT *gw = (const CT *)a.latgeom + gi;
 #pragma unroll
                 for (int i = 0; i < 10; ++i) {
-                    g0[i] = gw[i];
-                    g1[i] = gw[10 + i];
+                    g0[i] = lane_value(rec, REC_GEOM + i);
+                    g1[i] = lane_value(rec, REC_GEOM + 10 + i);
                 }
             } else {
+                const int xcc = min(max(xc, 0), nc - 1), ycc = min(max(yc, 0), nc - 1);
                 const T *gw = a.latgeom + ((((int64_t)sd * nc + xcc) * nc + ycc) * 2) * 10;
 #pragma unroll
                 for (int i = 0; i < 10; ++i) {
@@ -373,12 +450,15 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
             const T b00 = wave_shr1(s00[K][c]), b01 = wave_shr1(s01[K][c]);
 #pragma unroll
             for (int p = 0; p < K; ++p) {
-                xch[buf][v][0][p][c][rl] = s00[p][c] + (p == 0 && rl > 0 ? b00 : T(0));
-                xch[buf][v][1][p][c][rl] = s01[p][c] + (p == 0 && rl > 0 ? b01 : T(0));
+                xch[buf][vi][0][p][c][rl] = s00[p][c] + (p == 0 && rl > 0 ? b00 : T(0));
+                xch[buf][vi][1][p][c][rl] = s01[p][c] + (p == 0 && rl > 0 ? b01 : T(0));
             }
             if (ovf_lane) {
-                xovf[buf][v][0][c] = s00[K][c];
-                xovf[buf][v][1][c] = s01[K][c];
+                xovf[buf][vi][0][c] = s00[K][c];
+                xovf[buf][vi][1][c] = s01[K][c];
+                if constexpr (RG > 1) {
+                    if (!top_chunk) xtop[buf][vi][c] = acc_cur[K][c];
+                }
             }
         }
         __syncthreads();
@@ -387,12 +467,12 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
             for (int c = 0; c < 3; ++c) {
 #pragma unroll
                 for (int p = 0; p < K; ++p) {
-                    acc_cur[p][c] += xch[buf][v + 1][0][p][c][rl];
-                    acc_next[p][c] += xch[buf][v + 1][1][p][c][rl];
+                    acc_cur[p][c] += xch[buf][vi + 1][0][p][c][rl];
+                    acc_next[p][c] += xch[buf][vi + 1][1][p][c][rl];
                 }
                 if (ovf_lane) {
-                    acc_cur[K][c] += xovf[buf][v + 1][0][c];
-                    acc_next[K][c] += xovf[buf][v + 1][1][c];
+                    acc_cur[K][c] += xovf[buf][vi + 1][0][c];
+                    acc_next[K][c] += xovf[buf][vi + 1][1][c];
                 }
             }
         }
@@ -403,47 +483,68 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
         for (int c = 0; c < 3; ++c) {
             const T up = wave_shr1(acc_cur[K][c]);
             below[c] = rl > 0 ? up : T(0);
+            if constexpr (RG > 1) {
+                // lane 0 of an upper chunk: the lower chunk's top level
+                if (rl == 0 && !bot_chunk && v + 1 < VX)
+                    below[c] = xtop[buf][vi - VX][c] + xovf[buf][vi - VX + 1][0][c];
+            }
         }
         const int Y = yc;
         if (Y >= Y0 && v < VX - 1 && X < nx && lane_has) {
             const int64_t lcol = ((int64_t)sd * nx + X) * nx + Y;
             const int64_t base = lcol * nr;
-            T pb[3] = {T(0), T(0), T(0)}, cc[3] = {T(0), T(0), T(0)};
-            if (a.pen) {
-                pb[0] = (T)a.pen[0];
-                pb[1] = (T)a.pen[1];
-                pb[2] = (T)a.pen[2];
-                cc[0] = a.coords[3 * lcol];
-                cc[1] = a.coords[3 * lcol + 1];
-                cc[2] = a.coords[3 * lcol + 2];
+            // unit normal of the output node (free-slip rows, penalty)
+            T cc[3] = {T(0), T(0), T(0)};
+            if (a.pen || any_fs) {
+                if constexpr (R == 64) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) cc[c] = lane_value(rec, REC_NOUT + c);
+                } else {
+                    cc[0] = a.coords[3 * lcol];
+                    cc[1] = a.coords[3 * lcol + 1];
+                    cc[2] = a.coords[3 * lcol + 2];
+                }
             }
 #pragma unroll
             for (int p = 0; p < KL; ++p) {
-                if (p == K && !ovf_lane) continue;
-                const int l = rl * K + p;
+                if (p == K && !(ovf_lane && top_chunk)) continue;
+                const int l = hc * nch + rl * K + p;
                 T val[3];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) val[c] = acc_cur[p][c] + (p == 0 ? below[c] : T(0));
-                const bool at_cmb = p == 0 && rl == 0 && has_cmb && a.bc_cmb != TNG_NEUMANN;
-                const bool at_surf = p == K && has_surf && a.bc_surf != TNG_NEUMANN;
+                const bool at_cmb = p == 0 && rl == 0 && bot_chunk && has_cmb && a.bc_cmb != TNG_NEUMANN;
+                const bool at_surf = p == K && top_chunk && has_surf && a.bc_surf != TNG_NEUMANN;
                 if (at_cmb || at_surf) {
                     const int bcv = at_cmb ? a.bc_cmb : a.bc_surf;
-                    const T *bt = a.btab + (lcol * 2 + (at_cmb ? 0 : 1)) * 3;
-                    const T u0 = a.src0[base + l], u1 = a.src1[base + l], u2 = a.src2[base + l];
-                    if (bcv == TNG_DIRICHLET) {
-                        val[0] = bt[0] * u0;
-                        val[1] = bt[1] * u1;
-                        val[2] = bt[2] * u2;
-                    } else {
-                        const T *n = a.coords + 3 * lcol;
-                        const T dv = n[0] * val[0] + n[1] * val[1] + n[2] * val[2];
-                        const T un = bt[0] * (n[0] * u0 + n[1] * u1 + n[2] * u2);
+                    T bt[3], u[3];
+                    if constexpr (R == 64) {
 #pragma unroll
-                        for (int c = 0; c < 3; ++c) val[c] = val[c] - dv * n[c] + un * n[c];
+                        for (int c = 0; c < 3; ++c) {
+                            bt[c] = at_cmb ? lane_value(rec, REC_BTAB + c) : lane_value(rec, REC_BTAB + 3 + c);
+                            u[c] = at_cmb ? lane_value(rec, REC_UCMB + c) : lane_value(rec, REC_USURF + c);
+                        }
+                    } else {
+                        const T *btp = a.btab + (lcol * 2 + (at_cmb ? 0 : 1)) * 3;
+                        bt[0] = btp[0];
+                        bt[1] = btp[1];
+                        bt[2] = btp[2];
+                        u[0] = a.src0[base + l];
+                        u[1] = a.src1[base + l];
+                        u[2] = a.src2[base + l];
+                    }
+                    if (bcv == TNG_DIRICHLET) {
+                        val[0] = bt[0] * u[0];
+                        val[1] = bt[1] * u[1];
+                        val[2] = bt[2] * u[2];
+                    } else {
+                        const T dv = cc[0] * val[0] + cc[1] * val[1] + cc[2] * val[2];
+                        const T un = bt[0] * (cc[0] * u[0] + cc[1] * u[1] + cc[2] * u[2]);
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) val[c] = val[c] - dv * cc[c] + un * cc[c];
                     }
                 }
                 if (a.pen) {
-                    const T rad = a.radii[(int64_t)sd * nr + l];
+                    const T rad = radl[p];
                     const T x0 = cc[0] * rad, x1 = cc[1] * rad, x2 = cc[2] * rad;
                     val[0] += pb[1] * x2 - pb[2] * x1;
                     val[1] += pb[2] * x0 - pb[0] * x2;
@@ -462,12 +563,32 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
 #pragma unroll
         for (int i = 0; i < KL; ++i)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                acc_cur[i][c] = acc_next[i][c];
-                acc_next[i][c] = T(0);
-            }
-        B0 = T0;
-        B1 = T1;
+            for (int c = 0; c < 3; ++c) acc_cur[i][c] = T(0);
+    };
+
+    const int nsteps = Yend - (Y0 - 1);
+    NodeCol<T, K> C0, C1;
+    T rec2 = T(0);
+    if constexpr (R == 64 && K == 1) {
+        // two steps per iteration with alternating roles (no register copies)
+        for (int step = 0; step < nsteps; step += 2) {
+            do_step(step, B0, B1, C0, C1, accA, accB, rec, rec2);
+            if (step + 1 < nsteps) do_step(step + 1, C0, C1, B0, B1, accB, accA, rec2, rec);
+        }
+    } else {
+        for (int step = 0; step < nsteps; ++step) {
+            do_step(step, B0, B1, C0, C1, accA, accB, rec, rec2);
+            B0 = C0;
+            B1 = C1;
+            rec = rec2;
+#pragma unroll
+            for (int i = 0; i < KL; ++i)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    accA[i][c] = accB[i][c];
+                    accB[i][c] = T(0);
+                }
+        }
     }
 }
 
@@ -733,9 +854,10 @@ template <> MeshPtrs<float> mesh_ptrs<float>(const tng_domain &m)
     return {m.d_coords_f.as<float>(), m.d_radii_f.as<float>(), m.d_latgeom_f.as<float>()};
 }
 
-template <typename T, int R, int K, int NW> void launch_apply_rk(const ApplyArgs<T> &a, int blocks, hipStream_t s)
+template <typename T, int R, int K, int NW, int RG>
+void launch_apply_rk(const ApplyArgs<T> &a, int blocks, hipStream_t s)
 {
-    hipLaunchKernelGGL((epsdivdiv_apply_kernel<T, R, K, NW>), dim3(blocks), dim3(NW * 64), 0, s, a);
+    hipLaunchKernelGGL((epsdivdiv_apply_kernel<T, R, K, NW, RG>), dim3(blocks), dim3(NW * 64), 0, s, a);
 }
 
 int env_int(const char *name, int dflt)
@@ -747,18 +869,25 @@ int env_int(const char *name, int dflt)
 template <typename T> void launch_apply(ApplyArgs<T> a, hipStream_t s)
 {
     TNG_REQUIRE(a.ncells >= 1 && (a.ncells & (a.ncells - 1)) == 0, "radial layers per subdomain must be a power of 2");
-    int R, K;
+    // shapes: up to 64 radial cells one cell per lane (R lanes per column);
+    // 128 cells in two radial chunks of one wave each; more cells K per lane
+    int R, K, RG = 1;
     if (a.ncells <= 64) {
         K = 1;
         R = 4;
         while (R < a.ncells) R *= 2;
+    } else if (a.ncells == 128 && env_int("TNG_APPLY_RADIAL_CHUNKS", 2) == 2) {
+        R = 64;
+        K = 1;
+        RG = 2;
     } else {
         R = 64;
         K = a.ncells / 64;
         TNG_REQUIRE(K <= 4, "more than 256 radial layers per subdomain is not supported");
     }
-    const int NW = K == 1 ? 8 : 4;
-    const int VX = NW * (64 / R);
+    int NW = K == 1 ? 8 : 4;
+    if (K == 1 && R == 64 && RG == 1 && env_int("TNG_APPLY_WAVES", 8) == 4) NW = 4;
+    const int VX = NW / RG * (64 / R);
     a.strips = (a.nx + (VX - 1) - 1) / (VX - 1);
     // y-segment length: fewer, longer segments recompute fewer halo rows;
     // enough workgroups are needed to fill the 256 CUs
@@ -773,18 +902,23 @@ template <typename T> void launch_apply(ApplyArgs<T> a, hipStream_t s)
     a.seg_len = seg;
     a.segs = (a.nx + seg - 1) / seg;
     const int blocks = (int)(lateral_groups * a.segs);
-    if (K == 1) {
+    if (RG == 2) {
+        launch_apply_rk<T, 64, 1, 8, 2>(a, blocks, s);
+    } else if (K == 1) {
         switch (R) {
-        case 4: launch_apply_rk<T, 4, 1, 8>(a, blocks, s); break;
-        case 8: launch_apply_rk<T, 8, 1, 8>(a, blocks, s); break;
-        case 16: launch_apply_rk<T, 16, 1, 8>(a, blocks, s); break;
-        case 32: launch_apply_rk<T, 32, 1, 8>(a, blocks, s); break;
-        default: launch_apply_rk<T, 64, 1, 8>(a, blocks, s); break;
+        case 4: launch_apply_rk<T, 4, 1, 8, 1>(a, blocks, s); break;
+        case 8: launch_apply_rk<T, 8, 1, 8, 1>(a, blocks, s); break;
+        case 16: launch_apply_rk<T, 16, 1, 8, 1>(a, blocks, s); break;
+        case 32: launch_apply_rk<T, 32, 1, 8, 1>(a, blocks, s); break;
+        default:
+            if (NW == 4) launch_apply_rk<T, 64, 1, 4, 1>(a, blocks, s);
+            else launch_apply_rk<T, 64, 1, 8, 1>(a, blocks, s);
+            break;
         }
     } else if (K == 2) {
-        launch_apply_rk<T, 64, 2, 4>(a, blocks, s);
+        launch_apply_rk<T, 64, 2, 4, 1>(a, blocks, s);
     } else {
-        launch_apply_rk<T, 64, 4, 4>(a, blocks, s);
+        launch_apply_rk<T, 64, 4, 4, 1>(a, blocks, s);
     }
     TNG_HIP_CHECK(hipGetLastError());
 }

@@ -85,6 +85,26 @@ def test_parity_shapes(gpu_ctx, L, Lr, ls, rs, bcs):
     assert rel_err(g, ref) < 1e-12
 
 
+@pytest.mark.parametrize("L,Lr,env", [(2, 7, {"TNG_APPLY_RADIAL_CHUNKS": "1"}), (3, 6, {"TNG_APPLY_WAVES": "4"}),
+                                       (1, 8, {}), (3, 7, {})])
+@pytest.mark.parametrize("bcs", [(D, D), (F, F), (N, N)])
+def test_parity_launch_variants(gpu_ctx, L, Lr, env, bcs, monkeypatch):
+    """Launch shapes chosen by layer count or tuning variable: 128 layers as
+    two radial chunks (default) or two cells per lane, 4-wave workgroups at
+    64 layers, four cells per lane at 256 layers."""
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    dom, om, w = setup(L, Lr)
+    k = w.eta_lognormal(102)
+    u = w.velocity(101)
+    for add in (False, True):
+        d0 = w.velocity(103)
+        g = gpu_apply(dom, k, u, bcs, eps=1.0, apply_mode=tng.ADD if add else tng.REPLACE, dst0=d0)
+        ref = om.epsdivdiv(k, u, *bcs, path="fast", penalty_eps=1.0, apply_mode=tng.ADD if add else tng.REPLACE,
+                           dst=d0)
+        assert rel_err(g, ref) < 1e-12
+
+
 @pytest.mark.parametrize("mode", [(tng.REPLACE, tng.SKIP_COMMUNICATION), (tng.ADD, tng.SKIP_COMMUNICATION),
                                   (tng.ADD, tng.COMMUNICATE_ADDITIVELY)])
 @pytest.mark.parametrize("bcs", [(D, D), (F, F)])
