@@ -57,8 +57,6 @@ tng_status tng_ctx_create(int device_ordinal, tng_ctx *out)
         set_device(ctx);
         TNG_HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         ctx->red_partials.alloc(4096 * 3 * sizeof(double));
-        ctx->red_counter.alloc(64);
-        TNG_HIP_CHECK(hipMemset(ctx->red_counter.p, 0, 64));
         ctx->red_result.alloc(4096);
         TNG_HIP_CHECK(hipDeviceSynchronize());
     });
@@ -85,8 +83,6 @@ tng_status tng_ctx_create_with_stream(int device_ordinal, void *hip_stream, tng_
         ctx->stream = static_cast<hipStream_t>(hip_stream);
         ctx->owns_stream = false;
         ctx->red_partials.alloc(4096 * 3 * sizeof(double));
-        ctx->red_counter.alloc(64);
-        TNG_HIP_CHECK(hipMemsetAsync(ctx->red_counter.p, 0, 64, ctx->stream));
         ctx->red_result.alloc(4096);
         TNG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     });

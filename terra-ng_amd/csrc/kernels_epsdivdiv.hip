@@ -751,17 +751,16 @@ template <typename T> __global__ void epsdivdiv_diag_kernel(const DiagArgs<T> a)
 }
 
 // ----------------------------------------------------------------------------
-// fs/fs rotation penalty, first half: m = sum_OWNED x cross u, then on the
-// last block b = eps * G^{-1} m (G = Gram matrix of the rotation fields).
-// Two-stage deterministic reduction: per-block partials in fixed order, the
-// block that takes the last ticket sums them and resets the ticket.
+// fs/fs rotation penalty, first half: m = sum_OWNED x cross u, then
+// b = eps * G^{-1} m (G = Gram matrix of the rotation fields).  Two launches:
+// per-block partials, then one block sums them in a fixed order, so the
+// result is deterministic.
 // ----------------------------------------------------------------------------
 template <typename T> struct PenArgs {
     const T *src0, *src1, *src2;
     const T *coords, *radii;
     const uint8_t *omask;
     double *partials;
-    unsigned int *counter;
     double *out_b;
     double eps;
     double ginv[9];
@@ -775,71 +774,65 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-template <typename T> __global__ __launch_bounds__(256) void penalty_moment_kernel(const PenArgs<T> a)
+// Rotation moments m = sum over OWNED copies of x (x) u (x = radius * unit
+// coordinate): per-block partials, flat over the node copies (coalesced
+// streams of src and the mask), four independent nodes per thread and step.
+template <typename T> __global__ __launch_bounds__(256) void penalty_partial_kernel(const PenArgs<T> a)
 {
     __shared__ double red[4][3];
-    __shared__ bool last;
     double m0 = 0, m1 = 0, m2 = 0;
-    // one wave per lateral column, lanes along r (coalesced, no per-element index arithmetic)
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int64_t ncol = (int64_t)a.nsd * a.nx * a.nx;
-    const int64_t nxx = (int64_t)a.nx * a.nx;
-    for (int64_t col = (int64_t)blockIdx.x * 4 + wave; col < ncol; col += (int64_t)gridDim.x * 4) {
-        const int sd = (int)(col / nxx);
-        const double c0 = (double)a.coords[3 * col], c1 = (double)a.coords[3 * col + 1],
-                     c2 = (double)a.coords[3 * col + 2];
-        const int64_t base = col * a.nr;
-        for (int l = lane; l < a.nr; l += 64) {
-            const double w = a.omask[base + l] == OWNED ? 1.0 : 0.0;
-            const double rad = (double)a.radii[(int64_t)sd * a.nr + l] * w;
-            const double x0 = c0 * rad, x1 = c1 * rad, x2 = c2 * rad;
-            const double u0 = a.src0[base + l], u1 = a.src1[base + l], u2 = a.src2[base + l];
-            m0 += x1 * u2 - x2 * u1;
-            m1 += x2 * u0 - x0 * u2;
-            m2 += x0 * u1 - x1 * u0;
+    const int n = (int)a.n, nr = a.nr, nxx = a.nx * a.nx;
+    const int stride = gridDim.x * blockDim.x;
+    for (int i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += 4 * stride) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = i0 + q * stride;
+            if (i < n) {
+                const int col = i / nr, l = i - col * nr, sd = col / nxx;
+                const double w = a.omask[i] == OWNED ? 1.0 : 0.0;
+                const double rad = (double)a.radii[sd * nr + l] * w;
+                const double x0 = (double)a.coords[3 * col] * rad, x1 = (double)a.coords[3 * col + 1] * rad,
+                             x2 = (double)a.coords[3 * col + 2] * rad;
+                const double u0 = a.src0[i], u1 = a.src1[i], u2 = a.src2[i];
+                m0 += x1 * u2 - x2 * u1;
+                m1 += x2 * u0 - x0 * u2;
+                m2 += x0 * u1 - x1 * u0;
+            }
         }
     }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     m0 = wave_sum(m0); m1 = wave_sum(m1); m2 = wave_sum(m2);
     if (lane == 0) { red[wave][0] = m0; red[wave][1] = m1; red[wave][2] = m2; }
     __syncthreads();
     if (threadIdx.x == 0) {
         double s[3] = {0, 0, 0};
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+        for (int w = 0; w < 4; ++w)
             for (int c = 0; c < 3; ++c) s[c] += red[w][c];
         for (int c = 0; c < 3; ++c) a.partials[3 * blockIdx.x + c] = s[c];
-        __threadfence();
-        const unsigned int ticket = atomicAdd(a.counter, 1u);
-        last = (ticket == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (last) {
-        // the last block sums the per-block partials in a fixed order: thread t
-        // takes partials t, t+256, ..., then a fixed wave/block tree
-        __threadfence();
-        double s[3] = {0, 0, 0};
-        for (unsigned int bb = threadIdx.x; bb < gridDim.x; bb += blockDim.x)
-            for (int c = 0; c < 3; ++c) {
-                const unsigned long long bits = __hip_atomic_load(
-                    reinterpret_cast<unsigned long long *>(&a.partials[3 * bb + c]), __ATOMIC_RELAXED,
-                    __HIP_MEMORY_SCOPE_AGENT);
-                s[c] += __longlong_as_double((long long)bits);
-            }
-        for (int c = 0; c < 3; ++c) s[c] = wave_sum(s[c]);
-        __syncthreads();
-        if (lane == 0) { red[wave][0] = s[0]; red[wave][1] = s[1]; red[wave][2] = s[2]; }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double m[3] = {0, 0, 0};
-            for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
-                for (int c = 0; c < 3; ++c) m[c] += red[w][c];
-            for (int r = 0; r < 3; ++r)
-                a.out_b[r] = a.eps * (a.ginv[3 * r] * m[0] + a.ginv[3 * r + 1] * m[1] + a.ginv[3 * r + 2] * m[2]);
-            atomicExch(a.counter, 0u);
-        }
     }
 }
 
-constexpr int kPenBlocks = 512;
+// One block: the partials in a fixed order, then b = eps G^{-1} m.
+template <typename T> __global__ __launch_bounds__(256) void penalty_final_kernel(const PenArgs<T> a, int nb)
+{
+    __shared__ double red[4][3];
+    double s[3] = {0, 0, 0};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        for (int c = 0; c < 3; ++c) s[c] += a.partials[3 * b + c];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int c = 0; c < 3; ++c) s[c] = wave_sum(s[c]);
+    if (lane == 0) { red[wave][0] = s[0]; red[wave][1] = s[1]; red[wave][2] = s[2]; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m[3] = {0, 0, 0};
+        for (int w = 0; w < 4; ++w)
+            for (int c = 0; c < 3; ++c) m[c] += red[w][c];
+        for (int r = 0; r < 3; ++r)
+            a.out_b[r] = a.eps * (a.ginv[3 * r] * m[0] + a.ginv[3 * r + 1] * m[1] + a.ginv[3 * r + 2] * m[2]);
+    }
+}
+
+constexpr int kPenBlocks = 2048;
 
 template <typename T> struct MeshPtrs {
     const T *coords, *radii, *latgeom;
@@ -960,12 +953,13 @@ template <typename T> void apply_impl(tng_operator &op, const tng_vector &src, t
         p.src0 = (const T *)src.comp(0); p.src1 = (const T *)src.comp(1); p.src2 = (const T *)src.comp(2);
         p.coords = mp.coords; p.radii = mp.radii; p.omask = m.d_omask.as<uint8_t>();
         p.partials = ctx.red_partials.as<double>();
-        p.counter = ctx.red_counter.as<unsigned int>();
         p.out_b = op.d_penalty_b.as<double>();
         p.eps = op.penalty_eps;
         for (int i = 0; i < 9; ++i) p.ginv[i] = m.rot_gram_inv[i];
         p.nsd = m.nsd; p.nx = m.nx; p.nr = m.nr; p.n = m.nalloc;
-        hipLaunchKernelGGL(penalty_moment_kernel<T>, dim3(kPenBlocks), dim3(256), 0, s, p);
+        TNG_REQUIRE(m.nalloc < (int64_t)1 << 31, "penalty moment: node copies exceed 32-bit indexing");
+        hipLaunchKernelGGL(penalty_partial_kernel<T>, dim3(kPenBlocks), dim3(256), 0, s, p);
+        hipLaunchKernelGGL(penalty_final_kernel<T>, dim3(1), dim3(256), 0, s, p, kPenBlocks);
         TNG_HIP_CHECK(hipGetLastError());
     };
     if (op.stored) {

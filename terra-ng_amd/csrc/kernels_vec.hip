@@ -159,56 +159,53 @@ enum RedOp { RED_DOT = 0, RED_MAXABS = 1, RED_NONFINITE = 2 };
 
 // op RED_DOT: sum over OWNED copies of sum_c x_c y_c (y may alias x)
 // op RED_MAXABS: max |x| over all copies; RED_NONFINITE: count of non-finite entries
+// Two launches: per-block partials, then one block combines them in a fixed
+// order (deterministic for a given grid; no inter-block fences or atomics).
+template <int OP> __device__ __forceinline__ double red_combine(double a, double b)
+{
+    return OP == RED_MAXABS ? fmax(a, b) : a + b;
+}
+
+template <int OP> __device__ __forceinline__ double block_reduce(double acc, double *red)
+{
+    acc = OP == RED_MAXABS ? wmax(acc) : wsum(acc);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) red[wave] = acc;
+    __syncthreads();
+    double s = red[0];
+    for (int w = 1; w < 4; ++w) s = red_combine<OP>(s, red[w]);
+    return s;
+}
+
 template <typename T, int OP>
-__global__ __launch_bounds__(256) void reduce_kernel(const Comp3<T> x, const Comp3<T> y, int ncomp,
-                                                     const uint8_t *omask, int64_t n, double *partials,
-                                                     unsigned int *counter, double *out)
+__global__ __launch_bounds__(256) void reduce_partial_kernel(const Comp3<T> x, const Comp3<T> y, int ncomp,
+                                                             const uint8_t *omask, int64_t n, double *partials)
 {
     __shared__ double red[4];
-    __shared__ bool last;
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         if (OP == RED_DOT) {
-            if (omask[i] != OWNED) continue;
-            for (int c = 0; c < ncomp; ++c) acc += (double)x.p[c][i] * (double)y.p[c][i];
+            const bool own = omask[i] == OWNED;
+            double v = 0.0;
+            for (int c = 0; c < ncomp; ++c) v += (double)x.p[c][i] * (double)y.p[c][i];
+            acc += own ? v : 0.0;
         } else if (OP == RED_MAXABS) {
             for (int c = 0; c < ncomp; ++c) acc = fmax(acc, fabs((double)x.p[c][i]));
         } else {
             for (int c = 0; c < ncomp; ++c) acc += isfinite((double)x.p[c][i]) ? 0.0 : 1.0;
         }
     }
-    acc = OP == RED_MAXABS ? wmax(acc) : wsum(acc);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (lane == 0) red[wave] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = red[0];
-        for (int w = 1; w < 4; ++w) s = OP == RED_MAXABS ? fmax(s, red[w]) : s + red[w];
-        partials[blockIdx.x] = s;
-        __threadfence();
-        last = atomicAdd(counter, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (last) {
-        __threadfence();
-        double s = 0.0;
-        for (unsigned int b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
-            const unsigned long long bits = __hip_atomic_load(reinterpret_cast<unsigned long long *>(&partials[b]),
-                                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const double v = __longlong_as_double((long long)bits);
-            s = OP == RED_MAXABS ? fmax(s, v) : s + v;
-        }
-        s = OP == RED_MAXABS ? wmax(s) : wsum(s);
-        __syncthreads();
-        if (lane == 0) red[wave] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double t = red[0];
-            for (int w = 1; w < 4; ++w) t = OP == RED_MAXABS ? fmax(t, red[w]) : t + red[w];
-            *out = t;
-            atomicExch(counter, 0u);
-        }
-    }
+    const double s = block_reduce<OP>(acc, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+template <int OP> __global__ __launch_bounds__(256) void reduce_final_kernel(const double *partials, int nb, double *out)
+{
+    __shared__ double red[4];
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) acc = red_combine<OP>(acc, partials[b]);
+    const double s = block_reduce<OP>(acc, red);
+    if (threadIdx.x == 0) *out = s;
 }
 
 constexpr int kRedBlocks = 1024;
@@ -272,11 +269,13 @@ void launch_reduce(int op, const tng_vector &x, const tng_vector *y, int slot, h
     double *out = ctx.red_result.as<double>() + slot;
     const uint8_t *om = x.mesh->d_omask.as<uint8_t>();
     double *part = ctx.red_partials.as<double>();
-    unsigned int *cnt = ctx.red_counter.as<unsigned int>();
     const tng_vector &yy = y ? *y : x;
 #define TNG_RED(T, OPV)                                                                                            \
-    hipLaunchKernelGGL((reduce_kernel<T, OPV>), dim3(g), dim3(256), 0, s, comps<T>(x), comps<T>(yy), x.ncomp, om, n, \
-                       part, cnt, out)
+    do {                                                                                                           \
+        hipLaunchKernelGGL((reduce_partial_kernel<T, OPV>), dim3(g), dim3(256), 0, s, comps<T>(x), comps<T>(yy),   \
+                           x.ncomp, om, n, part);                                                                  \
+        hipLaunchKernelGGL((reduce_final_kernel<OPV>), dim3(1), dim3(256), 0, s, part, (int)g, out);               \
+    } while (0)
     if (x.dtype == TNG_F64) {
         if (op == RED_DOT) TNG_RED(double, RED_DOT);
         else if (op == RED_MAXABS) TNG_RED(double, RED_MAXABS);

@@ -71,7 +71,6 @@ struct tng_context {
     bool profiling = false;
     // Scratch for two-stage reductions: partial sums + a ticket counter.
     tng::DevBuf red_partials;   // doubles
-    tng::DevBuf red_counter;    // unsigned int, kept at zero between launches
     tng::DevBuf red_result;     // small device result slots (doubles)
     // Profiling: per apply four events (begin, main kernel begin/end, end)
     // recorded on the stream without synchronising; read back in bulk.
