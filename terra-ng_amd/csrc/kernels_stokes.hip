@@ -192,10 +192,42 @@ template <typename T> struct GradArgs {
     int bc_cmb, bc_surf, add;
 };
 
-// B^T per fine node: sum over the <= 12 fine wedges containing the node.
-template <typename T> __global__ __launch_bounds__(256) void gradient_kernel(const GradArgs<T> a)
+// Phase 1 of B^T: per fine wedge, f = -|det J| p(centroid), with p
+// interpolated from coarse wedge w of the coarse cell.  One thread per fine
+// cell (both wedges), r fastest.
+template <typename T> __global__ __launch_bounds__(256) void grad_weight_kernel(const GradArgs<T> a, T *fw)
 {
-    const int nx = a.nx, nr = a.nr, nc = nx - 1;
+    const int nx = a.nx, nr = a.nr, nc = nx - 1, ncr = nr - 1;
+    const int64_t n = (int64_t)a.nsd * nc * nc * ncr;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n) return;
+    const int rc = (int)(gid % ncr);
+    int64_t t = gid / ncr;
+    const int yc = (int)(t % nc); t /= nc;
+    const int xc = (int)(t % nc);
+    const int sd = (int)(t / nc);
+    const T r0 = a.radii[(int64_t)sd * nr + rc], r1 = a.radii[(int64_t)sd * nr + rc + 1];
+    const T rmid = T(0.5) * (r0 + r1), h = T(0.5) * (r1 - r0);
+    const int cx = xc / 2, cy = yc / 2, cr = rc / 2;
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+        const T *g = a.latgeom + ((((int64_t)sd * nc + xc) * nc + yc) * 2 + w) * 10;
+        const int lat = fine_lat_idx(xc, yc, w);
+        T pi = T(0);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const int jx = cx + kWC[w][j % 3][0], jy = cy + kWC[w][j % 3][1], jr = cr + j / 3;
+            pi += shape_coarse_c<T>(j, lat, rc & 1) * a.p[(((int64_t)sd * a.nxc + jx) * a.nxc + jy) * a.nrc + jr];
+        }
+        fw[gid * 2 + w] = -(g[9] * rmid * rmid * h) * pi;
+    }
+}
+
+// Phase 2 of B^T per fine node: sum over the <= 12 fine wedges containing the
+// node of f_w g_node, with g_node = J^{-T} grad N at the centroid.
+template <typename T> __global__ __launch_bounds__(256) void gradient_kernel(const GradArgs<T> a, const T *fw)
+{
+    const int nx = a.nx, nr = a.nr, nc = nx - 1, ncr = nr - 1;
     const int64_t n = (int64_t)a.nsd * nx * nx * nr;
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= n) return;
@@ -207,52 +239,55 @@ template <typename T> __global__ __launch_bounds__(256) void gradient_kernel(con
     const int64_t lcol = t * nx + Y;
     const int corner_of[2][2][2] = {{{0, 2}, {1, -1}}, {{-1, 1}, {2, 0}}};
     const bool has_cmb = a.sd_cmb[sd] != 0, has_surf = a.sd_surf[sd] != 0;
+    // radial factors of the (at most) two cells below / above the node
+    T sfac[2], dzf[2];
+    int cbc[2];
+    bool onb[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int rc = l - 1 + q;
+        const int rcc = min(max(rc, 0), ncr - 1);
+        const T r0 = a.radii[(int64_t)sd * nr + rcc], r1 = a.radii[(int64_t)sd * nr + rcc + 1];
+        const T rmid = T(0.5) * (r0 + r1), h = T(0.5) * (r1 - r0);
+        sfac[q] = T(1) / rmid;
+        dzf[q] = (q == 1 ? T(-1) : T(1)) / (T(6) * h);
+        int blayer;
+        cbc[q] = cell_bc(rcc, nr, has_cmb, has_surf, a.bc_cmb, a.bc_surf, blayer);
+        onb[q] = blayer == (q == 1 ? 0 : 1);
+    }
     // o: interior contributions; ob: contributions of boundary cells in which
     // this node lies on the boundary layer (free-slip: projected at the end)
     T o[3] = {T(0), T(0), T(0)}, ob[3] = {T(0), T(0), T(0)};
-    int bbc = TNG_NEUMANN;
+    bool any_fs = false;
     for (int dxc = -1; dxc <= 0; ++dxc) {
         const int xc = X + dxc;
         if (xc < 0 || xc >= nc) continue;
         for (int dyc = -1; dyc <= 0; ++dyc) {
             const int yc = Y + dyc;
             if (yc < 0 || yc >= nc) continue;
+            const int64_t cb = (((int64_t)sd * nc + xc) * nc + yc) * ncr;
             for (int w = 0; w < 2; ++w) {
                 const int k = corner_of[w][X - xc][Y - yc];
                 if (k < 0) continue;
                 const T *g = a.latgeom + ((((int64_t)sd * nc + xc) * nc + yc) * 2 + w) * 10;
                 const T l0 = k == 0 ? T(-0.5) : (k == 1 ? T(0.5) : T(0));
                 const T l1 = k == 0 ? T(-0.5) : (k == 1 ? T(0) : T(0.5));
-                const int lat = fine_lat_idx(xc, yc, w);
-                for (int rc = l - 1; rc <= l; ++rc) {
-                    if (rc < 0 || rc >= nr - 1) continue;
-                    int blayer;
-                    const int cbc = cell_bc(rc, nr, has_cmb, has_surf, a.bc_cmb, a.bc_surf, blayer);
-                    const bool on_bd = blayer == (rc == l ? 0 : 1);
-                    if (on_bd && cbc == TNG_DIRICHLET) continue;
-                    const T r0 = a.radii[(int64_t)sd * nr + rc], r1 = a.radii[(int64_t)sd * nr + rc + 1];
-                    const T rmid = T(0.5) * (r0 + r1), h = T(0.5) * (r1 - r0);
-                    const T s = T(1) / rmid, dz = (rc == l ? T(-1) : T(1)) / (T(6) * h);
-                    // pressure interpolated at the centroid from coarse wedge w of cell (xc/2, yc/2, rc/2)
-                    const int cx = xc / 2, cy = yc / 2, cr = rc / 2;
-                    T pi = T(0);
-                    for (int j = 0; j < 6; ++j) {
-                        const int jx = cx + kWC[w][j % 3][0], jy = cy + kWC[w][j % 3][1], jr = cr + j / 3;
-                        pi += shape_coarse_c<T>(j, lat, rc & 1) *
-                              a.p[(((int64_t)sd * a.nxc + jx) * a.nxc + jy) * a.nrc + jr];
-                    }
-                    const T f = -(g[9] * rmid * rmid * h) * pi;
-                    T *acc = o;
-                    if (on_bd && cbc == TNG_FREESLIP) {
-                        acc = ob;
-                        bbc = TNG_FREESLIP;
-                    }
-                    for (int d = 0; d < 3; ++d) acc[d] += f * (s * (l0 * g[d] + l1 * g[3 + d]) + dz * g[6 + d]);
+                const T gl[3] = {l0 * g[0] + l1 * g[3], l0 * g[1] + l1 * g[4], l0 * g[2] + l1 * g[5]};
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    const int rc = l - 1 + q;
+                    if (rc < 0 || rc >= ncr) continue;
+                    if (onb[q] && cbc[q] == TNG_DIRICHLET) continue;
+                    const T f = fw[(cb + rc) * 2 + w];
+                    const bool to_b = onb[q] && cbc[q] == TNG_FREESLIP;
+                    any_fs = any_fs || to_b;
+                    T *acc = to_b ? ob : o;
+                    for (int d = 0; d < 3; ++d) acc[d] += f * (sfac[q] * gl[d] + dzf[q] * g[6 + d]);
                 }
             }
         }
     }
-    if (bbc == TNG_FREESLIP) {
+    if (any_fs) {
         bc_project(a.coords, lcol, TNG_FREESLIP, ob);
         o[0] += ob[0]; o[1] += ob[1]; o[2] += ob[2];
     }
@@ -513,7 +548,12 @@ template <typename T> void grad_impl(tng_operator &op, const tng_vector &src, tn
     a.sd_cmb = f.d_sd_cmb.as<uint8_t>(); a.sd_surf = f.d_sd_surf.as<uint8_t>();
     a.nsd = f.nsd; a.nx = f.nx; a.nr = f.nr; a.nxc = c.nx; a.nrc = c.nr;
     a.bc_cmb = op.bc_cmb; a.bc_surf = op.bc_surf; a.add = op.apply_mode == TNG_ADD ? 1 : 0;
-    hipLaunchKernelGGL(gradient_kernel<T>, dim3(blocks_for(f.nalloc)), dim3(256), 0, s, a);
+    const int64_t nfw = (int64_t)f.nsd * (f.nx - 1) * (f.nx - 1) * (f.nr - 1) * 2;
+    const size_t bytes = (size_t)nfw * sizeof(T);
+    if (op.scratch.bytes < bytes) op.scratch.alloc(bytes);
+    hipLaunchKernelGGL(grad_weight_kernel<T>, dim3(blocks_for(nfw / 2)), dim3(256), 0, s, a, op.scratch.as<T>());
+    hipLaunchKernelGGL(gradient_kernel<T>, dim3(blocks_for(f.nalloc)), dim3(256), 0, s, a,
+                       (const T *)op.scratch.p);
     TNG_HIP_CHECK(hipGetLastError());
 }
 

@@ -1,5 +1,7 @@
 // Vector kernels: fill, copy/convert, and the additive halo sum over
 // duplicated interface nodes.
+#include <cfloat>
+
 #include "tng_internal.hpp"
 
 namespace tng {
@@ -138,6 +140,30 @@ template <typename T> __global__ __launch_bounds__(256) void mult_kernel(const C
         for (int c = 0; c < ncomp; ++c) y.p[c][i] *= x.p[c][i];
 }
 
+// One Chebyshev sub-step (linalg/solvers/chebyshev.hpp:102-165) in one pass:
+//   z = invd * (b - Ax)   (Ax absent: x = 0)
+//   d = cz * z + cd * d   (cd absent: d is overwritten)
+//   x = x + d             (x absent: x = 0, written without being read)
+// with the rounding of the separate lincomb / scale kernels.
+template <typename T>
+__global__ __launch_bounds__(256) void cheb_update_kernel(const Comp3<T> x, const Comp3<T> d, const Comp3<T> b,
+                                                          const Comp3<T> ax, const Comp3<T> invd, int has_ax,
+                                                          int read_d, int read_x, double cz, double cd, int64_t n)
+{
+    const T az = (T)cz, ad = (T)cd;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            T r = b.p[c][i];
+            if (has_ax) r = r - ax.p[c][i];
+            const T z = r * invd.p[c][i];
+            T dn = az * z;
+            if (read_d) dn = fma(ad, d.p[c][i], dn);
+            d.p[c][i] = dn;
+            x.p[c][i] = read_x ? x.p[c][i] + dn : dn;
+        }
+}
+
 template <typename T> __global__ __launch_bounds__(256) void invert_kernel(const Comp3<T> y, int ncomp, int64_t n)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -208,6 +234,110 @@ template <int OP> __global__ __launch_bounds__(256) void reduce_final_kernel(con
     if (threadIdx.x == 0) *out = s;
 }
 
+// ----------------------------------------------------------------------------
+// PCG with the scalars on the device (linalg/solvers/pcg.hpp:81-170, identity
+// preconditioner, so z = r).  State, in doubles: rr, initial norm, alpha,
+// beta, done, iterations, final relative residual.  A finished solve turns
+// every later step into a no-op, so the host issues iterations without
+// waiting for the dots.  Dots use the reduction grid and per-thread order of
+// reduce_partial_kernel, so every scalar is bitwise that of the host-synced
+// formulation.
+// ----------------------------------------------------------------------------
+enum { PCG_RR = 0, PCG_INIT, PCG_ALPHA, PCG_BETA, PCG_DONE, PCG_ITERS, PCG_REL, PCG_NSTATE };
+
+__device__ __forceinline__ double block_sum_partials(const double *partials, int nb, double *red)
+{
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += partials[b];
+    return block_reduce<RED_DOT>(acc, red);
+}
+
+__global__ __launch_bounds__(256) void pcg_begin_kernel(const double *partials, int nb, double *st, double atol)
+{
+    __shared__ double red[4];
+    const double rr = block_sum_partials(partials, nb, red);
+    if (threadIdx.x == 0) {
+        const double init = sqrt(rr);
+        st[PCG_RR] = rr;
+        st[PCG_INIT] = init;
+        st[PCG_ALPHA] = 0.0;
+        st[PCG_BETA] = 0.0;
+        st[PCG_DONE] = init < atol ? 1.0 : 0.0;
+        st[PCG_ITERS] = 0.0;
+        st[PCG_REL] = 1.0;
+    }
+}
+
+__global__ __launch_bounds__(256) void pcg_alpha_kernel(const double *partials, int nb, double *st, int it)
+{
+    __shared__ double red[4];
+    const double pap = block_sum_partials(partials, nb, red);
+    if (threadIdx.x == 0 && st[PCG_DONE] == 0.0) {
+        if (fabs(pap) < DBL_MIN) {
+            st[PCG_DONE] = 1.0;
+            st[PCG_ITERS] = it;
+        } else {
+            st[PCG_ALPHA] = st[PCG_RR] / pap;
+        }
+    }
+}
+
+// x += alpha p; r -= alpha ap; per-block partials of r.r (OWNED copies)
+template <typename T>
+__global__ __launch_bounds__(256) void pcg_update_kernel(const Comp3<T> x, const Comp3<T> r, const Comp3<T> p,
+                                                         const Comp3<T> ap, int ncomp, const uint8_t *omask, int64_t n,
+                                                         const double *st, double *partials)
+{
+    __shared__ double red[4];
+    if (st[PCG_DONE] != 0.0) return;
+    const T alpha = (T)st[PCG_ALPHA], malpha = (T)(-st[PCG_ALPHA]);
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const bool own = omask[i] == OWNED;
+        double v = 0.0;
+        for (int c = 0; c < ncomp; ++c) {
+            x.p[c][i] = fma(alpha, p.p[c][i], x.p[c][i]);
+            const T rn = fma(malpha, ap.p[c][i], r.p[c][i]);
+            r.p[c][i] = rn;
+            v += (double)rn * (double)rn;
+        }
+        acc += own ? v : 0.0;
+    }
+    const double sblk = block_reduce<RED_DOT>(acc, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = sblk;
+}
+
+__global__ __launch_bounds__(256) void pcg_beta_kernel(const double *partials, int nb, double *st, double rtol,
+                                                       double atol, int it)
+{
+    __shared__ double red[4];
+    if (st[PCG_DONE] != 0.0) return;  // uniform over the block
+    const double rr_new = block_sum_partials(partials, nb, red);
+    if (threadIdx.x == 0) {
+        const double res = sqrt(rr_new), rel = res / st[PCG_INIT];
+        st[PCG_REL] = rel;
+        const double rr_old = st[PCG_RR];
+        if (rel <= rtol || res < atol || fabs(rr_old) < DBL_MIN) {
+            st[PCG_DONE] = 1.0;
+            st[PCG_ITERS] = it;
+        } else {
+            st[PCG_BETA] = rr_new / rr_old;
+        }
+        st[PCG_RR] = rr_new;
+    }
+}
+
+// p = r + beta p
+template <typename T>
+__global__ __launch_bounds__(256) void pcg_direction_kernel(const Comp3<T> p, const Comp3<T> r, int ncomp, int64_t n,
+                                                            const double *st)
+{
+    if (st[PCG_DONE] != 0.0) return;
+    const T beta = (T)st[PCG_BETA];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        for (int c = 0; c < ncomp; ++c) p.p[c][i] = fma(beta, p.p[c][i], r.p[c][i]);
+}
+
 constexpr int kRedBlocks = 1024;
 
 unsigned gs(int64_t n)
@@ -246,6 +376,27 @@ void launch_mult(tng_vector &y, const tng_vector &x, hipStream_t s)
     else
         hipLaunchKernelGGL(mult_kernel<float>, dim3(gs(n)), dim3(256), 0, s, comps<float>(y), comps<float>(x), y.ncomp,
                            n);
+    TNG_HIP_CHECK(hipGetLastError());
+}
+
+void launch_cheb_update(tng_vector &x, tng_vector &d, const tng_vector &b, const tng_vector *ax,
+                        const tng_vector &invd, double cz, double cd, bool read_d, bool x_zero, hipStream_t s)
+{
+    const tng_vector *const others[3] = {&d, &b, &invd};
+    for (const tng_vector *v : others)
+        TNG_REQUIRE(v->mesh == x.mesh && v->dtype == x.dtype && v->ncomp == 3 && x.ncomp == 3,
+                    "chebyshev: vectors must share mesh and dtype and have 3 components");
+    if (ax) TNG_REQUIRE(ax->mesh == x.mesh && ax->dtype == x.dtype && ax->ncomp == 3, "chebyshev: mismatched A x");
+    const int64_t n = x.mesh->nalloc;
+    const tng_vector &axv = ax ? *ax : b;
+    if (x.dtype == TNG_F64)
+        hipLaunchKernelGGL(cheb_update_kernel<double>, dim3(gs(n)), dim3(256), 0, s, comps<double>(x), comps<double>(d),
+                           comps<double>(b), comps<double>(axv), comps<double>(invd), ax ? 1 : 0, read_d ? 1 : 0,
+                           x_zero ? 0 : 1, cz, cd, n);
+    else
+        hipLaunchKernelGGL(cheb_update_kernel<float>, dim3(gs(n)), dim3(256), 0, s, comps<float>(x), comps<float>(d),
+                           comps<float>(b), comps<float>(axv), comps<float>(invd), ax ? 1 : 0, read_d ? 1 : 0,
+                           x_zero ? 0 : 1, cz, cd, n);
     TNG_HIP_CHECK(hipGetLastError());
 }
 
@@ -297,6 +448,88 @@ double reduce_to_host(int op, const tng_vector &x, const tng_vector *y, hipStrea
     TNG_HIP_CHECK(hipMemcpyAsync(&v, ctx.red_result.p, sizeof(double), hipMemcpyDeviceToHost, s));
     TNG_HIP_CHECK(hipStreamSynchronize(s));
     return v;
+}
+
+// PCG on the device: the state lives in ctx.red_result[kPcgSlot ...]
+constexpr int kPcgSlot = 64;
+
+static unsigned red_grid(int64_t n) { return std::min<unsigned>(kRedBlocks, gs(n)); }
+
+void launch_pcg_begin(const tng_vector &r, double atol, hipStream_t s)
+{
+    tng_context &ctx = *r.mesh->ctx;
+    const int64_t n = r.mesh->nalloc;
+    const unsigned g = red_grid(n);
+    double *part = ctx.red_partials.as<double>(), *st = ctx.red_result.as<double>() + kPcgSlot;
+    const uint8_t *om = r.mesh->d_omask.as<uint8_t>();
+    if (r.dtype == TNG_F64)
+        hipLaunchKernelGGL((reduce_partial_kernel<double, RED_DOT>), dim3(g), dim3(256), 0, s, comps<double>(r),
+                           comps<double>(r), r.ncomp, om, n, part);
+    else
+        hipLaunchKernelGGL((reduce_partial_kernel<float, RED_DOT>), dim3(g), dim3(256), 0, s, comps<float>(r),
+                           comps<float>(r), r.ncomp, om, n, part);
+    hipLaunchKernelGGL(pcg_begin_kernel, dim3(1), dim3(256), 0, s, part, (int)g, st, atol);
+    TNG_HIP_CHECK(hipGetLastError());
+}
+
+void launch_pcg_alpha(const tng_vector &ap, const tng_vector &p, int it, hipStream_t s)
+{
+    tng_context &ctx = *p.mesh->ctx;
+    const int64_t n = p.mesh->nalloc;
+    const unsigned g = red_grid(n);
+    double *part = ctx.red_partials.as<double>(), *st = ctx.red_result.as<double>() + kPcgSlot;
+    const uint8_t *om = p.mesh->d_omask.as<uint8_t>();
+    if (p.dtype == TNG_F64)
+        hipLaunchKernelGGL((reduce_partial_kernel<double, RED_DOT>), dim3(g), dim3(256), 0, s, comps<double>(ap),
+                           comps<double>(p), p.ncomp, om, n, part);
+    else
+        hipLaunchKernelGGL((reduce_partial_kernel<float, RED_DOT>), dim3(g), dim3(256), 0, s, comps<float>(ap),
+                           comps<float>(p), p.ncomp, om, n, part);
+    hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(256), 0, s, part, (int)g, st, it);
+    TNG_HIP_CHECK(hipGetLastError());
+}
+
+void launch_pcg_update(tng_vector &x, tng_vector &r, const tng_vector &p, const tng_vector &ap, double rtol,
+                       double atol, int it, hipStream_t s)
+{
+    tng_context &ctx = *p.mesh->ctx;
+    const int64_t n = p.mesh->nalloc;
+    const unsigned g = red_grid(n);
+    double *part = ctx.red_partials.as<double>(), *st = ctx.red_result.as<double>() + kPcgSlot;
+    const uint8_t *om = p.mesh->d_omask.as<uint8_t>();
+    if (p.dtype == TNG_F64)
+        hipLaunchKernelGGL(pcg_update_kernel<double>, dim3(g), dim3(256), 0, s, comps<double>(x), comps<double>(r),
+                           comps<double>(p), comps<double>(ap), p.ncomp, om, n, (const double *)st, part);
+    else
+        hipLaunchKernelGGL(pcg_update_kernel<float>, dim3(g), dim3(256), 0, s, comps<float>(x), comps<float>(r),
+                           comps<float>(p), comps<float>(ap), p.ncomp, om, n, (const double *)st, part);
+    hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(256), 0, s, part, (int)g, st, rtol, atol, it);
+    TNG_HIP_CHECK(hipGetLastError());
+}
+
+void launch_pcg_direction(tng_vector &p, const tng_vector &r, hipStream_t s)
+{
+    tng_context &ctx = *p.mesh->ctx;
+    const int64_t n = p.mesh->nalloc;
+    const double *st = ctx.red_result.as<double>() + kPcgSlot;
+    if (p.dtype == TNG_F64)
+        hipLaunchKernelGGL(pcg_direction_kernel<double>, dim3(gs(n)), dim3(256), 0, s, comps<double>(p),
+                           comps<double>(r), p.ncomp, n, st);
+    else
+        hipLaunchKernelGGL(pcg_direction_kernel<float>, dim3(gs(n)), dim3(256), 0, s, comps<float>(p),
+                           comps<float>(r), p.ncomp, n, st);
+    TNG_HIP_CHECK(hipGetLastError());
+}
+
+void pcg_read_state(const tng_vector &v, int *done, int *iters, double *final_rel, hipStream_t s)
+{
+    tng_context &ctx = *v.mesh->ctx;
+    double st[PCG_NSTATE];
+    TNG_HIP_CHECK(hipMemcpyAsync(st, ctx.red_result.as<double>() + kPcgSlot, sizeof(st), hipMemcpyDeviceToHost, s));
+    TNG_HIP_CHECK(hipStreamSynchronize(s));
+    if (done) *done = st[PCG_DONE] != 0.0;
+    if (iters) *iters = (int)st[PCG_ITERS];
+    if (final_rel) *final_rel = st[PCG_REL];
 }
 
 }  // namespace tng

@@ -89,7 +89,7 @@ double power_iteration(tng_operator &A, const tng_vector &inv_diag, tng_vector &
 // ---------------------------------------------------------------------------
 // Chebyshev smoother (chebyshev.hpp:102-165)
 // ---------------------------------------------------------------------------
-void Chebyshev::solve(tng_operator &A, tng_vector &x, const tng_vector &b)
+void Chebyshev::solve(tng_operator &A, tng_vector &x, const tng_vector &b, bool x_zero)
 {
     if (need_estimate) {
         lmax_estimate = power_iteration(A, *inv_diag, *d, *z, power_iterations, seed);
@@ -100,20 +100,18 @@ void Chebyshev::solve(tng_operator &A, tng_vector &x, const tng_vector &b)
     const double theta = 0.5 * (lambda_max + lambda_min), delta = 0.5 * (lambda_max - lambda_min);
     const double sigma = theta / delta;
     double rho = 1.0 / sigma;  // carried across iterations, as in the reference
+    hipStream_t s = stream_of(x);
     for (int iter = 0; iter < iterations; ++iter) {
-        apply(A, x, *z);
-        lincomb(*z, {1.0, -1.0}, {&b, z});
-        scale_in_place(*z, *inv_diag);
-        lincomb(*d, {1.0 / theta}, {z});
-        lincomb(x, {1.0, 1.0}, {&x, d});
+        // z = D^{-1}(b - A x); d = z / theta; x += d.  With x = 0 on entry A x
+        // is exactly zero and is not applied.
+        const bool zero = x_zero && iter == 0;
+        if (!zero) apply(A, x, *z);
+        launch_cheb_update(x, *d, b, zero ? nullptr : z, *inv_diag, 1.0 / theta, 0.0, false, zero, s);
         for (int i = 2; i <= order; ++i) {
             const double rho_new = 1.0 / (2.0 * sigma - rho);
             const double beta = rho_new * rho, alpha = 2.0 * rho_new / delta;
             apply(A, x, *z);
-            lincomb(*z, {1.0, -1.0}, {&b, z});
-            scale_in_place(*z, *inv_diag);
-            lincomb(*d, {alpha, beta}, {z, d});
-            lincomb(x, {1.0, 1.0}, {&x, d});
+            launch_cheb_update(x, *d, b, z, *inv_diag, alpha, beta, true, false, s);
             rho = rho_new;
         }
     }
@@ -123,39 +121,43 @@ void Chebyshev::solve(tng_operator &A, tng_vector &x, const tng_vector &b)
 // PCG, identity preconditioner (pcg.hpp:81-170)
 // ---------------------------------------------------------------------------
 int pcg_solve(tng_operator &A, tng_vector &x, const tng_vector &b, tng_vector &r, tng_vector &p, tng_vector &ap,
-              tng_vector &z, int max_iterations, double rtol, double atol, double *final_rel)
+              tng_vector & /*z: identity preconditioner, z = r*/, int max_iterations, double rtol, double atol,
+              double *final_rel, bool x_zero)
 {
-    apply(A, x, r);
-    lincomb(r, {1.0, -1.0}, {&b, &r});
-    assign(z, r);  // identity preconditioner
-    assign(p, z);
-    const double initial = std::sqrt(dot(r, r));
-    if (final_rel) *final_rel = 1.0;
-    if (initial < atol) return 0;
-    for (int it = 1; it <= max_iterations; ++it) {
-        const double alpha_num = dot(z, r);
-        apply(A, p, ap);
-        const double alpha_den = dot(ap, p);
-        if (std::abs(alpha_den) < std::numeric_limits<double>::min()) return it;
-        const double alpha = alpha_num / alpha_den;
-        lincomb(x, {1.0, alpha}, {&x, &p});
-        lincomb(r, {1.0, -alpha}, {&r, &ap});
-        const double abs_res = std::sqrt(dot(r, r));
-        const double rel = abs_res / initial;
-        if (final_rel) *final_rel = rel;
-        if (rel <= rtol || abs_res < atol) return it;
-        assign(z, r);
-        const double beta_num = dot(z, r);
-        if (std::abs(alpha_num) < std::numeric_limits<double>::min()) return it;
-        lincomb(p, {1.0, beta_num / alpha_num}, {&z, &p});
+    // The scalars (dots, alpha, beta, the stopping test) stay on the device;
+    // the host only checks every few iterations whether the solve finished.
+    constexpr int kCheckEvery = 8;
+    hipStream_t s = stream_of(x);
+    if (x_zero) {
+        assign(r, b);  // A x = 0 exactly
+    } else {
+        apply(A, x, r);
+        lincomb(r, {1.0, -1.0}, {&b, &r});
     }
-    return max_iterations;
+    assign(p, r);  // z = r, p = z
+    launch_pcg_begin(r, atol, s);
+    int done = 0, iters = 0;
+    double rel = 1.0;
+    int it = 1;
+    for (; it <= max_iterations; ++it) {
+        apply(A, p, ap);
+        launch_pcg_alpha(ap, p, it, s);
+        launch_pcg_update(x, r, p, ap, rtol, atol, it, s);
+        launch_pcg_direction(p, r, s);
+        if (it % kCheckEvery == 0 && it < max_iterations) {
+            pcg_read_state(x, &done, &iters, &rel, s);
+            if (done) break;
+        }
+    }
+    pcg_read_state(x, &done, &iters, &rel, s);
+    if (final_rel) *final_rel = rel;
+    return done ? iters : max_iterations;
 }
 
 // ---------------------------------------------------------------------------
 // Multigrid V-cycle (multigrid.hpp:216-432)
 // ---------------------------------------------------------------------------
-void Multigrid::coarse_solve(tng_vector &e0, const tng_vector &r0)
+void Multigrid::coarse_solve(tng_vector &e0, const tng_vector &r0, bool x_zero)
 {
     // The coarsest solve runs in FP64 whatever the precision of the levels.
     tng_vector &x = e0.dtype == TNG_F64 ? e0 : *cg_x;
@@ -166,31 +168,31 @@ void Multigrid::coarse_solve(tng_vector &e0, const tng_vector &r0)
         b = cg_b;
     }
     coarse_iterations += pcg_solve(*coarse_op, x, *b, *cg_tmp[0], *cg_tmp[1], *cg_tmp[2], *cg_tmp[3], coarse_max_it,
-                                   coarse_rtol, coarse_atol, nullptr);
+                                   coarse_rtol, coarse_atol, nullptr, x_zero);
     if (e0.dtype != TNG_F64) assign(e0, *cg_x);
 }
 
-void Multigrid::recurse(int level, tng_vector &x, const tng_vector &b)
+void Multigrid::recurse(int level, tng_vector &x, const tng_vector &b, bool x_zero)
 {
     if (level == 0) {
-        coarse_solve(x, b);
+        coarse_solve(x, b, x_zero);
         return;
     }
     Level &L = levels[level];
-    pre[level].solve(*L.A, x, b);
+    pre[level].solve(*L.A, x, b, x_zero);
     apply(*L.A, x, *L.tmp);
     lincomb(*L.tmp, {1.0, -1.0}, {&b, L.tmp});
     Level &C = levels[level - 1];
     apply(*L.R, *L.tmp, *C.r);
     assign(*C.e, 0.0);
-    recurse(level - 1, *C.e, *C.r);
+    recurse(level - 1, *C.e, *C.r, true);
     apply(*L.P, *C.e, x);  // Add mode
-    post[level].solve(*L.A, x, b);
+    post[level].solve(*L.A, x, b, false);
 }
 
-void Multigrid::solve(tng_vector &x, const tng_vector &b)
+void Multigrid::solve(tng_vector &x, const tng_vector &b, bool x_zero)
 {
-    for (int c = 0; c < num_cycles; ++c) recurse((int)levels.size() - 1, x, b);
+    for (int c = 0; c < num_cycles; ++c) recurse((int)levels.size() - 1, x, b, x_zero && c == 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -376,27 +378,28 @@ void StokesSolver::apply_stokes(const SVec &src, SVec &dst)
 }
 
 // velocity block preconditioner: V-cycle(s) on A x = b, x as given
-void StokesSolver::precondition_velocity(tng_vector &x, const tng_vector &b)
+void StokesSolver::precondition_velocity(tng_vector &x, const tng_vector &b, bool x_zero)
 {
     if (!mg_x) {
-        mg.solve(x, b);
+        mg.solve(x, b, x_zero);
         return;
     }
-    assign(*mg_x, x);
+    if (x_zero) assign(*mg_x, 0.0);
+    else assign(*mg_x, x);
     assign(*mg_b, b);
-    mg.solve(*mg_x, *mg_b);
+    mg.solve(*mg_x, *mg_b, x_zero);
     assign(x, *mg_x);
 }
 
 // block-triangular preconditioner (block_preconditioner_2x2.hpp:205-221)
-void StokesSolver::precondition(SVec &x, const SVec &b)
+void StokesSolver::precondition(SVec &x, const SVec &b, bool x_zero)
 {
     assign(*x.p, *b.p);
     scale_in_place(*x.p, *inv_lumped_pmass);
     apply(*Bt, *x.p, *prec_tmp_u);  // Replace, Communicate
     lincomb(*prec_tmp_u, {1.0, 1.0}, {prec_tmp_u, b.u});
     lincomb(*x.p, {-1.0}, {x.p});
-    precondition_velocity(*x.u, *prec_tmp_u);
+    precondition_velocity(*x.u, *prec_tmp_u, x_zero);
 }
 
 namespace {
@@ -465,7 +468,7 @@ void StokesSolver::fgmres(SVec &x, const SVec &b, tng_solve_report &rep)
             SVec &vj = fg[offV + j], &zj = fg[offZ + j];
             rep.iterations += 1;  // preconditioned Arnoldi steps
             s_assign(zj, 0.0);
-            precondition(zj, vj);
+            precondition(zj, vj, true);
             if (s_nonfinite(zj)) {
                 rep.nonfinite_preconditioner += 1;
                 s_assign(zj, vj);  // skip the preconditioner in this iteration

@@ -190,9 +190,18 @@ void stokes_block_apply(tng_operator &op, const tng_vector &src, tng_vector &dst
 void launch_lincomb(tng_vector &y, int nx, const double *c, const tng_vector *const *x, double c0, hipStream_t s);
 void launch_mult(tng_vector &y, const tng_vector &x, hipStream_t s);
 void launch_invert(tng_vector &y, hipStream_t s);
+void launch_cheb_update(tng_vector &x, tng_vector &d, const tng_vector &b, const tng_vector *ax,
+                        const tng_vector &invd, double cz, double cd, bool read_d, bool x_zero, hipStream_t s);
 enum { REDUCE_DOT = 0, REDUCE_MAXABS = 1, REDUCE_NONFINITE = 2 };
 void launch_reduce(int op, const tng_vector &x, const tng_vector *y, int slot, hipStream_t s);
 double reduce_to_host(int op, const tng_vector &x, const tng_vector *y, hipStream_t s);
+// PCG with device-side scalars (identity preconditioner)
+void launch_pcg_begin(const tng_vector &r, double atol, hipStream_t s);
+void launch_pcg_alpha(const tng_vector &ap, const tng_vector &p, int it, hipStream_t s);
+void launch_pcg_update(tng_vector &x, tng_vector &r, const tng_vector &p, const tng_vector &ap, double rtol,
+                       double atol, int it, hipStream_t s);
+void launch_pcg_direction(tng_vector &p, const tng_vector &r, hipStream_t s);
+void pcg_read_state(const tng_vector &v, int *done, int *iters, double *final_rel, hipStream_t s);
 
 // generic operator apply (dispatch on kind)
 void op_apply(tng_operator &op, const tng_vector &src, tng_vector &dst);

@@ -12,7 +12,8 @@ double power_iteration(tng_operator &A, const tng_vector &inv_diag, tng_vector &
                        uint64_t seed);
 
 int pcg_solve(tng_operator &A, tng_vector &x, const tng_vector &b, tng_vector &r, tng_vector &p, tng_vector &ap,
-              tng_vector &z, int max_iterations, double rtol, double atol, double *final_rel);
+              tng_vector &z, int max_iterations, double rtol, double atol, double *final_rel,
+              bool x_zero = false);
 
 // Chebyshev-accelerated Jacobi smoother on D^{-1} A.
 struct Chebyshev {
@@ -22,7 +23,7 @@ struct Chebyshev {
     bool need_estimate = true;
     double lmax_estimate = 0.0;  // estimate of the largest eigenvalue of D^{-1} A
     uint64_t seed = 12345;
-    void solve(tng_operator &A, tng_vector &x, const tng_vector &b);
+    void solve(tng_operator &A, tng_vector &x, const tng_vector &b, bool x_zero = false);
 };
 
 // Geometric multigrid V-cycle; level 0 is the coarsest.
@@ -43,9 +44,9 @@ struct Multigrid {
     double coarse_rtol = 1e-6, coarse_atol = 1e-16;
     long coarse_iterations = 0;
 
-    void solve(tng_vector &x, const tng_vector &b);
-    void recurse(int level, tng_vector &x, const tng_vector &b);
-    void coarse_solve(tng_vector &e0, const tng_vector &r0);
+    void solve(tng_vector &x, const tng_vector &b, bool x_zero = false);
+    void recurse(int level, tng_vector &x, const tng_vector &b, bool x_zero);
+    void coarse_solve(tng_vector &e0, const tng_vector &r0, bool x_zero);
 };
 
 struct StokesSolver {
@@ -72,8 +73,8 @@ struct StokesSolver {
     void set_viscosity(const tng_vector &eta_fine);
     void viscosity_updated();
     void apply_stokes(const SVec &src, SVec &dst);
-    void precondition_velocity(tng_vector &x, const tng_vector &b);
-    void precondition(SVec &x, const SVec &b);
+    void precondition_velocity(tng_vector &x, const tng_vector &b, bool x_zero = false);
+    void precondition(SVec &x, const SVec &b, bool x_zero = false);
     void fgmres(SVec &x, const SVec &b, tng_solve_report &rep);
     void solve(SVec &x, const SVec &b, tng_solve_report &rep);
     void enforce_rhs(tng_vector &f_u);
