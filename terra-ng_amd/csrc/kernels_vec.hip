@@ -338,6 +338,50 @@ __global__ __launch_bounds__(256) void pcg_direction_kernel(const Comp3<T> p, co
         for (int c = 0; c < ncomp; ++c) p.p[c][i] = fma(beta, p.p[c][i], r.p[c][i]);
 }
 
+// ----------------------------------------------------------------------------
+// Modified Gram-Schmidt step of FGMRES (linalg/solvers/fgmres.hpp:211-224)
+// on one block of a Stokes vector, in one pass:
+//   w -= h_prev v_prev   (h_prev read from the device; skipped when absent)
+//   partial of (w . v_next) over OWNED copies   (v_next absent: w . w)
+// The dot uses the reduction grid and per-thread order of
+// reduce_partial_kernel; mgs_final_kernel adds the velocity and pressure dots
+// as the host formulation did.
+// ----------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void mgs_block_kernel(const Comp3<T> w, const Comp3<T> vprev, const Comp3<T> vnext,
+                                                        int ncomp, const uint8_t *omask, int64_t n,
+                                                        const double *hprev, int self, double *partials)
+{
+    __shared__ double red[4];
+    const T mh = hprev ? (T)(-hprev[0]) : T(0);
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const bool own = omask[i] == OWNED;
+        double v = 0.0;
+        for (int c = 0; c < ncomp; ++c) {
+            T wc = w.p[c][i];
+            if (hprev) {
+                wc = fma(mh, vprev.p[c][i], wc);
+                w.p[c][i] = wc;
+            }
+            const T vc = self ? wc : vnext.p[c][i];
+            v += (double)wc * (double)vc;
+        }
+        acc += own ? v : 0.0;
+    }
+    const double sblk = block_reduce<RED_DOT>(acc, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = sblk;
+}
+
+__global__ __launch_bounds__(256) void mgs_final_kernel(const double *pu, int gu, const double *pp, int gp, double *out)
+{
+    __shared__ double red[4];
+    const double su = block_sum_partials(pu, gu, red);
+    __syncthreads();
+    const double sp = block_sum_partials(pp, gp, red);
+    if (threadIdx.x == 0) *out = su + sp;
+}
+
 constexpr int kRedBlocks = 1024;
 
 unsigned gs(int64_t n)
@@ -520,6 +564,39 @@ void launch_pcg_direction(tng_vector &p, const tng_vector &r, hipStream_t s)
                            comps<float>(r), p.ncomp, n, st);
     TNG_HIP_CHECK(hipGetLastError());
 }
+
+// MGS step on a Stokes vector (u, p): h_out = (w -= h_prev v_prev) . v_next
+void launch_mgs_step(tng_vector &wu, tng_vector &wp, const tng_vector *vprev_u, const tng_vector *vprev_p,
+                     const double *h_prev_dev, const tng_vector *vnext_u, const tng_vector *vnext_p, double *h_out_dev,
+                     hipStream_t s)
+{
+    tng_context &ctx = *wu.mesh->ctx;
+    double *part = ctx.red_partials.as<double>();
+    unsigned g[2] = {0, 0};
+    tng_vector *w[2] = {&wu, &wp};
+    const tng_vector *vp[2] = {vprev_u, vprev_p}, *vn[2] = {vnext_u, vnext_p};
+    for (int b = 0; b < 2; ++b) {
+        const tng_vector &wb = *w[b];
+        const int64_t n = wb.mesh->nalloc;
+        g[b] = red_grid(n);
+        const tng_vector &pv = vp[b] ? *vp[b] : wb, &nv = vn[b] ? *vn[b] : wb;
+        TNG_REQUIRE(pv.mesh == wb.mesh && nv.mesh == wb.mesh && pv.dtype == wb.dtype && nv.dtype == wb.dtype,
+                    "mgs: mismatched vectors");
+        const uint8_t *om = wb.mesh->d_omask.as<uint8_t>();
+        double *pb = part + b * kRedBlocks;
+        if (wb.dtype == TNG_F64)
+            hipLaunchKernelGGL(mgs_block_kernel<double>, dim3(g[b]), dim3(256), 0, s, comps<double>(wb),
+                               comps<double>(pv), comps<double>(nv), wb.ncomp, om, n, h_prev_dev, vn[b] ? 0 : 1, pb);
+        else
+            hipLaunchKernelGGL(mgs_block_kernel<float>, dim3(g[b]), dim3(256), 0, s, comps<float>(wb),
+                               comps<float>(pv), comps<float>(nv), wb.ncomp, om, n, h_prev_dev, vn[b] ? 0 : 1, pb);
+    }
+    hipLaunchKernelGGL(mgs_final_kernel, dim3(1), dim3(256), 0, s, (const double *)part, (int)g[0],
+                       (const double *)(part + kRedBlocks), (int)g[1], h_out_dev);
+    TNG_HIP_CHECK(hipGetLastError());
+}
+
+double *device_slots(tng_context &ctx, int first) { return ctx.red_result.as<double>() + first; }
 
 void pcg_read_state(const tng_vector &v, int *done, int *iters, double *final_rel, hipStream_t s)
 {

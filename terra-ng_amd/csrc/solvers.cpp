@@ -452,7 +452,9 @@ void StokesSolver::fgmres(SVec &x, const SVec &b, tng_solve_report &rep)
         return;
     }
     const int offV = 1, offZ = offV + m + 1, idxW = offZ + m, idxAcc = idxW + 1;
-    std::vector<double> H((size_t)(m + 1) * m), g(m + 1), cs(m + 1), sn(m + 1), y;
+    std::vector<double> H((size_t)(m + 1) * m), g(m + 1), cs(m + 1), sn(m + 1), y, hcol(m + 2);
+    TNG_REQUIRE(m + 2 <= 256, "krylov_restart above 254 is not supported");
+    double *hdev = device_slots(*x.u->mesh->ctx, 256);
     auto h = [&](int i, int j) -> double & { return H[(size_t)i * m + j]; };
     int total = 0;
     while (total < params.krylov_max_iterations) {
@@ -475,13 +477,21 @@ void StokesSolver::fgmres(SVec &x, const SVec &b, tng_solve_report &rep)
             }
             SVec &w = fg[idxW];
             apply_stokes(zj, w);
-            for (int i = 0; i <= j; ++i) {
-                SVec &vi = fg[offV + i];
-                const double hij = s_dot(w, vi);
-                h(i, j) = hij;
-                s_lincomb(w, {1.0, -hij}, {&w, &vi});
+            // modified Gram-Schmidt: h_ij = w . v_i, then w -= h_ij v_i, one
+            // fused pass per i with the h values kept on the device; one copy
+            // to the host per Arnoldi step
+            for (int i = 0; i <= j + 1; ++i) {
+                const SVec *vprev = i > 0 ? &fg[offV + i - 1] : nullptr;
+                const SVec *vnext = i <= j ? &fg[offV + i] : nullptr;
+                launch_mgs_step(*w.u, *w.p, vprev ? vprev->u : nullptr, vprev ? vprev->p : nullptr,
+                                i > 0 ? hdev + i - 1 : nullptr, vnext ? vnext->u : nullptr,
+                                vnext ? vnext->p : nullptr, hdev + i, stream_of(*w.u));
             }
-            const double hj1 = std::sqrt(s_dot(w, w));
+            TNG_HIP_CHECK(hipMemcpyAsync(hcol.data(), hdev, sizeof(double) * (j + 2), hipMemcpyDeviceToHost,
+                                         stream_of(*w.u)));
+            TNG_HIP_CHECK(hipStreamSynchronize(stream_of(*w.u)));
+            for (int i = 0; i <= j; ++i) h(i, j) = hcol[i];
+            const double hj1 = std::sqrt(hcol[j + 1]);
             if (hj1 < std::numeric_limits<double>::epsilon() * initial) {
                 h(j + 1, j) = 0;
                 inner = j + 1;

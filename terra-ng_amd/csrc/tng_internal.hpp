@@ -202,6 +202,11 @@ void launch_pcg_update(tng_vector &x, tng_vector &r, const tng_vector &p, const 
                        double atol, int it, hipStream_t s);
 void launch_pcg_direction(tng_vector &p, const tng_vector &r, hipStream_t s);
 void pcg_read_state(const tng_vector &v, int *done, int *iters, double *final_rel, hipStream_t s);
+// FGMRES modified Gram-Schmidt step on a Stokes vector; h values live on the device
+void launch_mgs_step(tng_vector &wu, tng_vector &wp, const tng_vector *vprev_u, const tng_vector *vprev_p,
+                     const double *h_prev_dev, const tng_vector *vnext_u, const tng_vector *vnext_p, double *h_out_dev,
+                     hipStream_t s);
+double *device_slots(tng_context &ctx, int first);  // ctx.red_result + first (doubles, 512 in total)
 
 // generic operator apply (dispatch on kind)
 void op_apply(tng_operator &op, const tng_vector &src, tng_vector &dst);
