@@ -89,6 +89,18 @@ __device__ __forceinline__ float lane_value(float v, int j)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
 }
 
+// Value of lane src (per-lane index) through the LDS crossbar.
+__device__ __forceinline__ double lane_fetch(double v, int src)
+{
+    const int lo = __builtin_amdgcn_ds_bpermute(src * 4, __double2loint(v));
+    const int hi = __builtin_amdgcn_ds_bpermute(src * 4, __double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float lane_fetch(float v, int src)
+{
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src * 4, __float_as_int(v)));
+}
+
 // Lateral record of one y-step of a wave that owns one cell column (R = 64):
 // lane j holds element j, loaded one step ahead with a single vector load.
 enum : int {
@@ -303,23 +315,44 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
     };
 
     // R == 64: lateral data of the next step, one element per lane
-    auto load_record = [&](int yc) -> T {
-        // every lane computes a valid address and loads unconditionally (a
-        // branch-free load keeps the wait counts of the pipeline exact)
+    // Per-lane address of the record element: base + stride * clamp(yc + dy,
+    // 0, ymax), fixed for the whole sweep (every lane loads unconditionally
+    // from a valid address, which keeps the wait counts of the pipeline exact).
+    const T *rec_base = a.latgeom;
+    int rec_stride = 0, rec_dy = 0, rec_ymax = 0;
+    if constexpr (R == 64) {
         const int j = lane;
-        const int xcc = min(max(xc, 0), nc - 1), ycc = min(max(yc, 0), nc - 1);
-        const int64_t out_col = colbase(xc + 1, yc);
-        const int cs = (j - REC_UCMB) % 3;
-        const T *sc = cs == 0 ? a.src0 : (cs == 1 ? a.src1 : a.src2);
-        const T *ptr = a.latgeom + (((int64_t)sd * nc + xcc) * nc + ycc) * 20 + min(j, REC_NT0 - 1);
-        ptr = j >= REC_NT0 ? a.coords + 3 * colbase(xc, yc + 1) + (j - REC_NT0) : ptr;
-        ptr = j >= REC_NT1 ? a.coords + 3 * colbase(xc + 1, yc + 1) + (j - REC_NT1) : ptr;
-        ptr = j >= REC_NOUT ? a.coords + 3 * out_col + (j - REC_NOUT) : ptr;
-        ptr = j >= REC_BTAB && a.btab ? a.btab + 6 * out_col + (j - REC_BTAB) : ptr;
-        ptr = j >= REC_UCMB ? sc + out_col * nr + (j < REC_USURF ? 0 : nr - 1) : ptr;
-        ptr = j >= REC_LEN ? a.latgeom : ptr;
-        const T val = *ptr;
-        return j < REC_LEN ? val : T(0);
+        const int xcc = min(max(xc, 0), nc - 1);
+        const int xq0 = min(max(xc, 0), nx - 1), xq1 = min(max(xc + 1, 0), nx - 1);
+        const int64_t col0 = ((int64_t)sd * nx + xq0) * nx, col1 = ((int64_t)sd * nx + xq1) * nx;
+        if (j < REC_NT0) {
+            rec_base = a.latgeom + ((int64_t)sd * nc + xcc) * nc * 20 + j;
+            rec_stride = 20, rec_ymax = nc - 1;
+        } else if (j < REC_NT1) {
+            rec_base = a.coords + 3 * col0 + (j - REC_NT0);
+            rec_stride = 3, rec_dy = 1, rec_ymax = nx - 1;
+        } else if (j < REC_NOUT) {
+            rec_base = a.coords + 3 * col1 + (j - REC_NT1);
+            rec_stride = 3, rec_dy = 1, rec_ymax = nx - 1;
+        } else if (j < REC_BTAB) {
+            rec_base = a.coords + 3 * col1 + (j - REC_NOUT);
+            rec_stride = 3, rec_ymax = nx - 1;
+        } else if (j < REC_UCMB) {
+            if (a.btab) {
+                rec_base = a.btab + 6 * col1 + (j - REC_BTAB);
+                rec_stride = 6, rec_ymax = nx - 1;
+            }
+        } else if (j < REC_LEN) {
+            const int c = (j - REC_UCMB) % 3;
+            const T *sc = c == 0 ? a.src0 : (c == 1 ? a.src1 : a.src2);
+            rec_base = sc + col1 * nr + (j < REC_USURF ? 0 : nr - 1);
+            rec_stride = nr, rec_ymax = nx - 1;
+        }
+    }
+    auto load_record = [&](int yc) -> T {
+        const int y = min(max(yc + rec_dy, 0), rec_ymax);
+        const T val = rec_base[(int64_t)y * rec_stride];
+        return lane < REC_LEN ? val : T(0);
     };
 
     NodeCol<T, K> B0, B1, T0, T1;
@@ -386,11 +419,7 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
         load_raw<T, K>(a, colbase(xc, yc + 2) * nr, lvl0, P0);
         load_raw<T, K>(a, colbase(xc + 1, yc + 2) * nr, lvl0, P1);
 
-        T s00[KL][3], s01[KL][3];
-#pragma unroll
-        for (int i = 0; i < KL; ++i)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) s00[i][c] = s01[i][c] = T(0);
+        T s00[KL][3], s01[KL][3];  // written by the corner loop below
 
         const bool cell_ok = xc >= 0 && xc < nc && yc >= 0 && yc < nc;
         {
@@ -413,32 +442,33 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
 #pragma unroll
             for (int p = 0; p < K; ++p) {
                 const T cvp = cell_ok ? cv[p] : T(0);
-                T A0[3], A1[3], Cc[3];
+                T A0[3], A1[3], Cc[3], E0[3], E1[3], Cd[3];
                 // wedge 0: (c00, c10, c01)
                 wedge_stress<T>(g0, B0.su[p], B1.su[p], T0.su[p], B0.du[p], B1.du[p], T0.du[p],
                                 B0.kk[p] + B1.kk[p] + T0.kk[p], qq[p], cvp, A0, A1, Cc);
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const T x00 = -(A0[c] + A1[c]);
-                    s00[p][c] += x00 - Cc[c];
-                    s00[p + 1][c] += x00 + Cc[c];
-                    acc_cur[p][c] += A0[c] - Cc[c];
-                    acc_cur[p + 1][c] += A0[c] + Cc[c];
-                    s01[p][c] += A1[c] - Cc[c];
-                    s01[p + 1][c] += A1[c] + Cc[c];
-                }
                 // wedge 1: (c11, c01, c10)
                 wedge_stress<T>(g1, T1.su[p], T0.su[p], B1.su[p], T1.du[p], T0.du[p], B1.du[p],
-                                T1.kk[p] + T0.kk[p] + B1.kk[p], qq[p], cvp, A0, A1, Cc);
+                                T1.kk[p] + T0.kk[p] + B1.kk[p], qq[p], cvp, E0, E1, Cd);
+                // corners: c00 and c11 get one wedge each, c10 and c01 both;
+                // lateral parts L and radial parts C enter the bottom node as
+                // L - C and the top node as L + C
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const T x11 = -(A0[c] + A1[c]);
-                    acc_next[p][c] += x11 - Cc[c];
-                    acc_next[p + 1][c] += x11 + Cc[c];
-                    s01[p][c] += A0[c] - Cc[c];
-                    s01[p + 1][c] += A0[c] + Cc[c];
-                    acc_cur[p][c] += A1[c] - Cc[c];
-                    acc_cur[p + 1][c] += A1[c] + Cc[c];
+                    const T x00 = -(A0[c] + A1[c]), x11 = -(E0[c] + E1[c]);
+                    const T l10 = A0[c] + E1[c], l01 = A1[c] + E0[c], c2 = Cc[c] + Cd[c];
+                    if (p == 0) {
+                        s00[p][c] = x00 - Cc[c];
+                        s01[p][c] = l01 - c2;
+                    } else {
+                        s00[p][c] += x00 - Cc[c];
+                        s01[p][c] += l01 - c2;
+                    }
+                    s00[p + 1][c] = x00 + Cc[c];
+                    s01[p + 1][c] = l01 + c2;
+                    acc_cur[p][c] += l10 - c2;
+                    acc_cur[p + 1][c] += l10 + c2;
+                    acc_next[p][c] += x11 - Cd[c];
+                    acc_next[p + 1][c] += x11 + Cd[c];
                 }
             }
         }
@@ -518,10 +548,12 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
                     const int bcv = at_cmb ? a.bc_cmb : a.bc_surf;
                     T bt[3], u[3];
                     if constexpr (R == 64) {
+                        // lane 0 takes the CMB entries, the top lane the surface ones
+                        const int off = at_cmb ? 0 : 3;
 #pragma unroll
                         for (int c = 0; c < 3; ++c) {
-                            bt[c] = at_cmb ? lane_value(rec, REC_BTAB + c) : lane_value(rec, REC_BTAB + 3 + c);
-                            u[c] = at_cmb ? lane_value(rec, REC_UCMB + c) : lane_value(rec, REC_USURF + c);
+                            bt[c] = lane_fetch(rec, REC_BTAB + off + c);
+                            u[c] = lane_fetch(rec, REC_UCMB + off + c);
                         }
                     } else {
                         const T *btp = a.btab + (lcol * 2 + (at_cmb ? 0 : 1)) * 3;
