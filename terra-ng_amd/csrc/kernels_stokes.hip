@@ -57,6 +57,7 @@ template <typename T> struct DivArgs {
     int nsd, nx, nr;             // fine
     int nxc, nrc;                // coarse
     int bc_cmb, bc_surf;
+    Index4 cidx;                 // coarse cells (nsd, nxc-1, nxc-1, nrc-1)
 };
 
 // Boundary kind of fine cell layer r (boundary_setup of the reference's
@@ -96,14 +97,11 @@ template <typename T> __device__ __forceinline__ void bc_project(const T *coords
 template <typename T> __global__ __launch_bounds__(256) void div_partial_kernel(const DivArgs<T> a)
 {
     const int ncc = a.nxc - 1, nrcc = a.nrc - 1;
-    const int64_t n = (int64_t)a.nsd * ncc * ncc * nrcc;
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = (uint32_t)a.nsd * ncc * ncc * nrcc;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= n) return;
-    const int rc = (int)(gid % nrcc);
-    int64_t t = gid / nrcc;
-    const int yc = (int)(t % ncc); t /= ncc;
-    const int xc = (int)(t % ncc);
-    const int sd = (int)(t / ncc);
+    int sd, xc, yc, rc;
+    a.cidx.decode(gid, sd, xc, yc, rc);
     const int nx = a.nx, nr = a.nr, nc = nx - 1;
     const bool has_cmb = a.sd_cmb[sd] != 0, has_surf = a.sd_surf[sd] != 0;
     T acc[2][6];
@@ -149,16 +147,14 @@ template <typename T> __global__ __launch_bounds__(256) void div_partial_kernel(
 
 // Phase 2 of B: coarse node = sum of the coarse-wedge partials it belongs to.
 template <typename T>
-__global__ __launch_bounds__(256) void wedge_gather_kernel(const T *partial, T *dst, int add, int nsd, int nx, int nr)
+__global__ __launch_bounds__(256) void wedge_gather_kernel(const T *partial, T *dst, int add, int nsd, int nx, int nr,
+                                                           const Index4 idx)
 {
-    const int64_t n = (int64_t)nsd * nx * nx * nr;
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = (uint32_t)nsd * nx * nx * nr;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= n) return;
-    const int l = (int)(gid % nr);
-    int64_t t = gid / nr;
-    const int Y = (int)(t % nx); t /= nx;
-    const int X = (int)(t % nx);
-    const int sd = (int)(t / nx);
+    int sd, X, Y, l;
+    idx.decode(gid, sd, X, Y, l);
     const int nc = nx - 1, ncr = nr - 1;
     // lateral corner index of (dx, dy) in wedge w (-1: not a corner)
     const int corner_of[2][2][2] = {{{0, 2}, {1, -1}}, {{-1, 1}, {2, 0}}};
@@ -190,6 +186,7 @@ template <typename T> struct GradArgs {
     const uint8_t *sd_cmb, *sd_surf;
     int nsd, nx, nr, nxc, nrc;
     int bc_cmb, bc_surf, add;
+    Index4 fcell, fnode;          // fine cells (nsd, nx-1, nx-1, nr-1) and fine nodes
 };
 
 // Phase 1 of B^T: per fine wedge, f = -|det J| p(centroid), with p
@@ -198,14 +195,11 @@ template <typename T> struct GradArgs {
 template <typename T> __global__ __launch_bounds__(256) void grad_weight_kernel(const GradArgs<T> a, T *fw)
 {
     const int nx = a.nx, nr = a.nr, nc = nx - 1, ncr = nr - 1;
-    const int64_t n = (int64_t)a.nsd * nc * nc * ncr;
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = (uint32_t)a.nsd * nc * nc * ncr;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= n) return;
-    const int rc = (int)(gid % ncr);
-    int64_t t = gid / ncr;
-    const int yc = (int)(t % nc); t /= nc;
-    const int xc = (int)(t % nc);
-    const int sd = (int)(t / nc);
+    int sd, xc, yc, rc;
+    a.fcell.decode(gid, sd, xc, yc, rc);
     const T r0 = a.radii[(int64_t)sd * nr + rc], r1 = a.radii[(int64_t)sd * nr + rc + 1];
     const T rmid = T(0.5) * (r0 + r1), h = T(0.5) * (r1 - r0);
     const int cx = xc / 2, cy = yc / 2, cr = rc / 2;
@@ -219,7 +213,7 @@ template <typename T> __global__ __launch_bounds__(256) void grad_weight_kernel(
             const int jx = cx + kWC[w][j % 3][0], jy = cy + kWC[w][j % 3][1], jr = cr + j / 3;
             pi += shape_coarse_c<T>(j, lat, rc & 1) * a.p[(((int64_t)sd * a.nxc + jx) * a.nxc + jy) * a.nrc + jr];
         }
-        fw[gid * 2 + w] = -(g[9] * rmid * rmid * h) * pi;
+        fw[(size_t)gid * 2 + w] = -(g[9] * rmid * rmid * h) * pi;
     }
 }
 
@@ -228,15 +222,12 @@ template <typename T> __global__ __launch_bounds__(256) void grad_weight_kernel(
 template <typename T> __global__ __launch_bounds__(256) void gradient_kernel(const GradArgs<T> a, const T *fw)
 {
     const int nx = a.nx, nr = a.nr, nc = nx - 1, ncr = nr - 1;
-    const int64_t n = (int64_t)a.nsd * nx * nx * nr;
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = (uint32_t)a.nsd * nx * nx * nr;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= n) return;
-    const int l = (int)(gid % nr);
-    int64_t t = gid / nr;
-    const int Y = (int)(t % nx); t /= nx;
-    const int X = (int)(t % nx);
-    const int sd = (int)(t / nx);
-    const int64_t lcol = t * nx + Y;
+    int sd, X, Y, l;
+    a.fnode.decode(gid, sd, X, Y, l);
+    const int64_t lcol = ((int64_t)sd * nx + X) * nx + Y;
     const int corner_of[2][2][2] = {{{0, 2}, {1, -1}}, {{-1, 1}, {2, 0}}};
     const bool has_cmb = a.sd_cmb[sd] != 0, has_surf = a.sd_surf[sd] != 0;
     // radial factors of the (at most) two cells below / above the node
@@ -301,16 +292,13 @@ template <typename T> __global__ __launch_bounds__(256) void gradient_kernel(con
 // P1 interpolation per fine node from the coarse nodes of its position.
 template <typename T>
 __global__ __launch_bounds__(256) void prolongate_kernel(const Comp3<T> src, const Comp3<T> dst, int ncomp, int add,
-                                                         int nsd, int nxf, int nrf, int nxc, int nrc)
+                                                         int nsd, int nxf, int nrf, int nxc, int nrc, const Index4 idx)
 {
-    const int64_t n = (int64_t)nsd * nxf * nxf * nrf;
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = (uint32_t)nsd * nxf * nxf * nrf;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= n) return;
-    const int rf = (int)(gid % nrf);
-    int64_t t = gid / nrf;
-    const int yf = (int)(t % nxf); t /= nxf;
-    const int xf = (int)(t % nxf);
-    const int sd = (int)(t / nxf);
+    int sd, xf, yf, rf;
+    idx.decode(gid, sd, xf, yf, rf);
     // lateral parents (up to 2) with weights
     int px[2], py[2], np = 0;
     T pw;
@@ -337,16 +325,13 @@ __global__ __launch_bounds__(256) void prolongate_kernel(const Comp3<T> src, con
 template <typename T>
 __global__ __launch_bounds__(256) void restrict_kernel(const Comp3<T> src, const Comp3<T> dst, int ncomp, int add,
                                                        const uint8_t *omask_f, int nsd, int nxf, int nrf, int nxc,
-                                                       int nrc)
+                                                       int nrc, const Index4 idx)
 {
-    const int64_t n = (int64_t)nsd * nxc * nxc * nrc;
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = (uint32_t)nsd * nxc * nxc * nrc;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= n) return;
-    const int rc = (int)(gid % nrc);
-    int64_t t = gid / nrc;
-    const int yc = (int)(t % nxc); t /= nxc;
-    const int xc = (int)(t % nxc);
-    const int sd = (int)(t / nxc);
+    int sd, xc, yc, rc;
+    idx.decode(gid, sd, xc, yc, rc);
     const int st[7][2] = {{-1, 0}, {-1, 1}, {0, 1}, {1, 0}, {1, -1}, {0, -1}, {0, 0}};
     T s[3] = {T(0), T(0), T(0)};
     for (int dr = -1; dr <= 1; ++dr) {
@@ -501,7 +486,12 @@ __global__ __launch_bounds__(256) void buoyancy_kernel(const T *temp, const Comp
     for (int d = 0; d < 3; ++d) u.p[d][gid] = v * c[d] * inv;
 }
 
-unsigned blocks_for(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+// one thread per element; the node kernels index with 32 bits
+unsigned blocks_for(int64_t n, int bs = 256)
+{
+    TNG_REQUIRE(n < ((int64_t)1 << 31), "more than 2^31 elements per launch");
+    return (unsigned)((n + bs - 1) / bs);
+}
 
 template <typename T> struct MeshP {
     const T *coords, *radii, *latgeom;
@@ -530,10 +520,12 @@ template <typename T> void div_impl(tng_operator &op, const tng_vector &src, tng
     a.partial = op.scratch.as<T>();
     a.nsd = f.nsd; a.nx = f.nx; a.nr = f.nr; a.nxc = c.nx; a.nrc = c.nr;
     a.bc_cmb = op.bc_cmb; a.bc_surf = op.bc_surf;
+    a.cidx = Index4(c.nx - 1, c.nr - 1);
     hipLaunchKernelGGL(div_partial_kernel<T>, dim3(blocks_for(ncell_c)), dim3(256), 0, s, a);
     TNG_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(wedge_gather_kernel<T>, dim3(blocks_for(c.nalloc)), dim3(256), 0, s,
-                       (const T *)op.scratch.p, (T *)dst.comp(0), op.apply_mode == TNG_ADD ? 1 : 0, c.nsd, c.nx, c.nr);
+                       (const T *)op.scratch.p, (T *)dst.comp(0), op.apply_mode == TNG_ADD ? 1 : 0, c.nsd, c.nx, c.nr,
+                       Index4(c.nx, c.nr));
     TNG_HIP_CHECK(hipGetLastError());
 }
 
@@ -548,6 +540,8 @@ template <typename T> void grad_impl(tng_operator &op, const tng_vector &src, tn
     a.sd_cmb = f.d_sd_cmb.as<uint8_t>(); a.sd_surf = f.d_sd_surf.as<uint8_t>();
     a.nsd = f.nsd; a.nx = f.nx; a.nr = f.nr; a.nxc = c.nx; a.nrc = c.nr;
     a.bc_cmb = op.bc_cmb; a.bc_surf = op.bc_surf; a.add = op.apply_mode == TNG_ADD ? 1 : 0;
+    a.fcell = Index4(f.nx - 1, f.nr - 1);
+    a.fnode = Index4(f.nx, f.nr);
     const int64_t nfw = (int64_t)f.nsd * (f.nx - 1) * (f.nx - 1) * (f.nr - 1) * 2;
     const size_t bytes = (size_t)nfw * sizeof(T);
     if (op.scratch.bytes < bytes) op.scratch.alloc(bytes);
@@ -561,7 +555,8 @@ template <typename T> void prolong_impl(tng_operator &op, const tng_vector &src,
 {
     const tng_domain &c = *op.mesh2, &f = *op.mesh;  // mesh = fine (dst), mesh2 = coarse (src)
     hipLaunchKernelGGL(prolongate_kernel<T>, dim3(blocks_for(f.nalloc)), dim3(256), 0, s, comps<T>(src),
-                       comps<T>(dst), src.ncomp, op.apply_mode == TNG_ADD ? 1 : 0, f.nsd, f.nx, f.nr, c.nx, c.nr);
+                       comps<T>(dst), src.ncomp, op.apply_mode == TNG_ADD ? 1 : 0, f.nsd, f.nx, f.nr, c.nx, c.nr,
+                       Index4(f.nx, f.nr));
     TNG_HIP_CHECK(hipGetLastError());
 }
 
@@ -570,7 +565,7 @@ template <typename T> void restrict_impl(tng_operator &op, const tng_vector &src
     const tng_domain &f = *op.mesh, &c = *op.mesh2;  // mesh = fine (src), mesh2 = coarse (dst)
     hipLaunchKernelGGL(restrict_kernel<T>, dim3(blocks_for(c.nalloc)), dim3(256), 0, s, comps<T>(src),
                        comps<T>(dst), src.ncomp, op.apply_mode == TNG_ADD ? 1 : 0, f.d_omask.as<uint8_t>(), f.nsd,
-                       f.nx, f.nr, c.nx, c.nr);
+                       f.nx, f.nr, c.nx, c.nr, Index4(c.nx, c.nr));
     TNG_HIP_CHECK(hipGetLastError());
 }
 

@@ -37,6 +37,50 @@ struct Error : std::runtime_error {
 enum : uint8_t { SBF_INNER = 1, SBF_CMB = 2 | 4, SBF_SURFACE = 2 | 8 };
 enum : uint8_t { OWNED = 1 };
 
+// Division by a launch-constant divisor through a multiply-high (round-up
+// method of Granlund and Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994): q = (mulhi(m, n) + n) >> l with
+// l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1.  Exact for n < 2^31
+// (the sum then fits 32 bits); index decoding in the node kernels uses it
+// instead of 64-bit division.
+struct FastDiv {
+    uint32_t d = 1, m = 1;
+    int l = 0;
+    FastDiv() = default;
+    explicit FastDiv(uint32_t div) : d(div)
+    {
+        l = 0;
+        while (((uint64_t)1 << l) < div) ++l;
+        m = (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << l) - div)) / div + 1);
+    }
+    __host__ __device__ __forceinline__ uint32_t div(uint32_t n) const
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        const uint32_t t = __umulhi(m, n);
+#else
+        const uint32_t t = (uint32_t)(((uint64_t)m * n) >> 32);
+#endif
+        return (t + n) >> l;
+    }
+};
+
+// (sd, x, y, r) of a linear index over a (nsd, nx, nx, nr) grid, r fastest
+struct Index4 {
+    FastDiv r, x;
+    Index4() = default;
+    Index4(int nx, int nr) : r((uint32_t)nr), x((uint32_t)nx) {}
+    __host__ __device__ __forceinline__ void decode(uint32_t i, int &sd, int &X, int &Y, int &l) const
+    {
+        const uint32_t t = r.div(i);
+        l = (int)(i - t * r.d);
+        const uint32_t t2 = x.div(t);
+        Y = (int)(t - t2 * x.d);
+        const uint32_t t3 = x.div(t2);
+        X = (int)(t2 - t3 * x.d);
+        sd = (int)t3;
+    }
+};
+
 // Device buffer RAII.
 struct DevBuf {
     void *p = nullptr;
