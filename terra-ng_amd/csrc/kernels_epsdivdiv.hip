@@ -268,11 +268,27 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
     }
     const int seg = b % a.segs;
     b /= a.segs;
-    const int strip = b % a.strips;
-    const int sd = b / a.strips;
     const int nx = a.nx, nc = nx - 1, nr = a.nr, ncells = a.ncells;
-    const int XG = strip * (VX - 1);
-    const int xc = XG - 1 + v;
+    int sd, xc;
+    bool col_ok = true;  // the virtual column exists
+    if constexpr (R == 64) {
+        // One wave per column: strips run over the virtual columns of all
+        // subdomains in a row (xc = -1 .. nc-1 per subdomain; xc = -1 has no
+        // cells and only outputs node column 0), so only the last strip is
+        // ragged.  A neighbour in the next subdomain has no cells there and
+        // passes zeros.
+        const int nvc = a.nsd * nx;
+        const int vcol = b * (VX - 1) + v;
+        col_ok = vcol < nvc;
+        const int vc = min(vcol, nvc - 1);
+        sd = vc / nx;
+        // past the last subdomain: a column without cells (zero contributions)
+        xc = col_ok ? vc - sd * nx - 1 : nc;
+    } else {
+        const int strip = b % a.strips;
+        sd = b / a.strips;
+        xc = strip * (VX - 1) - 1 + v;
+    }
     const int X = xc + 1;
     const int Y0 = seg * a.seg_len;
     const int Yend = min(Y0 + a.seg_len, nx);
@@ -520,7 +536,7 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
             }
         }
         const int Y = yc;
-        if (Y >= Y0 && v < VX - 1 && X < nx && lane_has) {
+        if (Y >= Y0 && v < VX - 1 && X < nx && col_ok && lane_has) {
             const int64_t lcol = ((int64_t)sd * nx + X) * nx + Y;
             const int64_t base = lcol * nr;
             // unit normal of the output node (free-slip rows, penalty)
@@ -879,16 +895,59 @@ template <> MeshPtrs<float> mesh_ptrs<float>(const tng_domain &m)
     return {m.d_coords_f.as<float>(), m.d_radii_f.as<float>(), m.d_latgeom_f.as<float>()};
 }
 
-template <typename T, int R, int K, int NW, int RG>
-void launch_apply_rk(const ApplyArgs<T> &a, int blocks, hipStream_t s)
-{
-    hipLaunchKernelGGL((epsdivdiv_apply_kernel<T, R, K, NW, RG>), dim3(blocks), dim3(NW * 64), 0, s, a);
-}
-
 int env_int(const char *name, int dflt)
 {
     const char *v = std::getenv(name);
     return v ? std::atoi(v) : dflt;
+}
+
+// Workgroups resident at once on the current device for one kernel.
+int resident_groups(const void *kernel, int threads)
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    TNG_HIP_CHECK(hipGetDevice(&dev));
+    TNG_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    TNG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0));
+    return std::max(1, per_cu) * std::max(1, cus);
+}
+
+// y-segment length.  A workgroup sweeps seg rows plus one halo row, after a
+// prologue of about 1.5 row-steps (the first two rows' loads); the grid runs
+// in ceil(blocks / resident) rounds.  Pick the length with the least
+// rounds * (seg + 2.5).
+int pick_segment(int nx, int64_t lateral_groups, int resident)
+{
+    int best = nx;
+    double best_cost = 1e300;
+    for (int seg = std::min(4, nx); seg <= nx; ++seg) {
+        const int64_t segs = (nx + seg - 1) / seg;
+        const int64_t rounds = (lateral_groups * segs + resident - 1) / resident;
+        const double cost = (double)rounds * (seg + 2.5);
+        if (cost < best_cost - 1e-9) {
+            best_cost = cost;
+            best = seg;
+        }
+    }
+    return best;
+}
+
+template <typename T, int R, int K, int NW, int RG>
+void launch_apply_rk(ApplyArgs<T> a, int64_t lateral_groups, hipStream_t s)
+{
+    const void *kern = reinterpret_cast<const void *>(&epsdivdiv_apply_kernel<T, R, K, NW, RG>);
+    static int cached_dev = -1, cached_resident = 0;
+    int dev = 0;
+    TNG_HIP_CHECK(hipGetDevice(&dev));
+    if (dev != cached_dev) {
+        cached_resident = resident_groups(kern, NW * 64);
+        cached_dev = dev;
+    }
+    int seg = env_int("TNG_APPLY_SEG", 0);
+    if (seg <= 0) seg = pick_segment(a.nx, lateral_groups, cached_resident);
+    a.seg_len = seg;
+    a.segs = (a.nx + seg - 1) / seg;
+    const int blocks = (int)(lateral_groups * a.segs);
+    hipLaunchKernelGGL((epsdivdiv_apply_kernel<T, R, K, NW, RG>), dim3(blocks), dim3(NW * 64), 0, s, a);
 }
 
 template <typename T> void launch_apply(ApplyArgs<T> a, hipStream_t s)
@@ -913,37 +972,27 @@ template <typename T> void launch_apply(ApplyArgs<T> a, hipStream_t s)
     int NW = K == 1 ? 8 : 4;
     if (K == 1 && R == 64 && RG == 1 && env_int("TNG_APPLY_WAVES", 8) == 4) NW = 4;
     const int VX = NW / RG * (64 / R);
-    a.strips = (a.nx + (VX - 1) - 1) / (VX - 1);
-    // y-segment length: fewer, longer segments recompute fewer halo rows;
-    // enough workgroups are needed to fill the 256 CUs
-    const int64_t lateral_groups = (int64_t)a.nsd * a.strips;
-    int seg = env_int("TNG_APPLY_SEG", 0);
-    if (seg <= 0) {
-        const int64_t target = env_int("TNG_APPLY_BLOCKS", 480);
-        int nseg = (int)std::max<int64_t>(1, (target + lateral_groups - 1) / lateral_groups);
-        nseg = std::min(nseg, std::max(1, a.nx / 8));
-        seg = (a.nx + nseg - 1) / nseg;
-    }
-    a.seg_len = seg;
-    a.segs = (a.nx + seg - 1) / seg;
-    const int blocks = (int)(lateral_groups * a.segs);
+    // R = 64: strips over the columns of all subdomains in a row; otherwise
+    // strips per subdomain
+    a.strips = R == 64 ? (a.nsd * a.nx + (VX - 1) - 1) / (VX - 1) : (a.nx + (VX - 1) - 1) / (VX - 1);
+    const int64_t lg = R == 64 ? (int64_t)a.strips : (int64_t)a.nsd * a.strips;
     if (RG == 2) {
-        launch_apply_rk<T, 64, 1, 8, 2>(a, blocks, s);
+        launch_apply_rk<T, 64, 1, 8, 2>(a, lg, s);
     } else if (K == 1) {
         switch (R) {
-        case 4: launch_apply_rk<T, 4, 1, 8, 1>(a, blocks, s); break;
-        case 8: launch_apply_rk<T, 8, 1, 8, 1>(a, blocks, s); break;
-        case 16: launch_apply_rk<T, 16, 1, 8, 1>(a, blocks, s); break;
-        case 32: launch_apply_rk<T, 32, 1, 8, 1>(a, blocks, s); break;
+        case 4: launch_apply_rk<T, 4, 1, 8, 1>(a, lg, s); break;
+        case 8: launch_apply_rk<T, 8, 1, 8, 1>(a, lg, s); break;
+        case 16: launch_apply_rk<T, 16, 1, 8, 1>(a, lg, s); break;
+        case 32: launch_apply_rk<T, 32, 1, 8, 1>(a, lg, s); break;
         default:
-            if (NW == 4) launch_apply_rk<T, 64, 1, 4, 1>(a, blocks, s);
-            else launch_apply_rk<T, 64, 1, 8, 1>(a, blocks, s);
+            if (NW == 4) launch_apply_rk<T, 64, 1, 4, 1>(a, lg, s);
+            else launch_apply_rk<T, 64, 1, 8, 1>(a, lg, s);
             break;
         }
     } else if (K == 2) {
-        launch_apply_rk<T, 64, 2, 4, 1>(a, blocks, s);
+        launch_apply_rk<T, 64, 2, 4, 1>(a, lg, s);
     } else {
-        launch_apply_rk<T, 64, 4, 4, 1>(a, blocks, s);
+        launch_apply_rk<T, 64, 4, 4, 1>(a, lg, s);
     }
     TNG_HIP_CHECK(hipGetLastError());
 }
