@@ -19,12 +19,17 @@
 // tau g_k with g_k = J^{-T} grad N_k.  This is the reference's arithmetic
 // regrouped (about 130 flops per wedge instead of about 450).
 //
-// Work decomposition.  Lanes run along r (one radial cell per lane, K cells
-// per lane for more than 64 layers) so every global access is a contiguous
-// run of the r-fastest layout.  A "virtual column" is a group of R lanes that
-// owns one lateral cell column (xc) and sweeps y.  The VX virtual columns of
-// a workgroup cover the cell columns XG-1 .. XG+VX-2; virtual column v
-// outputs node column XG+v.  The contributions of a cell to its left corners
+// Work decomposition.  Lanes run along r (one radial cell per lane; 128
+// layers as two radial chunks of one wave each; K cells per lane beyond) so
+// every global access is a contiguous run of the r-fastest layout.  A
+// "virtual column" is a group of R lanes that owns one lateral cell column
+// (xc) and sweeps a segment of y.  The VX virtual columns of a workgroup cover
+// consecutive cell columns (xc = -1 stands for the empty column left of a
+// subdomain; with one column per wave the strips run across subdomains);
+// virtual column v outputs node column xc+1 for v < VX-1, so neighbouring
+// strips overlap by one column.  The lateral data of the next step (geometry,
+// normals, boundary rows) is one vector load of a per-wave record, broadcast
+// with readlane.  The contributions of a cell to its left corners
 // belong to the neighbour on the left and are passed through LDS once per
 // y-step, already combined radially; radial neighbours are combined with a
 // DPP wave shift.  The next row of node columns is prefetched one step ahead.
