@@ -41,6 +41,10 @@
 
 #include "tng_internal.hpp"
 
+#ifndef TNG_APPLY_SMEM_GEOM
+#define TNG_APPLY_SMEM_GEOM 1
+#endif
+
 namespace tng {
 
 namespace {
@@ -418,6 +422,19 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
     auto do_step = [&](int step, NodeCol<T, K> &B0, NodeCol<T, K> &B1, NodeCol<T, K> &T0, NodeCol<T, K> &T1,
                        T (&acc_cur)[KL][3], T (&acc_next)[KL][3], T &rec, T &rec_next) {
         const int yc = Y0 - 1 + step;
+        // one cell per lane and column per wave: the cell's geometry is
+        // wave-uniform; scalar loads issued here have the processing of the
+        // new row to arrive (more cells per lane run out of scalar registers
+        // and take it from the record instead)
+        T sg[20];
+        if constexpr (R == 64 && K == 1 && TNG_APPLY_SMEM_GEOM) {
+            using CT = const T __attribute__((address_space(4)));
+            const int xcc = min(max(xc, 0), nc - 1), ycc = min(max(yc, 0), nc - 1);
+            const int gi = __builtin_amdgcn_readfirstlane(((sd * nc + xcc) * nc + ycc) * 20);
+            const CT *gw = (const CT *)a.latgeom + gi;
+#pragma unroll
+            for (int i = 0; i < 20; ++i) sg[i] = gw[i];
+        }
         // the prefetched row yc+1 becomes the top row; prefetch row yc+2
         {
             T n0[3] = {T(0), T(0), T(0)}, n1[3] = {T(0), T(0), T(0)};
@@ -445,7 +462,13 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
         const bool cell_ok = xc >= 0 && xc < nc && yc >= 0 && yc < nc;
         {
             T g0[10], g1[10];
-            if constexpr (R == 64) {
+            if constexpr (R == 64 && K == 1 && TNG_APPLY_SMEM_GEOM) {
+#pragma unroll
+                for (int i = 0; i < 10; ++i) {
+                    g0[i] = sg[i];
+                    g1[i] = sg[10 + i];
+                }
+            } else if constexpr (R == 64) {
 #pragma unroll
                 for (int i = 0; i < 10; ++i) {
                     g0[i] = lane_value(rec, REC_GEOM + i);
