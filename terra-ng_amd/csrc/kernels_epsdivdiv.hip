@@ -74,16 +74,17 @@ template <typename T> __device__ __forceinline__ void normal_of(const T *c, T n[
     n[2] = c[2] * inv;
 }
 
-// DPP wave_shr:1 -- lane i receives lane i-1 (lane 0 receives 0).
+// DPP wave_shr:1 -- lane i receives lane i-1; lane 0 has no source and the
+// bound control writes 0 there (no initialisation of the destination).
 __device__ __forceinline__ double wave_shr1(double v)
 {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, false);
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x138, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x138, 0xf, 0xf, true);
     return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ float wave_shr1(float v)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
 }
 
 // Value of lane j (a compile-time constant) as a wave-uniform scalar.
@@ -158,13 +159,19 @@ template <typename T> struct LaneBC {
     T z_lo, f_lo, z_hi, f_hi;
 };
 
-template <typename T, int K>
+constexpr int BCM_RUNTIME = 7;
+
+template <typename T, int K, int BCM>
 __device__ __forceinline__ void process(const RawCol<T, K> &r, const T n[3], const LaneBC<T> &bc, bool any_fs,
                                         bool any_dir, NodeCol<T, K> &o)
 {
+    // BCM: bit 0 free-slip, bit 1 Dirichlet compiled in and applied to every
+    // lane (the factors are neutral away from the boundary); BCM_RUNTIME:
+    // decided per subdomain
+    constexpr bool rt = BCM == BCM_RUNTIME;
     T lo[3] = {r.u[0][0], r.u[0][1], r.u[0][2]};
     T hi[3] = {r.u[K][0], r.u[K][1], r.u[K][2]};
-    if (any_fs) {
+    if (rt ? any_fs : (BCM & 1) != 0) {
         const T dl = bc.f_lo * (n[0] * lo[0] + n[1] * lo[1] + n[2] * lo[2]);
         const T dh = bc.f_hi * (n[0] * hi[0] + n[1] * hi[1] + n[2] * hi[2]);
 #pragma unroll
@@ -173,7 +180,7 @@ __device__ __forceinline__ void process(const RawCol<T, K> &r, const T n[3], con
             hi[c] -= dh * n[c];
         }
     }
-    if (any_dir) {
+    if (rt ? any_dir : (BCM & 2) != 0) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             lo[c] *= bc.z_lo;
@@ -240,7 +247,7 @@ __device__ __forceinline__ void wedge_stress(const T *g /*M0,M1,M2,detL*/, const
     }
 }
 
-template <typename T, int R, int K, int NW, int RG>
+template <typename T, int R, int K, int NW, int RG, int BCM>
 __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArgs<T> a)
 {
     // RG > 1: the radial cells of a column are split into RG chunks of 64,
@@ -391,8 +398,8 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
             load_normal(xc, Y0 - 1, n0);
             load_normal(xc + 1, Y0 - 1, n1);
         }
-        process<T, K>(P0, n0, lbc, any_fs, any_dir, B0);
-        process<T, K>(P1, n1, lbc, any_fs, any_dir, B1);
+        process<T, K, BCM>(P0, n0, lbc, any_fs, any_dir, B0);
+        process<T, K, BCM>(P1, n1, lbc, any_fs, any_dir, B1);
         load_raw<T, K>(a, colbase(xc, Y0) * nr, lvl0, P0);
         load_raw<T, K>(a, colbase(xc + 1, Y0) * nr, lvl0, P1);
         if constexpr (R == 64) rec = load_record(Y0 - 1);
@@ -438,7 +445,7 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
         // the prefetched row yc+1 becomes the top row; prefetch row yc+2
         {
             T n0[3] = {T(0), T(0), T(0)}, n1[3] = {T(0), T(0), T(0)};
-            if (any_fs) {
+            if (BCM == BCM_RUNTIME ? any_fs : (BCM & 1) != 0) {
                 if constexpr (R == 64) {
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
@@ -450,8 +457,8 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
                     load_normal(xc + 1, yc + 1, n1);
                 }
             }
-            process<T, K>(P0, n0, lbc, any_fs, any_dir, T0);
-            process<T, K>(P1, n1, lbc, any_fs, any_dir, T1);
+            process<T, K, BCM>(P0, n0, lbc, any_fs, any_dir, T0);
+            process<T, K, BCM>(P1, n1, lbc, any_fs, any_dir, T1);
         }
         if constexpr (R == 64) rec_next = load_record(yc + 1);
         load_raw<T, K>(a, colbase(xc, yc + 2) * nr, lvl0, P0);
@@ -959,10 +966,10 @@ int pick_segment(int nx, int64_t lateral_groups, int resident)
     return best;
 }
 
-template <typename T, int R, int K, int NW, int RG>
+template <typename T, int R, int K, int NW, int RG, int BCM = BCM_RUNTIME>
 void launch_apply_rk(ApplyArgs<T> a, int64_t lateral_groups, hipStream_t s)
 {
-    const void *kern = reinterpret_cast<const void *>(&epsdivdiv_apply_kernel<T, R, K, NW, RG>);
+    const void *kern = reinterpret_cast<const void *>(&epsdivdiv_apply_kernel<T, R, K, NW, RG, BCM>);
     static int cached_dev = -1, cached_resident = 0;
     int dev = 0;
     TNG_HIP_CHECK(hipGetDevice(&dev));
@@ -975,7 +982,7 @@ void launch_apply_rk(ApplyArgs<T> a, int64_t lateral_groups, hipStream_t s)
     a.seg_len = seg;
     a.segs = (a.nx + seg - 1) / seg;
     const int blocks = (int)(lateral_groups * a.segs);
-    hipLaunchKernelGGL((epsdivdiv_apply_kernel<T, R, K, NW, RG>), dim3(blocks), dim3(NW * 64), 0, s, a);
+    hipLaunchKernelGGL((epsdivdiv_apply_kernel<T, R, K, NW, RG, BCM>), dim3(blocks), dim3(NW * 64), 0, s, a);
 }
 
 template <typename T> void launch_apply(ApplyArgs<T> a, hipStream_t s)
@@ -1004,18 +1011,30 @@ template <typename T> void launch_apply(ApplyArgs<T> a, hipStream_t s)
     // strips per subdomain
     a.strips = R == 64 ? (a.nsd * a.nx + (VX - 1) - 1) / (VX - 1) : (a.nx + (VX - 1) - 1) / (VX - 1);
     const int64_t lg = R == 64 ? (int64_t)a.strips : (int64_t)a.nsd * a.strips;
+    // the one-column kernels are compiled per boundary-condition mix
+    const int bcm = ((a.bc_cmb == TNG_FREESLIP || a.bc_surf == TNG_FREESLIP) ? 1 : 0) |
+                    ((a.bc_cmb == TNG_DIRICHLET || a.bc_surf == TNG_DIRICHLET) ? 2 : 0);
     if (RG == 2) {
-        launch_apply_rk<T, 64, 1, 8, 2>(a, lg, s);
+        switch (bcm) {
+        case 0: launch_apply_rk<T, 64, 1, 8, 2, 0>(a, lg, s); break;
+        case 1: launch_apply_rk<T, 64, 1, 8, 2, 1>(a, lg, s); break;
+        case 2: launch_apply_rk<T, 64, 1, 8, 2, 2>(a, lg, s); break;
+        default: launch_apply_rk<T, 64, 1, 8, 2, 3>(a, lg, s); break;
+        }
+    } else if (K == 1 && R == 64 && NW == 8) {
+        switch (bcm) {
+        case 0: launch_apply_rk<T, 64, 1, 8, 1, 0>(a, lg, s); break;
+        case 1: launch_apply_rk<T, 64, 1, 8, 1, 1>(a, lg, s); break;
+        case 2: launch_apply_rk<T, 64, 1, 8, 1, 2>(a, lg, s); break;
+        default: launch_apply_rk<T, 64, 1, 8, 1, 3>(a, lg, s); break;
+        }
     } else if (K == 1) {
         switch (R) {
         case 4: launch_apply_rk<T, 4, 1, 8, 1>(a, lg, s); break;
         case 8: launch_apply_rk<T, 8, 1, 8, 1>(a, lg, s); break;
         case 16: launch_apply_rk<T, 16, 1, 8, 1>(a, lg, s); break;
         case 32: launch_apply_rk<T, 32, 1, 8, 1>(a, lg, s); break;
-        default:
-            if (NW == 4) launch_apply_rk<T, 64, 1, 4, 1>(a, lg, s);
-            else launch_apply_rk<T, 64, 1, 8, 1>(a, lg, s);
-            break;
+        default: launch_apply_rk<T, 64, 1, 4, 1>(a, lg, s); break;
         }
     } else if (K == 2) {
         launch_apply_rk<T, 64, 2, 4, 1>(a, lg, s);

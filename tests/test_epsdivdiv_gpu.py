@@ -87,12 +87,13 @@ def test_parity_shapes(gpu_ctx, L, Lr, ls, rs, bcs):
 
 
 @pytest.mark.parametrize("L,Lr,env", [(2, 7, {"TNG_APPLY_RADIAL_CHUNKS": "1"}), (3, 6, {"TNG_APPLY_WAVES": "4"}),
-                                       (1, 8, {}), (3, 7, {})])
-@pytest.mark.parametrize("bcs", [(D, D), (F, F), (N, N)])
+                                       (1, 8, {}), (3, 7, {}), (3, 6, {})])
+@pytest.mark.parametrize("bcs", [(D, D), (F, F), (N, N), (D, F), (N, F), (F, D)])
 def test_parity_launch_variants(gpu_ctx, L, Lr, env, bcs, monkeypatch):
     """Launch shapes chosen by layer count or tuning variable: 128 layers as
     two radial chunks (default) or two cells per lane, 4-wave workgroups at
-    64 layers, four cells per lane at 256 layers."""
+    64 layers, four cells per lane at 256 layers; every boundary-condition
+    mix of the one-column kernels."""
     for k, v in env.items():
         monkeypatch.setenv(k, v)
     dom, om, w = setup(L, Lr)
