@@ -576,7 +576,7 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
             const int64_t base = lcol * nr;
             // unit normal of the output node (free-slip rows, penalty)
             T cc[3] = {T(0), T(0), T(0)};
-            if (a.pen || any_fs) {
+            if (a.pen || (BCM == BCM_RUNTIME ? any_fs : (BCM & 1) != 0)) {
                 if constexpr (R == 64) {
 #pragma unroll
                     for (int c = 0; c < 3; ++c) cc[c] = lane_value(rec, REC_NOUT + c);
@@ -585,6 +585,12 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
                     cc[1] = a.coords[3 * lcol + 1];
                     cc[2] = a.coords[3 * lcol + 2];
                 }
+            }
+            T pbn[3] = {T(0), T(0), T(0)};
+            if (a.pen) {
+                pbn[0] = pb[1] * cc[2] - pb[2] * cc[1];
+                pbn[1] = pb[2] * cc[0] - pb[0] * cc[2];
+                pbn[2] = pb[0] * cc[1] - pb[1] * cc[0];
             }
 #pragma unroll
             for (int p = 0; p < KL; ++p) {
@@ -615,11 +621,12 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
                         u[1] = a.src1[base + l];
                         u[2] = a.src2[base + l];
                     }
-                    if (bcv == TNG_DIRICHLET) {
+                    constexpr bool rt = BCM == BCM_RUNTIME, has_d = rt || (BCM & 2), has_f = rt || (BCM & 1);
+                    if (has_d && (!has_f || bcv == TNG_DIRICHLET)) {
                         val[0] = bt[0] * u[0];
                         val[1] = bt[1] * u[1];
                         val[2] = bt[2] * u[2];
-                    } else {
+                    } else if (has_f) {
                         const T dv = cc[0] * val[0] + cc[1] * val[1] + cc[2] * val[2];
                         const T un = bt[0] * (cc[0] * u[0] + cc[1] * u[1] + cc[2] * u[2]);
 #pragma unroll
@@ -627,11 +634,11 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
                     }
                 }
                 if (a.pen) {
+                    // b x (r n) = r (b x n), b x n once per output column
                     const T rad = radl[p];
-                    const T x0 = cc[0] * rad, x1 = cc[1] * rad, x2 = cc[2] * rad;
-                    val[0] += pb[1] * x2 - pb[2] * x1;
-                    val[1] += pb[2] * x0 - pb[0] * x2;
-                    val[2] += pb[0] * x1 - pb[1] * x0;
+                    val[0] += rad * pbn[0];
+                    val[1] += rad * pbn[1];
+                    val[2] += rad * pbn[2];
                 }
                 if (a.add) {
                     val[0] += a.dst0[base + l];
