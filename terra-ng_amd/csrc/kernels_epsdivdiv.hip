@@ -531,8 +531,10 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
             const T b00 = wave_shr1(s00[K][c]), b01 = wave_shr1(s01[K][c]);
 #pragma unroll
             for (int p = 0; p < K; ++p) {
-                xch[buf][vi][0][p][c][rl] = s00[p][c] + (p == 0 && rl > 0 ? b00 : T(0));
-                xch[buf][vi][1][p][c][rl] = s01[p][c] + (p == 0 && rl > 0 ? b01 : T(0));
+                // one column per wave: the bound-controlled shift already
+                // gives lane 0 a zero
+                xch[buf][vi][0][p][c][rl] = s00[p][c] + (p == 0 && (R == 64 || rl > 0) ? b00 : T(0));
+                xch[buf][vi][1][p][c][rl] = s01[p][c] + (p == 0 && (R == 64 || rl > 0) ? b01 : T(0));
             }
             if (ovf_lane) {
                 xovf[buf][vi][0][c] = s00[K][c];
@@ -563,7 +565,7 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const T up = wave_shr1(acc_cur[K][c]);
-            below[c] = rl > 0 ? up : T(0);
+            below[c] = (R == 64 || rl > 0) ? up : T(0);
             if constexpr (RG > 1) {
                 // lane 0 of an upper chunk: the lower chunk's top level
                 if (rl == 0 && !bot_chunk && v + 1 < VX)
@@ -995,6 +997,7 @@ void launch_apply_rk(ApplyArgs<T> a, int64_t lateral_groups, hipStream_t s)
 template <typename T> void launch_apply(ApplyArgs<T> a, hipStream_t s)
 {
     TNG_REQUIRE(a.ncells >= 1 && (a.ncells & (a.ncells - 1)) == 0, "radial layers per subdomain must be a power of 2");
+    TNG_REQUIRE((int64_t)a.nsd * a.nx * a.nx * a.nr < ((int64_t)1 << 31), "node copies exceed 32-bit indexing");
     // shapes: up to 64 radial cells one cell per lane (R lanes per column);
     // 128 cells in two radial chunks of one wave each; more cells K per lane
     int R, K, RG = 1;
