@@ -858,6 +858,7 @@ template <typename T> struct PenArgs {
     double ginv[9];
     int nsd, nx, nr;
     int64_t n;
+    FastDiv div_nr, div_nxx;  // node copy -> column, column -> subdomain
 };
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -873,14 +874,15 @@ template <typename T> __global__ __launch_bounds__(256) void penalty_partial_ker
 {
     __shared__ double red[4][3];
     double m0 = 0, m1 = 0, m2 = 0;
-    const int n = (int)a.n, nr = a.nr, nxx = a.nx * a.nx;
+    const int n = (int)a.n, nr = a.nr;
     const int stride = gridDim.x * blockDim.x;
     for (int i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += 4 * stride) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int i = i0 + q * stride;
             if (i < n) {
-                const int col = i / nr, l = i - col * nr, sd = col / nxx;
+                const int col = (int)a.div_nr.div((uint32_t)i), l = i - col * nr;
+                const int sd = (int)a.div_nxx.div((uint32_t)col);
                 const double w = a.omask[i] == OWNED ? 1.0 : 0.0;
                 const double rad = (double)a.radii[sd * nr + l] * w;
                 const double x0 = (double)a.coords[3 * col] * rad, x1 = (double)a.coords[3 * col + 1] * rad,
@@ -924,7 +926,7 @@ template <typename T> __global__ __launch_bounds__(256) void penalty_final_kerne
     }
 }
 
-constexpr int kPenBlocks = 2048;
+constexpr int kPenBlocks = 1024;
 
 template <typename T> struct MeshPtrs {
     const T *coords, *radii, *latgeom;
@@ -1095,6 +1097,8 @@ template <typename T> void apply_impl(tng_operator &op, const tng_vector &src, t
         p.eps = op.penalty_eps;
         for (int i = 0; i < 9; ++i) p.ginv[i] = m.rot_gram_inv[i];
         p.nsd = m.nsd; p.nx = m.nx; p.nr = m.nr; p.n = m.nalloc;
+        p.div_nr = FastDiv((uint32_t)m.nr);
+        p.div_nxx = FastDiv((uint32_t)(m.nx * m.nx));
         TNG_REQUIRE(m.nalloc < (int64_t)1 << 31, "penalty moment: node copies exceed 32-bit indexing");
         hipLaunchKernelGGL(penalty_partial_kernel<T>, dim3(kPenBlocks), dim3(256), 0, s, p);
         hipLaunchKernelGGL(penalty_final_kernel<T>, dim3(1), dim3(256), 0, s, p, kPenBlocks);

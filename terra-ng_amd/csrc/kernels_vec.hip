@@ -110,9 +110,9 @@ void launch_halo_sum(const tng_domain &m, tng_vector &v, hipStream_t s)
 
 // ----------------------------------------------------------------------------
 // BLAS-1 over all components (grid_operations.hpp:11-273) and reductions over
-// OWNED node copies (grid_operations.hpp:275-733).  Reductions are two-stage
-// and deterministic: per-block partials in fixed order, the block that draws
-// the last ticket sums them and resets the ticket.
+// OWNED node copies (grid_operations.hpp:275-733).  Reductions are two
+// launches and deterministic: per-block partials, then one block sums them
+// in a fixed order.
 // ----------------------------------------------------------------------------
 namespace tng {
 
