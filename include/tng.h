@@ -171,6 +171,22 @@ tng_status tng_op_copy_local_matrices(tng_op dst, tng_op src);
  * A_fine is the fine operator's stored matrices, or its Neumann local
  * matrices assembled from its (FP64) k when it stores none. */
 tng_status tng_gca_assemble(tng_op fine, tng_op coarse);
+/* Adaptive GCA (gca = 2).  linalg/solvers/gca/gca_elements_collector.hpp:20-96
+ * GCAElementsCollector(fine_domain, k, level_range, GCAElements): per fine
+ * wedge, |grad k| in reference coordinates at the centroid; above 10 the
+ * coarsest-level ancestor cell (cell index >> level_range) is flagged.
+ * host_flags receives one byte per coarsest cell, layout (sd, x_cell, y_cell,
+ * r_cell) of `coarsest`, 1 = flagged; k is an FP64 scalar on a mesh
+ * level_range refinements above `coarsest`. */
+tng_status tng_gca_elements_collect(tng_vec k, int level_range, tng_mesh coarsest, unsigned char *host_flags);
+/* TwoGridGCA with element selection (gca.hpp:150-175) into an operator in
+ * Selective stored-matrix mode (epsilon_divdiv_kerngen.hpp:503-575,
+ * 1040-1075): wedges of coarse cells whose coarsest ancestor
+ * (cell >> level_range) is flagged in host_flags receive the Galerkin matrix,
+ * all others the Neumann matrix assembled from the coarse operator's own k.
+ * The coarse mesh is level_range refinements above `coarsest`. */
+tng_status tng_gca_assemble_selective(tng_op fine, tng_op coarse, int level_range, tng_mesh coarsest,
+                                      const unsigned char *host_flags);
 
 /* fe/wedge/operators/shell/divergence_kerngen.hpp:121-257 DivergenceKerngen
  * (fine velocity mesh -> coarse pressure mesh, one refinement apart): B u,

@@ -58,6 +58,7 @@ def lib():
         L.orc_kmass_lumped_apply.argtypes = [vp, dp, dp, dp]
         L.orc_epsdivdiv_apply_stored.argtypes = [vp, dp, ip, ip, ip, ip, ip, C.c_double, dp, dp]
         L.orc_epsdivdiv_local_matrices.argtypes = [vp, dp, dp]
+        L.orc_gca_elements.argtypes = [vp, dp, ip, ip, ip, C.POINTER(C.c_uint8)]
         L.orc_gca.argtypes = [vp, vp, dp, dp, ip]
         L.orc_epsdivdiv_log_k_gradient.argtypes = [vp, dp, ip, ip, dp, dp, dp]
         L.orc_vector_mass_apply.argtypes = [vp, dp, dp]
@@ -226,6 +227,28 @@ def gca(fine, coarse, lmat_fine, exact_top=False):
     out = np.zeros((coarse.nsd, coarse.nx - 1, coarse.nx - 1, coarse.nr - 1, 2, 18, 18))
     lib().orc_gca(fine.h, coarse.h, _dp(lf), _dp(out), int(exact_top))
     return out
+
+
+def gca_elements(mesh, k, level_range, coarsest):
+    """GCAElementsCollector (gca_elements_collector.hpp:20-96): uint8 flags per cell of `coarsest`."""
+    k = _c(k)
+    out = np.zeros((coarsest.nsd, coarsest.nx - 1, coarsest.nx - 1, coarsest.nr - 1), dtype=np.uint8)
+    lib().orc_gca_elements(mesh.h, _dp(k), int(level_range), coarsest.nx - 1, coarsest.nr - 1,
+                           out.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return out
+
+
+def gca_selective(fine, coarse, lmat_fine, k_coarse, level_range, flags):
+    """TwoGridGCA with element selection (gca.hpp:150-175) into a Selective stored-matrix operator
+    (epsilon_divdiv_kerngen.hpp:1040-1075): Galerkin matrices on wedges whose coarsest ancestor
+    (cell >> level_range) is flagged, the on-the-fly Neumann matrices of k_coarse elsewhere."""
+    g = gca(fine, coarse, lmat_fine)
+    loc = local_matrices(coarse, k_coarse)
+    nc, ncr = coarse.nx - 1, coarse.nr - 1
+    xs = np.arange(nc) >> level_range
+    rs = np.arange(ncr) >> level_range
+    sel = np.asarray(flags, dtype=bool)[:, xs][:, :, xs][:, :, :, rs]
+    return np.where(sel[..., None, None, None], g, loc)
 
 
 def epsdivdiv_stored(mesh, lmat, src, bc_cmb, bc_surf, diagonal=False, apply_mode=REPLACE, comm=COMMUNICATE,

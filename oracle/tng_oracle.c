@@ -840,6 +840,34 @@ void orc_epsdivdiv_local_matrices(const orc_mesh *m, const double *k, double *ou
                     }
 }
 
+/* GCAElementsCollector (linalg/solvers/gca/gca_elements_collector.hpp:20-96):
+ * per fine wedge, grad_ref k = sum_j grad_shape(j, qp) k_j at the 1-point
+ * quadrature point (quadrature.hpp:36-39); a norm above 10 flags the
+ * coarsest ancestor, map_to_coarse_element = level_range halvings (:20-28).
+ * flags: (sd, ncx0, ncx0, ncr0) bytes, zeroed by the caller. */
+void orc_gca_elements(const orc_mesh *m, const double *k, int level_range, int ncx0, int ncr0, uint8_t *flags)
+{
+    const int nc = m->nx - 1, ncr = m->nr - 1;
+    const double xi = 1.0 / 3.0, eta = 1.0 / 3.0, zeta = 0.0;
+    for (int sd = 0; sd < m->nsd; ++sd)
+        for (int xc = 0; xc < nc; ++xc)
+            for (int yc = 0; yc < nc; ++yc)
+                for (int rc = 0; rc < ncr; ++rc)
+                    for (int w = 0; w < 2; ++w) {
+                        double g[3] = {0.0, 0.0, 0.0};
+                        for (int n = 0; n < 6; ++n) {
+                            const double kn = k[lin(m, sd, xc + WOFF[w][n][0], yc + WOFF[w][n][1], rc + WOFF[w][n][2])];
+                            const vec3 gs = grad_shape_w(n, xi, eta, zeta);
+                            for (int d = 0; d < 3; ++d) g[d] += gs.v[d] * kn;
+                        }
+                        if (sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]) > 10.0) {
+                            int x0 = xc, y0 = yc, r0 = rc;
+                            for (int l = 0; l < level_range; ++l) { x0 /= 2; y0 /= 2; r0 /= 2; }
+                            flags[(((size_t)sd * ncx0 + x0) * ncx0 + y0) * ncr0 + r0] = 1;
+                        }
+                    }
+}
+
 /* ------------------------------------------------------------------------- */
 /* TwoGridGCA (linalg/solvers/gca/gca.hpp:41-531), constant interpolation,   */
 /* all elements (gca = 1), no boundary treatment (the block is disabled,     */

@@ -656,6 +656,34 @@ tng_status tng_gca_assemble(tng_op fine, tng_op coarse)
     });
 }
 
+tng_status tng_gca_elements_collect(tng_vec k, int level_range, tng_mesh coarsest, unsigned char *host_flags)
+{
+    if (!k || !coarsest || !host_flags) return TNG_INVALID_ARGUMENT;
+    return guarded(coarsest->ctx, [&] {
+        set_device(coarsest->ctx);
+        hipStream_t s = coarsest->ctx->stream;
+        DevBuf flags;
+        gca_collect(*k, level_range, *coarsest, flags, s);
+        TNG_HIP_CHECK(hipMemcpyAsync(host_flags, flags.p, flags.bytes, hipMemcpyDeviceToHost, s));
+        TNG_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+tng_status tng_gca_assemble_selective(tng_op fine, tng_op coarse, int level_range, tng_mesh coarsest,
+                                      const unsigned char *host_flags)
+{
+    if (!fine || !coarse || !coarsest || !host_flags || fine->kind != OP_EPSDIVDIV || coarse->kind != OP_EPSDIVDIV)
+        return TNG_INVALID_ARGUMENT;
+    return guarded(coarse->mesh->ctx, [&] {
+        set_device(coarse->mesh->ctx);
+        TNG_REQUIRE(coarse->k->dtype == TNG_F64, "GCA assembles FP64 matrices: the coarse operator must be FP64");
+        hipStream_t s = coarse->mesh->ctx->stream;
+        DevBuf flags((size_t)coarsest->nsd * (coarsest->nx - 1) * (coarsest->nx - 1) * (coarsest->nr - 1));
+        TNG_HIP_CHECK(hipMemcpyAsync(flags.p, host_flags, flags.bytes, hipMemcpyHostToDevice, s));
+        gca_assemble_selective(*fine, *coarse, level_range, *coarsest, flags, s);
+    });
+}
+
 tng_status tng_op_copy_local_matrices(tng_op dst, tng_op src)
 {
     if (!dst || !src || dst->kind != OP_EPSDIVDIV || src->kind != OP_EPSDIVDIV) return TNG_INVALID_ARGUMENT;

@@ -401,6 +401,62 @@ template <typename T> __global__ __launch_bounds__(256) void stored_apply_kernel
     }
 }
 
+// Adaptive GCA element selection (linalg/solvers/gca/gca_elements_collector.hpp
+// :20-96): per fine wedge, the gradient of k in reference coordinates at the
+// centroid (1/3, 1/3, 0); a norm above 10 flags the coarsest-level ancestor
+// cell (cell index >> level_range in x, y and r).  With s_a = k_bottom + k_top
+// and d_a = k_top - k_bottom at the three lateral corners the gradient is
+// (s_1 - s_0, s_2 - s_0, (d_0 + d_1 + d_2) / 3) / 2.
+__global__ __launch_bounds__(256) void gca_collect_kernel(const double *k, int nsd, int nx, int nr, int level_range,
+                                                          uint8_t *flags, int ncx0, int ncr0)
+{
+    const int nc = nx - 1, ncr = nr - 1;
+    const int64_t n = (int64_t)nsd * nc * nc * ncr;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n) return;
+    const int rc = (int)(gid % ncr);
+    int64_t t = gid / ncr;
+    const int yc = (int)(t % nc); t /= nc;
+    const int xc = (int)(t % nc);
+    const int sd = (int)(t / nc);
+    bool hit = false;
+    for (int w = 0; w < 2; ++w) {
+        double sa[3], da = 0.0;
+        for (int c = 0; c < 3; ++c) {
+            const int64_t i = (((int64_t)sd * nx + xc + kWN[w][c][0]) * nx + yc + kWN[w][c][1]) * nr + rc;
+            const double kb = k[i], kt = k[i + 1];
+            sa[c] = kb + kt;
+            da += kt - kb;
+        }
+        const double g0 = 0.5 * (sa[1] - sa[0]), g1 = 0.5 * (sa[2] - sa[0]), g2 = da * (1.0 / 6.0);
+        if (sqrt(g0 * g0 + g1 * g1 + g2 * g2) > 10.0) hit = true;
+    }
+    if (hit)
+        flags[(((int64_t)sd * ncx0 + (xc >> level_range)) * ncx0 + (yc >> level_range)) * ncr0 + (rc >> level_range)] = 1;
+}
+
+// Selective storage (local_matrix_storage.hpp:28-243, OperatorStoredMatrixMode::
+// Selective): wedges of cells whose coarsest ancestor is not flagged take the
+// on-the-fly local matrix (epsilon_divdiv_kerngen.hpp:1047-1060) instead of the
+// Galerkin one.
+__global__ __launch_bounds__(256) void gca_select_kernel(double *lmat, const double *local, const uint8_t *flags,
+                                                         int nsd, int nc, int ncr, int level_range, int ncx0, int ncr0)
+{
+    const int64_t ncell = (int64_t)nsd * nc * nc * ncr;
+    const int64_t n = ncell * 2 * 324;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t cell = e / (2 * 324);
+        const int rc = (int)(cell % ncr);
+        int64_t t = cell / ncr;
+        const int yc = (int)(t % nc); t /= nc;
+        const int xc = (int)(t % nc);
+        const int sd = (int)(t / nc);
+        const uint8_t f =
+            flags[(((int64_t)sd * ncx0 + (xc >> level_range)) * ncx0 + (yc >> level_range)) * ncr0 + (rc >> level_range)];
+        if (!f) lmat[e] = local[e];
+    }
+}
+
 }  // namespace
 
 void epsdivdiv_local_matrices(const tng_operator &op, DevBuf &out, hipStream_t s)
@@ -440,6 +496,44 @@ void gca_assemble(const tng_operator &fine, tng_operator &coarse, hipStream_t s)
     TNG_HIP_CHECK(hipGetLastError());
     coarse.stored = true;
     coarse.lmat_dtype = TNG_F64;
+}
+
+void gca_collect(const tng_vector &k_fine, int level_range, const tng_domain &coarsest, DevBuf &flags, hipStream_t s)
+{
+    const tng_domain &f = *k_fine.mesh;
+    TNG_REQUIRE(k_fine.dtype == TNG_F64 && k_fine.ncomp == 1, "GCA collector: FP64 scalar coefficient");
+    TNG_REQUIRE(level_range >= 0 && level_range < 16 && f.nsd == coarsest.nsd &&
+                    (f.nx - 1) == ((coarsest.nx - 1) << level_range) && (f.nr - 1) == ((coarsest.nr - 1) << level_range),
+                "GCA collector: the coefficient's mesh must be level_range refinements above the coarsest mesh");
+    const int ncx0 = coarsest.nx - 1, ncr0 = coarsest.nr - 1;
+    const size_t bytes = (size_t)coarsest.nsd * ncx0 * ncx0 * ncr0;
+    if (flags.bytes != bytes) flags.alloc(bytes);
+    TNG_HIP_CHECK(hipMemsetAsync(flags.p, 0, bytes, s));
+    const int64_t n = (int64_t)f.nsd * (f.nx - 1) * (f.nx - 1) * (f.nr - 1);
+    hipLaunchKernelGGL(gca_collect_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       (const double *)k_fine.comp(0), f.nsd, f.nx, f.nr, level_range, flags.as<uint8_t>(), ncx0, ncr0);
+    TNG_HIP_CHECK(hipGetLastError());
+}
+
+void gca_assemble_selective(const tng_operator &fine, tng_operator &coarse, int level_range,
+                            const tng_domain &coarsest, const DevBuf &flags, hipStream_t s)
+{
+    const tng_domain &c = *coarse.mesh;
+    const int ncx0 = coarsest.nx - 1, ncr0 = coarsest.nr - 1;
+    TNG_REQUIRE(flags.bytes == (size_t)coarsest.nsd * ncx0 * ncx0 * ncr0, "GCA: flags do not match the coarsest mesh");
+    TNG_REQUIRE(level_range >= 0 && c.nsd == coarsest.nsd && (c.nx - 1) == (ncx0 << level_range) &&
+                    (c.nr - 1) == (ncr0 << level_range),
+                "GCA: the coarse operator's mesh must be level_range refinements above the coarsest mesh");
+    gca_assemble(fine, coarse, s);
+    DevBuf local;
+    epsdivdiv_local_matrices(coarse, local, s);
+    const int64_t n = (int64_t)stored_matrix_count(c) * 324;
+    const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(gca_select_kernel, dim3(g), dim3(256), 0, s, coarse.d_lmat.as<double>(),
+                       (const double *)local.p, flags.as<uint8_t>(), c.nsd, c.nx - 1, c.nr - 1, level_range, ncx0,
+                       ncr0);
+    TNG_HIP_CHECK(hipGetLastError());
+    TNG_HIP_CHECK(hipStreamSynchronize(s));  // `local` is freed on return
 }
 
 void copy_stored_matrices(tng_operator &dst, const tng_operator &src, tng_dtype dtype, hipStream_t s)

@@ -207,7 +207,8 @@ StokesSolver::StokesSolver(tng_context *ctx, int min_level, int max_level, int r
     TNG_REQUIRE(prm.krylov_restart >= 1 && prm.krylov_max_iterations >= 0, "invalid Krylov parameters");
     TNG_REQUIRE(prm.viscous_pc_chebyshev_order >= 1, "Chebyshev order must be at least 1");
     TNG_REQUIRE(prm.viscous_pc_dtype == TNG_F64 || prm.viscous_pc_dtype == TNG_F32, "invalid multigrid precision");
-    TNG_REQUIRE(prm.gca == 0 || prm.gca == 1, "gca must be 0 (re-discretised) or 1 (Galerkin on all coarse levels)");
+    TNG_REQUIRE(prm.gca >= 0 && prm.gca <= 2,
+                "gca must be 0 (re-discretised), 1 (Galerkin on all coarse levels) or 2 (adaptive Galerkin)");
     for (int l = min_level; l <= max_level; ++l) {
         meshes.emplace_back(new_mesh(ctx, l, l + radial_extra, r_min, r_max, lat_sdr, rad_sdr));
         meshes.back()->solver_owned = true;
@@ -228,7 +229,7 @@ StokesSolver::StokesSolver(tng_context *ctx, int min_level, int max_level, int r
     B.reset(new_two_level(OP_DIVERGENCE, vel, pre_mesh, bc_cmb, bc_surf));
 
     // Galerkin coarse operators (FP64 stored matrices), assembled top-down in viscosity_updated()
-    if (prm.gca == 1) {
+    if (prm.gca >= 1) {
         TNG_REQUIRE(meshes[0]->nr >= 3, "GCA needs at least two radial layers on the coarsest level");
         for (int l = 0; l < nl - 1; ++l)
             gca64.emplace_back(new_epsdivdiv(meshes[l].get(), eta[l], bc_cmb, bc_surf, false));
@@ -242,7 +243,7 @@ StokesSolver::StokesSolver(tng_context *ctx, int min_level, int max_level, int r
         tng_domain *m = meshes[l].get();
         if (l == nl - 1 && mgt == TNG_F64) {
             L.A = A.get();
-        } else if (l < nl - 1 && prm.gca == 1 && mgt == TNG_F64) {
+        } else if (l < nl - 1 && prm.gca >= 1 && mgt == TNG_F64) {
             L.A = gca64[l].get();
         } else {
             mg_ops.emplace_back(new_epsdivdiv(m, eta_mg[l], bc_cmb, bc_surf, false));
@@ -280,7 +281,7 @@ StokesSolver::StokesSolver(tng_context *ctx, int min_level, int max_level, int r
     mg.coarse_atol = prm.coarse_absolute_tolerance;
     if (mgt == TNG_F64) {
         mg.coarse_op = mg.levels[0].A;
-    } else if (prm.gca == 1) {
+    } else if (prm.gca >= 1) {
         mg.coarse_op = gca64[0].get();
         mg.cg_x = vec(pool, meshes[0].get(), TNG_F64, 3);
         mg.cg_b = vec(pool, meshes[0].get(), TNG_F64, 3);
@@ -331,11 +332,21 @@ void StokesSolver::viscosity_updated()
     if (coarse_op64) coarse_op64->btab_valid = false;
     for (auto &op : mg_ops)
         if (op->kind == OP_EPSDIVDIV) op->btab_valid = false;
-    if (params.gca == 1) {
+    if (params.gca >= 1) {
         // TwoGridGCA top-down (stokes_solver.hpp:375-388): the finest step
-        // uses the Neumann local matrices of the velocity operator
+        // uses the Neumann local matrices of the velocity operator.  gca = 2
+        // (stokes_solver.hpp:278-287, 361-365): coarsest cells flagged from
+        // the gradient of the velocity-level viscosity keep the Galerkin
+        // matrices of all their descendants, the rest the re-discretised ones.
         hipStream_t s = vel->ctx->stream;
-        for (int l = nl - 2; l >= 0; --l) gca_assemble(l == nl - 2 ? *A : *gca64[l + 1], *gca64[l], s);
+        if (params.gca == 2) gca_collect(*eta[nl - 1], nl - 1, *meshes[0], gca_flags, s);
+        for (int l = nl - 2; l >= 0; --l) {
+            const tng_operator &fine = l == nl - 2 ? *A : *gca64[l + 1];
+            if (params.gca == 2)
+                gca_assemble_selective(fine, *gca64[l], l, *meshes[0], gca_flags, s);
+            else
+                gca_assemble(fine, *gca64[l], s);
+        }
         if (params.viscous_pc_dtype != TNG_F64)
             for (int l = 0; l < nl - 1; ++l)
                 copy_stored_matrices(*mg.levels[l].A, *gca64[l], (tng_dtype)params.viscous_pc_dtype, s);

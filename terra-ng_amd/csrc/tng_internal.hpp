@@ -267,6 +267,11 @@ tng_operator *new_kmass(tng_domain *mesh, tng_vector *k);
 size_t stored_matrix_count(const tng_domain &m);  // wedges = matrices
 void epsdivdiv_local_matrices(const tng_operator &op, DevBuf &out, hipStream_t s);  // FP64, from k
 void gca_assemble(const tng_operator &fine, tng_operator &coarse, hipStream_t s);
+// adaptive GCA: flags of the coarsest cells from the fine coefficient, and the
+// selective assembly (Galerkin on flagged cells, on-the-fly matrices elsewhere)
+void gca_collect(const tng_vector &k_fine, int level_range, const tng_domain &coarsest, DevBuf &flags, hipStream_t s);
+void gca_assemble_selective(const tng_operator &fine, tng_operator &coarse, int level_range,
+                            const tng_domain &coarsest, const DevBuf &flags, hipStream_t s);
 void copy_stored_matrices(tng_operator &dst, const tng_operator &src, tng_dtype dtype, hipStream_t s);
 template <typename T>
 void stored_apply(tng_operator &op, const tng_vector &src, tng_vector &dst, const double *pen, bool add, hipStream_t s);

@@ -61,7 +61,8 @@ struct StokesSolver {
     std::vector<tng_vector *> eta, eta_mg;           // per level: FP64 viscosity, multigrid-precision copy
     std::unique_ptr<tng_operator> A, Bt, B, coarse_op64, pmass;
     std::vector<std::unique_ptr<tng_operator>> mg_ops;
-    std::vector<std::unique_ptr<tng_operator>> gca64;  // FP64 Galerkin operators of the coarse levels (gca = 1)
+    std::vector<std::unique_ptr<tng_operator>> gca64;  // FP64 Galerkin operators of the coarse levels (gca >= 1)
+    DevBuf gca_flags;  // gca = 2: one byte per coarsest-level cell, 1 = Galerkin hierarchy
     Multigrid mg;
     tng_vector *k_pm = nullptr, *inv_lumped_pmass = nullptr, *ones_p = nullptr, *ones_v = nullptr;
     tng_vector *prec_tmp_u = nullptr, *mg_x = nullptr, *mg_b = nullptr;

@@ -99,6 +99,8 @@ EXPORTS = [
     ("tng_op_clear_local_matrices", _i, [_vp]),
     ("tng_op_copy_local_matrices", _i, [_vp, _vp]),
     ("tng_gca_assemble", _i, [_vp, _vp]),
+    ("tng_gca_elements_collect", _i, [_vp, _i, _vp, _vp]),
+    ("tng_gca_assemble_selective", _i, [_vp, _vp, _i, _vp, _vp]),
     ("tng_epsdivdiv_log_coefficient_gradient", _i, [_vp, _vp, _vp, _vp]),
     ("tng_stokes_adjoint_gradient", _i, [_vp, _vp, _vp, _vp, _vp]),
     ("tng_pcg_solve", _i, [_vp, _vp, _vp, _i, _d, _d, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
@@ -398,6 +400,30 @@ class EpsilonDivDivKerngen(_Operator):
 def gca_assemble(fine_op, coarse_op):
     """TwoGridGCA (linalg/solvers/gca/gca.hpp): coarse stored matrices from the fine operator."""
     coarse_op.domain.ctx.check(load_library().tng_gca_assemble(fine_op.h, coarse_op.h))
+
+
+def _cell_shape(domain):
+    return (domain.nsd, domain.nx - 1, domain.nx - 1, domain.nr - 1)
+
+
+def gca_elements_collect(k, level_range, domain_coarsest):
+    """GCAElementsCollector (linalg/solvers/gca/gca_elements_collector.hpp:20-96): uint8 flags per
+    coarsest-level cell (sd, x_cell, y_cell, r_cell), 1 where a descendant wedge on k's mesh has a
+    reference-coordinate |grad k| above 10 at its centroid."""
+    out = np.zeros(_cell_shape(domain_coarsest), dtype=np.uint8)
+    domain_coarsest.ctx.check(load_library().tng_gca_elements_collect(
+        k.h, int(level_range), domain_coarsest.h, out.ctypes.data_as(C.c_void_p)))
+    return out
+
+
+def gca_assemble_selective(fine_op, coarse_op, level_range, domain_coarsest, flags):
+    """TwoGridGCA with element selection (gca.hpp:150-175) into a Selective stored-matrix operator:
+    Galerkin matrices where the coarsest ancestor is flagged, re-discretised ones elsewhere."""
+    f = np.ascontiguousarray(flags, dtype=np.uint8)
+    if f.shape != _cell_shape(domain_coarsest):
+        raise ValueError(f"flags must have shape {_cell_shape(domain_coarsest)}, got {f.shape}")
+    coarse_op.domain.ctx.check(load_library().tng_gca_assemble_selective(
+        fine_op.h, coarse_op.h, int(level_range), domain_coarsest.h, f.ctypes.data_as(C.c_void_p)))
 
 
 class DivergenceKerngen(_Operator):

@@ -1,7 +1,8 @@
 """Stored local matrices, the stored-matrix slow path and TwoGridGCA on the GPU
 against the oracle (epsilon_divdiv_kerngen.hpp:503-575, 1013-1242, 2315-2392;
-linalg/solvers/gca/gca.hpp:41-531).  FP64 bar 1e-12 relative (matrices
-1e-13), FP32 operators 2e-6."""
+linalg/solvers/gca/gca.hpp:41-531), adaptive GCA (gca_elements_collector.hpp
+:20-96, Selective storage).  FP64 bar 1e-12 relative (matrices 1e-13), FP32
+operators 2e-6, element flags bit-exact."""
 import numpy as np
 import pytest
 
@@ -123,15 +124,68 @@ def test_gca_operator_apply_freeslip(gpu_ctx):
     assert rel_err(out.download(), ref) < 1e-12
 
 
+def contrast_k(om, seed, scale, jump_r=0.8, inner=40.0, outer=1.0):
+    """Lognormal viscosity with a radial jump (factor `inner` below jump_r,
+    `outer` above): flags some coarsest cells, not all."""
+    r = np.broadcast_to(om.radii[:, None, None, :], om.shape)
+    return np.where(r < jump_r, inner, outer) * np.exp(scale * field(om, 1, seed))
+
+
+@pytest.mark.parametrize("lat,level_range,scale", [(3, 1, 1.0), (4, 2, 1.0), (4, 1, 2.0), (2, 0, 3.0)])
+def test_gca_elements_collect_matches_oracle(lat, level_range, scale, gpu_ctx):
+    df, of = setup(lat)
+    d0, o0 = setup(lat - level_range)
+    k = contrast_k(of, 13 + lat, scale)
+    got = tng.gca_elements_collect(vec(df, 1, k), level_range, d0)
+    ref = O.gca_elements(of, k, level_range, o0)
+    assert 0 < ref.sum() < ref.size
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("lat,level_range,kargs", [(4, 1, (1.0, 0.8, 40.0, 1.0)), (4, 0, (1.0, 0.8, 40.0, 1.0)),
+                                                   (3, 1, (0.3, 0.95, 1.0, 25.0))])
+def test_gca_assemble_selective_matches_oracle(lat, level_range, kargs, gpu_ctx):
+    df, of = setup(lat)
+    dc, oc = setup(lat - 1)
+    d0, o0 = setup(lat - 1 - level_range)
+    k = contrast_k(of, 20 + lat, *kargs)
+    kc = np.ascontiguousarray(k[:, ::2, ::2, ::2])
+    flags = O.gca_elements(of, k, level_range + 1, o0)
+    assert 0 < flags.sum() < flags.size
+    fine, kf = op_on(df, k, (D, D))
+    coarse, kcv = op_on(dc, kc, (D, D))
+    tng.gca_assemble_selective(fine, coarse, level_range, d0, flags)
+    ref = O.gca_selective(of, oc, O.local_matrices(of, k), kc, level_range, flags)
+    assert rel_err(coarse.get_local_matrices(), ref) < 1e-13
+    # and through the stored-matrix apply
+    x = field(oc, 3, 24)
+    out = vec(dc, 3)
+    coarse.apply(vec(dc, 3, x), out)
+    assert rel_err(out.download(), O.epsdivdiv_stored(oc, ref, x, D, D)) < 1e-12
+
+
+def test_gca_selective_rejects_mismatched_meshes(gpu_ctx):
+    df, of = setup(3)
+    dc, oc = setup(2)
+    d0, o0 = setup(1)
+    fine, kf = op_on(df, np.ones(of.shape), (D, D))
+    coarse, kc = op_on(dc, np.ones(oc.shape), (D, D))
+    flags = np.zeros((o0.nsd, o0.nx - 1, o0.nx - 1, o0.nr - 1), dtype=np.uint8)
+    with pytest.raises(tng.TngError):
+        tng.gca_assemble_selective(fine, coarse, 2, d0, flags)  # coarse is one refinement above d0
+    with pytest.raises(ValueError):
+        tng.gca_assemble_selective(fine, coarse, 1, d0, flags[:, :1])
+
+
 @pytest.mark.parametrize("dt", [tng.F64, tng.F32])
 def test_stokes_solve_with_gca(dt, gpu_ctx):
-    """gca = 1 preconditioner reaches the same solution as gca = 0."""
+    """gca = 1 and adaptive gca = 2 preconditioners reach the same solution as gca = 0."""
     of, oc = O.Mesh(4), O.Mesh(3)
     r = np.broadcast_to(of.radii[:, None, None, :], of.shape)
     eta = np.where(r < 0.85, 30.0, 1.0) * np.exp(0.5 * field(of, 1, 11))
     f = field(of, 3, 12)
     sols = {}
-    for gca in (0, 1):
+    for gca in (0, 1, 2):
         S = tng.StokesSolver(2, 4, 0.55, 1.0, (D, D), tng.StokesParams.defaults(
             gca=gca, viscous_pc_dtype=dt, krylov_restart=30, krylov_max_iterations=400,
             krylov_relative_tolerance=1e-9, krylov_absolute_tolerance=1e-30))
@@ -144,3 +198,4 @@ def test_stokes_solve_with_gca(dt, gpu_ctx):
         sols[gca] = u.download()
     own = of.ownership_mask == 1
     assert rel_err(sols[1][:, own], sols[0][:, own]) < 1e-5
+    assert rel_err(sols[2][:, own], sols[0][:, own]) < 1e-5
