@@ -63,6 +63,13 @@ template <typename T> struct ApplyArgs {
     int bc_cmb, bc_surf;
     int add;
     int strips, segs, seg_len;
+    // fused Chebyshev sub-step (FUSE kernels): on node copies no other
+    // subdomain shares, d = az * (b - A src) * invd [+ ad * d] and
+    // xo = src + d replace the store of A src
+    const T *cb0, *cb1, *cb2, *ci0, *ci1, *ci2;
+    T *cd0, *cd1, *cd2, *cx0, *cx1, *cx2;
+    T caz, cad;
+    int cread_d;
 };
 
 template <typename T> __device__ __forceinline__ void normal_of(const T *c, T n[3])
@@ -121,7 +128,12 @@ enum : int {
     REC_BTAB = 29,      // 6: boundary rows of the output node (CMB 3, surface 3)
     REC_UCMB = 35,      // 3: src of the output node at the CMB level
     REC_USURF = 38,     // 3: src of the output node at the surface level
-    REC_LEN = 41
+    REC_LEN = 41,
+    // fused Chebyshev step: b, invd, d of the output node at the surface level
+    REC_SB = 41,
+    REC_SI = 44,
+    REC_SD = 47,
+    REC_LEN_FUSE = 50
 };
 
 // Raw node values of one node column at the K+1 levels spanned by a lane.
@@ -172,12 +184,15 @@ __device__ __forceinline__ void process(const RawCol<T, K> &r, const T n[3], con
     T lo[3] = {r.u[0][0], r.u[0][1], r.u[0][2]};
     T hi[3] = {r.u[K][0], r.u[K][1], r.u[K][2]};
     if (rt ? any_fs : (BCM & 1) != 0) {
-        const T dl = bc.f_lo * (n[0] * lo[0] + n[1] * lo[1] + n[2] * lo[2]);
-        const T dh = bc.f_hi * (n[0] * hi[0] + n[1] * hi[1] + n[2] * hi[2]);
+        // explicit fused products: the rounding does not depend on what the
+        // compiler contracts in a given instantiation
+#pragma clang fp contract(off)
+        const T dl = bc.f_lo * fma(n[2], lo[2], fma(n[1], lo[1], n[0] * lo[0]));
+        const T dh = bc.f_hi * fma(n[2], hi[2], fma(n[1], hi[1], n[0] * hi[0]));
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            lo[c] -= dl * n[c];
-            hi[c] -= dh * n[c];
+            lo[c] = fma(-dl, n[c], lo[c]);
+            hi[c] = fma(-dh, n[c], hi[c]);
         }
     }
     if (rt ? any_dir : (BCM & 2) != 0) {
@@ -247,7 +262,39 @@ __device__ __forceinline__ void wedge_stress(const T *g /*M0,M1,M2,detL*/, const
     }
 }
 
-template <typename T, int R, int K, int NW, int RG, int BCM>
+// The Chebyshev sub-step of one node copy, with the rounding of the separate
+// update kernel (no contraction: every product is rounded before it is added).
+// in: b, invd, d (if read), x at the node.
+template <typename T> __device__ __forceinline__ void cheb_load(const ApplyArgs<T> &a, int64_t i, T in[4][3])
+{
+    const T *cb[3] = {a.cb0, a.cb1, a.cb2}, *ci[3] = {a.ci0, a.ci1, a.ci2}, *xs[3] = {a.src0, a.src1, a.src2};
+    const T *cd[3] = {a.cd0, a.cd1, a.cd2};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        in[0][c] = cb[c][i];
+        in[1][c] = ci[c][i];
+        in[2][c] = a.cread_d ? cd[c][i] : T(0);
+        in[3][c] = xs[c][i];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void cheb_update(const ApplyArgs<T> &a, int64_t i, const T val[3], const T in[4][3])
+{
+#pragma clang fp contract(off)
+    T *cd[3] = {a.cd0, a.cd1, a.cd2}, *cx[3] = {a.cx0, a.cx1, a.cx2};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const T r = in[0][c] - val[c];
+        const T z = r * in[1][c];
+        T t = a.caz * z;
+        if (a.cread_d) t = fma(a.cad, in[2][c], t);
+        cd[c][i] = t;
+        cx[c][i] = in[3][c] + t;
+    }
+}
+
+template <typename T, int R, int K, int NW, int RG, int BCM, bool FUSE>
 __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArgs<T> a)
 {
     // RG > 1: the radial cells of a column are split into RG chunks of 64,
@@ -379,12 +426,21 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
             const T *sc = c == 0 ? a.src0 : (c == 1 ? a.src1 : a.src2);
             rec_base = sc + col1 * nr + (j < REC_USURF ? 0 : nr - 1);
             rec_stride = nr, rec_ymax = nx - 1;
+        } else if (FUSE && j < REC_LEN_FUSE) {
+            const int c = (j - REC_SB) % 3, w = (j - REC_SB) / 3;
+            const T *vc;  // selects, not a table: a table indexed per lane lives in scratch
+            if (w == 0) vc = c == 0 ? a.cb0 : (c == 1 ? a.cb1 : a.cb2);
+            else if (w == 1) vc = c == 0 ? a.ci0 : (c == 1 ? a.ci1 : a.ci2);
+            else vc = c == 0 ? a.cd0 : (c == 1 ? a.cd1 : a.cd2);
+            rec_base = vc + col1 * nr + nr - 1;
+            rec_stride = nr, rec_ymax = nx - 1;
         }
     }
+    constexpr int rec_len = FUSE ? REC_LEN_FUSE : REC_LEN;
     auto load_record = [&](int yc) -> T {
         const int y = min(max(yc + rec_dy, 0), rec_ymax);
         const T val = rec_base[(int64_t)y * rec_stride];
-        return lane < REC_LEN ? val : T(0);
+        return lane < rec_len ? val : T(0);
     };
 
     NodeCol<T, K> B0, B1, T0, T1;
@@ -441,6 +497,19 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
             const CT *gw = (const CT *)a.latgeom + gi;
 #pragma unroll
             for (int i = 0; i < 20; ++i) sg[i] = gw[i];
+        }
+        // fused Chebyshev step.  FP64: the inputs of this step's output node
+        // are requested ahead of the next row's prefetch, so that waiting for
+        // them in the store phase does not wait for the prefetch (FP32 loads
+        // them in the store phase: the registers would halve its occupancy).
+        // The surface level comes with the record.
+        constexpr bool early = FUSE && sizeof(T) == 8;
+        T cin[4][3];
+        bool fuse_here = false;
+        if constexpr (FUSE) {
+            fuse_here = X > 0 && X < nx - 1 && yc > 0 && yc < nx - 1;
+            if (early && fuse_here)
+                cheb_load<T>(a, (((int64_t)sd * nx + X) * nx + yc) * nr + min(hc * nch + rl * K, nr - 1), cin);
         }
         // the prefetched row yc+1 becomes the top row; prefetch row yc+2
         {
@@ -590,9 +659,10 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
             }
             T pbn[3] = {T(0), T(0), T(0)};
             if (a.pen) {
-                pbn[0] = pb[1] * cc[2] - pb[2] * cc[1];
-                pbn[1] = pb[2] * cc[0] - pb[0] * cc[2];
-                pbn[2] = pb[0] * cc[1] - pb[1] * cc[0];
+#pragma clang fp contract(off)
+                pbn[0] = fma(pb[1], cc[2], -(pb[2] * cc[1]));
+                pbn[1] = fma(pb[2], cc[0], -(pb[0] * cc[2]));
+                pbn[2] = fma(pb[0], cc[1], -(pb[1] * cc[0]));
             }
 #pragma unroll
             for (int p = 0; p < KL; ++p) {
@@ -629,23 +699,50 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
                         val[1] = bt[1] * u[1];
                         val[2] = bt[2] * u[2];
                     } else if (has_f) {
-                        const T dv = cc[0] * val[0] + cc[1] * val[1] + cc[2] * val[2];
-                        const T un = bt[0] * (cc[0] * u[0] + cc[1] * u[1] + cc[2] * u[2]);
+#pragma clang fp contract(off)
+                        const T dv = fma(cc[2], val[2], fma(cc[1], val[1], cc[0] * val[0]));
+                        const T un = bt[0] * fma(cc[2], u[2], fma(cc[1], u[1], cc[0] * u[0]));
 #pragma unroll
-                        for (int c = 0; c < 3; ++c) val[c] = val[c] - dv * cc[c] + un * cc[c];
+                        for (int c = 0; c < 3; ++c) val[c] = fma(un, cc[c], fma(-dv, cc[c], val[c]));
                     }
                 }
                 if (a.pen) {
+#pragma clang fp contract(off)
                     // b x (r n) = r (b x n), b x n once per output column
                     const T rad = radl[p];
-                    val[0] += rad * pbn[0];
-                    val[1] += rad * pbn[1];
-                    val[2] += rad * pbn[2];
+                    val[0] = fma(rad, pbn[0], val[0]);
+                    val[1] = fma(rad, pbn[1], val[1]);
+                    val[2] = fma(rad, pbn[2], val[2]);
                 }
                 if (a.add) {
                     val[0] += a.dst0[base + l];
                     val[1] += a.dst1[base + l];
                     val[2] += a.dst2[base + l];
+                }
+                if constexpr (FUSE) {
+                    // columns off the subdomain's lateral edges are not shared
+                    // (every subdomain spans the whole shell radially here)
+                    if (fuse_here) {
+                        if (p == K) {
+                            // the top lane's extra (surface) level
+                            T in[4][3];
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) {
+                                in[0][c] = lane_value(rec, REC_SB + c);
+                                in[1][c] = lane_value(rec, REC_SI + c);
+                                in[2][c] = a.cread_d ? lane_value(rec, REC_SD + c) : T(0);
+                                in[3][c] = lane_value(rec, REC_USURF + c);
+                            }
+                            cheb_update<T>(a, base + l, val, in);
+                        } else if constexpr (early) {
+                            cheb_update<T>(a, base + l, val, cin);
+                        } else {
+                            T in[4][3];
+                            cheb_load<T>(a, base + l, in);
+                            cheb_update<T>(a, base + l, val, in);
+                        }
+                        continue;
+                    }
                 }
                 a.dst0[base + l] = val[0];
                 a.dst1[base + l] = val[1];
@@ -977,10 +1074,10 @@ int pick_segment(int nx, int64_t lateral_groups, int resident)
     return best;
 }
 
-template <typename T, int R, int K, int NW, int RG, int BCM = BCM_RUNTIME>
+template <typename T, int R, int K, int NW, int RG, int BCM = BCM_RUNTIME, bool FUSE = false>
 void launch_apply_rk(ApplyArgs<T> a, int64_t lateral_groups, hipStream_t s)
 {
-    const void *kern = reinterpret_cast<const void *>(&epsdivdiv_apply_kernel<T, R, K, NW, RG, BCM>);
+    const void *kern = reinterpret_cast<const void *>(&epsdivdiv_apply_kernel<T, R, K, NW, RG, BCM, FUSE>);
     static int cached_dev = -1, cached_resident = 0;
     int dev = 0;
     TNG_HIP_CHECK(hipGetDevice(&dev));
@@ -993,31 +1090,41 @@ void launch_apply_rk(ApplyArgs<T> a, int64_t lateral_groups, hipStream_t s)
     a.seg_len = seg;
     a.segs = (a.nx + seg - 1) / seg;
     const int blocks = (int)(lateral_groups * a.segs);
-    hipLaunchKernelGGL((epsdivdiv_apply_kernel<T, R, K, NW, RG, BCM>), dim3(blocks), dim3(NW * 64), 0, s, a);
+    hipLaunchKernelGGL((epsdivdiv_apply_kernel<T, R, K, NW, RG, BCM, FUSE>), dim3(blocks), dim3(NW * 64), 0, s, a);
 }
 
-template <typename T> void launch_apply(ApplyArgs<T> a, hipStream_t s)
+// Shape of the apply launch for a mesh: R lanes per column, K cells per lane,
+// RG radial chunks, NW waves per workgroup.
+struct ApplyShape {
+    int R, K, RG, NW;
+};
+
+ApplyShape apply_shape(int ncells)
+{
+    ApplyShape sh{64, 1, 1, 8};
+    if (ncells <= 64) {
+        sh.R = 4;
+        while (sh.R < ncells) sh.R *= 2;
+    } else if (ncells == 128 && env_int("TNG_APPLY_RADIAL_CHUNKS", 2) == 2) {
+        sh.RG = 2;
+    } else {
+        sh.K = ncells / 64;
+        sh.NW = 4;
+    }
+    if (sh.K == 1 && sh.R == 64 && sh.RG == 1 && env_int("TNG_APPLY_WAVES", 8) == 4) sh.NW = 4;
+    return sh;
+}
+
+template <typename T, bool FUSE = false> void launch_apply(ApplyArgs<T> a, hipStream_t s)
 {
     TNG_REQUIRE(a.ncells >= 1 && (a.ncells & (a.ncells - 1)) == 0, "radial layers per subdomain must be a power of 2");
     TNG_REQUIRE((int64_t)a.nsd * a.nx * a.nx * a.nr < ((int64_t)1 << 31), "node copies exceed 32-bit indexing");
     // shapes: up to 64 radial cells one cell per lane (R lanes per column);
     // 128 cells in two radial chunks of one wave each; more cells K per lane
-    int R, K, RG = 1;
-    if (a.ncells <= 64) {
-        K = 1;
-        R = 4;
-        while (R < a.ncells) R *= 2;
-    } else if (a.ncells == 128 && env_int("TNG_APPLY_RADIAL_CHUNKS", 2) == 2) {
-        R = 64;
-        K = 1;
-        RG = 2;
-    } else {
-        R = 64;
-        K = a.ncells / 64;
-        TNG_REQUIRE(K <= 4, "more than 256 radial layers per subdomain is not supported");
-    }
-    int NW = K == 1 ? 8 : 4;
-    if (K == 1 && R == 64 && RG == 1 && env_int("TNG_APPLY_WAVES", 8) == 4) NW = 4;
+    const ApplyShape sh = apply_shape(a.ncells);
+    const int R = sh.R, K = sh.K, RG = sh.RG, NW = sh.NW;
+    TNG_REQUIRE(K <= 4, "more than 256 radial layers per subdomain is not supported");
+    TNG_REQUIRE(!FUSE || (R == 64 && K == 1 && NW == 8), "the fused Chebyshev step needs the one-column kernels");
     const int VX = NW / RG * (64 / R);
     // R = 64: strips over the columns of all subdomains in a row; otherwise
     // strips per subdomain
@@ -1028,18 +1135,20 @@ template <typename T> void launch_apply(ApplyArgs<T> a, hipStream_t s)
                     ((a.bc_cmb == TNG_DIRICHLET || a.bc_surf == TNG_DIRICHLET) ? 2 : 0);
     if (RG == 2) {
         switch (bcm) {
-        case 0: launch_apply_rk<T, 64, 1, 8, 2, 0>(a, lg, s); break;
-        case 1: launch_apply_rk<T, 64, 1, 8, 2, 1>(a, lg, s); break;
-        case 2: launch_apply_rk<T, 64, 1, 8, 2, 2>(a, lg, s); break;
-        default: launch_apply_rk<T, 64, 1, 8, 2, 3>(a, lg, s); break;
+        case 0: launch_apply_rk<T, 64, 1, 8, 2, 0, FUSE>(a, lg, s); break;
+        case 1: launch_apply_rk<T, 64, 1, 8, 2, 1, FUSE>(a, lg, s); break;
+        case 2: launch_apply_rk<T, 64, 1, 8, 2, 2, FUSE>(a, lg, s); break;
+        default: launch_apply_rk<T, 64, 1, 8, 2, 3, FUSE>(a, lg, s); break;
         }
     } else if (K == 1 && R == 64 && NW == 8) {
         switch (bcm) {
-        case 0: launch_apply_rk<T, 64, 1, 8, 1, 0>(a, lg, s); break;
-        case 1: launch_apply_rk<T, 64, 1, 8, 1, 1>(a, lg, s); break;
-        case 2: launch_apply_rk<T, 64, 1, 8, 1, 2>(a, lg, s); break;
-        default: launch_apply_rk<T, 64, 1, 8, 1, 3>(a, lg, s); break;
+        case 0: launch_apply_rk<T, 64, 1, 8, 1, 0, FUSE>(a, lg, s); break;
+        case 1: launch_apply_rk<T, 64, 1, 8, 1, 1, FUSE>(a, lg, s); break;
+        case 2: launch_apply_rk<T, 64, 1, 8, 1, 2, FUSE>(a, lg, s); break;
+        default: launch_apply_rk<T, 64, 1, 8, 1, 3, FUSE>(a, lg, s); break;
         }
+    } else if constexpr (FUSE) {
+        // unreachable: rejected above
     } else if (K == 1) {
         switch (R) {
         case 4: launch_apply_rk<T, 4, 1, 8, 1>(a, lg, s); break;
@@ -1080,6 +1189,45 @@ template <typename T> void run_diag_kernel(tng_operator &op, const tng_vector *s
     TNG_HIP_CHECK(hipGetLastError());
 }
 
+// fs/fs penalty: the rotation moment b of src into op.d_penalty_b
+template <typename T> void penalty_moment(tng_operator &op, const tng_vector &src, hipStream_t s)
+{
+    const tng_domain &m = *op.mesh;
+    PenArgs<T> p{};
+    const MeshPtrs<T> mp = mesh_ptrs<T>(m);
+    p.src0 = (const T *)src.comp(0); p.src1 = (const T *)src.comp(1); p.src2 = (const T *)src.comp(2);
+    p.coords = mp.coords; p.radii = mp.radii; p.omask = m.d_omask.as<uint8_t>();
+    p.partials = m.ctx->red_partials.as<double>();
+    p.out_b = op.d_penalty_b.as<double>();
+    p.eps = op.penalty_eps;
+    for (int i = 0; i < 9; ++i) p.ginv[i] = m.rot_gram_inv[i];
+    p.nsd = m.nsd; p.nx = m.nx; p.nr = m.nr; p.n = m.nalloc;
+    p.div_nr = FastDiv((uint32_t)m.nr);
+    p.div_nxx = FastDiv((uint32_t)(m.nx * m.nx));
+    TNG_REQUIRE(m.nalloc < (int64_t)1 << 31, "penalty moment: node copies exceed 32-bit indexing");
+    hipLaunchKernelGGL(penalty_partial_kernel<T>, dim3(kPenBlocks), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(penalty_final_kernel<T>, dim3(1), dim3(256), 0, s, p, kPenBlocks);
+    TNG_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+ApplyArgs<T> apply_args(const tng_operator &op, const tng_vector &src, tng_vector &dst, bool pen, bool add)
+{
+    const tng_domain &m = *op.mesh;
+    ApplyArgs<T> a{};
+    const MeshPtrs<T> mp = mesh_ptrs<T>(m);
+    a.src0 = (const T *)src.comp(0); a.src1 = (const T *)src.comp(1); a.src2 = (const T *)src.comp(2);
+    a.dst0 = (T *)dst.comp(0); a.dst1 = (T *)dst.comp(1); a.dst2 = (T *)dst.comp(2);
+    a.k = (const T *)op.k->comp(0);
+    a.radii = mp.radii; a.latgeom = mp.latgeom; a.coords = mp.coords;
+    a.btab = op.d_btab.as<T>();
+    a.pen = pen ? op.d_penalty_b.as<double>() : nullptr;
+    a.sd_cmb = m.d_sd_cmb.as<uint8_t>(); a.sd_surf = m.d_sd_surf.as<uint8_t>();
+    a.nsd = m.nsd; a.nx = m.nx; a.nr = m.nr; a.ncells = m.nr - 1;
+    a.bc_cmb = op.bc_cmb; a.bc_surf = op.bc_surf; a.add = add;
+    return a;
+}
+
 template <typename T> void apply_impl(tng_operator &op, const tng_vector &src, tng_vector &dst, hipStream_t s)
 {
     tng_domain &m = *op.mesh;
@@ -1087,23 +1235,7 @@ template <typename T> void apply_impl(tng_operator &op, const tng_vector &src, t
     const bool add = op.apply_mode == TNG_ADD;
     hipEvent_t *ev = ctx.profiling ? ctx.prof_next_set() : nullptr;
     if (ev) TNG_HIP_CHECK(hipEventRecord(ev[0], s));
-    auto penalty_moment = [&]() {
-        PenArgs<T> p{};
-        const MeshPtrs<T> mp = mesh_ptrs<T>(m);
-        p.src0 = (const T *)src.comp(0); p.src1 = (const T *)src.comp(1); p.src2 = (const T *)src.comp(2);
-        p.coords = mp.coords; p.radii = mp.radii; p.omask = m.d_omask.as<uint8_t>();
-        p.partials = ctx.red_partials.as<double>();
-        p.out_b = op.d_penalty_b.as<double>();
-        p.eps = op.penalty_eps;
-        for (int i = 0; i < 9; ++i) p.ginv[i] = m.rot_gram_inv[i];
-        p.nsd = m.nsd; p.nx = m.nx; p.nr = m.nr; p.n = m.nalloc;
-        p.div_nr = FastDiv((uint32_t)m.nr);
-        p.div_nxx = FastDiv((uint32_t)(m.nx * m.nx));
-        TNG_REQUIRE(m.nalloc < (int64_t)1 << 31, "penalty moment: node copies exceed 32-bit indexing");
-        hipLaunchKernelGGL(penalty_partial_kernel<T>, dim3(kPenBlocks), dim3(256), 0, s, p);
-        hipLaunchKernelGGL(penalty_final_kernel<T>, dim3(1), dim3(256), 0, s, p, kPenBlocks);
-        TNG_HIP_CHECK(hipGetLastError());
-    };
+    auto penalty_moment = [&]() { tng::penalty_moment<T>(op, src, s); };
     if (op.stored) {
         // slow path on stored matrices (Dirichlet/free-slip handled per row)
         const bool pen = op.penalty_active && !op.diagonal;
@@ -1119,23 +1251,50 @@ template <typename T> void apply_impl(tng_operator &op, const tng_vector &src, t
         if (!op.btab_valid) epsdivdiv_update_boundary_table(op, s);
         const bool pen = op.penalty_active;
         if (pen) penalty_moment();
-        ApplyArgs<T> a{};
-        const MeshPtrs<T> mp = mesh_ptrs<T>(m);
-        a.src0 = (const T *)src.comp(0); a.src1 = (const T *)src.comp(1); a.src2 = (const T *)src.comp(2);
-        a.dst0 = (T *)dst.comp(0); a.dst1 = (T *)dst.comp(1); a.dst2 = (T *)dst.comp(2);
-        a.k = (const T *)op.k->comp(0);
-        a.radii = mp.radii; a.latgeom = mp.latgeom; a.coords = mp.coords;
-        a.btab = op.d_btab.as<T>();
-        a.pen = pen ? op.d_penalty_b.as<double>() : nullptr;
-        a.sd_cmb = m.d_sd_cmb.as<uint8_t>(); a.sd_surf = m.d_sd_surf.as<uint8_t>();
-        a.nsd = m.nsd; a.nx = m.nx; a.nr = m.nr; a.ncells = m.nr - 1;
-        a.bc_cmb = op.bc_cmb; a.bc_surf = op.bc_surf; a.add = add;
+        ApplyArgs<T> a = apply_args<T>(op, src, dst, pen, add);
         if (ev) TNG_HIP_CHECK(hipEventRecord(ev[1], s));
         launch_apply<T>(a, s);
         if (ev) TNG_HIP_CHECK(hipEventRecord(ev[2], s));
     }
     if (op.comm_mode == TNG_COMMUNICATE_ADDITIVELY) launch_halo_sum(m, dst, s);
     if (ev) TNG_HIP_CHECK(hipEventRecord(ev[3], s));
+}
+
+// One Chebyshev sub-step with the update fused into the apply: node copies
+// that no other subdomain shares are updated in the apply kernel's store
+// phase, the shared ones (lateral subdomain edges) after their A x has been
+// summed over the copies.  Returns false (nothing launched) where the fused
+// kernels do not apply.
+template <typename T>
+bool apply_cheb_impl(tng_operator &op, const tng_vector &x, tng_vector &ax, tng_vector &x_out, tng_vector &d,
+                     const tng_vector &b, const tng_vector &invd, double az, double ad, bool read_d, hipStream_t s)
+{
+    tng_domain &m = *op.mesh;
+    if (op.stored || op.diagonal || op.apply_mode != TNG_REPLACE || op.comm_mode != TNG_COMMUNICATE_ADDITIVELY ||
+        m.rad_sdr != 0)
+        return false;
+    const ApplyShape sh = apply_shape(m.nr - 1);
+    if (!(sh.R == 64 && sh.K == 1 && sh.NW == 8)) return false;
+    tng_context &ctx = *m.ctx;
+    hipEvent_t *ev = ctx.profiling ? ctx.prof_next_set() : nullptr;
+    if (ev) TNG_HIP_CHECK(hipEventRecord(ev[0], s));
+    if (!op.btab_valid) epsdivdiv_update_boundary_table(op, s);
+    const bool pen = op.penalty_active;
+    if (pen) penalty_moment<T>(op, x, s);
+    ApplyArgs<T> a = apply_args<T>(op, x, ax, pen, false);
+    a.cb0 = (const T *)b.comp(0); a.cb1 = (const T *)b.comp(1); a.cb2 = (const T *)b.comp(2);
+    a.ci0 = (const T *)invd.comp(0); a.ci1 = (const T *)invd.comp(1); a.ci2 = (const T *)invd.comp(2);
+    a.cd0 = (T *)d.comp(0); a.cd1 = (T *)d.comp(1); a.cd2 = (T *)d.comp(2);
+    a.cx0 = (T *)x_out.comp(0); a.cx1 = (T *)x_out.comp(1); a.cx2 = (T *)x_out.comp(2);
+    a.caz = (T)az;
+    a.cad = (T)ad;
+    a.cread_d = read_d ? 1 : 0;
+    if (ev) TNG_HIP_CHECK(hipEventRecord(ev[1], s));
+    launch_apply<T, true>(a, s);
+    if (ev) TNG_HIP_CHECK(hipEventRecord(ev[2], s));
+    launch_halo_cheb(m, ax, b, invd, d, x, x_out, az, ad, read_d, s);
+    if (ev) TNG_HIP_CHECK(hipEventRecord(ev[3], s));
+    return true;
 }
 
 }  // namespace
@@ -1155,6 +1314,19 @@ void epsdivdiv_apply(tng_operator &op, const tng_vector &src, tng_vector &dst, h
 {
     if (src.dtype == TNG_F64) apply_impl<double>(op, src, dst, s);
     else apply_impl<float>(op, src, dst, s);
+}
+
+bool epsdivdiv_apply_cheb(tng_operator &op, const tng_vector &x, tng_vector &ax, tng_vector &x_out, tng_vector &d,
+                          const tng_vector &b, const tng_vector &invd, double az, double ad, bool read_d,
+                          hipStream_t s)
+{
+    if (op.kind != OP_EPSDIVDIV) return false;
+    const tng_vector *vs[6] = {&x, &ax, &x_out, &d, &b, &invd};
+    for (const tng_vector *v : vs)
+        if (v->mesh != op.mesh || v->dtype != x.dtype || v->ncomp != 3) return false;
+    if (x.comp(0) == x_out.comp(0)) return false;
+    if (x.dtype == TNG_F64) return apply_cheb_impl<double>(op, x, ax, x_out, d, b, invd, az, ad, read_d, s);
+    return apply_cheb_impl<float>(op, x, ax, x_out, d, b, invd, az, ad, read_d, s);
 }
 
 }  // namespace tng

@@ -45,6 +45,38 @@ __global__ void halo_sum_kernel(T *d0, T *d1, T *d2, int ncomp, const int32_t *_
     }
 }
 
+// Shared node copies of a fused Chebyshev sub-step: A x summed over the
+// copies (halo_sum_kernel's order), then the update of every copy with the
+// rounding of the separate update kernel.
+template <typename T>
+__global__ void halo_cheb_kernel(const Comp3<T> ax, const Comp3<T> b, const Comp3<T> invd, const Comp3<T> d,
+                                 const Comp3<T> x, const Comp3<T> xo, T az, T ad, int read_d,
+                                 const int32_t *__restrict__ idx, int64_t n, int maxc)
+{
+#pragma clang fp contract(off)
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int32_t ci[16];
+    int nc = 0;
+    for (int j = 0; j < maxc; ++j) {
+        const int32_t i = idx[(int64_t)j * n + t];
+        if (i >= 0) ci[nc++] = i;
+    }
+    for (int c = 0; c < 3; ++c) {
+        T sum = T(0);
+        for (int j = 0; j < nc; ++j) sum += ax.p[c][ci[j]];
+        for (int j = 0; j < nc; ++j) {
+            const int32_t i = ci[j];
+            const T r = b.p[c][i] - sum;
+            const T z = r * invd.p[c][i];
+            T tt = az * z;
+            if (read_d) tt = fma(ad, d.p[c][i], tt);
+            d.p[c][i] = tt;
+            xo.p[c][i] = x.p[c][i] + tt;
+        }
+    }
+}
+
 unsigned grid_for(int64_t n, int bs = 256)
 {
     int64_t g = (n + bs - 1) / bs;
@@ -106,6 +138,26 @@ void launch_halo_sum(const tng_domain &m, tng_vector &v, hipStream_t s)
     TNG_HIP_CHECK(hipGetLastError());
 }
 
+void launch_halo_cheb(const tng_domain &m, const tng_vector &ax, const tng_vector &b, const tng_vector &invd,
+                      tng_vector &d, const tng_vector &x, tng_vector &x_out, double az, double ad, bool read_d,
+                      hipStream_t s)
+{
+    if (m.halo_n == 0) return;
+    TNG_REQUIRE(m.halo_maxc <= 16, "halo: more than 16 copies of a node");
+    const unsigned g = (unsigned)((m.halo_n + 255) / 256);
+    auto run = [&](auto tag) {
+        using T = decltype(tag);
+        hipLaunchKernelGGL(halo_cheb_kernel<T>, dim3(g), dim3(256), 0, s, comps<T>(ax), comps<T>(b), comps<T>(invd),
+                           comps<T>(d), comps<T>(x), comps<T>(x_out), (T)az, (T)ad, read_d ? 1 : 0,
+                           m.d_halo_idx.as<int32_t>(), m.halo_n, m.halo_maxc);
+    };
+    if (x.dtype == TNG_F64)
+        run(double());
+    else
+        run(float());
+    TNG_HIP_CHECK(hipGetLastError());
+}
+
 }  // namespace tng
 
 // ----------------------------------------------------------------------------
@@ -118,20 +170,106 @@ namespace tng {
 
 namespace {
 
-template <typename T>
+// The streaming kernels move V consecutive elements of a component per
+// access: 16 bytes (V = 16 / sizeof(T)) when every pointer is 16-byte aligned,
+// else V = 1; the n % V trailing elements take V = 1.  The pointers may alias
+// as far as the compiler knows, so loads are not moved past stores by
+// themselves: a chunk either loads all components before its first store
+// (kLoadFirst: faster for FP32 vectors and for the Gram-Schmidt pass) or
+// finishes one component before the next (measured faster for the FP64
+// Chebyshev update and linear combination).
+template <typename T, int V> struct Pack {
+    T e[V];
+};
+
+template <int V, typename T> __device__ __forceinline__ Pack<T, V> ldv(const T *p, int64_t i)
+{
+    Pack<T, V> o;
+    if constexpr (V == 1) {
+        o.e[0] = p[i];
+    } else {
+        typedef T VT __attribute__((ext_vector_type(V)));
+        const VT v = reinterpret_cast<const VT *>(p)[i];
+#pragma unroll
+        for (int k = 0; k < V; ++k) o.e[k] = v[k];
+    }
+    return o;
+}
+
+template <int V, typename T> __device__ __forceinline__ void stv(T *p, int64_t i, const Pack<T, V> &o)
+{
+    if constexpr (V == 1) {
+        p[i] = o.e[0];
+    } else {
+        typedef T VT __attribute__((ext_vector_type(V)));
+        VT v;
+#pragma unroll
+        for (int k = 0; k < V; ++k) v[k] = o.e[k];
+        reinterpret_cast<VT *>(p)[i] = v;
+    }
+}
+
+template <typename T> constexpr int kVec = 16 / (int)sizeof(T);
+template <typename T> constexpr bool kLoadFirst = sizeof(T) == 4;
+
+template <typename T> bool aligned16(const Comp3<T> &v, int ncomp)
+{
+    for (int c = 0; c < ncomp; ++c)
+        if (reinterpret_cast<uintptr_t>(v.p[c]) & 15) return false;
+    return true;
+}
+
+template <int V, typename T>
+__device__ __forceinline__ void lincomb_chunk(const Comp3<T> &y, const Comp3<T> &x0, const Comp3<T> &x1,
+                                              const Comp3<T> &x2, int nx, T a0, T a1, T a2, T a3, int ncomp,
+                                              int64_t i)
+{
+    Pack<T, V> v[3][3];
+    auto load = [&](int c) {
+        if (nx > 0) v[c][0] = ldv<V>(x0.p[c], i);
+        if (nx > 1) v[c][1] = ldv<V>(x1.p[c], i);
+        if (nx > 2) v[c][2] = ldv<V>(x2.p[c], i);
+    };
+    auto update = [&](int c) {
+        Pack<T, V> o;
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+            T t = a0;
+            if (nx > 0) t += a1 * v[c][0].e[k];
+            if (nx > 1) t += a2 * v[c][1].e[k];
+            if (nx > 2) t += a3 * v[c][2].e[k];
+            o.e[k] = t;
+        }
+        stv<V>(y.p[c], i, o);
+    };
+    if constexpr (kLoadFirst<T>) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if (c < ncomp) load(c);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if (c < ncomp) update(c);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if (c < ncomp) {
+                load(c);
+                update(c);
+            }
+    }
+}
+
+template <typename T, int V>
 __global__ __launch_bounds__(256) void lincomb_kernel(const Comp3<T> y, const Comp3<T> x0, const Comp3<T> x1,
                                                       const Comp3<T> x2, int nx, double c0, double c1, double c2,
                                                       double c3, int ncomp, int64_t n)
 {
     const T a0 = (T)c0, a1 = (T)c1, a2 = (T)c2, a3 = (T)c3;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        for (int c = 0; c < ncomp; ++c) {
-            T v = a0;
-            if (nx > 0) v += a1 * x0.p[c][i];
-            if (nx > 1) v += a2 * x1.p[c][i];
-            if (nx > 2) v += a3 * x2.p[c][i];
-            y.p[c][i] = v;
-        }
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t nv = n / V;
+    for (int64_t i = t0; i < nv; i += stride) lincomb_chunk<V>(y, x0, x1, x2, nx, a0, a1, a2, a3, ncomp, i);
+    if constexpr (V > 1)
+        for (int64_t i = nv * V + t0; i < n; i += stride) lincomb_chunk<1>(y, x0, x1, x2, nx, a0, a1, a2, a3, ncomp, i);
 }
 
 template <typename T> __global__ __launch_bounds__(256) void mult_kernel(const Comp3<T> y, const Comp3<T> x, int ncomp, int64_t n)
@@ -145,23 +283,61 @@ template <typename T> __global__ __launch_bounds__(256) void mult_kernel(const C
 //   d = cz * z + cd * d   (cd absent: d is overwritten)
 //   x = x + d             (x absent: x = 0, written without being read)
 // with the rounding of the separate lincomb / scale kernels.
-template <typename T>
+template <int V, typename T>
+__device__ __forceinline__ void cheb_chunk(const Comp3<T> &x, const Comp3<T> &d, const Comp3<T> &b,
+                                           const Comp3<T> &ax, const Comp3<T> &invd, int has_ax, int read_d,
+                                           int read_x, T az, T ad, int64_t i)
+{
+    Pack<T, V> vb[3], va[3], vi[3], vd[3], vx[3];
+    auto load = [&](int c) {
+        vb[c] = ldv<V>(b.p[c], i);
+        if (has_ax) va[c] = ldv<V>(ax.p[c], i);
+        vi[c] = ldv<V>(invd.p[c], i);
+        if (read_d) vd[c] = ldv<V>(d.p[c], i);
+        if (read_x) vx[c] = ldv<V>(x.p[c], i);
+    };
+    auto update = [&](int c) {
+#pragma clang fp contract(off)
+        Pack<T, V> dn, xn;
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+            T r = vb[c].e[k];
+            if (has_ax) r = r - va[c].e[k];
+            const T z = r * vi[c].e[k];
+            T t = az * z;
+            if (read_d) t = fma(ad, vd[c].e[k], t);
+            dn.e[k] = t;
+            xn.e[k] = read_x ? vx[c].e[k] + t : t;
+        }
+        stv<V>(d.p[c], i, dn);
+        stv<V>(x.p[c], i, xn);
+    };
+    if constexpr (kLoadFirst<T>) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) load(c);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) update(c);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            load(c);
+            update(c);
+        }
+    }
+}
+
+template <typename T, int V>
 __global__ __launch_bounds__(256) void cheb_update_kernel(const Comp3<T> x, const Comp3<T> d, const Comp3<T> b,
                                                           const Comp3<T> ax, const Comp3<T> invd, int has_ax,
                                                           int read_d, int read_x, double cz, double cd, int64_t n)
 {
     const T az = (T)cz, ad = (T)cd;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            T r = b.p[c][i];
-            if (has_ax) r = r - ax.p[c][i];
-            const T z = r * invd.p[c][i];
-            T dn = az * z;
-            if (read_d) dn = fma(ad, d.p[c][i], dn);
-            d.p[c][i] = dn;
-            x.p[c][i] = read_x ? x.p[c][i] + dn : dn;
-        }
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t nv = n / V;
+    for (int64_t i = t0; i < nv; i += stride) cheb_chunk<V>(x, d, b, ax, invd, has_ax, read_d, read_x, az, ad, i);
+    if constexpr (V > 1)
+        for (int64_t i = nv * V + t0; i < n; i += stride)
+            cheb_chunk<1>(x, d, b, ax, invd, has_ax, read_d, read_x, az, ad, i);
 }
 
 template <typename T> __global__ __launch_bounds__(256) void invert_kernel(const Comp3<T> y, int ncomp, int64_t n)
@@ -347,28 +523,55 @@ __global__ __launch_bounds__(256) void pcg_direction_kernel(const Comp3<T> p, co
 // reduce_partial_kernel; mgs_final_kernel adds the velocity and pressure dots
 // as the host formulation did.
 // ----------------------------------------------------------------------------
-template <typename T>
+template <int V, typename T>
+__device__ __forceinline__ double mgs_chunk(const Comp3<T> &w, const Comp3<T> &vprev, const Comp3<T> &vnext,
+                                            int ncomp, const uint8_t *omask, bool upd, T mh, int self, int64_t i)
+{
+    Pack<T, V> vw[3], vp[3], vn[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (c >= ncomp) break;
+        vw[c] = ldv<V>(w.p[c], i);
+        if (upd) vp[c] = ldv<V>(vprev.p[c], i);
+        if (!self) vn[c] = ldv<V>(vnext.p[c], i);
+    }
+    double v[V];
+#pragma unroll
+    for (int k = 0; k < V; ++k) v[k] = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (c >= ncomp) break;
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+            T wc = vw[c].e[k];
+            if (upd) wc = fma(mh, vp[c].e[k], wc);
+            vw[c].e[k] = wc;
+            const T vc = self ? wc : vn[c].e[k];
+            v[k] += (double)wc * (double)vc;
+        }
+        if (upd) stv<V>(w.p[c], i, vw[c]);
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < V; ++k) acc += omask[i * V + k] == OWNED ? v[k] : 0.0;
+    return acc;
+}
+
+template <typename T, int V>
 __global__ __launch_bounds__(256) void mgs_block_kernel(const Comp3<T> w, const Comp3<T> vprev, const Comp3<T> vnext,
                                                         int ncomp, const uint8_t *omask, int64_t n,
                                                         const double *hprev, int self, double *partials)
 {
     __shared__ double red[4];
-    const T mh = hprev ? (T)(-hprev[0]) : T(0);
+    const bool upd = hprev != nullptr;
+    const T mh = upd ? (T)(-hprev[0]) : T(0);
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t nv = n / V;
     double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const bool own = omask[i] == OWNED;
-        double v = 0.0;
-        for (int c = 0; c < ncomp; ++c) {
-            T wc = w.p[c][i];
-            if (hprev) {
-                wc = fma(mh, vprev.p[c][i], wc);
-                w.p[c][i] = wc;
-            }
-            const T vc = self ? wc : vnext.p[c][i];
-            v += (double)wc * (double)vc;
-        }
-        acc += own ? v : 0.0;
-    }
+    for (int64_t i = t0; i < nv; i += stride) acc += mgs_chunk<V>(w, vprev, vnext, ncomp, omask, upd, mh, self, i);
+    if constexpr (V > 1)
+        for (int64_t i = nv * V + t0; i < n; i += stride)
+            acc += mgs_chunk<1>(w, vprev, vnext, ncomp, omask, upd, mh, self, i);
     const double sblk = block_reduce<RED_DOT>(acc, red);
     if (threadIdx.x == 0) partials[blockIdx.x] = sblk;
 }
@@ -401,12 +604,21 @@ void launch_lincomb(tng_vector &y, int nx, const double *c, const tng_vector *co
     const int64_t n = y.mesh->nalloc;
     const double cc[3] = {nx > 0 ? c[0] : 0, nx > 1 ? c[1] : 0, nx > 2 ? c[2] : 0};
     const tng_vector &x0 = nx > 0 ? *x[0] : y, &x1 = nx > 1 ? *x[1] : y, &x2 = nx > 2 ? *x[2] : y;
+    auto run = [&](auto tag) {
+        using T = decltype(tag);
+        const Comp3<T> py = comps<T>(y), p0 = comps<T>(x0), p1 = comps<T>(x1), p2 = comps<T>(x2);
+        const int nc = y.ncomp;
+        if (aligned16(py, nc) && aligned16(p0, nc) && aligned16(p1, nc) && aligned16(p2, nc))
+            hipLaunchKernelGGL((lincomb_kernel<T, kVec<T>>), dim3(gs(n / kVec<T>)), dim3(256), 0, s, py, p0, p1, p2, nx,
+                               c0, cc[0], cc[1], cc[2], nc, n);
+        else
+            hipLaunchKernelGGL((lincomb_kernel<T, 1>), dim3(gs(n)), dim3(256), 0, s, py, p0, p1, p2, nx, c0, cc[0],
+                               cc[1], cc[2], nc, n);
+    };
     if (y.dtype == TNG_F64)
-        hipLaunchKernelGGL(lincomb_kernel<double>, dim3(gs(n)), dim3(256), 0, s, comps<double>(y), comps<double>(x0),
-                           comps<double>(x1), comps<double>(x2), nx, c0, cc[0], cc[1], cc[2], y.ncomp, n);
+        run(double());
     else
-        hipLaunchKernelGGL(lincomb_kernel<float>, dim3(gs(n)), dim3(256), 0, s, comps<float>(y), comps<float>(x0),
-                           comps<float>(x1), comps<float>(x2), nx, c0, cc[0], cc[1], cc[2], y.ncomp, n);
+        run(float());
     TNG_HIP_CHECK(hipGetLastError());
 }
 
@@ -433,14 +645,20 @@ void launch_cheb_update(tng_vector &x, tng_vector &d, const tng_vector &b, const
     if (ax) TNG_REQUIRE(ax->mesh == x.mesh && ax->dtype == x.dtype && ax->ncomp == 3, "chebyshev: mismatched A x");
     const int64_t n = x.mesh->nalloc;
     const tng_vector &axv = ax ? *ax : b;
+    auto run = [&](auto tag) {
+        using T = decltype(tag);
+        const Comp3<T> px = comps<T>(x), pd = comps<T>(d), pb = comps<T>(b), pa = comps<T>(axv), pi = comps<T>(invd);
+        if (aligned16(px, 3) && aligned16(pd, 3) && aligned16(pb, 3) && aligned16(pa, 3) && aligned16(pi, 3))
+            hipLaunchKernelGGL((cheb_update_kernel<T, kVec<T>>), dim3(gs(n / kVec<T>)), dim3(256), 0, s, px, pd, pb,
+                               pa, pi, ax ? 1 : 0, read_d ? 1 : 0, x_zero ? 0 : 1, cz, cd, n);
+        else
+            hipLaunchKernelGGL((cheb_update_kernel<T, 1>), dim3(gs(n)), dim3(256), 0, s, px, pd, pb, pa, pi, ax ? 1 : 0,
+                               read_d ? 1 : 0, x_zero ? 0 : 1, cz, cd, n);
+    };
     if (x.dtype == TNG_F64)
-        hipLaunchKernelGGL(cheb_update_kernel<double>, dim3(gs(n)), dim3(256), 0, s, comps<double>(x), comps<double>(d),
-                           comps<double>(b), comps<double>(axv), comps<double>(invd), ax ? 1 : 0, read_d ? 1 : 0,
-                           x_zero ? 0 : 1, cz, cd, n);
+        run(double());
     else
-        hipLaunchKernelGGL(cheb_update_kernel<float>, dim3(gs(n)), dim3(256), 0, s, comps<float>(x), comps<float>(d),
-                           comps<float>(b), comps<float>(axv), comps<float>(invd), ax ? 1 : 0, read_d ? 1 : 0,
-                           x_zero ? 0 : 1, cz, cd, n);
+        run(float());
     TNG_HIP_CHECK(hipGetLastError());
 }
 
@@ -578,18 +796,29 @@ void launch_mgs_step(tng_vector &wu, tng_vector &wp, const tng_vector *vprev_u, 
     for (int b = 0; b < 2; ++b) {
         const tng_vector &wb = *w[b];
         const int64_t n = wb.mesh->nalloc;
-        g[b] = red_grid(n);
         const tng_vector &pv = vp[b] ? *vp[b] : wb, &nv = vn[b] ? *vn[b] : wb;
         TNG_REQUIRE(pv.mesh == wb.mesh && nv.mesh == wb.mesh && pv.dtype == wb.dtype && nv.dtype == wb.dtype,
                     "mgs: mismatched vectors");
         const uint8_t *om = wb.mesh->d_omask.as<uint8_t>();
         double *pb = part + b * kRedBlocks;
+        auto run = [&](auto tag) {
+            using T = decltype(tag);
+            const Comp3<T> pw = comps<T>(wb), pp = comps<T>(pv), pn = comps<T>(nv);
+            const int nc = wb.ncomp;
+            if (aligned16(pw, nc) && aligned16(pp, nc) && aligned16(pn, nc)) {
+                g[b] = red_grid(n / kVec<T>);
+                hipLaunchKernelGGL((mgs_block_kernel<T, kVec<T>>), dim3(g[b]), dim3(256), 0, s, pw, pp, pn, nc, om, n,
+                                   h_prev_dev, vn[b] ? 0 : 1, pb);
+            } else {
+                g[b] = red_grid(n);
+                hipLaunchKernelGGL((mgs_block_kernel<T, 1>), dim3(g[b]), dim3(256), 0, s, pw, pp, pn, nc, om, n,
+                                   h_prev_dev, vn[b] ? 0 : 1, pb);
+            }
+        };
         if (wb.dtype == TNG_F64)
-            hipLaunchKernelGGL(mgs_block_kernel<double>, dim3(g[b]), dim3(256), 0, s, comps<double>(wb),
-                               comps<double>(pv), comps<double>(nv), wb.ncomp, om, n, h_prev_dev, vn[b] ? 0 : 1, pb);
+            run(double());
         else
-            hipLaunchKernelGGL(mgs_block_kernel<float>, dim3(g[b]), dim3(256), 0, s, comps<float>(wb),
-                               comps<float>(pv), comps<float>(nv), wb.ncomp, om, n, h_prev_dev, vn[b] ? 0 : 1, pb);
+            run(float());
     }
     hipLaunchKernelGGL(mgs_final_kernel, dim3(1), dim3(256), 0, s, (const double *)part, (int)g[0],
                        (const double *)(part + kRedBlocks), (int)g[1], h_out_dev);

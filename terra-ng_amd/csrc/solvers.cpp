@@ -16,6 +16,7 @@
 // transfers) while FGMRES, the Stokes operator, the Schur scaling and the
 // coarsest-level PCG stay in FP64.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -57,7 +58,18 @@ void apply(tng_operator &op, const tng_vector &src, tng_vector &dst) { op_apply(
 tng_vector *vec(std::vector<VecPtr> &pool, tng_domain *m, tng_dtype dt, int ncomp)
 {
     pool.emplace_back(new_vector(m, dt, ncomp));
+    pool.back()->swappable = true;  // solver work vectors are only reached through their objects
     return pool.back().get();
+}
+
+// Exchange the storage of two owned work vectors of one shape.
+void swap_storage(tng_vector &a, tng_vector &b)
+{
+    TNG_REQUIRE(a.mesh == b.mesh && a.dtype == b.dtype && a.ncomp == b.ncomp && a.buf.p && b.buf.p &&
+                    a.buf.bytes == b.buf.bytes,
+                "swap: vectors of different shape or not owned");
+    std::swap(a.cp, b.cp);
+    std::swap(a.buf.p, b.buf.p);
 }
 
 }  // namespace
@@ -101,17 +113,28 @@ void Chebyshev::solve(tng_operator &A, tng_vector &x, const tng_vector &b, bool 
     const double sigma = theta / delta;
     double rho = 1.0 / sigma;  // carried across iterations, as in the reference
     hipStream_t s = stream_of(x);
+    // d = az D^{-1}(b - A x) [+ ad d]; x += d.  With a spare vector the update
+    // runs inside the apply and writes x + d to the spare, whose storage then
+    // becomes x's (the apply still reads the old x).
+    auto sub_step = [&](double az, double ad, bool read_d) {
+        if (xb && x.swappable && epsdivdiv_apply_cheb(A, x, *z, *xb, *d, b, *inv_diag, az, ad, read_d, s)) {
+            swap_storage(x, *xb);
+            return;
+        }
+        apply(A, x, *z);
+        launch_cheb_update(x, *d, b, z, *inv_diag, az, ad, read_d, false, s);
+    };
     for (int iter = 0; iter < iterations; ++iter) {
-        // z = D^{-1}(b - A x); d = z / theta; x += d.  With x = 0 on entry A x
-        // is exactly zero and is not applied.
-        const bool zero = x_zero && iter == 0;
-        if (!zero) apply(A, x, *z);
-        launch_cheb_update(x, *d, b, zero ? nullptr : z, *inv_diag, 1.0 / theta, 0.0, false, zero, s);
+        // first sub-step: d = z / theta.  With x = 0 on entry A x is exactly
+        // zero and is not applied.
+        if (x_zero && iter == 0)
+            launch_cheb_update(x, *d, b, nullptr, *inv_diag, 1.0 / theta, 0.0, false, true, s);
+        else
+            sub_step(1.0 / theta, 0.0, false);
         for (int i = 2; i <= order; ++i) {
             const double rho_new = 1.0 / (2.0 * sigma - rho);
             const double beta = rho_new * rho, alpha = 2.0 * rho_new / delta;
-            apply(A, x, *z);
-            launch_cheb_update(x, *d, b, z, *inv_diag, alpha, beta, true, false, s);
+            sub_step(alpha, beta, true);
             rho = rho_new;
         }
     }
@@ -265,6 +288,15 @@ StokesSolver::StokesSolver(tng_context *ctx, int min_level, int max_level, int r
     }
     mg.pre.resize(nl);
     mg.post.resize(nl);
+    // spare solution vectors for the fused Chebyshev sub-steps (one-column
+    // apply kernels: 64 or 128 radial cells); TNG_CHEB_FUSE=0 turns them off
+    const char *fuse_env = std::getenv("TNG_CHEB_FUSE");
+    const bool fuse = !fuse_env || std::atoi(fuse_env) != 0;
+    for (int l = 0; l < nl; ++l) {
+        const int ncells = meshes[l]->nr - 1;
+        if (fuse && (ncells == 64 || ncells == 128) && meshes[l]->rad_sdr == 0)
+            mg.pre[l].xb = mg.post[l].xb = vec(pool, meshes[l].get(), mgt, 3);
+    }
     for (int l = 0; l < nl; ++l)
         for (Chebyshev *s : {&mg.pre[l], &mg.post[l]}) {
             s->order = prm.viscous_pc_chebyshev_order;

@@ -170,6 +170,7 @@ struct tng_vector {
     int ncomp = 1;
     void *cp[3] = {nullptr, nullptr, nullptr};  // component base addresses
     tng::DevBuf buf;                            // storage of an owned vector (empty when wrapped)
+    bool swappable = false;                     // internal work vector: its storage may be exchanged
     size_t elem_size() const { return dtype == TNG_F64 ? 8 : 4; }
     void *comp(int c) const { return cp[c]; }
 };
@@ -228,6 +229,15 @@ void launch_copy(tng_vector &dst, const tng_vector &src, hipStream_t s);
 void launch_halo_sum(const tng_domain &m, tng_vector &v, hipStream_t s);
 void epsdivdiv_update_boundary_table(tng_operator &op, hipStream_t s);
 void epsdivdiv_apply(tng_operator &op, const tng_vector &src, tng_vector &dst, hipStream_t s);
+// One Chebyshev sub-step fused into the apply: ax = A x on shared node copies
+// (scratch), d = az (b - A x) invd [+ ad d], x_out = x + d everywhere.
+// Returns false, launching nothing, where the fused kernels do not apply.
+bool epsdivdiv_apply_cheb(tng_operator &op, const tng_vector &x, tng_vector &ax, tng_vector &x_out, tng_vector &d,
+                          const tng_vector &b, const tng_vector &invd, double az, double ad, bool read_d,
+                          hipStream_t s);
+void launch_halo_cheb(const tng_domain &m, const tng_vector &ax, const tng_vector &b, const tng_vector &invd,
+                      tng_vector &d, const tng_vector &x, tng_vector &x_out, double az, double ad, bool read_d,
+                      hipStream_t s);
 void stokes_block_apply(tng_operator &op, const tng_vector &src, tng_vector &dst, hipStream_t s);
 
 // BLAS-1 and reductions (kernels_vec.hip)

@@ -20,6 +20,7 @@ struct Chebyshev {
     int order = 2, iterations = 1, power_iterations = 10;
     tng_vector *inv_diag = nullptr;
     tng_vector *d = nullptr, *z = nullptr;  // work vectors
+    tng_vector *xb = nullptr;               // spare solution vector (fused sub-steps), optional
     bool need_estimate = true;
     double lmax_estimate = 0.0;  // estimate of the largest eigenvalue of D^{-1} A
     uint64_t seed = 12345;
