@@ -62,7 +62,8 @@ template <typename T> struct ApplyArgs {
     int nsd, nx, nr, ncells;
     int bc_cmb, bc_surf;
     int add;
-    int strips, segs, seg_len;
+    int strips, groups;  // strips per subdomain row (or of all subdomains), lateral groups
+    int segs, seg_len;   // one-segment launches: segments per strip, rows per segment
     // fused Chebyshev sub-step (FUSE kernels): on node copies no other
     // subdomain shares, d = az * (b - A src) * invd [+ ad * d] and
     // xo = src + d replace the store of A src
@@ -329,456 +330,476 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
         const int nb = gridDim.x, x = blockIdx.x & 7, i = blockIdx.x >> 3, q = nb >> 3, r = nb & 7;
         b = x * q + min(x, r) + i;
     }
-    const int seg = b % a.segs;
-    b /= a.segs;
     const int nx = a.nx, nc = nx - 1, nr = a.nr, ncells = a.ncells;
-    int sd, xc;
-    bool col_ok = true;  // the virtual column exists
-    if constexpr (R == 64) {
-        // One wave per column: strips run over the virtual columns of all
-        // subdomains in a row (xc = -1 .. nc-1 per subdomain; xc = -1 has no
-        // cells and only outputs node column 0), so only the last strip is
-        // ragged.  A neighbour in the next subdomain has no cells there and
-        // passes zeros.
-        const int nvc = a.nsd * nx;
-        const int vcol = b * (VX - 1) + v;
-        col_ok = vcol < nvc;
-        const int vc = min(vcol, nvc - 1);
-        sd = vc / nx;
-        // past the last subdomain: a column without cells (zero contributions)
-        xc = col_ok ? vc - sd * nx - 1 : nc;
+    // Work: the node rows Y = 0 .. nx-1 of every lateral group (strip), one
+    // strip after the other.  BAL: workgroup b sweeps an equal share [q, q1)
+    // of them, so the whole grid runs in one round, and a share that crosses
+    // into the next strip starts a new sweep there with its own warm-up row.
+    // Otherwise one segment of seg_len rows of one strip per workgroup (the
+    // loop costs registers that FP32 and the fused kernels do not have).
+    constexpr bool BAL = sizeof(T) == 8 && !FUSE;
+    int64_t q, q1;
+    if constexpr (BAL) {
+        const int64_t rows_total = (int64_t)a.groups * nx;
+        q = rows_total * b / gridDim.x;
+        q1 = rows_total * (b + 1) / gridDim.x;
     } else {
-        const int strip = b % a.strips;
-        sd = b / a.strips;
-        xc = strip * (VX - 1) - 1 + v;
+        const int gb = b / a.segs, sb = b % a.segs;
+        q = (int64_t)gb * nx + sb * a.seg_len;
+        q1 = (int64_t)gb * nx + min(nx, (sb + 1) * a.seg_len);
     }
-    const int X = xc + 1;
-    const int Y0 = seg * a.seg_len;
-    const int Yend = min(Y0 + a.seg_len, nx);
-    const bool has_cmb = a.sd_cmb[sd] != 0, has_surf = a.sd_surf[sd] != 0;
-    const bool any_fs = (has_cmb && a.bc_cmb == TNG_FREESLIP) || (has_surf && a.bc_surf == TNG_FREESLIP);
-    const bool any_dir = (has_cmb && a.bc_cmb == TNG_DIRICHLET) || (has_surf && a.bc_surf == TNG_DIRICHLET);
+    do {
+        const int g = (int)(q / nx);
+        const int Y0 = (int)(q - (int64_t)g * nx);
+        const int Yend = (int)min((int64_t)nx, Y0 + (q1 - q));
+        q += Yend - Y0;
+        int sd, xc;
+        bool col_ok = true;  // the virtual column exists
+        if constexpr (R == 64) {
+            // One wave per column: strips run over the virtual columns of all
+            // subdomains in a row (xc = -1 .. nc-1 per subdomain; xc = -1 has no
+            // cells and only outputs node column 0), so only the last strip is
+            // ragged.  A neighbour in the next subdomain has no cells there and
+            // passes zeros.
+            const int nvc = a.nsd * nx;
+            const int vcol = g * (VX - 1) + v;
+            col_ok = vcol < nvc;
+            const int vc = min(vcol, nvc - 1);
+            sd = vc / nx;
+            // past the last subdomain: a column without cells (zero contributions)
+            xc = col_ok ? vc - sd * nx - 1 : nc;
+        } else {
+            const int strip = g % a.strips;
+            sd = g / a.strips;
+            xc = strip * (VX - 1) - 1 + v;
+        }
+        const int X = xc + 1;
+        const bool has_cmb = a.sd_cmb[sd] != 0, has_surf = a.sd_surf[sd] != 0;
+        const bool any_fs = (has_cmb && a.bc_cmb == TNG_FREESLIP) || (has_surf && a.bc_surf == TNG_FREESLIP);
+        const bool any_dir = (has_cmb && a.bc_cmb == TNG_DIRICHLET) || (has_surf && a.bc_surf == TNG_DIRICHLET);
 
-    // lanes: cells hc*nch + rl*K .. +K-1 (nch is either <= R with K = 1, or R*K)
-    const int nch = ncells / RG;
-    const bool lane_has = rl * K < nch;
-    const bool ovf_lane = rl * K + K == nch;  // lane holding the last cell of the chunk
-    const bool bot_chunk = hc == 0, top_chunk = hc == RG - 1;
-    const int lvl0 = hc * nch + (lane_has ? rl * K : nch - K);  // clamped first level (loads stay in bounds)
-    LaneBC<T> lbc;
-    lbc.z_lo = (has_cmb && bot_chunk && rl == 0 && a.bc_cmb == TNG_DIRICHLET) ? T(0) : T(1);
-    lbc.f_lo = (has_cmb && bot_chunk && rl == 0 && a.bc_cmb == TNG_FREESLIP) ? T(1) : T(0);
-    lbc.z_hi = (has_surf && top_chunk && ovf_lane && a.bc_surf == TNG_DIRICHLET) ? T(0) : T(1);
-    lbc.f_hi = (has_surf && top_chunk && ovf_lane && a.bc_surf == TNG_FREESLIP) ? T(1) : T(0);
+        // lanes: cells hc*nch + rl*K .. +K-1 (nch is either <= R with K = 1, or R*K)
+        const int nch = ncells / RG;
+        const bool lane_has = rl * K < nch;
+        const bool ovf_lane = rl * K + K == nch;  // lane holding the last cell of the chunk
+        const bool bot_chunk = hc == 0, top_chunk = hc == RG - 1;
+        const int lvl0 = hc * nch + (lane_has ? rl * K : nch - K);  // clamped first level (loads stay in bounds)
+        LaneBC<T> lbc;
+        lbc.z_lo = (has_cmb && bot_chunk && rl == 0 && a.bc_cmb == TNG_DIRICHLET) ? T(0) : T(1);
+        lbc.f_lo = (has_cmb && bot_chunk && rl == 0 && a.bc_cmb == TNG_FREESLIP) ? T(1) : T(0);
+        lbc.z_hi = (has_surf && top_chunk && ovf_lane && a.bc_surf == TNG_DIRICHLET) ? T(0) : T(1);
+        lbc.f_hi = (has_surf && top_chunk && ovf_lane && a.bc_surf == TNG_FREESLIP) ? T(1) : T(0);
 
-    T qq[K], cv[K];
-#pragma unroll
-    for (int p = 0; p < K; ++p) {
-        const int l = lvl0 + p;
-        const T r0 = a.radii[(int64_t)sd * nr + l], r1 = a.radii[(int64_t)sd * nr + l + 1];
-        const T rmid = T(0.5) * (r0 + r1), h = T(0.5) * (r1 - r0);
-        const T hs = T(0.5) / rmid, tt = T(1) / (T(6) * h);
-        qq[p] = tt / hs;
-        cv[p] = lane_has ? hs * hs * rmid * rmid * h * T(1.0 / 6.0) : T(0);
-    }
+        T qq[K], cv[K];
+    #pragma unroll
+        for (int p = 0; p < K; ++p) {
+            const int l = lvl0 + p;
+            const T r0 = a.radii[(int64_t)sd * nr + l], r1 = a.radii[(int64_t)sd * nr + l + 1];
+            const T rmid = T(0.5) * (r0 + r1), h = T(0.5) * (r1 - r0);
+            const T hs = T(0.5) / rmid, tt = T(1) / (T(6) * h);
+            qq[p] = tt / hs;
+            cv[p] = lane_has ? hs * hs * rmid * rmid * h * T(1.0 / 6.0) : T(0);
+        }
 
-    auto colbase = [&](int Xq, int Yq) -> int64_t {
-        const int xq = min(max(Xq, 0), nx - 1), yq = min(max(Yq, 0), nx - 1);
-        return (((int64_t)sd * nx + xq) * nx + yq);
-    };
-    auto load_normal = [&](int Xq, int Yq, T n[3]) {
-        const T *c = a.coords + 3 * colbase(Xq, Yq);
-        n[0] = c[0];
-        n[1] = c[1];
-        n[2] = c[2];
-    };
+        auto colbase = [&](int Xq, int Yq) -> int64_t {
+            const int xq = min(max(Xq, 0), nx - 1), yq = min(max(Yq, 0), nx - 1);
+            return (((int64_t)sd * nx + xq) * nx + yq);
+        };
+        auto load_normal = [&](int Xq, int Yq, T n[3]) {
+            const T *c = a.coords + 3 * colbase(Xq, Yq);
+            n[0] = c[0];
+            n[1] = c[1];
+            n[2] = c[2];
+        };
 
-    // R == 64: lateral data of the next step, one element per lane
-    // Per-lane address of the record element: base + stride * clamp(yc + dy,
-    // 0, ymax), fixed for the whole sweep (every lane loads unconditionally
-    // from a valid address, which keeps the wait counts of the pipeline exact).
-    const T *rec_base = a.latgeom;
-    int rec_stride = 0, rec_dy = 0, rec_ymax = 0;
-    if constexpr (R == 64) {
-        const int j = lane;
-        const int xcc = min(max(xc, 0), nc - 1);
-        const int xq0 = min(max(xc, 0), nx - 1), xq1 = min(max(xc + 1, 0), nx - 1);
-        const int64_t col0 = ((int64_t)sd * nx + xq0) * nx, col1 = ((int64_t)sd * nx + xq1) * nx;
-        if (j < REC_NT0) {
-            rec_base = a.latgeom + ((int64_t)sd * nc + xcc) * nc * 20 + j;
-            rec_stride = 20, rec_ymax = nc - 1;
-        } else if (j < REC_NT1) {
-            rec_base = a.coords + 3 * col0 + (j - REC_NT0);
-            rec_stride = 3, rec_dy = 1, rec_ymax = nx - 1;
-        } else if (j < REC_NOUT) {
-            rec_base = a.coords + 3 * col1 + (j - REC_NT1);
-            rec_stride = 3, rec_dy = 1, rec_ymax = nx - 1;
-        } else if (j < REC_BTAB) {
-            rec_base = a.coords + 3 * col1 + (j - REC_NOUT);
-            rec_stride = 3, rec_ymax = nx - 1;
-        } else if (j < REC_UCMB) {
-            if (a.btab) {
-                rec_base = a.btab + 6 * col1 + (j - REC_BTAB);
-                rec_stride = 6, rec_ymax = nx - 1;
+        // R == 64: lateral data of the next step, one element per lane
+        // Per-lane address of the record element: base + stride * clamp(yc + dy,
+        // 0, ymax), fixed for the whole sweep (every lane loads unconditionally
+        // from a valid address, which keeps the wait counts of the pipeline exact).
+        const T *rec_base = a.latgeom;
+        int rec_stride = 0, rec_dy = 0, rec_ymax = 0;
+        if constexpr (R == 64) {
+            const int j = lane;
+            const int xcc = min(max(xc, 0), nc - 1);
+            const int xq0 = min(max(xc, 0), nx - 1), xq1 = min(max(xc + 1, 0), nx - 1);
+            const int64_t col0 = ((int64_t)sd * nx + xq0) * nx, col1 = ((int64_t)sd * nx + xq1) * nx;
+            if (j < REC_NT0) {
+                rec_base = a.latgeom + ((int64_t)sd * nc + xcc) * nc * 20 + j;
+                rec_stride = 20, rec_ymax = nc - 1;
+            } else if (j < REC_NT1) {
+                rec_base = a.coords + 3 * col0 + (j - REC_NT0);
+                rec_stride = 3, rec_dy = 1, rec_ymax = nx - 1;
+            } else if (j < REC_NOUT) {
+                rec_base = a.coords + 3 * col1 + (j - REC_NT1);
+                rec_stride = 3, rec_dy = 1, rec_ymax = nx - 1;
+            } else if (j < REC_BTAB) {
+                rec_base = a.coords + 3 * col1 + (j - REC_NOUT);
+                rec_stride = 3, rec_ymax = nx - 1;
+            } else if (j < REC_UCMB) {
+                if (a.btab) {
+                    rec_base = a.btab + 6 * col1 + (j - REC_BTAB);
+                    rec_stride = 6, rec_ymax = nx - 1;
+                }
+            } else if (j < REC_LEN) {
+                const int c = (j - REC_UCMB) % 3;
+                const T *sc = c == 0 ? a.src0 : (c == 1 ? a.src1 : a.src2);
+                rec_base = sc + col1 * nr + (j < REC_USURF ? 0 : nr - 1);
+                rec_stride = nr, rec_ymax = nx - 1;
+            } else if (FUSE && j < REC_LEN_FUSE) {
+                const int c = (j - REC_SB) % 3, w = (j - REC_SB) / 3;
+                const T *vc;  // selects, not a table: a table indexed per lane lives in scratch
+                if (w == 0) vc = c == 0 ? a.cb0 : (c == 1 ? a.cb1 : a.cb2);
+                else if (w == 1) vc = c == 0 ? a.ci0 : (c == 1 ? a.ci1 : a.ci2);
+                else vc = c == 0 ? a.cd0 : (c == 1 ? a.cd1 : a.cd2);
+                rec_base = vc + col1 * nr + nr - 1;
+                rec_stride = nr, rec_ymax = nx - 1;
             }
-        } else if (j < REC_LEN) {
-            const int c = (j - REC_UCMB) % 3;
-            const T *sc = c == 0 ? a.src0 : (c == 1 ? a.src1 : a.src2);
-            rec_base = sc + col1 * nr + (j < REC_USURF ? 0 : nr - 1);
-            rec_stride = nr, rec_ymax = nx - 1;
-        } else if (FUSE && j < REC_LEN_FUSE) {
-            const int c = (j - REC_SB) % 3, w = (j - REC_SB) / 3;
-            const T *vc;  // selects, not a table: a table indexed per lane lives in scratch
-            if (w == 0) vc = c == 0 ? a.cb0 : (c == 1 ? a.cb1 : a.cb2);
-            else if (w == 1) vc = c == 0 ? a.ci0 : (c == 1 ? a.ci1 : a.ci2);
-            else vc = c == 0 ? a.cd0 : (c == 1 ? a.cd1 : a.cd2);
-            rec_base = vc + col1 * nr + nr - 1;
-            rec_stride = nr, rec_ymax = nx - 1;
         }
-    }
-    constexpr int rec_len = FUSE ? REC_LEN_FUSE : REC_LEN;
-    auto load_record = [&](int yc) -> T {
-        const int y = min(max(yc + rec_dy, 0), rec_ymax);
-        const T val = rec_base[(int64_t)y * rec_stride];
-        return lane < rec_len ? val : T(0);
-    };
+        constexpr int rec_len = FUSE ? REC_LEN_FUSE : REC_LEN;
+        auto load_record = [&](int yc) -> T {
+            const int y = min(max(yc + rec_dy, 0), rec_ymax);
+            const T val = rec_base[(int64_t)y * rec_stride];
+            return lane < rec_len ? val : T(0);
+        };
 
-    NodeCol<T, K> B0, B1, T0, T1;
-    RawCol<T, K> P0, P1;
-    T rec = T(0);
-    {
-        T n0[3] = {T(0), T(0), T(0)}, n1[3] = {T(0), T(0), T(0)};
-        load_raw<T, K>(a, colbase(xc, Y0 - 1) * nr, lvl0, P0);
-        load_raw<T, K>(a, colbase(xc + 1, Y0 - 1) * nr, lvl0, P1);
-        if (any_fs) {
-            load_normal(xc, Y0 - 1, n0);
-            load_normal(xc + 1, Y0 - 1, n1);
-        }
-        process<T, K, BCM>(P0, n0, lbc, any_fs, any_dir, B0);
-        process<T, K, BCM>(P1, n1, lbc, any_fs, any_dir, B1);
-        load_raw<T, K>(a, colbase(xc, Y0) * nr, lvl0, P0);
-        load_raw<T, K>(a, colbase(xc + 1, Y0) * nr, lvl0, P1);
-        if constexpr (R == 64) rec = load_record(Y0 - 1);
-    }
-
-    // loop invariants of the output phase
-    T pb[3] = {T(0), T(0), T(0)};
-    if (a.pen) {
-        pb[0] = (T)a.pen[0];
-        pb[1] = (T)a.pen[1];
-        pb[2] = (T)a.pen[2];
-    }
-    T radl[KL];
-#pragma unroll
-    for (int p = 0; p < KL; ++p) radl[p] = a.radii[(int64_t)sd * nr + min(hc * nch + rl * K + p, nr - 1)];
-
-    T accA[KL][3], accB[KL][3];
-#pragma unroll
-    for (int i = 0; i < KL; ++i)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) accA[i][c] = accB[i][c] = T(0);
-
-    // One y-step: bottom rows B0/B1 (processed), the prefetched raw row yc+1
-    // becomes the top rows T0/T1; acc_cur collects node row yc, acc_next row
-    // yc+1.  The caller alternates the roles of the two row pairs and the two
-    // accumulators, so nothing is copied between steps.
-    auto do_step = [&](int step, NodeCol<T, K> &B0, NodeCol<T, K> &B1, NodeCol<T, K> &T0, NodeCol<T, K> &T1,
-                       T (&acc_cur)[KL][3], T (&acc_next)[KL][3], T &rec, T &rec_next) {
-        const int yc = Y0 - 1 + step;
-        // one cell per lane and column per wave: the cell's geometry is
-        // wave-uniform; scalar loads issued here have the processing of the
-        // new row to arrive (more cells per lane run out of scalar registers
-        // and take it from the record instead)
-        T sg[20];
-        if constexpr (R == 64 && K == 1 && TNG_APPLY_SMEM_GEOM) {
-            using CT = const T __attribute__((address_space(4)));
-            const int xcc = min(max(xc, 0), nc - 1), ycc = min(max(yc, 0), nc - 1);
-            const int gi = __builtin_amdgcn_readfirstlane(((sd * nc + xcc) * nc + ycc) * 20);
-            const CT *gw = (const CT *)a.latgeom + gi;
-#pragma unroll
-            for (int i = 0; i < 20; ++i) sg[i] = gw[i];
-        }
-        // fused Chebyshev step.  FP64: the inputs of this step's output node
-        // are requested ahead of the next row's prefetch, so that waiting for
-        // them in the store phase does not wait for the prefetch (FP32 loads
-        // them in the store phase: the registers would halve its occupancy).
-        // The surface level comes with the record.
-        constexpr bool early = FUSE && sizeof(T) == 8;
-        T cin[4][3];
-        bool fuse_here = false;
-        if constexpr (FUSE) {
-            fuse_here = X > 0 && X < nx - 1 && yc > 0 && yc < nx - 1;
-            if (early && fuse_here)
-                cheb_load<T>(a, (((int64_t)sd * nx + X) * nx + yc) * nr + min(hc * nch + rl * K, nr - 1), cin);
-        }
-        // the prefetched row yc+1 becomes the top row; prefetch row yc+2
+        NodeCol<T, K> B0, B1, T0, T1;
+        RawCol<T, K> P0, P1;
+        T rec = T(0);
         {
             T n0[3] = {T(0), T(0), T(0)}, n1[3] = {T(0), T(0), T(0)};
-            if (BCM == BCM_RUNTIME ? any_fs : (BCM & 1) != 0) {
-                if constexpr (R == 64) {
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        n0[c] = lane_value(rec, REC_NT0 + c);
-                        n1[c] = lane_value(rec, REC_NT1 + c);
+            load_raw<T, K>(a, colbase(xc, Y0 - 1) * nr, lvl0, P0);
+            load_raw<T, K>(a, colbase(xc + 1, Y0 - 1) * nr, lvl0, P1);
+            if (any_fs) {
+                load_normal(xc, Y0 - 1, n0);
+                load_normal(xc + 1, Y0 - 1, n1);
+            }
+            process<T, K, BCM>(P0, n0, lbc, any_fs, any_dir, B0);
+            process<T, K, BCM>(P1, n1, lbc, any_fs, any_dir, B1);
+            load_raw<T, K>(a, colbase(xc, Y0) * nr, lvl0, P0);
+            load_raw<T, K>(a, colbase(xc + 1, Y0) * nr, lvl0, P1);
+            if constexpr (R == 64) rec = load_record(Y0 - 1);
+        }
+
+        // loop invariants of the output phase
+        T pb[3] = {T(0), T(0), T(0)};
+        if (a.pen) {
+            pb[0] = (T)a.pen[0];
+            pb[1] = (T)a.pen[1];
+            pb[2] = (T)a.pen[2];
+        }
+        T radl[KL];
+    #pragma unroll
+        for (int p = 0; p < KL; ++p) radl[p] = a.radii[(int64_t)sd * nr + min(hc * nch + rl * K + p, nr - 1)];
+
+        T accA[KL][3], accB[KL][3];
+    #pragma unroll
+        for (int i = 0; i < KL; ++i)
+    #pragma unroll
+            for (int c = 0; c < 3; ++c) accA[i][c] = accB[i][c] = T(0);
+
+        // One y-step: bottom rows B0/B1 (processed), the prefetched raw row yc+1
+        // becomes the top rows T0/T1; acc_cur collects node row yc, acc_next row
+        // yc+1.  The caller alternates the roles of the two row pairs and the two
+        // accumulators, so nothing is copied between steps.
+        auto do_step = [&](int step, NodeCol<T, K> &B0, NodeCol<T, K> &B1, NodeCol<T, K> &T0, NodeCol<T, K> &T1,
+                           T (&acc_cur)[KL][3], T (&acc_next)[KL][3], T &rec, T &rec_next) {
+            const int yc = Y0 - 1 + step;
+            // one cell per lane and column per wave: the cell's geometry is
+            // wave-uniform; scalar loads issued here have the processing of the
+            // new row to arrive (more cells per lane run out of scalar registers
+            // and take it from the record instead)
+            T sg[20];
+            if constexpr (R == 64 && K == 1 && TNG_APPLY_SMEM_GEOM) {
+                using CT = const T __attribute__((address_space(4)));
+                const int xcc = min(max(xc, 0), nc - 1), ycc = min(max(yc, 0), nc - 1);
+                const int gi = __builtin_amdgcn_readfirstlane(((sd * nc + xcc) * nc + ycc) * 20);
+                const CT *gw = (const CT *)a.latgeom + gi;
+    #pragma unroll
+                for (int i = 0; i < 20; ++i) sg[i] = gw[i];
+            }
+            // fused Chebyshev step.  FP64: the inputs of this step's output node
+            // are requested ahead of the next row's prefetch, so that waiting for
+            // them in the store phase does not wait for the prefetch (FP32 loads
+            // them in the store phase: the registers would halve its occupancy).
+            // The surface level comes with the record.
+            constexpr bool early = FUSE && sizeof(T) == 8;
+            T cin[4][3];
+            bool fuse_here = false;
+            if constexpr (FUSE) {
+                fuse_here = X > 0 && X < nx - 1 && yc > 0 && yc < nx - 1;
+                if (early && fuse_here)
+                    cheb_load<T>(a, (((int64_t)sd * nx + X) * nx + yc) * nr + min(hc * nch + rl * K, nr - 1), cin);
+            }
+            // the prefetched row yc+1 becomes the top row; prefetch row yc+2
+            {
+                T n0[3] = {T(0), T(0), T(0)}, n1[3] = {T(0), T(0), T(0)};
+                if (BCM == BCM_RUNTIME ? any_fs : (BCM & 1) != 0) {
+                    if constexpr (R == 64) {
+    #pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            n0[c] = lane_value(rec, REC_NT0 + c);
+                            n1[c] = lane_value(rec, REC_NT1 + c);
+                        }
+                    } else {
+                        load_normal(xc, yc + 1, n0);
+                        load_normal(xc + 1, yc + 1, n1);
+                    }
+                }
+                process<T, K, BCM>(P0, n0, lbc, any_fs, any_dir, T0);
+                process<T, K, BCM>(P1, n1, lbc, any_fs, any_dir, T1);
+            }
+            if constexpr (R == 64) rec_next = load_record(yc + 1);
+            load_raw<T, K>(a, colbase(xc, yc + 2) * nr, lvl0, P0);
+            load_raw<T, K>(a, colbase(xc + 1, yc + 2) * nr, lvl0, P1);
+
+            T s00[KL][3], s01[KL][3];  // written by the corner loop below
+
+            const bool cell_ok = xc >= 0 && xc < nc && yc >= 0 && yc < nc;
+            {
+                T g0[10], g1[10];
+                if constexpr (R == 64 && K == 1 && TNG_APPLY_SMEM_GEOM) {
+    #pragma unroll
+                    for (int i = 0; i < 10; ++i) {
+                        g0[i] = sg[i];
+                        g1[i] = sg[10 + i];
+                    }
+                } else if constexpr (R == 64) {
+    #pragma unroll
+                    for (int i = 0; i < 10; ++i) {
+                        g0[i] = lane_value(rec, REC_GEOM + i);
+                        g1[i] = lane_value(rec, REC_GEOM + 10 + i);
                     }
                 } else {
-                    load_normal(xc, yc + 1, n0);
-                    load_normal(xc + 1, yc + 1, n1);
-                }
-            }
-            process<T, K, BCM>(P0, n0, lbc, any_fs, any_dir, T0);
-            process<T, K, BCM>(P1, n1, lbc, any_fs, any_dir, T1);
-        }
-        if constexpr (R == 64) rec_next = load_record(yc + 1);
-        load_raw<T, K>(a, colbase(xc, yc + 2) * nr, lvl0, P0);
-        load_raw<T, K>(a, colbase(xc + 1, yc + 2) * nr, lvl0, P1);
-
-        T s00[KL][3], s01[KL][3];  // written by the corner loop below
-
-        const bool cell_ok = xc >= 0 && xc < nc && yc >= 0 && yc < nc;
-        {
-            T g0[10], g1[10];
-            if constexpr (R == 64 && K == 1 && TNG_APPLY_SMEM_GEOM) {
-#pragma unroll
-                for (int i = 0; i < 10; ++i) {
-                    g0[i] = sg[i];
-                    g1[i] = sg[10 + i];
-                }
-            } else if constexpr (R == 64) {
-#pragma unroll
-                for (int i = 0; i < 10; ++i) {
-                    g0[i] = lane_value(rec, REC_GEOM + i);
-                    g1[i] = lane_value(rec, REC_GEOM + 10 + i);
-                }
-            } else {
-                const int xcc = min(max(xc, 0), nc - 1), ycc = min(max(yc, 0), nc - 1);
-                const T *gw = a.latgeom + ((((int64_t)sd * nc + xcc) * nc + ycc) * 2) * 10;
-#pragma unroll
-                for (int i = 0; i < 10; ++i) {
-                    g0[i] = gw[i];
-                    g1[i] = gw[10 + i];
-                }
-            }
-#pragma unroll
-            for (int p = 0; p < K; ++p) {
-                const T cvp = cell_ok ? cv[p] : T(0);
-                T A0[3], A1[3], Cc[3], E0[3], E1[3], Cd[3];
-                // wedge 0: (c00, c10, c01)
-                wedge_stress<T>(g0, B0.su[p], B1.su[p], T0.su[p], B0.du[p], B1.du[p], T0.du[p],
-                                B0.kk[p] + B1.kk[p] + T0.kk[p], qq[p], cvp, A0, A1, Cc);
-                // wedge 1: (c11, c01, c10)
-                wedge_stress<T>(g1, T1.su[p], T0.su[p], B1.su[p], T1.du[p], T0.du[p], B1.du[p],
-                                T1.kk[p] + T0.kk[p] + B1.kk[p], qq[p], cvp, E0, E1, Cd);
-                // corners: c00 and c11 get one wedge each, c10 and c01 both;
-                // lateral parts L and radial parts C enter the bottom node as
-                // L - C and the top node as L + C
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const T x00 = -(A0[c] + A1[c]), x11 = -(E0[c] + E1[c]);
-                    const T l10 = A0[c] + E1[c], l01 = A1[c] + E0[c], c2 = Cc[c] + Cd[c];
-                    if (p == 0) {
-                        s00[p][c] = x00 - Cc[c];
-                        s01[p][c] = l01 - c2;
-                    } else {
-                        s00[p][c] += x00 - Cc[c];
-                        s01[p][c] += l01 - c2;
+                    const int xcc = min(max(xc, 0), nc - 1), ycc = min(max(yc, 0), nc - 1);
+                    const T *gw = a.latgeom + ((((int64_t)sd * nc + xcc) * nc + ycc) * 2) * 10;
+    #pragma unroll
+                    for (int i = 0; i < 10; ++i) {
+                        g0[i] = gw[i];
+                        g1[i] = gw[10 + i];
                     }
-                    s00[p + 1][c] = x00 + Cc[c];
-                    s01[p + 1][c] = l01 + c2;
-                    acc_cur[p][c] += l10 - c2;
-                    acc_cur[p + 1][c] += l10 + c2;
-                    acc_next[p][c] += x11 - Cd[c];
-                    acc_next[p + 1][c] += x11 + Cd[c];
                 }
-            }
-        }
-
-        // left corners -> neighbour v-1, combined radially
-        const int buf = step & 1;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const T b00 = wave_shr1(s00[K][c]), b01 = wave_shr1(s01[K][c]);
-#pragma unroll
-            for (int p = 0; p < K; ++p) {
-                // one column per wave: the bound-controlled shift already
-                // gives lane 0 a zero
-                xch[buf][vi][0][p][c][rl] = s00[p][c] + (p == 0 && (R == 64 || rl > 0) ? b00 : T(0));
-                xch[buf][vi][1][p][c][rl] = s01[p][c] + (p == 0 && (R == 64 || rl > 0) ? b01 : T(0));
-            }
-            if (ovf_lane) {
-                xovf[buf][vi][0][c] = s00[K][c];
-                xovf[buf][vi][1][c] = s01[K][c];
-                if constexpr (RG > 1) {
-                    if (!top_chunk) xtop[buf][vi][c] = acc_cur[K][c];
-                }
-            }
-        }
-        __syncthreads();
-        if (v + 1 < VX) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-#pragma unroll
+    #pragma unroll
                 for (int p = 0; p < K; ++p) {
-                    acc_cur[p][c] += xch[buf][vi + 1][0][p][c][rl];
-                    acc_next[p][c] += xch[buf][vi + 1][1][p][c][rl];
+                    const T cvp = cell_ok ? cv[p] : T(0);
+                    T A0[3], A1[3], Cc[3], E0[3], E1[3], Cd[3];
+                    // wedge 0: (c00, c10, c01)
+                    wedge_stress<T>(g0, B0.su[p], B1.su[p], T0.su[p], B0.du[p], B1.du[p], T0.du[p],
+                                    B0.kk[p] + B1.kk[p] + T0.kk[p], qq[p], cvp, A0, A1, Cc);
+                    // wedge 1: (c11, c01, c10)
+                    wedge_stress<T>(g1, T1.su[p], T0.su[p], B1.su[p], T1.du[p], T0.du[p], B1.du[p],
+                                    T1.kk[p] + T0.kk[p] + B1.kk[p], qq[p], cvp, E0, E1, Cd);
+                    // corners: c00 and c11 get one wedge each, c10 and c01 both;
+                    // lateral parts L and radial parts C enter the bottom node as
+                    // L - C and the top node as L + C
+    #pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const T x00 = -(A0[c] + A1[c]), x11 = -(E0[c] + E1[c]);
+                        const T l10 = A0[c] + E1[c], l01 = A1[c] + E0[c], c2 = Cc[c] + Cd[c];
+                        if (p == 0) {
+                            s00[p][c] = x00 - Cc[c];
+                            s01[p][c] = l01 - c2;
+                        } else {
+                            s00[p][c] += x00 - Cc[c];
+                            s01[p][c] += l01 - c2;
+                        }
+                        s00[p + 1][c] = x00 + Cc[c];
+                        s01[p + 1][c] = l01 + c2;
+                        acc_cur[p][c] += l10 - c2;
+                        acc_cur[p + 1][c] += l10 + c2;
+                        acc_next[p][c] += x11 - Cd[c];
+                        acc_next[p + 1][c] += x11 + Cd[c];
+                    }
+                }
+            }
+
+            // left corners -> neighbour v-1, combined radially
+            const int buf = step & 1;
+    #pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const T b00 = wave_shr1(s00[K][c]), b01 = wave_shr1(s01[K][c]);
+    #pragma unroll
+                for (int p = 0; p < K; ++p) {
+                    // one column per wave: the bound-controlled shift already
+                    // gives lane 0 a zero
+                    xch[buf][vi][0][p][c][rl] = s00[p][c] + (p == 0 && (R == 64 || rl > 0) ? b00 : T(0));
+                    xch[buf][vi][1][p][c][rl] = s01[p][c] + (p == 0 && (R == 64 || rl > 0) ? b01 : T(0));
                 }
                 if (ovf_lane) {
-                    acc_cur[K][c] += xovf[buf][vi + 1][0][c];
-                    acc_next[K][c] += xovf[buf][vi + 1][1][c];
-                }
-            }
-        }
-
-        // node row yc is complete: combine radially and store
-        T below[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const T up = wave_shr1(acc_cur[K][c]);
-            below[c] = (R == 64 || rl > 0) ? up : T(0);
-            if constexpr (RG > 1) {
-                // lane 0 of an upper chunk: the lower chunk's top level
-                if (rl == 0 && !bot_chunk && v + 1 < VX)
-                    below[c] = xtop[buf][vi - VX][c] + xovf[buf][vi - VX + 1][0][c];
-            }
-        }
-        const int Y = yc;
-        if (Y >= Y0 && v < VX - 1 && X < nx && col_ok && lane_has) {
-            const int64_t lcol = ((int64_t)sd * nx + X) * nx + Y;
-            const int64_t base = lcol * nr;
-            // unit normal of the output node (free-slip rows, penalty)
-            T cc[3] = {T(0), T(0), T(0)};
-            if (a.pen || (BCM == BCM_RUNTIME ? any_fs : (BCM & 1) != 0)) {
-                if constexpr (R == 64) {
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) cc[c] = lane_value(rec, REC_NOUT + c);
-                } else {
-                    cc[0] = a.coords[3 * lcol];
-                    cc[1] = a.coords[3 * lcol + 1];
-                    cc[2] = a.coords[3 * lcol + 2];
-                }
-            }
-            T pbn[3] = {T(0), T(0), T(0)};
-            if (a.pen) {
-#pragma clang fp contract(off)
-                pbn[0] = fma(pb[1], cc[2], -(pb[2] * cc[1]));
-                pbn[1] = fma(pb[2], cc[0], -(pb[0] * cc[2]));
-                pbn[2] = fma(pb[0], cc[1], -(pb[1] * cc[0]));
-            }
-#pragma unroll
-            for (int p = 0; p < KL; ++p) {
-                if (p == K && !(ovf_lane && top_chunk)) continue;
-                const int l = hc * nch + rl * K + p;
-                T val[3];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) val[c] = acc_cur[p][c] + (p == 0 ? below[c] : T(0));
-                const bool at_cmb = p == 0 && rl == 0 && bot_chunk && has_cmb && a.bc_cmb != TNG_NEUMANN;
-                const bool at_surf = p == K && top_chunk && has_surf && a.bc_surf != TNG_NEUMANN;
-                if (at_cmb || at_surf) {
-                    const int bcv = at_cmb ? a.bc_cmb : a.bc_surf;
-                    T bt[3], u[3];
-                    if constexpr (R == 64) {
-                        // lane 0 takes the CMB entries, the top lane the surface ones
-                        const int off = at_cmb ? 0 : 3;
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) {
-                            bt[c] = lane_fetch(rec, REC_BTAB + off + c);
-                            u[c] = lane_fetch(rec, REC_UCMB + off + c);
-                        }
-                    } else {
-                        const T *btp = a.btab + (lcol * 2 + (at_cmb ? 0 : 1)) * 3;
-                        bt[0] = btp[0];
-                        bt[1] = btp[1];
-                        bt[2] = btp[2];
-                        u[0] = a.src0[base + l];
-                        u[1] = a.src1[base + l];
-                        u[2] = a.src2[base + l];
-                    }
-                    constexpr bool rt = BCM == BCM_RUNTIME, has_d = rt || (BCM & 2), has_f = rt || (BCM & 1);
-                    if (has_d && (!has_f || bcv == TNG_DIRICHLET)) {
-                        val[0] = bt[0] * u[0];
-                        val[1] = bt[1] * u[1];
-                        val[2] = bt[2] * u[2];
-                    } else if (has_f) {
-#pragma clang fp contract(off)
-                        const T dv = fma(cc[2], val[2], fma(cc[1], val[1], cc[0] * val[0]));
-                        const T un = bt[0] * fma(cc[2], u[2], fma(cc[1], u[1], cc[0] * u[0]));
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) val[c] = fma(un, cc[c], fma(-dv, cc[c], val[c]));
+                    xovf[buf][vi][0][c] = s00[K][c];
+                    xovf[buf][vi][1][c] = s01[K][c];
+                    if constexpr (RG > 1) {
+                        if (!top_chunk) xtop[buf][vi][c] = acc_cur[K][c];
                     }
                 }
-                if (a.pen) {
-#pragma clang fp contract(off)
-                    // b x (r n) = r (b x n), b x n once per output column
-                    const T rad = radl[p];
-                    val[0] = fma(rad, pbn[0], val[0]);
-                    val[1] = fma(rad, pbn[1], val[1]);
-                    val[2] = fma(rad, pbn[2], val[2]);
-                }
-                if (a.add) {
-                    val[0] += a.dst0[base + l];
-                    val[1] += a.dst1[base + l];
-                    val[2] += a.dst2[base + l];
-                }
-                if constexpr (FUSE) {
-                    // columns off the subdomain's lateral edges are not shared
-                    // (every subdomain spans the whole shell radially here)
-                    if (fuse_here) {
-                        if (p == K) {
-                            // the top lane's extra (surface) level
-                            T in[4][3];
-#pragma unroll
-                            for (int c = 0; c < 3; ++c) {
-                                in[0][c] = lane_value(rec, REC_SB + c);
-                                in[1][c] = lane_value(rec, REC_SI + c);
-                                in[2][c] = a.cread_d ? lane_value(rec, REC_SD + c) : T(0);
-                                in[3][c] = lane_value(rec, REC_USURF + c);
-                            }
-                            cheb_update<T>(a, base + l, val, in);
-                        } else if constexpr (early) {
-                            cheb_update<T>(a, base + l, val, cin);
-                        } else {
-                            T in[4][3];
-                            cheb_load<T>(a, base + l, in);
-                            cheb_update<T>(a, base + l, val, in);
-                        }
-                        continue;
-                    }
-                }
-                a.dst0[base + l] = val[0];
-                a.dst1[base + l] = val[1];
-                a.dst2[base + l] = val[2];
             }
-        }
-#pragma unroll
-        for (int i = 0; i < KL; ++i)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) acc_cur[i][c] = T(0);
-    };
-
-    const int nsteps = Yend - (Y0 - 1);
-    NodeCol<T, K> C0, C1;
-    T rec2 = T(0);
-    if constexpr (R == 64 && K == 1) {
-        // two steps per iteration with alternating roles (no register copies)
-        for (int step = 0; step < nsteps; step += 2) {
-            do_step(step, B0, B1, C0, C1, accA, accB, rec, rec2);
-            if (step + 1 < nsteps) do_step(step + 1, C0, C1, B0, B1, accB, accA, rec2, rec);
-        }
-    } else {
-        for (int step = 0; step < nsteps; ++step) {
-            do_step(step, B0, B1, C0, C1, accA, accB, rec, rec2);
-            B0 = C0;
-            B1 = C1;
-            rec = rec2;
-#pragma unroll
-            for (int i = 0; i < KL; ++i)
-#pragma unroll
+            __syncthreads();
+            if (v + 1 < VX) {
+    #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    accA[i][c] = accB[i][c];
-                    accB[i][c] = T(0);
+    #pragma unroll
+                    for (int p = 0; p < K; ++p) {
+                        acc_cur[p][c] += xch[buf][vi + 1][0][p][c][rl];
+                        acc_next[p][c] += xch[buf][vi + 1][1][p][c][rl];
+                    }
+                    if (ovf_lane) {
+                        acc_cur[K][c] += xovf[buf][vi + 1][0][c];
+                        acc_next[K][c] += xovf[buf][vi + 1][1][c];
+                    }
                 }
+            }
+
+            // node row yc is complete: combine radially and store
+            T below[3];
+    #pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const T up = wave_shr1(acc_cur[K][c]);
+                below[c] = (R == 64 || rl > 0) ? up : T(0);
+                if constexpr (RG > 1) {
+                    // lane 0 of an upper chunk: the lower chunk's top level
+                    if (rl == 0 && !bot_chunk && v + 1 < VX)
+                        below[c] = xtop[buf][vi - VX][c] + xovf[buf][vi - VX + 1][0][c];
+                }
+            }
+            const int Y = yc;
+            if (Y >= Y0 && v < VX - 1 && X < nx && col_ok && lane_has) {
+                const int64_t lcol = ((int64_t)sd * nx + X) * nx + Y;
+                const int64_t base = lcol * nr;
+                // unit normal of the output node (free-slip rows, penalty)
+                T cc[3] = {T(0), T(0), T(0)};
+                if (a.pen || (BCM == BCM_RUNTIME ? any_fs : (BCM & 1) != 0)) {
+                    if constexpr (R == 64) {
+    #pragma unroll
+                        for (int c = 0; c < 3; ++c) cc[c] = lane_value(rec, REC_NOUT + c);
+                    } else {
+                        cc[0] = a.coords[3 * lcol];
+                        cc[1] = a.coords[3 * lcol + 1];
+                        cc[2] = a.coords[3 * lcol + 2];
+                    }
+                }
+                T pbn[3] = {T(0), T(0), T(0)};
+                if (a.pen) {
+    #pragma clang fp contract(off)
+                    pbn[0] = fma(pb[1], cc[2], -(pb[2] * cc[1]));
+                    pbn[1] = fma(pb[2], cc[0], -(pb[0] * cc[2]));
+                    pbn[2] = fma(pb[0], cc[1], -(pb[1] * cc[0]));
+                }
+    #pragma unroll
+                for (int p = 0; p < KL; ++p) {
+                    if (p == K && !(ovf_lane && top_chunk)) continue;
+                    const int l = hc * nch + rl * K + p;
+                    T val[3];
+    #pragma unroll
+                    for (int c = 0; c < 3; ++c) val[c] = acc_cur[p][c] + (p == 0 ? below[c] : T(0));
+                    const bool at_cmb = p == 0 && rl == 0 && bot_chunk && has_cmb && a.bc_cmb != TNG_NEUMANN;
+                    const bool at_surf = p == K && top_chunk && has_surf && a.bc_surf != TNG_NEUMANN;
+                    if (at_cmb || at_surf) {
+                        const int bcv = at_cmb ? a.bc_cmb : a.bc_surf;
+                        T bt[3], u[3];
+                        if constexpr (R == 64) {
+                            // lane 0 takes the CMB entries, the top lane the surface ones
+                            const int off = at_cmb ? 0 : 3;
+    #pragma unroll
+                            for (int c = 0; c < 3; ++c) {
+                                bt[c] = lane_fetch(rec, REC_BTAB + off + c);
+                                u[c] = lane_fetch(rec, REC_UCMB + off + c);
+                            }
+                        } else {
+                            const T *btp = a.btab + (lcol * 2 + (at_cmb ? 0 : 1)) * 3;
+                            bt[0] = btp[0];
+                            bt[1] = btp[1];
+                            bt[2] = btp[2];
+                            u[0] = a.src0[base + l];
+                            u[1] = a.src1[base + l];
+                            u[2] = a.src2[base + l];
+                        }
+                        constexpr bool rt = BCM == BCM_RUNTIME, has_d = rt || (BCM & 2), has_f = rt || (BCM & 1);
+                        if (has_d && (!has_f || bcv == TNG_DIRICHLET)) {
+                            val[0] = bt[0] * u[0];
+                            val[1] = bt[1] * u[1];
+                            val[2] = bt[2] * u[2];
+                        } else if (has_f) {
+    #pragma clang fp contract(off)
+                            const T dv = fma(cc[2], val[2], fma(cc[1], val[1], cc[0] * val[0]));
+                            const T un = bt[0] * fma(cc[2], u[2], fma(cc[1], u[1], cc[0] * u[0]));
+    #pragma unroll
+                            for (int c = 0; c < 3; ++c) val[c] = fma(un, cc[c], fma(-dv, cc[c], val[c]));
+                        }
+                    }
+                    if (a.pen) {
+    #pragma clang fp contract(off)
+                        // b x (r n) = r (b x n), b x n once per output column
+                        const T rad = radl[p];
+                        val[0] = fma(rad, pbn[0], val[0]);
+                        val[1] = fma(rad, pbn[1], val[1]);
+                        val[2] = fma(rad, pbn[2], val[2]);
+                    }
+                    if (a.add) {
+                        val[0] += a.dst0[base + l];
+                        val[1] += a.dst1[base + l];
+                        val[2] += a.dst2[base + l];
+                    }
+                    if constexpr (FUSE) {
+                        // columns off the subdomain's lateral edges are not shared
+                        // (every subdomain spans the whole shell radially here)
+                        if (fuse_here) {
+                            if (p == K) {
+                                // the top lane's extra (surface) level
+                                T in[4][3];
+    #pragma unroll
+                                for (int c = 0; c < 3; ++c) {
+                                    in[0][c] = lane_value(rec, REC_SB + c);
+                                    in[1][c] = lane_value(rec, REC_SI + c);
+                                    in[2][c] = a.cread_d ? lane_value(rec, REC_SD + c) : T(0);
+                                    in[3][c] = lane_value(rec, REC_USURF + c);
+                                }
+                                cheb_update<T>(a, base + l, val, in);
+                            } else if constexpr (early) {
+                                cheb_update<T>(a, base + l, val, cin);
+                            } else {
+                                T in[4][3];
+                                cheb_load<T>(a, base + l, in);
+                                cheb_update<T>(a, base + l, val, in);
+                            }
+                            continue;
+                        }
+                    }
+                    a.dst0[base + l] = val[0];
+                    a.dst1[base + l] = val[1];
+                    a.dst2[base + l] = val[2];
+                }
+            }
+    #pragma unroll
+            for (int i = 0; i < KL; ++i)
+    #pragma unroll
+                for (int c = 0; c < 3; ++c) acc_cur[i][c] = T(0);
+        };
+
+        const int nsteps = Yend - (Y0 - 1);
+        NodeCol<T, K> C0, C1;
+        T rec2 = T(0);
+        if constexpr (R == 64 && K == 1) {
+            // two steps per iteration with alternating roles (no register copies)
+            for (int step = 0; step < nsteps; step += 2) {
+                do_step(step, B0, B1, C0, C1, accA, accB, rec, rec2);
+                if (step + 1 < nsteps) do_step(step + 1, C0, C1, B0, B1, accB, accA, rec2, rec);
+            }
+        } else {
+            for (int step = 0; step < nsteps; ++step) {
+                do_step(step, B0, B1, C0, C1, accA, accB, rec, rec2);
+                B0 = C0;
+                B1 = C1;
+                rec = rec2;
+    #pragma unroll
+                for (int i = 0; i < KL; ++i)
+    #pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        accA[i][c] = accB[i][c];
+                        accB[i][c] = T(0);
+                    }
+            }
         }
-    }
+        if constexpr (BAL) __syncthreads();  // the next sweep reuses the exchange buffers
+    } while (BAL && q < q1);
 }
 
 // ----------------------------------------------------------------------------
@@ -1054,9 +1075,9 @@ int resident_groups(const void *kernel, int threads)
     return std::max(1, per_cu) * std::max(1, cus);
 }
 
-// y-segment length.  A workgroup sweeps seg rows plus one halo row, after a
-// prologue of about 1.5 row-steps (the first two rows' loads); the grid runs
-// in ceil(blocks / resident) rounds.  Pick the length with the least
+// One-segment launches: a workgroup sweeps seg rows plus one warm-up row,
+// after a prologue of about 1.5 row-steps; the grid runs in
+// ceil(blocks / resident) rounds.  Pick the length with the least
 // rounds * (seg + 2.5).
 int pick_segment(int nx, int64_t lateral_groups, int resident)
 {
@@ -1085,11 +1106,22 @@ void launch_apply_rk(ApplyArgs<T> a, int64_t lateral_groups, hipStream_t s)
         cached_resident = resident_groups(kern, NW * 64);
         cached_dev = dev;
     }
-    int seg = env_int("TNG_APPLY_SEG", 0);
-    if (seg <= 0) seg = pick_segment(a.nx, lateral_groups, cached_resident);
-    a.seg_len = seg;
-    a.segs = (a.nx + seg - 1) / seg;
-    const int blocks = (int)(lateral_groups * a.segs);
+    a.groups = (int)lateral_groups;
+    int blocks;
+    if (sizeof(T) == 8 && !FUSE) {
+        // one round of resident workgroups, each sweeping an equal share of
+        // the node rows (TNG_APPLY_BLOCKS overrides the count)
+        const int64_t rows = lateral_groups * a.nx;
+        int64_t want = env_int("TNG_APPLY_BLOCKS", 0);
+        if (want <= 0) want = cached_resident;
+        blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, rows));
+    } else {
+        int seg = env_int("TNG_APPLY_SEG", 0);
+        if (seg <= 0) seg = pick_segment(a.nx, lateral_groups, cached_resident);
+        a.seg_len = seg;
+        a.segs = (a.nx + seg - 1) / seg;
+        blocks = (int)(lateral_groups * a.segs);
+    }
     hipLaunchKernelGGL((epsdivdiv_apply_kernel<T, R, K, NW, RG, BCM, FUSE>), dim3(blocks), dim3(NW * 64), 0, s, a);
 }
 
