@@ -33,7 +33,8 @@ typedef enum {
     TNG_DEVICE_ERROR = 3,
     TNG_NOT_CONVERGED = 4,
     TNG_NONFINITE = 5,
-    TNG_UNSUPPORTED = 6
+    TNG_UNSUPPORTED = 6,
+    TNG_IO_ERROR = 7
 } tng_status;
 
 typedef enum { TNG_F64 = 0, TNG_F32 = 1 } tng_dtype;
@@ -49,6 +50,7 @@ typedef struct tng_context *tng_ctx;
 typedef struct tng_domain *tng_mesh;
 typedef struct tng_vector *tng_vec;
 typedef struct tng_operator *tng_op;
+typedef struct tng_xdmf_writer *tng_xdmf;
 
 /* ---------------------------------------------------------------- context */
 
@@ -357,6 +359,36 @@ tng_status tng_stokes_adjoint_gradient(tng_stokes_solver solver, tng_vec u, tng_
  * applies and the average durations since the last read, and resets. */
 tng_status tng_ctx_set_profiling(tng_ctx ctx, int on);
 tng_status tng_ctx_profile_read(tng_ctx ctx, int *num_applies, double *avg_main_kernel_ms, double *avg_total_ms);
+
+/* ------------------------------------------------------ XDMF output / checkpoints */
+
+/* io/xdmf.hpp:148-201 XDMFOutput(directory, domain, coords_shell, coords_radii,
+ * output_type_points, output_type_connectivity): a writer of node fields of
+ * `mesh` into `directory` (created on the first write if missing; existing
+ * files are overwritten).  points_bytes: 4 or 8 (node coordinates, default of
+ * the reference 4); connectivity_bytes: 4 or 8 (default 8). */
+tng_status tng_xdmf_create(tng_mesh mesh, const char *directory, int points_bytes, int connectivity_bytes,
+                           tng_xdmf *out);
+/* io/xdmf.hpp:208-281 add(data, output_type): a scalar or 3-component field,
+ * written with `label` (the reference's Kokkos::View label) in output_bytes
+ * (4 or 8) precision on every write().  Not allowed after the first write(). */
+tng_status tng_xdmf_add(tng_xdmf writer, tng_vec field, const char *label, int output_bytes);
+/* io/xdmf.hpp:203-207 set_write_counter */
+tng_status tng_xdmf_set_write_counter(tng_xdmf writer, int counter);
+/* io/xdmf.hpp:283-513 write(): first call geometry.bin, topology.bin,
+ * checkpoint_metadata.bin, README.md; every call <label>_<step>.bin per field
+ * and step_<step>.xmf last.  The fields are transposed to the file layout on
+ * the device.  pack_seconds / total_seconds (either may be NULL) receive the
+ * device time of the layout kernels and the wall time of the call. */
+tng_status tng_xdmf_write(tng_xdmf writer, double *pack_seconds, double *total_seconds);
+tng_status tng_xdmf_destroy(tng_xdmf writer);
+/* io/xdmf.hpp:1199-1441 read_xdmf_checkpoint_grid(directory, label, step,
+ * domain, grid): reads field `label` of write step `step` into `out` (scalar
+ * or 3 components, FP64 or FP32), locating every subdomain of out's mesh by
+ * its global subdomain id in the checkpoint metadata (:1094-1186).  The
+ * checkpoint must have the mesh's subdomain sizes.  TNG_IO_ERROR on a
+ * missing or malformed file. */
+tng_status tng_xdmf_checkpoint_read(const char *directory, const char *label, int step, tng_vec out);
 
 #ifdef __cplusplus
 }

@@ -87,6 +87,7 @@ class Mesh:
         self.shape = (self.nsd, self.nx, self.nx, self.nr)
         self.lat_level, self.rad_level = lat_level, rad_level
         self.r_min, self.r_max = r_min, r_max
+        self.lat_sdr, self.rad_sdr = lat_sdr, rad_sdr
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

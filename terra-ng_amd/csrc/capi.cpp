@@ -684,6 +684,64 @@ tng_status tng_gca_assemble_selective(tng_op fine, tng_op coarse, int level_rang
     });
 }
 
+tng_status tng_xdmf_create(tng_mesh mesh, const char *directory, int points_bytes, int connectivity_bytes,
+                           tng_xdmf *out)
+{
+    if (!mesh || !mesh->ctx || !directory || !out) return TNG_INVALID_ARGUMENT;
+    return guarded(mesh->ctx, [&] {
+        TNG_REQUIRE(points_bytes == 4 || points_bytes == 8, "XDMF: points_bytes must be 4 or 8");
+        TNG_REQUIRE(connectivity_bytes == 4 || connectivity_bytes == 8, "XDMF: connectivity_bytes must be 4 or 8");
+        std::unique_ptr<tng_xdmf_writer> w(new tng_xdmf_writer());
+        w->mesh = mesh;
+        w->dir = directory;
+        w->points_bytes = points_bytes;
+        w->conn_bytes = connectivity_bytes;
+        *out = w.release();
+    });
+}
+
+tng_status tng_xdmf_add(tng_xdmf writer, tng_vec field, const char *label, int output_bytes)
+{
+    if (!writer || !field || !label) return TNG_INVALID_ARGUMENT;
+    return guarded(writer->mesh->ctx, [&] { writer->add(field, label, output_bytes); });
+}
+
+tng_status tng_xdmf_set_write_counter(tng_xdmf writer, int counter)
+{
+    if (!writer || counter < 0) return TNG_INVALID_ARGUMENT;
+    writer->counter = counter;
+    return TNG_OK;
+}
+
+tng_status tng_xdmf_write(tng_xdmf writer, double *pack_seconds, double *total_seconds)
+{
+    if (!writer) return TNG_INVALID_ARGUMENT;
+    return guarded(writer->mesh->ctx, [&] {
+        set_device(writer->mesh->ctx);
+        writer->write();
+        if (pack_seconds) *pack_seconds = writer->last_pack_seconds;
+        if (total_seconds) *total_seconds = writer->last_write_seconds;
+    });
+}
+
+tng_status tng_xdmf_destroy(tng_xdmf writer)
+{
+    // like the other destroy calls: touches neither the mesh nor the context,
+    // which may already be gone when a garbage collector finalises in any order
+    if (!writer) return TNG_INVALID_ARGUMENT;
+    delete writer;
+    return TNG_OK;
+}
+
+tng_status tng_xdmf_checkpoint_read(const char *directory, const char *label, int step, tng_vec out)
+{
+    if (!directory || !label || !out || step < 0) return TNG_INVALID_ARGUMENT;
+    return guarded(out->mesh->ctx, [&] {
+        set_device(out->mesh->ctx);
+        xdmf_checkpoint_read(directory, label, step, *out);
+    });
+}
+
 tng_status tng_op_copy_local_matrices(tng_op dst, tng_op src)
 {
     if (!dst || !src || dst->kind != OP_EPSDIVDIV || src->kind != OP_EPSDIVDIV) return TNG_INVALID_ARGUMENT;

@@ -23,6 +23,15 @@ namespace tng {
 
 namespace {
 
+// Logical block index with contiguous ranges per XCD: the hardware deals
+// blocks round-robin to the 8 XCDs, each with its own L2, and neighbouring
+// node columns (which read the same cells) sit in neighbouring blocks.
+__device__ __forceinline__ uint32_t xcd_block()
+{
+    const uint32_t nb = gridDim.x, x = blockIdx.x & 7, i = blockIdx.x >> 3, q = nb >> 3, r = nb & 7;
+    return x * q + min(x, r) + i;
+}
+
 __host__ __device__ __forceinline__ int fine_lat_idx(int x, int y, int w)
 {
     // [w][y%2][x%2]
@@ -98,7 +107,7 @@ template <typename T> __global__ __launch_bounds__(256) void div_partial_kernel(
 {
     const int ncc = a.nxc - 1, nrcc = a.nrc - 1;
     const uint32_t n = (uint32_t)a.nsd * ncc * ncc * nrcc;
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t gid = xcd_block() * blockDim.x + threadIdx.x;
     if (gid >= n) return;
     int sd, xc, yc, rc;
     a.cidx.decode(gid, sd, xc, yc, rc);
@@ -151,7 +160,7 @@ __global__ __launch_bounds__(256) void wedge_gather_kernel(const T *partial, T *
                                                            const Index4 idx)
 {
     const uint32_t n = (uint32_t)nsd * nx * nx * nr;
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t gid = xcd_block() * blockDim.x + threadIdx.x;
     if (gid >= n) return;
     int sd, X, Y, l;
     idx.decode(gid, sd, X, Y, l);
@@ -196,7 +205,7 @@ template <typename T> __global__ __launch_bounds__(256) void grad_weight_kernel(
 {
     const int nx = a.nx, nr = a.nr, nc = nx - 1, ncr = nr - 1;
     const uint32_t n = (uint32_t)a.nsd * nc * nc * ncr;
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t gid = xcd_block() * blockDim.x + threadIdx.x;
     if (gid >= n) return;
     int sd, xc, yc, rc;
     a.fcell.decode(gid, sd, xc, yc, rc);
@@ -219,14 +228,42 @@ template <typename T> __global__ __launch_bounds__(256) void grad_weight_kernel(
 
 // Phase 2 of B^T per fine node: sum over the <= 12 fine wedges containing the
 // node of f_w g_node, with g_node = J^{-T} grad N at the centroid.
-template <typename T> __global__ __launch_bounds__(256) void gradient_kernel(const GradArgs<T> a, const T *fw)
+// STAGE (columns of >= 52 nodes): the block's threads cover at most
+// kGradCols node columns; the wedge geometry of the 4 cells around each is
+// staged in LDS once and read there by every node of the column (broadcast),
+// instead of 72 vector loads per node.
+constexpr int kGradCols = 6;
+template <typename T, bool STAGE>
+__global__ __launch_bounds__(256) void gradient_kernel(const GradArgs<T> a, const T *fw)
 {
+    __shared__ T sg[STAGE ? kGradCols : 1][4][2][9];
     const int nx = a.nx, nr = a.nr, nc = nx - 1, ncr = nr - 1;
     const uint32_t n = (uint32_t)a.nsd * nx * nx * nr;
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t b0 = xcd_block() * blockDim.x;
+    const uint32_t gid = b0 + threadIdx.x;
+    uint32_t col0 = 0;
+    if constexpr (STAGE) {
+        col0 = a.fnode.r.div(b0);
+        const uint32_t ncol = (uint32_t)a.nsd * nx * nx;
+        for (int e = threadIdx.x; e < kGradCols * 72; e += blockDim.x) {
+            const int slot = e / 72, rest = e - slot * 72, cell = rest / 18, wv = rest - cell * 18;
+            const uint32_t col = col0 + slot;
+            T val = T(0);
+            if (col < ncol) {
+                const uint32_t t2 = a.fnode.x.div(col), s3 = a.fnode.x.div(t2);
+                const int cy = (int)(col - t2 * a.fnode.x.d), cx = (int)(t2 - s3 * a.fnode.x.d);
+                const int xc = cx + (cell >> 1) - 1, yc = cy + (cell & 1) - 1;
+                if (xc >= 0 && xc < nc && yc >= 0 && yc < nc)
+                    val = a.latgeom[(((int64_t)s3 * nc + xc) * nc + yc) * 20 + (wv / 9) * 10 + wv % 9];
+            }
+            sg[slot][cell][wv / 9][wv % 9] = val;
+        }
+        __syncthreads();
+    }
     if (gid >= n) return;
     int sd, X, Y, l;
     a.fnode.decode(gid, sd, X, Y, l);
+    const int slot = STAGE ? (int)(a.fnode.r.div(gid) - col0) : 0;
     const int64_t lcol = ((int64_t)sd * nx + X) * nx + Y;
     const int corner_of[2][2][2] = {{{0, 2}, {1, -1}}, {{-1, 1}, {2, 0}}};
     const bool has_cmb = a.sd_cmb[sd] != 0, has_surf = a.sd_surf[sd] != 0;
@@ -260,7 +297,15 @@ template <typename T> __global__ __launch_bounds__(256) void gradient_kernel(con
             for (int w = 0; w < 2; ++w) {
                 const int k = corner_of[w][X - xc][Y - yc];
                 if (k < 0) continue;
-                const T *g = a.latgeom + ((((int64_t)sd * nc + xc) * nc + yc) * 2 + w) * 10;
+                T g[9];
+                if constexpr (STAGE) {
+#pragma unroll
+                    for (int v = 0; v < 9; ++v) g[v] = sg[slot][(dxc + 1) * 2 + (dyc + 1)][w][v];
+                } else {
+                    const T *gp = a.latgeom + ((((int64_t)sd * nc + xc) * nc + yc) * 2 + w) * 10;
+#pragma unroll
+                    for (int v = 0; v < 9; ++v) g[v] = gp[v];
+                }
                 const T l0 = k == 0 ? T(-0.5) : (k == 1 ? T(0.5) : T(0));
                 const T l1 = k == 0 ? T(-0.5) : (k == 1 ? T(0) : T(0.5));
                 const T gl[3] = {l0 * g[0] + l1 * g[3], l0 * g[1] + l1 * g[4], l0 * g[2] + l1 * g[5]};
@@ -295,7 +340,7 @@ __global__ __launch_bounds__(256) void prolongate_kernel(const Comp3<T> src, con
                                                          int nsd, int nxf, int nrf, int nxc, int nrc, const Index4 idx)
 {
     const uint32_t n = (uint32_t)nsd * nxf * nxf * nrf;
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t gid = xcd_block() * blockDim.x + threadIdx.x;
     if (gid >= n) return;
     int sd, xf, yf, rf;
     idx.decode(gid, sd, xf, yf, rf);
@@ -328,7 +373,7 @@ __global__ __launch_bounds__(256) void restrict_kernel(const Comp3<T> src, const
                                                        int nrc, const Index4 idx)
 {
     const uint32_t n = (uint32_t)nsd * nxc * nxc * nrc;
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t gid = xcd_block() * blockDim.x + threadIdx.x;
     if (gid >= n) return;
     int sd, xc, yc, rc;
     idx.decode(gid, sd, xc, yc, rc);
@@ -546,8 +591,13 @@ template <typename T> void grad_impl(tng_operator &op, const tng_vector &src, tn
     const size_t bytes = (size_t)nfw * sizeof(T);
     if (op.scratch.bytes < bytes) op.scratch.alloc(bytes);
     hipLaunchKernelGGL(grad_weight_kernel<T>, dim3(blocks_for(nfw / 2)), dim3(256), 0, s, a, op.scratch.as<T>());
-    hipLaunchKernelGGL(gradient_kernel<T>, dim3(blocks_for(f.nalloc)), dim3(256), 0, s, a,
-                       (const T *)op.scratch.p);
+    // a block of 256 threads spans at most 256 / nr + 2 node columns
+    if (f.nr >= 52)
+        hipLaunchKernelGGL((gradient_kernel<T, true>), dim3(blocks_for(f.nalloc)), dim3(256), 0, s, a,
+                           (const T *)op.scratch.p);
+    else
+        hipLaunchKernelGGL((gradient_kernel<T, false>), dim3(blocks_for(f.nalloc)), dim3(256), 0, s, a,
+                           (const T *)op.scratch.p);
     TNG_HIP_CHECK(hipGetLastError());
 }
 

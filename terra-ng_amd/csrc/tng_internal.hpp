@@ -176,6 +176,43 @@ struct tng_vector {
 };
 
 namespace tng {
+// Pinned host staging memory (hipHostMalloc), grown on demand.
+struct PinnedBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    void reserve(size_t n);
+    void release();
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf &) = delete;
+    PinnedBuf &operator=(const PinnedBuf &) = delete;
+    ~PinnedBuf() { release(); }
+};
+}  // namespace tng
+
+// XDMF output of node fields of one mesh (io/xdmf.hpp:148-513).
+struct tng_xdmf_writer {
+    struct Item {
+        tng_vector *v;
+        std::string label;
+        int bytes;
+    };
+    tng_domain *mesh = nullptr;
+    std::string dir;
+    int points_bytes = 4, conn_bytes = 8;
+    std::vector<Item> items;
+    int counter = 0;
+    bool first_done = false;
+    tng::DevBuf dev;
+    tng::PinnedBuf host;
+    double last_pack_seconds = 0.0, last_write_seconds = 0.0;
+    std::vector<const Item *> ordered() const;
+    void add(tng_vector *v, const std::string &label, int bytes);
+    void emit(const std::string &path, size_t bytes, hipStream_t s);
+    void write_metadata();
+    void write();
+};
+
+namespace tng {
 // Component pointers passed to kernels by value.
 template <typename T> struct Comp3 {
     T *p[3];
@@ -235,6 +272,11 @@ void epsdivdiv_apply(tng_operator &op, const tng_vector &src, tng_vector &dst, h
 bool epsdivdiv_apply_cheb(tng_operator &op, const tng_vector &x, tng_vector &ax, tng_vector &x_out, tng_vector &d,
                           const tng_vector &b, const tng_vector &invd, double az, double ad, bool read_d,
                           hipStream_t s);
+// XDMF field layout: pack (device layout -> file layout, out_bytes 4 or 8)
+// and unpack; checkpoint read of one field (io/xdmf.hpp:1199-1441).
+void xdmf_pack(const tng_vector &v, void *dst_device, int out_bytes, hipStream_t s);
+void xdmf_unpack(const void *src_device, int in_bytes, tng_vector &v, hipStream_t s);
+void xdmf_checkpoint_read(const std::string &dir, const std::string &label, int step, tng_vector &v);
 void launch_halo_cheb(const tng_domain &m, const tng_vector &ax, const tng_vector &b, const tng_vector &invd,
                       tng_vector &d, const tng_vector &x, tng_vector &x_out, double az, double ad, bool read_d,
                       hipStream_t s);

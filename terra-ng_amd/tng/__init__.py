@@ -30,7 +30,8 @@ F64, F32 = 0, 1
 INNER, CMB, SURFACE, OWNED = 1, 6, 10, 1
 
 _STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "DEVICE_ERROR", 4: "NOT_CONVERGED",
-           5: "NONFINITE", 6: "UNSUPPORTED"}
+           5: "NONFINITE", 6: "UNSUPPORTED", 7: "IO_ERROR"}
+INVALID_ARGUMENT, IO_ERROR = 1, 7
 
 
 class TngError(RuntimeError):
@@ -100,6 +101,12 @@ EXPORTS = [
     ("tng_op_copy_local_matrices", _i, [_vp, _vp]),
     ("tng_gca_assemble", _i, [_vp, _vp]),
     ("tng_gca_elements_collect", _i, [_vp, _i, _vp, _vp]),
+    ("tng_xdmf_create", _i, [_vp, C.c_char_p, _i, _i, _pv]),
+    ("tng_xdmf_add", _i, [_vp, _vp, C.c_char_p, _i]),
+    ("tng_xdmf_set_write_counter", _i, [_vp, _i]),
+    ("tng_xdmf_write", _i, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    ("tng_xdmf_destroy", _i, [_vp]),
+    ("tng_xdmf_checkpoint_read", _i, [C.c_char_p, C.c_char_p, _i, _vp]),
     ("tng_gca_assemble_selective", _i, [_vp, _vp, _i, _vp, _vp]),
     ("tng_epsdivdiv_log_coefficient_gradient", _i, [_vp, _vp, _vp, _vp]),
     ("tng_stokes_adjoint_gradient", _i, [_vp, _vp, _vp, _vp, _vp]),
@@ -424,6 +431,51 @@ def gca_assemble_selective(fine_op, coarse_op, level_range, domain_coarsest, fla
         raise ValueError(f"flags must have shape {_cell_shape(domain_coarsest)}, got {f.shape}")
     coarse_op.domain.ctx.check(load_library().tng_gca_assemble_selective(
         fine_op.h, coarse_op.h, int(level_range), domain_coarsest.h, f.ctypes.data_as(C.c_void_p)))
+
+
+class XDMFOutput:
+    """io/xdmf.hpp:148-513 XDMFOutput: node fields of one mesh written for XDMF readers and as checkpoints.
+
+    Fields are added once (``add``), then every ``write()`` writes one step; the first also writes the
+    geometry, the wedge topology and the checkpoint metadata."""
+
+    def __init__(self, domain, directory, points_bytes=4, connectivity_bytes=8):
+        self.domain = domain
+        self._fields = []  # keeps the added vectors alive
+        h = C.c_void_p()
+        domain.ctx.check(load_library().tng_xdmf_create(domain.h, os.fsencode(directory), int(points_bytes),
+                                                        int(connectivity_bytes), C.byref(h)))
+        self.h = h
+
+    def add(self, field, label, output_bytes=4):
+        self.domain.ctx.check(load_library().tng_xdmf_add(self.h, field.h, label.encode(), int(output_bytes)))
+        self._fields.append(field)
+
+    def set_write_counter(self, counter):
+        self.domain.ctx.check(load_library().tng_xdmf_set_write_counter(self.h, int(counter)))
+
+    def write(self):
+        """Writes one step; returns (device seconds of the layout kernels, wall seconds of the call)."""
+        pack, total = C.c_double(), C.c_double()
+        self.domain.ctx.check(load_library().tng_xdmf_write(self.h, C.byref(pack), C.byref(total)))
+        return pack.value, total.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            load_library().tng_xdmf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def read_xdmf_checkpoint(directory, label, step, out):
+    """io/xdmf.hpp:1199-1441 read_xdmf_checkpoint_grid into the vector ``out`` (its mesh must match)."""
+    out.domain.ctx.check(load_library().tng_xdmf_checkpoint_read(os.fsencode(directory), label.encode(), int(step),
+                                                                 out.h))
 
 
 class DivergenceKerngen(_Operator):

@@ -18,13 +18,14 @@ N, D, F = tng.NEUMANN, tng.DIRICHLET, tng.FREESLIP
 _cache = {}
 
 
-def levels(L, ls=0, rs=0):
-    key = (L, ls, rs)
+def levels(L, ls=0, rs=0, R=None):
+    R = L if R is None else R
+    key = (L, ls, rs, R)
     if key not in _cache:
-        df = tng.DistributedDomain.create_uniform(L, L, 0.55, 1.0, ls, rs)
-        dc = tng.DistributedDomain.create_uniform(L - 1, L - 1, 0.55, 1.0, ls, rs)
-        of = O.Mesh(L, L, 0.55, 1.0, ls, rs)
-        oc = O.Mesh(L - 1, L - 1, 0.55, 1.0, ls, rs)
+        df = tng.DistributedDomain.create_uniform(L, R, 0.55, 1.0, ls, rs)
+        dc = tng.DistributedDomain.create_uniform(L - 1, R - 1, 0.55, 1.0, ls, rs)
+        of = O.Mesh(L, R, 0.55, 1.0, ls, rs)
+        oc = O.Mesh(L - 1, R - 1, 0.55, 1.0, ls, rs)
         _cache[key] = (df, dc, of, oc)
     return _cache[key]
 
@@ -43,12 +44,14 @@ def vec(dom, ncomp, data=None, dtype=tng.F64):
 
 BC_SETS = [(N, N), (D, D), (F, F), (D, F), (F, N)]
 SHAPES = [(3, 0, 0), (4, 0, 0), (3, 1, 1), (4, 1, 0)]
+# tall columns (65 and 129 nodes per subdomain column): the LDS-staged kernels
+TALL = [(3, 0, 0, 6), (3, 0, 0, 7), (3, 1, 1, 7), (2, 0, 0, 7)]
 
 
 @pytest.mark.parametrize("bcs", BC_SETS)
-@pytest.mark.parametrize("L,ls,rs", SHAPES)
-def test_divergence(bcs, L, ls, rs, gpu_ctx):
-    df, dc, of, oc = levels(L, ls, rs)
+@pytest.mark.parametrize("L,ls,rs,R", [s + (None,) for s in SHAPES] + TALL)
+def test_divergence(bcs, L, ls, rs, R, gpu_ctx):
+    df, dc, of, oc = levels(L, ls, rs, R)
     u = field(of, 3, 1)
     op = tng.DivergenceKerngen(df, dc, bcs)
     out = vec(dc, 1)
@@ -58,9 +61,9 @@ def test_divergence(bcs, L, ls, rs, gpu_ctx):
 
 
 @pytest.mark.parametrize("bcs", BC_SETS)
-@pytest.mark.parametrize("L,ls,rs", SHAPES)
-def test_gradient(bcs, L, ls, rs, gpu_ctx):
-    df, dc, of, oc = levels(L, ls, rs)
+@pytest.mark.parametrize("L,ls,rs,R", [s + (None,) for s in SHAPES] + TALL)
+def test_gradient(bcs, L, ls, rs, R, gpu_ctx):
+    df, dc, of, oc = levels(L, ls, rs, R)
     p = field(oc, 1, 2)
     op = tng.GradientKerngen(df, dc, bcs)
     out = vec(df, 3)
