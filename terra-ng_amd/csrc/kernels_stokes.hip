@@ -57,6 +57,8 @@ template <typename T> __device__ __forceinline__ T shape_coarse_c(int i, int lat
 // wedge corner (dx, dy) per wedge and local lateral index
 __constant__ int kWC[2][3][2] = {{{0, 0}, {1, 0}, {0, 1}}, {{1, 1}, {0, 1}, {1, 0}}};
 
+template <typename T> using T2v = T __attribute__((ext_vector_type(2)));
+
 template <typename T> struct DivArgs {
     const T *u0, *u1, *u2;       // fine velocity
     const T *coords, *radii;     // fine mesh
@@ -67,6 +69,7 @@ template <typename T> struct DivArgs {
     int nxc, nrc;                // coarse
     int bc_cmb, bc_surf;
     Index4 cidx;                 // coarse cells (nsd, nxc-1, nxc-1, nrc-1)
+    Index4 fidx;                 // fine cells (nsd, nx-1, nx-1, nr-1)
 };
 
 // Boundary kind of fine cell layer r (boundary_setup of the reference's
@@ -101,9 +104,89 @@ template <typename T> __device__ __forceinline__ void bc_project(const T *coords
     }
 }
 
-// Phase 1 of B: per coarse cell and coarse wedge, the 6 nodal sums over the
-// 8 fine wedges it owns.  One thread per (coarse cell), r fastest.
-template <typename T> __global__ __launch_bounds__(256) void div_partial_kernel(const DivArgs<T> a)
+// Phase 1 of B, per fine cell: cdiv_w = -|det J| (div u) of both wedges.
+// One thread per fine cell, r fastest.  UNI (radial cells a multiple of 64):
+// every wave covers cells of one lateral column, so the wedge geometry is
+// wave-uniform and read with scalar loads.
+template <typename T, bool UNI>
+__global__ __launch_bounds__(256) void div_cell_kernel(const DivArgs<T> a, T *cdiv)
+{
+    const int nx = a.nx, nr = a.nr, nc = nx - 1, ncr = nr - 1;
+    const uint32_t n = (uint32_t)a.nsd * nc * nc * ncr;
+    const uint32_t gid = xcd_block() * blockDim.x + threadIdx.x;
+    if (gid >= n) return;
+    int sd, x, y, r;
+    a.fidx.decode(gid, sd, x, y, r);
+    const bool has_cmb = a.sd_cmb[sd] != 0, has_surf = a.sd_surf[sd] != 0;
+    const T r0 = a.radii[(int64_t)sd * nr + r], r1 = a.radii[(int64_t)sd * nr + r + 1];
+    const T rmid = T(0.5) * (r0 + r1), h = T(0.5) * (r1 - r0);
+    const T sfac = T(1) / rmid, tz = T(1) / (T(6) * h);
+    const T vol = rmid * rmid * h;
+    int blayer;
+    const int cbc = cell_bc(r, nr, has_cmb, has_surf, a.bc_cmb, a.bc_surf, blayer);
+    // the 8 nodes of the cell: lateral corners (dx, dy) at the two levels
+    T u[2][2][2][3];
+#pragma unroll
+    for (int dx = 0; dx < 2; ++dx)
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy) {
+            const int64_t lcol = ((int64_t)sd * nx + x + dx) * nx + y + dy;
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const int64_t i = lcol * nr + r + e;
+                u[dx][dy][e][0] = a.u0[i];
+                u[dx][dy][e][1] = a.u1[i];
+                u[dx][dy][e][2] = a.u2[i];
+            }
+        }
+    if (cbc != TNG_NEUMANN) {
+#pragma unroll
+        for (int dx = 0; dx < 2; ++dx)
+#pragma unroll
+            for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+                for (int e = 0; e < 2; ++e)
+                    if (e == blayer) bc_project(a.coords, ((int64_t)sd * nx + x + dx) * nx + y + dy, cbc, u[dx][dy][e]);
+    }
+    const int64_t gb = ((((int64_t)sd * nc + x) * nc + y) * 2) * 10;
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+        T g[10];
+        if constexpr (UNI) {
+            using CT = const T __attribute__((address_space(4)));
+            const int64_t gu = ((int64_t)__builtin_amdgcn_readfirstlane((int)(gb >> 32)) << 32) |
+                               (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)gb);
+            const CT *gw = (const CT *)a.latgeom + gu + w * 10;
+#pragma unroll
+            for (int v = 0; v < 10; ++v) g[v] = gw[v];
+        } else {
+#pragma unroll
+            for (int v = 0; v < 10; ++v) g[v] = a.latgeom[gb + w * 10 + v];
+        }
+        // div u = sum_j g_j . u_j over the 6 wedge nodes
+        T div = T(0);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int dx = w == 0 ? (k == 1 ? 1 : 0) : (k == 1 ? 0 : 1);
+            const int dy = w == 0 ? (k == 2 ? 1 : 0) : (k == 2 ? 0 : 1);
+            const T l0 = k == 0 ? T(-0.5) : (k == 1 ? T(0.5) : T(0));
+            const T l1 = k == 0 ? T(-0.5) : (k == 1 ? T(0) : T(0.5));
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const T dz = e == 0 ? -tz : tz;
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                    div += (sfac * (l0 * g[d] + l1 * g[3 + d]) + dz * g[6 + d]) * u[dx][dy][e][d];
+            }
+        }
+        cdiv[(size_t)gid * 2 + w] = -(g[9] * vol) * div;
+    }
+}
+
+// Phase 2 of B, per coarse cell and coarse wedge: the 6 nodal sums over the
+// 8 fine wedges it owns, cdiv N^c_i.  One thread per coarse cell, r fastest.
+template <typename T>
+__global__ __launch_bounds__(256) void div_coarse_kernel(const DivArgs<T> a, const T *cdiv)
 {
     const int ncc = a.nxc - 1, nrcc = a.nrc - 1;
     const uint32_t n = (uint32_t)a.nsd * ncc * ncc * nrcc;
@@ -111,46 +194,37 @@ template <typename T> __global__ __launch_bounds__(256) void div_partial_kernel(
     if (gid >= n) return;
     int sd, xc, yc, rc;
     a.cidx.decode(gid, sd, xc, yc, rc);
-    const int nx = a.nx, nr = a.nr, nc = nx - 1;
-    const bool has_cmb = a.sd_cmb[sd] != 0, has_surf = a.sd_surf[sd] != 0;
-    T acc[2][6];
-    for (int w = 0; w < 2; ++w)
-        for (int i = 0; i < 6; ++i) acc[w][i] = T(0);
+    const int nc = a.nx - 1, ncr = a.nr - 1;
+    T2v<T> f[2][2][2];
+#pragma unroll
     for (int fx = 0; fx < 2; ++fx)
+#pragma unroll
         for (int fy = 0; fy < 2; ++fy)
-            for (int fr = 0; fr < 2; ++fr) {
-                const int x = 2 * xc + fx, y = 2 * yc + fy, r = 2 * rc + fr;
-                const T r0 = a.radii[(int64_t)sd * nr + r], r1 = a.radii[(int64_t)sd * nr + r + 1];
-                const T rmid = T(0.5) * (r0 + r1), h = T(0.5) * (r1 - r0);
-                const T s = T(1) / rmid, tz = T(1) / (T(6) * h);
-                const T vol = rmid * rmid * h;
-                int blayer;
-                const int cbc = cell_bc(r, nr, has_cmb, has_surf, a.bc_cmb, a.bc_surf, blayer);
-                for (int w = 0; w < 2; ++w) {
-                    const T *g = a.latgeom + ((((int64_t)sd * nc + x) * nc + y) * 2 + w) * 10;
-                    // div u = sum_j g_j . u_j over the 6 wedge nodes
-                    T div = T(0);
-                    for (int k = 0; k < 3; ++k) {
-                        const int X = x + kWC[w][k][0], Y = y + kWC[w][k][1];
-                        const int64_t lcol = ((int64_t)sd * nx + X) * nx + Y;
-                        const T l0 = k == 0 ? T(-0.5) : (k == 1 ? T(0.5) : T(0));
-                        const T l1 = k == 0 ? T(-0.5) : (k == 1 ? T(0) : T(0.5));
-                        for (int e = 0; e < 2; ++e) {
-                            const int l = r + e;
-                            const int64_t i = lcol * nr + l;
-                            T u[3] = {a.u0[i], a.u1[i], a.u2[i]};
-                            if (e == blayer) bc_project(a.coords, lcol, cbc, u);
-                            const T dz = e == 0 ? -tz : tz;
-                            for (int d = 0; d < 3; ++d) div += (s * (l0 * g[d] + l1 * g[3 + d]) + dz * g[6 + d]) * u[d];
-                        }
-                    }
-                    const T cdiv = -(g[9] * vol) * div;
-                    const int lat = fine_lat_idx(x, y, w);
-                    for (int i = 0; i < 6; ++i) acc[w][i] += cdiv * shape_coarse_c<T>(i, lat, fr);
-                }
-            }
-    T *o = a.partial + gid * 12;
+#pragma unroll
+            for (int fr = 0; fr < 2; ++fr)
+                f[fx][fy][fr] = *reinterpret_cast<const T2v<T> *>(
+                    cdiv + ((((size_t)sd * nc + 2 * xc + fx) * nc + 2 * yc + fy) * ncr + 2 * rc + fr) * 2);
+    T acc[2][6];
+#pragma unroll
     for (int w = 0; w < 2; ++w)
+#pragma unroll
+        for (int i = 0; i < 6; ++i) acc[w][i] = T(0);
+#pragma unroll
+    for (int fx = 0; fx < 2; ++fx)
+#pragma unroll
+        for (int fy = 0; fy < 2; ++fy)
+#pragma unroll
+            for (int fr = 0; fr < 2; ++fr)
+#pragma unroll
+                for (int w = 0; w < 2; ++w) {
+                    const int lat = fine_lat_idx(fx, fy, w);
+#pragma unroll
+                    for (int i = 0; i < 6; ++i) acc[w][i] += f[fx][fy][fr][w] * shape_coarse_c<T>(i, lat, fr);
+                }
+    T *o = a.partial + (size_t)gid * 12;
+#pragma unroll
+    for (int w = 0; w < 2; ++w)
+#pragma unroll
         for (int i = 0; i < 6; ++i) o[w * 6 + i] = acc[w][i];
 }
 
@@ -283,19 +357,38 @@ __global__ __launch_bounds__(256) void gradient_kernel(const GradArgs<T> a, cons
         cbc[q] = cell_bc(rcc, nr, has_cmb, has_surf, a.bc_cmb, a.bc_surf, blayer);
         onb[q] = blayer == (q == 1 ? 0 : 1);
     }
+    // Every weight the node can use, requested before any of it is needed
+    // (clamped indices: always in bounds; the branches below only skip
+    // arithmetic), so the loads are in flight together.
+    typedef T T2 __attribute__((ext_vector_type(2)));
+    T2 fq[2][2][2];  // [dxc + 1][dyc + 1][q] -> (wedge 0, wedge 1)
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int xcc = min(max(X - 1 + i, 0), nc - 1), ycc = min(max(Y - 1 + j, 0), nc - 1);
+            const int64_t cbq = (((int64_t)sd * nc + xcc) * nc + ycc) * ncr;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int rcc = min(max(l - 1 + q, 0), ncr - 1);
+                fq[i][j][q] = *reinterpret_cast<const T2 *>(fw + (cbq + rcc) * 2);
+            }
+        }
     // o: interior contributions; ob: contributions of boundary cells in which
     // this node lies on the boundary layer (free-slip: projected at the end)
     T o[3] = {T(0), T(0), T(0)}, ob[3] = {T(0), T(0), T(0)};
     bool any_fs = false;
+#pragma unroll
     for (int dxc = -1; dxc <= 0; ++dxc) {
         const int xc = X + dxc;
         if (xc < 0 || xc >= nc) continue;
+#pragma unroll
         for (int dyc = -1; dyc <= 0; ++dyc) {
             const int yc = Y + dyc;
             if (yc < 0 || yc >= nc) continue;
-            const int64_t cb = (((int64_t)sd * nc + xc) * nc + yc) * ncr;
+#pragma unroll
             for (int w = 0; w < 2; ++w) {
-                const int k = corner_of[w][X - xc][Y - yc];
+                const int k = corner_of[w][-dxc][-dyc];
                 if (k < 0) continue;
                 T g[9];
                 if constexpr (STAGE) {
@@ -314,7 +407,7 @@ __global__ __launch_bounds__(256) void gradient_kernel(const GradArgs<T> a, cons
                     const int rc = l - 1 + q;
                     if (rc < 0 || rc >= ncr) continue;
                     if (onb[q] && cbc[q] == TNG_DIRICHLET) continue;
-                    const T f = fw[(cb + rc) * 2 + w];
+                    const T f = fq[dxc + 1][dyc + 1][q][w];
                     const bool to_b = onb[q] && cbc[q] == TNG_FREESLIP;
                     any_fs = any_fs || to_b;
                     T *acc = to_b ? ob : o;
@@ -555,7 +648,9 @@ template <typename T> void div_impl(tng_operator &op, const tng_vector &src, tng
 {
     const tng_domain &f = *op.mesh, &c = *op.mesh2;
     const int64_t ncell_c = (int64_t)c.nsd * (c.nx - 1) * (c.nx - 1) * (c.nr - 1);
-    const size_t bytes = (size_t)ncell_c * 12 * sizeof(T);
+    const int64_t ncell_f = (int64_t)f.nsd * (f.nx - 1) * (f.nx - 1) * (f.nr - 1);
+    // coarse-wedge partials (12 per coarse cell), then the fine-cell values (2 per fine cell)
+    const size_t bytes = (size_t)(ncell_c * 12 + ncell_f * 2) * sizeof(T);
     if (op.scratch.bytes < bytes) op.scratch.alloc(bytes);
     DivArgs<T> a{};
     const MeshP<T> m = mp<T>(f);
@@ -566,7 +661,13 @@ template <typename T> void div_impl(tng_operator &op, const tng_vector &src, tng
     a.nsd = f.nsd; a.nx = f.nx; a.nr = f.nr; a.nxc = c.nx; a.nrc = c.nr;
     a.bc_cmb = op.bc_cmb; a.bc_surf = op.bc_surf;
     a.cidx = Index4(c.nx - 1, c.nr - 1);
-    hipLaunchKernelGGL(div_partial_kernel<T>, dim3(blocks_for(ncell_c)), dim3(256), 0, s, a);
+    a.fidx = Index4(f.nx - 1, f.nr - 1);
+    T *cdiv = op.scratch.as<T>() + ncell_c * 12;
+    if ((f.nr - 1) % 64 == 0)
+        hipLaunchKernelGGL((div_cell_kernel<T, true>), dim3(blocks_for(ncell_f)), dim3(256), 0, s, a, cdiv);
+    else
+        hipLaunchKernelGGL((div_cell_kernel<T, false>), dim3(blocks_for(ncell_f)), dim3(256), 0, s, a, cdiv);
+    hipLaunchKernelGGL(div_coarse_kernel<T>, dim3(blocks_for(ncell_c)), dim3(256), 0, s, a, (const T *)cdiv);
     TNG_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(wedge_gather_kernel<T>, dim3(blocks_for(c.nalloc)), dim3(256), 0, s,
                        (const T *)op.scratch.p, (T *)dst.comp(0), op.apply_mode == TNG_ADD ? 1 : 0, c.nsd, c.nx, c.nr,
