@@ -371,7 +371,9 @@ tng_status tng_xdmf_create(tng_mesh mesh, const char *directory, int points_byte
                            tng_xdmf *out);
 /* io/xdmf.hpp:208-281 add(data, output_type): a scalar or 3-component field,
  * written with `label` (the reference's Kokkos::View label) in output_bytes
- * (4 or 8) precision on every write().  Not allowed after the first write(). */
+ * (4 or 8) precision on every write().  Not allowed after the first write().
+ * The writer keeps the handle: the field must outlive the writer (the
+ * reference holds a reference-counted view instead). */
 tng_status tng_xdmf_add(tng_xdmf writer, tng_vec field, const char *label, int output_bytes);
 /* io/xdmf.hpp:203-207 set_write_counter */
 tng_status tng_xdmf_set_write_counter(tng_xdmf writer, int counter);

@@ -389,9 +389,17 @@ void tng_xdmf_writer::write()
         TNG_HIP_CHECK(hipEventElapsedTime(&ms, a, z));
         pack += ms * 1e-3;
     };
-    hipEvent_t e0, e1;
-    TNG_HIP_CHECK(hipEventCreate(&e0));
-    TNG_HIP_CHECK(hipEventCreate(&e1));
+    struct Events {  // released on every exit path
+        hipEvent_t a = nullptr, z = nullptr;
+        ~Events()
+        {
+            if (a) (void)hipEventDestroy(a);
+            if (z) (void)hipEventDestroy(z);
+        }
+    } ev;
+    TNG_HIP_CHECK(hipEventCreate(&ev.a));
+    TNG_HIP_CHECK(hipEventCreate(&ev.z));
+    hipEvent_t e0 = ev.a, e1 = ev.z;
     if (!first_done) {
         tng::make_dirs(dir);
         const std::string readme =
@@ -466,8 +474,6 @@ void tng_xdmf_writer::write()
     const std::string text =
         "<?xml version=\"1.0\" ?>\n<!DOCTYPE Xdmf SYSTEM \"Xdmf.dtd\" []>\n" + xdmf.str();
     tng::write_file(dir + "/step_" + std::to_string(counter) + ".xmf", text.data(), text.size());
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     last_pack_seconds = pack;
     last_write_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     ++counter;
