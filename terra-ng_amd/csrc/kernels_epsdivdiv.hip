@@ -58,6 +58,10 @@ template <typename T> struct ApplyArgs {
     const T *coords;    // (nsd, nx, nx, 3) unit vectors (the radial normal)
     const T *btab;      // (nsd, nx, nx, 2, 3) boundary rows
     const double *pen;  // 3 doubles (penalty axis b), or nullptr
+    // or: the per-block rotation moments, combined by every workgroup
+    const double *pen_part;
+    int pen_nb;
+    double pen_eps, pen_ginv[9];
     const uint8_t *sd_cmb, *sd_surf;
     int nsd, nx, nr, ncells;
     int bc_cmb, bc_surf;
@@ -263,6 +267,39 @@ __device__ __forceinline__ void wedge_stress(const T *g /*M0,M1,M2,detL*/, const
     }
 }
 
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// fs/fs penalty axis b = eps G^{-1} m, m the sum of nb per-block moments
+// (3 doubles each), in a fixed order: threads 0..255 take every 256th
+// partial, a wave tree, then the four waves in order.  Every thread of the
+// block calls it (two barriers); out (3 doubles) is valid for all on return.
+__device__ __forceinline__ void penalty_combine(const double *partials, int nb, double eps, const double *ginv,
+                                                double *red /* 12, shared */, double *out)
+{
+#pragma clang fp contract(off)
+    double s[3] = {0.0, 0.0, 0.0};
+    if (threadIdx.x < 256)
+        for (int b = threadIdx.x; b < nb; b += 256)
+            for (int c = 0; c < 3; ++c) s[c] += partials[3 * b + c];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int c = 0; c < 3; ++c) s[c] = wave_sum(s[c]);
+    if (lane == 0 && wave < 4)
+        for (int c = 0; c < 3; ++c) red[3 * wave + c] = s[c];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m[3] = {0.0, 0.0, 0.0};
+        for (int w = 0; w < 4; ++w)
+            for (int c = 0; c < 3; ++c) m[c] += red[3 * w + c];
+        for (int r = 0; r < 3; ++r)
+            out[r] = eps * fma(ginv[3 * r + 2], m[2], fma(ginv[3 * r + 1], m[1], ginv[3 * r] * m[0]));
+    }
+    __syncthreads();
+}
+
 // The Chebyshev sub-step of one node copy, with the rounding of the separate
 // update kernel (no contraction: every product is rounded before it is added).
 // in: b, invd, d (if read), x at the node.
@@ -348,6 +385,9 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
         q = (int64_t)gb * nx + sb * a.seg_len;
         q1 = (int64_t)gb * nx + min(nx, (sb + 1) * a.seg_len);
     }
+    const bool pen_on = a.pen != nullptr || a.pen_part != nullptr;
+    __shared__ double pen_red[12], pen_b[3];
+    if (a.pen_part) penalty_combine(a.pen_part, a.pen_nb, a.pen_eps, a.pen_ginv, pen_red, pen_b);
     do {
         const int g = (int)(q / nx);
         const int Y0 = (int)(q - (int64_t)g * nx);
@@ -482,7 +522,11 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
 
         // loop invariants of the output phase
         T pb[3] = {T(0), T(0), T(0)};
-        if (a.pen) {
+        if (a.pen_part) {
+            pb[0] = (T)pen_b[0];
+            pb[1] = (T)pen_b[1];
+            pb[2] = (T)pen_b[2];
+        } else if (a.pen) {
             pb[0] = (T)a.pen[0];
             pb[1] = (T)a.pen[1];
             pb[2] = (T)a.pen[2];
@@ -666,7 +710,7 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
                 const int64_t base = lcol * nr;
                 // unit normal of the output node (free-slip rows, penalty)
                 T cc[3] = {T(0), T(0), T(0)};
-                if (a.pen || (BCM == BCM_RUNTIME ? any_fs : (BCM & 1) != 0)) {
+                if (pen_on || (BCM == BCM_RUNTIME ? any_fs : (BCM & 1) != 0)) {
                     if constexpr (R == 64) {
     #pragma unroll
                         for (int c = 0; c < 3; ++c) cc[c] = lane_value(rec, REC_NOUT + c);
@@ -677,7 +721,7 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
                     }
                 }
                 T pbn[3] = {T(0), T(0), T(0)};
-                if (a.pen) {
+                if (pen_on) {
     #pragma clang fp contract(off)
                     pbn[0] = fma(pb[1], cc[2], -(pb[2] * cc[1]));
                     pbn[1] = fma(pb[2], cc[0], -(pb[0] * cc[2]));
@@ -725,7 +769,7 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
                             for (int c = 0; c < 3; ++c) val[c] = fma(un, cc[c], fma(-dv, cc[c], val[c]));
                         }
                     }
-                    if (a.pen) {
+                    if (pen_on) {
     #pragma clang fp contract(off)
                         // b x (r n) = r (b x n), b x n once per output column
                         const T rad = radl[p];
@@ -979,12 +1023,6 @@ template <typename T> struct PenArgs {
     FastDiv div_nr, div_nxx;  // node copy -> column, column -> subdomain
 };
 
-__device__ __forceinline__ double wave_sum(double v)
-{
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
 // Rotation moments m = sum over OWNED copies of x (x) u (x = radius * unit
 // coordinate): per-block partials, flat over the node copies (coalesced
 // streams of src and the mask), four independent nodes per thread and step.
@@ -1027,21 +1065,9 @@ template <typename T> __global__ __launch_bounds__(256) void penalty_partial_ker
 // One block: the partials in a fixed order, then b = eps G^{-1} m.
 template <typename T> __global__ __launch_bounds__(256) void penalty_final_kernel(const PenArgs<T> a, int nb)
 {
-    __shared__ double red[4][3];
-    double s[3] = {0, 0, 0};
-    for (int b = threadIdx.x; b < nb; b += blockDim.x)
-        for (int c = 0; c < 3; ++c) s[c] += a.partials[3 * b + c];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int c = 0; c < 3; ++c) s[c] = wave_sum(s[c]);
-    if (lane == 0) { red[wave][0] = s[0]; red[wave][1] = s[1]; red[wave][2] = s[2]; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double m[3] = {0, 0, 0};
-        for (int w = 0; w < 4; ++w)
-            for (int c = 0; c < 3; ++c) m[c] += red[w][c];
-        for (int r = 0; r < 3; ++r)
-            a.out_b[r] = a.eps * (a.ginv[3 * r] * m[0] + a.ginv[3 * r + 1] * m[1] + a.ginv[3 * r + 2] * m[2]);
-    }
+    __shared__ double red[12], b[3];
+    penalty_combine(a.partials, nb, a.eps, a.ginv, red, b);
+    if (threadIdx.x < 3) a.out_b[threadIdx.x] = b[threadIdx.x];
 }
 
 constexpr int kPenBlocks = 1024;
@@ -1221,8 +1247,18 @@ template <typename T> void run_diag_kernel(tng_operator &op, const tng_vector *s
     TNG_HIP_CHECK(hipGetLastError());
 }
 
-// fs/fs penalty: the rotation moment b of src into op.d_penalty_b
-template <typename T> void penalty_moment(tng_operator &op, const tng_vector &src, hipStream_t s)
+template <typename T> PenArgs<T> pen_args(tng_operator &op, const tng_vector &src);
+
+// fs/fs penalty: the per-block rotation moments of src into the context's
+// partials (kPenBlocks x 3 doubles)
+template <typename T> void penalty_partials(tng_operator &op, const tng_vector &src, hipStream_t s)
+{
+    const PenArgs<T> p = pen_args<T>(op, src);
+    hipLaunchKernelGGL(penalty_partial_kernel<T>, dim3(kPenBlocks), dim3(256), 0, s, p);
+    TNG_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T> PenArgs<T> pen_args(tng_operator &op, const tng_vector &src)
 {
     const tng_domain &m = *op.mesh;
     PenArgs<T> p{};
@@ -1237,6 +1273,13 @@ template <typename T> void penalty_moment(tng_operator &op, const tng_vector &sr
     p.div_nr = FastDiv((uint32_t)m.nr);
     p.div_nxx = FastDiv((uint32_t)(m.nx * m.nx));
     TNG_REQUIRE(m.nalloc < (int64_t)1 << 31, "penalty moment: node copies exceed 32-bit indexing");
+    return p;
+}
+
+// fs/fs penalty: the rotation moment b of src into op.d_penalty_b
+template <typename T> void penalty_moment(tng_operator &op, const tng_vector &src, hipStream_t s)
+{
+    const PenArgs<T> p = pen_args<T>(op, src);
     hipLaunchKernelGGL(penalty_partial_kernel<T>, dim3(kPenBlocks), dim3(256), 0, s, p);
     hipLaunchKernelGGL(penalty_final_kernel<T>, dim3(1), dim3(256), 0, s, p, kPenBlocks);
     TNG_HIP_CHECK(hipGetLastError());
@@ -1281,9 +1324,16 @@ template <typename T> void apply_impl(tng_operator &op, const tng_vector &src, t
         if (ev) TNG_HIP_CHECK(hipEventRecord(ev[2], s));
     } else {
         if (!op.btab_valid) epsdivdiv_update_boundary_table(op, s);
+        // penalty: the per-block moments only; every apply workgroup combines them
         const bool pen = op.penalty_active;
-        if (pen) penalty_moment();
-        ApplyArgs<T> a = apply_args<T>(op, src, dst, pen, add);
+        ApplyArgs<T> a = apply_args<T>(op, src, dst, false, add);
+        if (pen) {
+            penalty_partials<T>(op, src, s);
+            a.pen_part = m.ctx->red_partials.as<double>();
+            a.pen_nb = kPenBlocks;
+            a.pen_eps = op.penalty_eps;
+            for (int i = 0; i < 9; ++i) a.pen_ginv[i] = m.rot_gram_inv[i];
+        }
         if (ev) TNG_HIP_CHECK(hipEventRecord(ev[1], s));
         launch_apply<T>(a, s);
         if (ev) TNG_HIP_CHECK(hipEventRecord(ev[2], s));
