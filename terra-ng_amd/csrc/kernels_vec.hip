@@ -25,24 +25,28 @@ template <typename TD, typename TS> __global__ void copy_kernel(TD *d, const TS 
 // thread per multi-copy node; copy indices are stored slot-major so that
 // consecutive threads (consecutive radial levels of the same interface
 // columns) read consecutive addresses.
+// One thread per (component, shared node): component-major, so consecutive
+// threads read consecutive entries of the copy table; the copies are loaded
+// before they are summed (in copy order).
 template <typename T>
 __global__ void halo_sum_kernel(T *d0, T *d1, T *d2, int ncomp, const int32_t *__restrict__ idx, int64_t n, int maxc)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+    if (t >= n * ncomp) return;
+    const int c = (int)(t / n);
+    const int64_t g = t - (int64_t)c * n;
+    T *d = c == 0 ? d0 : (c == 1 ? d1 : d2);
     int32_t ci[16];
+    T v[16];
     int nc = 0;
     for (int j = 0; j < maxc; ++j) {
-        const int32_t i = idx[(int64_t)j * n + t];
+        const int32_t i = idx[(int64_t)j * n + g];
         if (i >= 0) ci[nc++] = i;
     }
-    T *comp[3] = {d0, d1, d2};
-    for (int c = 0; c < ncomp; ++c) {
-        T *d = comp[c];
-        T s = T(0);
-        for (int j = 0; j < nc; ++j) s += d[ci[j]];
-        for (int j = 0; j < nc; ++j) d[ci[j]] = s;
-    }
+    for (int j = 0; j < nc; ++j) v[j] = d[ci[j]];
+    T s = T(0);
+    for (int j = 0; j < nc; ++j) s += v[j];
+    for (int j = 0; j < nc; ++j) d[ci[j]] = s;
 }
 
 // Shared node copies of a fused Chebyshev sub-step: A x summed over the
@@ -126,7 +130,7 @@ void launch_halo_sum(const tng_domain &m, tng_vector &v, hipStream_t s)
 {
     if (m.halo_n == 0) return;
     TNG_REQUIRE(m.halo_maxc <= 16, "halo: more than 16 copies of one node");
-    const unsigned g = (unsigned)((m.halo_n + 255) / 256);
+    const unsigned g = (unsigned)((m.halo_n * v.ncomp + 255) / 256);
     if (v.dtype == TNG_F64)
         hipLaunchKernelGGL(halo_sum_kernel<double>, dim3(g), dim3(256), 0, s, (double *)v.comp(0),
                            (double *)v.comp(v.ncomp > 1 ? 1 : 0), (double *)v.comp(v.ncomp > 2 ? 2 : 0), v.ncomp,
