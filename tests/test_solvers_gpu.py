@@ -235,6 +235,35 @@ def test_fused_chebyshev_matches_separate_kernels(dt, bcs, gpu_ctx, monkeypatch)
         assert bad.size == 0, (name, len(bad), bad[:8].tolist(), float(np.abs(a - c).max() / np.abs(a).max()))
 
 
+def test_fgmres_zero_rhs_returns_immediately(gpu_ctx):
+    """fgmres.hpp:130-133: a zero right-hand side converges with zero iterations and u = 0."""
+    S = stokes_solver(1, 3, (D, D))
+    u, p = vec(S.domain_velocity, 3), vec(S.domain_pressure, 1)
+    u.fill(0.0)
+    rep = S.solve(u, p, vec(S.domain_velocity, 3), vec(S.domain_pressure, 1))
+    assert rep["converged"] == 1 and rep["iterations"] == 0
+    assert not u.download().any() and not p.download().any()
+
+
+def test_fgmres_skips_nonfinite_preconditioner(gpu_ctx):
+    """fgmres.hpp:189-202: a preconditioner output with NaN/Inf is replaced by the
+    Arnoldi vector itself.  Forced from the host: one interior viscosity value
+    above the FP32 range leaves the FP64 operator finite but overflows the FP32
+    V-cycle, so every step takes the guard and FGMRES still reduces the
+    residual (unpreconditioned)."""
+    of = O.Mesh(3)
+    eta = np.ones(of.shape)
+    eta[3, 4, 4, 4] = 1e39  # interior node of one subdomain: a single copy
+    S = stokes_solver(1, 3, (D, D), viscous_pc_dtype=tng.F32, krylov_max_iterations=6, krylov_restart=10)
+    S.set_viscosity(vec(S.domain_velocity, 1, eta))
+    u, p = vec(S.domain_velocity, 3), vec(S.domain_pressure, 1)
+    rep = S.solve(u, p, vec(S.domain_velocity, 3, field(of, 3, 41)), vec(S.domain_pressure, 1))
+    assert rep["nonfinite_preconditioner"] == rep["iterations"] == 6, rep
+    assert rep["nonfinite_residual"] == 0
+    assert np.isfinite(rep["final_residual"]) and rep["final_residual"] < rep["initial_residual"]
+    assert np.isfinite(u.download()).all()
+
+
 def test_vcycle_reduces_error(gpu_ctx):
     """V-cycles with the reference's default smoother reduce the error; the
     contraction is limited by the low-energy modes of the one-point wedge
