@@ -472,6 +472,55 @@ class XDMFOutput:
             pass
 
 
+def read_xdmf_checkpoint_metadata(directory):
+    """io/xdmf.hpp:1060-1186 read_xdmf_checkpoint_metadata: the checkpoint's structure as a dict
+    (version, subdomains per diamond side / radially, subdomain sizes, radii, grid scalar bytes,
+    grid data files, subdomain ordering and the global subdomain ids in file order).  Host only."""
+    import struct
+    path = os.path.join(directory, "checkpoint_metadata.bin")
+    if not os.path.exists(path):
+        raise TngError(IO_ERROR, f"Could not find file: {path}")
+    with open(path, "rb") as f:
+        b = f.read()
+    pos = 0
+
+    def take(fmt):
+        nonlocal pos
+        size = struct.calcsize("<" + fmt)
+        if pos + size > len(b):
+            raise TngError(IO_ERROR, "Failed to read from input stream.")
+        v = struct.unpack_from("<" + fmt, b, pos)
+        pos += size
+        return v if len(v) > 1 else v[0]
+
+    md = {"version": take("i")}
+    md["num_subdomains_per_diamond_lateral_direction"] = take("i")
+    md["num_subdomains_per_diamond_radial_direction"] = take("i")
+    md["size_x"], md["size_y"], md["size_r"] = take("i"), take("i"), take("i")
+    nrad = md["num_subdomains_per_diamond_radial_direction"] * (md["size_r"] - 1) + 1
+    radii = [take("d") for _ in range(nrad)]
+    if any(radii[i] <= radii[i - 1] for i in range(1, nrad)):
+        raise TngError(IO_ERROR, "Radii are not sorted correctly in checkpoint.")
+    md["radii"] = radii
+    md["grid_scalar_bytes"] = take("i") if md["version"] > 0 else 0
+    files = []
+    for _ in range(take("i")):
+        n = take("i")
+        if pos + n > len(b):
+            raise TngError(IO_ERROR, "Failed to read from input stream.")
+        name = b[pos:pos + n].decode()
+        pos += n
+        dt, sb, vd = take("i"), take("i"), take("i")
+        files.append({"grid_name_string": name, "scalar_data_type": dt, "scalar_bytes": sb, "vec_dim": vd})
+    md["grid_data_files"] = files
+    md["checkpoint_subdomain_ordering"] = take("i")
+    if md["checkpoint_subdomain_ordering"] == 0:
+        s_lat = md["num_subdomains_per_diamond_lateral_direction"]
+        n_ids = 10 * s_lat * s_lat * md["num_subdomains_per_diamond_radial_direction"]
+        md["checkpoint_ordering_0_global_subdomain_ids"] = [take("q") for _ in range(n_ids)]
+    return md
+
+
 def read_xdmf_checkpoint(directory, label, step, out):
     """io/xdmf.hpp:1199-1441 read_xdmf_checkpoint_grid into the vector ``out`` (its mesh must match)."""
     out.domain.ctx.check(load_library().tng_xdmf_checkpoint_read(os.fsencode(directory), label.encode(), int(step),

@@ -133,3 +133,23 @@ def test_xmf_structure():
     assert attrs[1].attrib["AttributeType"] == "Vector"
     # attributes in name order, as util/xml.hpp's std::map writes them
     assert '<DataItem DataType="Float" Dimensions="' in text and "p&amp;q_3.bin" in text
+
+
+def test_python_metadata_reader(tmp_path):
+    """tng.read_xdmf_checkpoint_metadata (host only) parses what the format writes."""
+    import tng
+    m = O.Mesh(2, 3, 0.55, 1.0, 1, 1)
+    items = [{"label": "u", "ncomp": 3, "dtype": "f64", "bytes": 4},
+             {"label": "eta", "ncomp": 1, "dtype": "f32", "bytes": 8}]
+    (tmp_path / "checkpoint_metadata.bin").write_bytes(X.metadata_bytes(m, items, 8))
+    md = tng.read_xdmf_checkpoint_metadata(str(tmp_path))
+    assert md["version"] == 1 and md["grid_scalar_bytes"] == 8
+    assert (md["size_x"], md["size_y"], md["size_r"]) == (m.nx, m.nx, m.nr)
+    assert [f["grid_name_string"] for f in md["grid_data_files"]] == ["eta", "u"]
+    assert md["grid_data_files"][1]["vec_dim"] == 3 and md["grid_data_files"][0]["scalar_bytes"] == 8
+    assert md["checkpoint_ordering_0_global_subdomain_ids"] == [X.global_subdomain_id(*r) for r in m.subdomain_info]
+    (tmp_path / "checkpoint_metadata.bin").write_bytes(X.metadata_bytes(m, items, 8)[:-3])
+    with pytest.raises(tng.TngError):
+        tng.read_xdmf_checkpoint_metadata(str(tmp_path))
+    with pytest.raises(tng.TngError):
+        tng.read_xdmf_checkpoint_metadata(str(tmp_path / "nope"))
