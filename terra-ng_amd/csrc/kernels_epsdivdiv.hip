@@ -1163,8 +1163,10 @@ ApplyShape apply_shape(int ncells)
     if (ncells <= 64) {
         sh.R = 4;
         while (sh.R < ncells) sh.R *= 2;
-    } else if (ncells == 128 && env_int("TNG_APPLY_RADIAL_CHUNKS", 2) == 2) {
-        sh.RG = 2;
+    } else if ((ncells == 128 || ncells == 256) && env_int("TNG_APPLY_RADIAL_CHUNKS", 0) != 1) {
+        // one wave per radial chunk of 64 cells; the K-cells-per-lane kernels
+        // (TNG_APPLY_RADIAL_CHUNKS=1) exceed the register file at K = 4
+        sh.RG = ncells / 64;
     } else {
         sh.K = ncells / 64;
         sh.NW = 4;
@@ -1178,11 +1180,12 @@ template <typename T, bool FUSE = false> void launch_apply(ApplyArgs<T> a, hipSt
     TNG_REQUIRE(a.ncells >= 1 && (a.ncells & (a.ncells - 1)) == 0, "radial layers per subdomain must be a power of 2");
     TNG_REQUIRE((int64_t)a.nsd * a.nx * a.nx * a.nr < ((int64_t)1 << 31), "node copies exceed 32-bit indexing");
     // shapes: up to 64 radial cells one cell per lane (R lanes per column);
-    // 128 cells in two radial chunks of one wave each; more cells K per lane
+    // 128 and 256 cells in two or four radial chunks of one wave each; more cells K per lane
     const ApplyShape sh = apply_shape(a.ncells);
     const int R = sh.R, K = sh.K, RG = sh.RG, NW = sh.NW;
     TNG_REQUIRE(K <= 4, "more than 256 radial layers per subdomain is not supported");
-    TNG_REQUIRE(!FUSE || (R == 64 && K == 1 && NW == 8), "the fused Chebyshev step needs the one-column kernels");
+    TNG_REQUIRE(!FUSE || (R == 64 && K == 1 && NW == 8 && RG <= 2),
+                "the fused Chebyshev step needs the one-column kernels with at most 128 radial layers");
     const int VX = NW / RG * (64 / R);
     // R = 64: strips over the columns of all subdomains in a row; otherwise
     // strips per subdomain
@@ -1191,7 +1194,18 @@ template <typename T, bool FUSE = false> void launch_apply(ApplyArgs<T> a, hipSt
     // the one-column kernels are compiled per boundary-condition mix
     const int bcm = ((a.bc_cmb == TNG_FREESLIP || a.bc_surf == TNG_FREESLIP) ? 1 : 0) |
                     ((a.bc_cmb == TNG_DIRICHLET || a.bc_surf == TNG_DIRICHLET) ? 2 : 0);
-    if (RG == 2) {
+    if (RG == 4) {
+        if constexpr (FUSE) {
+            // unreachable: the fused step is limited to one or two chunks
+        } else {
+            switch (bcm) {
+            case 0: launch_apply_rk<T, 64, 1, 8, 4, 0>(a, lg, s); break;
+            case 1: launch_apply_rk<T, 64, 1, 8, 4, 1>(a, lg, s); break;
+            case 2: launch_apply_rk<T, 64, 1, 8, 4, 2>(a, lg, s); break;
+            default: launch_apply_rk<T, 64, 1, 8, 4, 3>(a, lg, s); break;
+            }
+        }
+    } else if (RG == 2) {
         switch (bcm) {
         case 0: launch_apply_rk<T, 64, 1, 8, 2, 0, FUSE>(a, lg, s); break;
         case 1: launch_apply_rk<T, 64, 1, 8, 2, 1, FUSE>(a, lg, s); break;
@@ -1356,7 +1370,7 @@ bool apply_cheb_impl(tng_operator &op, const tng_vector &x, tng_vector &ax, tng_
         m.rad_sdr != 0)
         return false;
     const ApplyShape sh = apply_shape(m.nr - 1);
-    if (!(sh.R == 64 && sh.K == 1 && sh.NW == 8)) return false;
+    if (!(sh.R == 64 && sh.K == 1 && sh.NW == 8 && sh.RG <= 2)) return false;
     tng_context &ctx = *m.ctx;
     hipEvent_t *ev = ctx.profiling ? ctx.prof_next_set() : nullptr;
     if (ev) TNG_HIP_CHECK(hipEventRecord(ev[0], s));

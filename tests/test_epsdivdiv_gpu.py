@@ -87,13 +87,14 @@ def test_parity_shapes(gpu_ctx, L, Lr, ls, rs, bcs):
 
 
 @pytest.mark.parametrize("L,Lr,env", [(2, 7, {"TNG_APPLY_RADIAL_CHUNKS": "1"}), (3, 6, {"TNG_APPLY_WAVES": "4"}),
-                                       (1, 8, {}), (3, 7, {}), (3, 6, {})])
+                                       (1, 8, {"TNG_APPLY_RADIAL_CHUNKS": "1"}), (1, 8, {}), (2, 8, {}),
+                                       (3, 7, {}), (3, 6, {})])
 @pytest.mark.parametrize("bcs", [(D, D), (F, F), (N, N), (D, F), (N, F), (F, D)])
 def test_parity_launch_variants(gpu_ctx, L, Lr, env, bcs, monkeypatch):
-    """Launch shapes chosen by layer count or tuning variable: 128 layers as
-    two radial chunks (default) or two cells per lane, 4-wave workgroups at
-    64 layers, four cells per lane at 256 layers; every boundary-condition
-    mix of the one-column kernels."""
+    """Launch shapes chosen by layer count or tuning variable: 128 and 256
+    layers as two and four radial chunks (default) or two and four cells per
+    lane, 4-wave workgroups at 64 layers; every boundary-condition mix of the
+    one-column kernels."""
     for k, v in env.items():
         monkeypatch.setenv(k, v)
     dom, om, w = setup(L, Lr)
@@ -132,14 +133,18 @@ def test_config2_shape_level5_freeslip(gpu_ctx):
     assert rel_err(g, ref) < 1e-12
 
 
-def test_fp32_apply(gpu_ctx):
-    dom, om, w = setup(4)
+# FP32 rounding is amplified by the cancellation between lateral and radial
+# stencil terms, which grows with the cells' aspect ratio (lateral width over
+# radial height: about 1.4 at (4, 4), 40 at (2, 7), 80 at (2, 8)).
+@pytest.mark.parametrize("L,Lr,tol", [(4, 4, 2e-6), (2, 7, 2e-5), (2, 8, 4e-5)])
+def test_fp32_apply(gpu_ctx, L, Lr, tol):
+    dom, om, w = setup(L, Lr)
     k = w.eta_layered(102)
     u = w.velocity(101)
     for bcs in [(D, D), (F, F)]:
         g = gpu_apply(dom, k, u, bcs, dtype=tng.F32)
         ref = om.epsdivdiv(k, u, *bcs, path="fast")
-        assert rel_err(g, ref) < 2e-6
+        assert rel_err(g, ref) < tol
 
 
 def test_deterministic(gpu_ctx):
