@@ -68,6 +68,13 @@ template <typename T> struct ApplyArgs {
     int add;
     int strips, groups;  // strips per subdomain row (or of all subdomains), lateral groups
     int segs, seg_len;   // one-segment launches: segments per strip, rows per segment
+    // rsplit > 1: every subdomain is swept as rsplit radial parts of
+    // ncells / rsplit cells, one after the other like separate subdomains;
+    // the node levels between two parts are written as two partial sums to
+    // iface ((interface, lower|upper, component) planes of nsd*nx*nx) and
+    // completed by epsdivdiv_iface_kernel
+    int rsplit;
+    T *iface;
     // fused Chebyshev sub-step (FUSE kernels): on node copies no other
     // subdomain shares, d = az * (b - A src) * invd [+ ad * d] and
     // xo = src + d replace the store of A src
@@ -332,7 +339,7 @@ __device__ __forceinline__ void cheb_update(const ApplyArgs<T> &a, int64_t i, co
     }
 }
 
-template <typename T, int R, int K, int NW, int RG, int BCM, bool FUSE>
+template <typename T, int R, int K, int NW, int RG, int BCM, bool FUSE, bool SPLIT>
 __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArgs<T> a)
 {
     // RG > 1: the radial cells of a column are split into RG chunks of 64,
@@ -340,6 +347,8 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
     // output by the upper chunk, which receives the lower chunk's partial
     // sums through LDS.
     static_assert(RG == 1 || (R == 64 && K == 1), "radial chunks need one cell per lane");
+    static_assert(!SPLIT || (R == 64 && K == 1 && !FUSE), "radial parts need the one-column kernels");
+    const int rsplit = SPLIT ? a.rsplit : 1;  // radial parts per subdomain
     constexpr int VPW = 64 / R;
     constexpr int VX = NW / RG * VPW;
     constexpr int NV = VX * RG;
@@ -393,7 +402,7 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
         const int Y0 = (int)(q - (int64_t)g * nx);
         const int Yend = (int)min((int64_t)nx, Y0 + (q1 - q));
         q += Yend - Y0;
-        int sd, xc;
+        int sd, xc, part = 0;
         bool col_ok = true;  // the virtual column exists
         if constexpr (R == 64) {
             // One wave per column: strips run over the virtual columns of all
@@ -401,29 +410,33 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
             // cells and only outputs node column 0), so only the last strip is
             // ragged.  A neighbour in the next subdomain has no cells there and
             // passes zeros.
-            const int nvc = a.nsd * nx;
+            const int nvc = a.nsd * rsplit * nx;
             const int vcol = g * (VX - 1) + v;
             col_ok = vcol < nvc;
             const int vc = min(vcol, nvc - 1);
-            sd = vc / nx;
+            const int psd = vc / nx;
+            sd = SPLIT ? psd / rsplit : psd;
+            part = psd - sd * rsplit;
             // past the last subdomain: a column without cells (zero contributions)
-            xc = col_ok ? vc - sd * nx - 1 : nc;
+            xc = col_ok ? vc - psd * nx - 1 : nc;
         } else {
             const int strip = g % a.strips;
             sd = g / a.strips;
             xc = strip * (VX - 1) - 1 + v;
         }
         const int X = xc + 1;
-        const bool has_cmb = a.sd_cmb[sd] != 0, has_surf = a.sd_surf[sd] != 0;
+        const bool has_cmb = a.sd_cmb[sd] != 0 && part == 0;
+        const bool has_surf = a.sd_surf[sd] != 0 && part == rsplit - 1;
         const bool any_fs = (has_cmb && a.bc_cmb == TNG_FREESLIP) || (has_surf && a.bc_surf == TNG_FREESLIP);
         const bool any_dir = (has_cmb && a.bc_cmb == TNG_DIRICHLET) || (has_surf && a.bc_surf == TNG_DIRICHLET);
 
-        // lanes: cells hc*nch + rl*K .. +K-1 (nch is either <= R with K = 1, or R*K)
-        const int nch = ncells / RG;
+        // lanes: cells c0 + rl*K .. +K-1 (nch is either <= R with K = 1, or R*K)
+        const int nch = ncells / (RG * rsplit);
+        const int c0 = (part * RG + hc) * nch;
         const bool lane_has = rl * K < nch;
         const bool ovf_lane = rl * K + K == nch;  // lane holding the last cell of the chunk
         const bool bot_chunk = hc == 0, top_chunk = hc == RG - 1;
-        const int lvl0 = hc * nch + (lane_has ? rl * K : nch - K);  // clamped first level (loads stay in bounds)
+        const int lvl0 = c0 + (lane_has ? rl * K : nch - K);  // clamped first level (loads stay in bounds)
         LaneBC<T> lbc;
         lbc.z_lo = (has_cmb && bot_chunk && rl == 0 && a.bc_cmb == TNG_DIRICHLET) ? T(0) : T(1);
         lbc.f_lo = (has_cmb && bot_chunk && rl == 0 && a.bc_cmb == TNG_FREESLIP) ? T(1) : T(0);
@@ -533,7 +546,7 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
         }
         T radl[KL];
     #pragma unroll
-        for (int p = 0; p < KL; ++p) radl[p] = a.radii[(int64_t)sd * nr + min(hc * nch + rl * K + p, nr - 1)];
+        for (int p = 0; p < KL; ++p) radl[p] = a.radii[(int64_t)sd * nr + min(c0 + rl * K + p, nr - 1)];
 
         T accA[KL][3], accB[KL][3];
     #pragma unroll
@@ -572,7 +585,7 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
             if constexpr (FUSE) {
                 fuse_here = X > 0 && X < nx - 1 && yc > 0 && yc < nx - 1;
                 if (early && fuse_here)
-                    cheb_load<T>(a, (((int64_t)sd * nx + X) * nx + yc) * nr + min(hc * nch + rl * K, nr - 1), cin);
+                    cheb_load<T>(a, (((int64_t)sd * nx + X) * nx + yc) * nr + min(c0 + rl * K, nr - 1), cin);
             }
             // the prefetched row yc+1 becomes the top row; prefetch row yc+2
             {
@@ -730,10 +743,22 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
     #pragma unroll
                 for (int p = 0; p < KL; ++p) {
                     if (p == K && !(ovf_lane && top_chunk)) continue;
-                    const int l = hc * nch + rl * K + p;
+                    const int l = c0 + rl * K + p;
                     T val[3];
     #pragma unroll
                     for (int c = 0; c < 3; ++c) val[c] = acc_cur[p][c] + (p == 0 ? below[c] : T(0));
+                    // level shared with the radial part above (lower partial)
+                    // or below (upper partial): completed by the interface kernel
+                    const bool if_lo = SPLIT && p == K && part < rsplit - 1;
+                    const bool if_up = SPLIT && p == 0 && rl == 0 && bot_chunk && part > 0;
+                    if (if_lo || if_up) {
+                        const int64_t plane = (int64_t)a.nsd * nx * nx;
+                        T *o = a.iface + ((int64_t)(((if_up ? part - 1 : part) * 2 + (if_up ? 1 : 0)) * 3)) * plane + lcol;
+                        o[0] = val[0];
+                        o[plane] = val[1];
+                        o[2 * plane] = val[2];
+                        continue;
+                    }
                     const bool at_cmb = p == 0 && rl == 0 && bot_chunk && has_cmb && a.bc_cmb != TNG_NEUMANN;
                     const bool at_surf = p == K && top_chunk && has_surf && a.bc_surf != TNG_NEUMANN;
                     if (at_cmb || at_surf) {
@@ -1070,6 +1095,45 @@ template <typename T> __global__ __launch_bounds__(256) void penalty_final_kerne
     if (threadIdx.x < 3) a.out_b[threadIdx.x] = b[threadIdx.x];
 }
 
+// The node levels between the radial parts of a split sweep: upper partial +
+// lower partial (the order in which a radial chunk adds the chunk below it),
+// then the penalty term and the Add mode as in the apply's store phase.  No
+// boundary rows: parts meet strictly inside the shell.
+template <typename T> __global__ __launch_bounds__(256) void epsdivdiv_iface_kernel(const ApplyArgs<T> a)
+{
+#pragma clang fp contract(off)
+    __shared__ double pen_red[12], pen_b[3];
+    if (a.pen_part) penalty_combine(a.pen_part, a.pen_nb, a.pen_eps, a.pen_ginv, pen_red, pen_b);
+    const bool pen_on = a.pen != nullptr || a.pen_part != nullptr;
+    T pb[3] = {T(0), T(0), T(0)};
+    for (int c = 0; c < 3; ++c) pb[c] = a.pen_part ? (T)pen_b[c] : (a.pen ? (T)a.pen[c] : T(0));
+    const int nx = a.nx, nr = a.nr, ncp = a.ncells / a.rsplit;
+    const int64_t plane = (int64_t)a.nsd * nx * nx, n = plane * (a.rsplit - 1);
+    T *dst[3] = {a.dst0, a.dst1, a.dst2};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int j = (int)(i / plane);  // between parts j and j + 1
+        const int64_t lcol = i - j * plane;
+        const int sd = (int)(lcol / ((int64_t)nx * nx));
+        const int l = (j + 1) * ncp;
+        const T *lo = a.iface + (int64_t)(j * 2) * 3 * plane + lcol;
+        const T *up = lo + 3 * plane;
+        T val[3];
+        for (int c = 0; c < 3; ++c) val[c] = up[c * plane] + lo[c * plane];
+        if (pen_on) {
+            const T cc[3] = {a.coords[3 * lcol], a.coords[3 * lcol + 1], a.coords[3 * lcol + 2]};
+            const T pbn[3] = {fma(pb[1], cc[2], -(pb[2] * cc[1])), fma(pb[2], cc[0], -(pb[0] * cc[2])),
+                              fma(pb[0], cc[1], -(pb[1] * cc[0]))};
+            const T rad = a.radii[(int64_t)sd * nr + l];
+            for (int c = 0; c < 3; ++c) val[c] = fma(rad, pbn[c], val[c]);
+        }
+        const int64_t node = lcol * nr + l;
+        for (int c = 0; c < 3; ++c) {
+            if (a.add) val[c] += dst[c][node];
+            dst[c][node] = val[c];
+        }
+    }
+}
+
 constexpr int kPenBlocks = 1024;
 
 template <typename T> struct MeshPtrs {
@@ -1121,10 +1185,10 @@ int pick_segment(int nx, int64_t lateral_groups, int resident)
     return best;
 }
 
-template <typename T, int R, int K, int NW, int RG, int BCM = BCM_RUNTIME, bool FUSE = false>
+template <typename T, int R, int K, int NW, int RG, int BCM = BCM_RUNTIME, bool FUSE = false, bool SPLIT = false>
 void launch_apply_rk(ApplyArgs<T> a, int64_t lateral_groups, hipStream_t s)
 {
-    const void *kern = reinterpret_cast<const void *>(&epsdivdiv_apply_kernel<T, R, K, NW, RG, BCM, FUSE>);
+    const void *kern = reinterpret_cast<const void *>(&epsdivdiv_apply_kernel<T, R, K, NW, RG, BCM, FUSE, SPLIT>);
     static int cached_dev = -1, cached_resident = 0;
     int dev = 0;
     TNG_HIP_CHECK(hipGetDevice(&dev));
@@ -1148,22 +1212,28 @@ void launch_apply_rk(ApplyArgs<T> a, int64_t lateral_groups, hipStream_t s)
         a.segs = (a.nx + seg - 1) / seg;
         blocks = (int)(lateral_groups * a.segs);
     }
-    hipLaunchKernelGGL((epsdivdiv_apply_kernel<T, R, K, NW, RG, BCM, FUSE>), dim3(blocks), dim3(NW * 64), 0, s, a);
+    hipLaunchKernelGGL((epsdivdiv_apply_kernel<T, R, K, NW, RG, BCM, FUSE, SPLIT>), dim3(blocks), dim3(NW * 64), 0, s, a);
 }
 
 // Shape of the apply launch for a mesh: R lanes per column, K cells per lane,
-// RG radial chunks, NW waves per workgroup.
+// RG radial chunks, NW waves per workgroup, S radial parts swept separately.
 struct ApplyShape {
-    int R, K, RG, NW;
+    int R, K, RG, NW, S;
 };
 
 ApplyShape apply_shape(int ncells)
 {
-    ApplyShape sh{64, 1, 1, 8};
-    if (ncells <= 64) {
+    ApplyShape sh{64, 1, 1, 8, 1};
+    const int chunks = env_int("TNG_APPLY_RADIAL_CHUNKS", 0);
+    if (ncells == 256 && chunks == 0) {
+        // four radial parts swept like separate subdomains (eight one-column
+        // waves per workgroup, seven new columns each), their shared levels
+        // completed by the interface kernel
+        sh.S = 4;
+    } else if (ncells <= 64) {
         sh.R = 4;
         while (sh.R < ncells) sh.R *= 2;
-    } else if ((ncells == 128 || ncells == 256) && env_int("TNG_APPLY_RADIAL_CHUNKS", 0) != 1) {
+    } else if ((ncells == 128 || ncells == 256) && chunks != 1) {
         // one wave per radial chunk of 64 cells; the K-cells-per-lane kernels
         // (TNG_APPLY_RADIAL_CHUNKS=1) exceed the register file at K = 4
         sh.RG = ncells / 64;
@@ -1184,17 +1254,30 @@ template <typename T, bool FUSE = false> void launch_apply(ApplyArgs<T> a, hipSt
     const ApplyShape sh = apply_shape(a.ncells);
     const int R = sh.R, K = sh.K, RG = sh.RG, NW = sh.NW;
     TNG_REQUIRE(K <= 4, "more than 256 radial layers per subdomain is not supported");
+    a.rsplit = sh.S;
+    TNG_REQUIRE(sh.S == 1 || (R == 64 && K == 1 && !FUSE && a.iface), "radial parts need the one-column kernels");
     TNG_REQUIRE(!FUSE || (R == 64 && K == 1 && NW == 8 && RG <= 2),
                 "the fused Chebyshev step needs the one-column kernels with at most 128 radial layers");
     const int VX = NW / RG * (64 / R);
     // R = 64: strips over the columns of all subdomains in a row; otherwise
     // strips per subdomain
-    a.strips = R == 64 ? (a.nsd * a.nx + (VX - 1) - 1) / (VX - 1) : (a.nx + (VX - 1) - 1) / (VX - 1);
+    a.strips = R == 64 ? (a.nsd * a.rsplit * a.nx + (VX - 1) - 1) / (VX - 1) : (a.nx + (VX - 1) - 1) / (VX - 1);
     const int64_t lg = R == 64 ? (int64_t)a.strips : (int64_t)a.nsd * a.strips;
     // the one-column kernels are compiled per boundary-condition mix
     const int bcm = ((a.bc_cmb == TNG_FREESLIP || a.bc_surf == TNG_FREESLIP) ? 1 : 0) |
                     ((a.bc_cmb == TNG_DIRICHLET || a.bc_surf == TNG_DIRICHLET) ? 2 : 0);
-    if (RG == 4) {
+    if (sh.S > 1) {
+        if constexpr (FUSE) {
+            // unreachable: rejected above
+        } else {
+            switch (bcm) {
+            case 0: launch_apply_rk<T, 64, 1, 8, 1, 0, false, true>(a, lg, s); break;
+            case 1: launch_apply_rk<T, 64, 1, 8, 1, 1, false, true>(a, lg, s); break;
+            case 2: launch_apply_rk<T, 64, 1, 8, 1, 2, false, true>(a, lg, s); break;
+            default: launch_apply_rk<T, 64, 1, 8, 1, 3, false, true>(a, lg, s); break;
+            }
+        }
+    } else if (RG == 4) {
         if constexpr (FUSE) {
             // unreachable: the fused step is limited to one or two chunks
         } else {
@@ -1233,6 +1316,11 @@ template <typename T, bool FUSE = false> void launch_apply(ApplyArgs<T> a, hipSt
         launch_apply_rk<T, 64, 2, 4, 1>(a, lg, s);
     } else {
         launch_apply_rk<T, 64, 4, 4, 1>(a, lg, s);
+    }
+    if (a.rsplit > 1) {
+        const int64_t n = (int64_t)a.nsd * a.nx * a.nx * (a.rsplit - 1);
+        const int blocks = (int)std::min<int64_t>(1024, (n + 255) / 256);
+        hipLaunchKernelGGL(epsdivdiv_iface_kernel<T>, dim3(blocks), dim3(256), 0, s, a);
     }
     TNG_HIP_CHECK(hipGetLastError());
 }
@@ -1341,6 +1429,13 @@ template <typename T> void apply_impl(tng_operator &op, const tng_vector &src, t
         // penalty: the per-block moments only; every apply workgroup combines them
         const bool pen = op.penalty_active;
         ApplyArgs<T> a = apply_args<T>(op, src, dst, false, add);
+        const ApplyShape sh = apply_shape(m.nr - 1);
+        if (sh.S > 1) {
+            // partial sums of the levels between radial parts
+            const size_t bytes = sizeof(T) * 6 * (sh.S - 1) * (size_t)m.nsd * m.nx * m.nx;
+            if (op.d_iface.bytes < bytes) op.d_iface.alloc(bytes);
+            a.iface = op.d_iface.as<T>();
+        }
         if (pen) {
             penalty_partials<T>(op, src, s);
             a.pen_part = m.ctx->red_partials.as<double>();
@@ -1370,7 +1465,7 @@ bool apply_cheb_impl(tng_operator &op, const tng_vector &x, tng_vector &ax, tng_
         m.rad_sdr != 0)
         return false;
     const ApplyShape sh = apply_shape(m.nr - 1);
-    if (!(sh.R == 64 && sh.K == 1 && sh.NW == 8 && sh.RG <= 2)) return false;
+    if (!(sh.R == 64 && sh.K == 1 && sh.NW == 8 && sh.RG <= 2 && sh.S == 1)) return false;
     tng_context &ctx = *m.ctx;
     hipEvent_t *ev = ctx.profiling ? ctx.prof_next_set() : nullptr;
     if (ev) TNG_HIP_CHECK(hipEventRecord(ev[0], s));

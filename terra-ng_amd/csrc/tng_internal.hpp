@@ -248,6 +248,7 @@ struct tng_operator {
     tng::DevBuf d_btab;
     bool btab_valid = false;
     tng::DevBuf d_penalty_b;  // 3 doubles: eps * G^{-1} m, written on device
+    tng::DevBuf d_iface;      // partial sums between radial parts (apply sweeps of 256 layers)
     // stored local matrices (OperatorStoredMatrixMode::Full): (sd, xc, yc, rc, wedge, 18, 18)
     bool stored = false;
     tng_dtype lmat_dtype = TNG_F64;
