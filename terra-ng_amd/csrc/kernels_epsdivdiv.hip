@@ -1051,9 +1051,12 @@ template <typename T> struct PenArgs {
 // Rotation moments m = sum over OWNED copies of x (x) u (x = radius * unit
 // coordinate): per-block partials, flat over the node copies (coalesced
 // streams of src and the mask), four independent nodes per thread and step.
-template <typename T> __global__ __launch_bounds__(256) void penalty_partial_kernel(const PenArgs<T> a)
+constexpr int kPenThreads = 512;
+
+template <typename T> __global__ __launch_bounds__(kPenThreads) void penalty_partial_kernel(const PenArgs<T> a)
 {
-    __shared__ double red[4][3];
+    constexpr int NWV = kPenThreads / 64;
+    __shared__ double red[NWV][3];
     double m0 = 0, m1 = 0, m2 = 0;
     const int n = (int)a.n, nr = a.nr;
     const int stride = gridDim.x * blockDim.x;
@@ -1081,7 +1084,7 @@ template <typename T> __global__ __launch_bounds__(256) void penalty_partial_ker
     __syncthreads();
     if (threadIdx.x == 0) {
         double s[3] = {0, 0, 0};
-        for (int w = 0; w < 4; ++w)
+        for (int w = 0; w < NWV; ++w)
             for (int c = 0; c < 3; ++c) s[c] += red[w][c];
         for (int c = 0; c < 3; ++c) a.partials[3 * blockIdx.x + c] = s[c];
     }
@@ -1221,15 +1224,16 @@ struct ApplyShape {
     int R, K, RG, NW, S;
 };
 
-ApplyShape apply_shape(int ncells)
+ApplyShape apply_shape(int ncells, bool fused = false)
 {
     ApplyShape sh{64, 1, 1, 8, 1};
     const int chunks = env_int("TNG_APPLY_RADIAL_CHUNKS", 0);
-    if (ncells == 256 && chunks == 0) {
-        // four radial parts swept like separate subdomains (eight one-column
-        // waves per workgroup, seven new columns each), their shared levels
-        // completed by the interface kernel
-        sh.S = 4;
+    const bool split = env_int("TNG_APPLY_RADIAL_SPLIT", 1) != 0;
+    if ((ncells == 128 || ncells == 256) && chunks == 0 && split && !fused) {
+        // radial parts of 64 cells swept like separate subdomains (eight
+        // one-column waves per workgroup, seven new columns each), their
+        // shared levels completed by the interface kernel
+        sh.S = ncells / 64;
     } else if (ncells <= 64) {
         sh.R = 4;
         while (sh.R < ncells) sh.R *= 2;
@@ -1241,7 +1245,7 @@ ApplyShape apply_shape(int ncells)
         sh.K = ncells / 64;
         sh.NW = 4;
     }
-    if (sh.K == 1 && sh.R == 64 && sh.RG == 1 && env_int("TNG_APPLY_WAVES", 8) == 4) sh.NW = 4;
+    if (sh.K == 1 && sh.R == 64 && sh.RG == 1 && sh.S == 1 && env_int("TNG_APPLY_WAVES", 8) == 4) sh.NW = 4;
     return sh;
 }
 
@@ -1251,7 +1255,7 @@ template <typename T, bool FUSE = false> void launch_apply(ApplyArgs<T> a, hipSt
     TNG_REQUIRE((int64_t)a.nsd * a.nx * a.nx * a.nr < ((int64_t)1 << 31), "node copies exceed 32-bit indexing");
     // shapes: up to 64 radial cells one cell per lane (R lanes per column);
     // 128 and 256 cells in two or four radial chunks of one wave each; more cells K per lane
-    const ApplyShape sh = apply_shape(a.ncells);
+    const ApplyShape sh = apply_shape(a.ncells, FUSE);
     const int R = sh.R, K = sh.K, RG = sh.RG, NW = sh.NW;
     TNG_REQUIRE(K <= 4, "more than 256 radial layers per subdomain is not supported");
     a.rsplit = sh.S;
@@ -1356,7 +1360,7 @@ template <typename T> PenArgs<T> pen_args(tng_operator &op, const tng_vector &sr
 template <typename T> void penalty_partials(tng_operator &op, const tng_vector &src, hipStream_t s)
 {
     const PenArgs<T> p = pen_args<T>(op, src);
-    hipLaunchKernelGGL(penalty_partial_kernel<T>, dim3(kPenBlocks), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(penalty_partial_kernel<T>, dim3(kPenBlocks), dim3(kPenThreads), 0, s, p);
     TNG_HIP_CHECK(hipGetLastError());
 }
 
@@ -1382,7 +1386,7 @@ template <typename T> PenArgs<T> pen_args(tng_operator &op, const tng_vector &sr
 template <typename T> void penalty_moment(tng_operator &op, const tng_vector &src, hipStream_t s)
 {
     const PenArgs<T> p = pen_args<T>(op, src);
-    hipLaunchKernelGGL(penalty_partial_kernel<T>, dim3(kPenBlocks), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(penalty_partial_kernel<T>, dim3(kPenBlocks), dim3(kPenThreads), 0, s, p);
     hipLaunchKernelGGL(penalty_final_kernel<T>, dim3(1), dim3(256), 0, s, p, kPenBlocks);
     TNG_HIP_CHECK(hipGetLastError());
 }
@@ -1464,7 +1468,7 @@ bool apply_cheb_impl(tng_operator &op, const tng_vector &x, tng_vector &ax, tng_
     if (op.stored || op.diagonal || op.apply_mode != TNG_REPLACE || op.comm_mode != TNG_COMMUNICATE_ADDITIVELY ||
         m.rad_sdr != 0)
         return false;
-    const ApplyShape sh = apply_shape(m.nr - 1);
+    const ApplyShape sh = apply_shape(m.nr - 1, true);
     if (!(sh.R == 64 && sh.K == 1 && sh.NW == 8 && sh.RG <= 2 && sh.S == 1)) return false;
     tng_context &ctx = *m.ctx;
     hipEvent_t *ev = ctx.profiling ? ctx.prof_next_set() : nullptr;

@@ -87,16 +87,16 @@ def test_parity_shapes(gpu_ctx, L, Lr, ls, rs, bcs):
 
 
 @pytest.mark.parametrize("L,Lr,env", [(2, 7, {"TNG_APPLY_RADIAL_CHUNKS": "1"}), (3, 6, {"TNG_APPLY_WAVES": "4"}),
-                                       (1, 8, {"TNG_APPLY_RADIAL_CHUNKS": "1"}), (1, 8, {"TNG_APPLY_RADIAL_CHUNKS": "4"}),
-                                       (1, 8, {}), (2, 8, {}),
+                                       (1, 8, {"TNG_APPLY_RADIAL_CHUNKS": "1"}), (1, 8, {"TNG_APPLY_RADIAL_SPLIT": "0"}),
+                                       (1, 8, {}), (2, 8, {}), (2, 7, {"TNG_APPLY_RADIAL_SPLIT": "1"}),
                                        (3, 7, {}), (3, 6, {})])
 @pytest.mark.parametrize("bcs", [(D, D), (F, F), (N, N), (D, F), (N, F), (F, D)])
 def test_parity_launch_variants(gpu_ctx, L, Lr, env, bcs, monkeypatch):
-    """Launch shapes chosen by layer count or tuning variable: 128 layers as
-    two radial chunks (default) or two cells per lane; 256 layers as four
-    radial parts swept separately (default), four radial chunks or four cells
-    per lane; 4-wave workgroups at 64 layers; every boundary-condition mix of
-    the one-column kernels."""
+    """Launch shapes chosen by layer count or tuning variable: 128 and 256
+    layers as radial parts of 64 cells swept separately, as radial chunks of
+    one wave each in one workgroup, or as two or four cells per lane; 4-wave
+    workgroups at 64 layers; every boundary-condition mix of the one-column
+    kernels."""
     for k, v in env.items():
         monkeypatch.setenv(k, v)
     dom, om, w = setup(L, Lr)
@@ -110,22 +110,23 @@ def test_parity_launch_variants(gpu_ctx, L, Lr, env, bcs, monkeypatch):
         assert rel_err(g, ref) < 1e-12
 
 
+@pytest.mark.parametrize("Lr", [7, 8])
 @pytest.mark.parametrize("bcs", [(D, D), (F, F), (F, D)])
 @pytest.mark.parametrize("dtype", [tng.F64, tng.F32])
-def test_radial_parts_match_radial_chunks(gpu_ctx, bcs, dtype, monkeypatch):
-    """256 layers: the four separately swept radial parts plus the interface
-    kernel give the same bits as four radial chunks in one workgroup (same
-    sums in the same order), in both apply modes."""
-    dom, om, w = setup(2, 8)
+def test_radial_parts_match_radial_chunks(gpu_ctx, Lr, bcs, dtype, monkeypatch):
+    """128 and 256 layers: the separately swept radial parts plus the
+    interface kernel give the same bits as radial chunks in one workgroup
+    (same sums in the same order), in both apply modes."""
+    dom, om, w = setup(2, Lr)
     k = w.eta_lognormal(102)
     u = w.velocity(101)
     d0 = w.velocity(103)
     out = {}
-    for env in ("0", "4"):
-        monkeypatch.setenv("TNG_APPLY_RADIAL_CHUNKS", env)
+    for env in ("1", "0"):
+        monkeypatch.setenv("TNG_APPLY_RADIAL_SPLIT", env)
         out[env] = [gpu_apply(dom, k, u, bcs, eps=1.0, dtype=dtype, apply_mode=m, dst0=d0)
                     for m in (tng.REPLACE, tng.ADD)]
-    for a, b in zip(out["0"], out["4"]):
+    for a, b in zip(out["1"], out["0"]):
         assert np.array_equal(a, b)
 
 
