@@ -130,6 +130,33 @@ def test_radial_parts_match_radial_chunks(gpu_ctx, Lr, bcs, dtype, monkeypatch):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("dtype,tol", [(tng.F64, 1e-12), (tng.F32, 2e-6)])
+def test_penalty_moment_unaligned_source(gpu_ctx, dtype, tol):
+    """Free slip on both surfaces (rotation penalty on) with wrapped source
+    components that start one value past an aligned address."""
+    dom, om, w = setup(3)
+    big, _, _ = setup(4)  # raw storage with room for one extra value
+    k = w.eta_layered(102)
+    u = w.velocity(101)
+    n = dom.num_alloc
+    store = []
+    for c in range(3):
+        raw = np.zeros(big.num_alloc)
+        raw[1:1 + n] = u[c].ravel()
+        store.append(tng.VectorQ1Scalar(big, dtype).upload(raw.reshape(big.shape)))
+    esize = 8 if dtype == tng.F64 else 4
+    src = tng.VectorQ1Vec.wrap(dom, dtype, [v.device_ptr() + esize for v in store], keepalive=store)
+    kv = tng.VectorQ1Scalar(dom, dtype).upload(k)
+    op = tng.EpsilonDivDivKerngen(dom, kv, (F, F))
+    op.set_penalty_epsilon(1.0)
+    d = tng.VectorQ1Vec(dom, dtype)
+    op.apply(src, d)
+    ref = om.epsdivdiv(k, u, F, F, path="fast", penalty_eps=1.0)
+    assert rel_err(d.download().astype(np.float64), ref) < tol
+    aligned = gpu_apply(dom, k, u, (F, F), eps=1.0, dtype=dtype)
+    assert rel_err(d.download().astype(np.float64), aligned) < 50 * np.finfo(np.float64 if esize == 8 else np.float32).eps
+
+
 @pytest.mark.parametrize("mode", [(tng.REPLACE, tng.SKIP_COMMUNICATION), (tng.ADD, tng.SKIP_COMMUNICATION),
                                   (tng.ADD, tng.COMMUNICATE_ADDITIVELY)])
 @pytest.mark.parametrize("bcs", [(D, D), (F, F)])
