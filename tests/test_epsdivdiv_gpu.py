@@ -73,11 +73,12 @@ def test_parity_bcs(gpu_ctx, bcs, diag):
 
 @pytest.mark.parametrize("L,Lr,ls,rs", [(1, 1, 0, 0), (2, 2, 0, 0), (2, 4, 0, 0), (3, 3, 1, 0), (3, 3, 0, 1),
                                         (3, 3, 1, 1), (2, 5, 0, 0), (2, 6, 0, 0), (2, 7, 0, 0), (2, 7, 0, 1),
-                                        (3, 6, 1, 0), (3, 7, 1, 0), (4, 6, 2, 0)])
+                                        (3, 6, 1, 0), (3, 7, 1, 0), (4, 6, 2, 0), (2, 8, 0, 1), (1, 9, 0, 1)])
 @pytest.mark.parametrize("bcs", [(D, D), (F, F)])
 def test_parity_shapes(gpu_ctx, L, Lr, ls, rs, bcs):
-    """Every lane mapping (4..64 radial lanes, 2 cells per lane for 128
-    layers) and subdomain split."""
+    """Every lane mapping (4..64 radial lanes, radial parts for 128 and 256
+    layers) and subdomain split, radial parts within radially split
+    subdomains included."""
     dom, om, w = setup(L, Lr, ls, rs)
     k = w.eta_lognormal(102)
     u = w.velocity(101)
