@@ -1229,7 +1229,7 @@ ApplyShape apply_shape(int ncells, bool fused = false)
     ApplyShape sh{64, 1, 1, 8, 1};
     const int chunks = env_int("TNG_APPLY_RADIAL_CHUNKS", 0);
     const bool split = env_int("TNG_APPLY_RADIAL_SPLIT", 1) != 0;
-    if ((ncells == 128 || ncells == 256) && chunks == 0 && split && !fused) {
+    if (ncells >= 128 && chunks == 0 && split && !fused) {
         // radial parts of 64 cells swept like separate subdomains (eight
         // one-column waves per workgroup, seven new columns each), their
         // shared levels completed by the interface kernel
@@ -1254,10 +1254,11 @@ template <typename T, bool FUSE = false> void launch_apply(ApplyArgs<T> a, hipSt
     TNG_REQUIRE(a.ncells >= 1 && (a.ncells & (a.ncells - 1)) == 0, "radial layers per subdomain must be a power of 2");
     TNG_REQUIRE((int64_t)a.nsd * a.nx * a.nx * a.nr < ((int64_t)1 << 31), "node copies exceed 32-bit indexing");
     // shapes: up to 64 radial cells one cell per lane (R lanes per column);
-    // 128 and 256 cells in two or four radial chunks of one wave each; more cells K per lane
+    // more as radial parts of 64 cells, or (128 and 256 cells) radial chunks
+    // of one wave each or K cells per lane
     const ApplyShape sh = apply_shape(a.ncells, FUSE);
     const int R = sh.R, K = sh.K, RG = sh.RG, NW = sh.NW;
-    TNG_REQUIRE(K <= 4, "more than 256 radial layers per subdomain is not supported");
+    TNG_REQUIRE(K <= 4, "more than 256 radial layers per subdomain need the radial parts");
     a.rsplit = sh.S;
     TNG_REQUIRE(sh.S == 1 || (R == 64 && K == 1 && !FUSE && a.iface), "radial parts need the one-column kernels");
     TNG_REQUIRE(!FUSE || (R == 64 && K == 1 && NW == 8 && RG <= 2),

@@ -89,15 +89,15 @@ def test_parity_shapes(gpu_ctx, L, Lr, ls, rs, bcs):
 
 @pytest.mark.parametrize("L,Lr,env", [(2, 7, {"TNG_APPLY_RADIAL_CHUNKS": "1"}), (3, 6, {"TNG_APPLY_WAVES": "4"}),
                                        (1, 8, {"TNG_APPLY_RADIAL_CHUNKS": "1"}), (1, 8, {"TNG_APPLY_RADIAL_SPLIT": "0"}),
-                                       (1, 8, {}), (2, 8, {}), (2, 7, {"TNG_APPLY_RADIAL_SPLIT": "1"}),
+                                       (1, 8, {}), (2, 8, {}), (2, 7, {"TNG_APPLY_RADIAL_SPLIT": "1"}), (1, 9, {}),
                                        (3, 7, {}), (3, 6, {})])
 @pytest.mark.parametrize("bcs", [(D, D), (F, F), (N, N), (D, F), (N, F), (F, D)])
 def test_parity_launch_variants(gpu_ctx, L, Lr, env, bcs, monkeypatch):
-    """Launch shapes chosen by layer count or tuning variable: 128 and 256
-    layers as radial parts of 64 cells swept separately, as radial chunks of
-    one wave each in one workgroup, or as two or four cells per lane; 4-wave
-    workgroups at 64 layers; every boundary-condition mix of the one-column
-    kernels."""
+    """Launch shapes chosen by layer count or tuning variable: 128, 256 and
+    512 layers as radial parts of 64 cells swept separately; 128 and 256 as
+    radial chunks of one wave each in one workgroup, or as two or four cells
+    per lane; 4-wave workgroups at 64 layers; every boundary-condition mix of
+    the one-column kernels."""
     for k, v in env.items():
         monkeypatch.setenv(k, v)
     dom, om, w = setup(L, Lr)
