@@ -33,12 +33,15 @@ sys.path.insert(0, os.path.join(ROOT, "terra-ng_amd"))
 
 METRIC = "EpsilonDivDiv GDoF/s; Stokes solve s to 1e-8 rel, FP32-MG vs FP64 (1 GPU)"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 vector
 
 
-def measured_traffic(level, bcs, dtype):
-    """HBM bytes per apply-kernel launch from the committed rocprof counter
-    record (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated against a copy
-    kernel), used only while the kernel source is the one that was profiled."""
+def profiled_record(level, bcs, dtype):
+    """The committed rocprof counter record of the apply kernel: HBM bytes per
+    launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated against a copy
+    kernel) and FP64 FLOP per launch (64 lanes x (ADD + MUL + TRANS + 2 FMA)
+    FP64 wave instructions).  Used only while the kernel source is the one that
+    was profiled."""
     if (level, bcs, dtype) != (6, "freeslip", "f64"):
         return None
     rec_path = os.path.join(ROOT, "profiles", "epsdivdiv_apply_L6_freeslip.traffic.json")
@@ -49,7 +52,7 @@ def measured_traffic(level, bcs, dtype):
             sha = hashlib.sha256(f.read()).hexdigest()
     except (OSError, KeyError, ValueError):
         return None
-    return rec["traffic_bytes"] if sha == rec["source_sha256"] else None
+    return rec if sha == rec["source_sha256"] else None
 
 
 def solve_leg(args, tng, inputs, np):
@@ -226,7 +229,10 @@ def main():
     achieved = alg_bytes / (main_ms * 1e-3) / 1e9 if main_ms > 0 else None
     # whole apply (adds the penalty's src re-read and the halo traffic)
     pen_bytes = (3 * esize + 1) * nalloc if bc == tng.FREESLIP else 0
-    traffic = measured_traffic(L, args.bcs, args.dtype)
+    rec = profiled_record(L, args.bcs, args.dtype)
+    traffic = rec["traffic_bytes"] if rec else None
+    flops = rec.get("fp64_flops_per_launch") if rec else None
+    tflops = flops / (main_ms * 1e-3) / 1e12 if flops and main_ms > 0 else None
 
     result = {
         "metric": METRIC,
@@ -262,6 +268,15 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
             "traffic": traffic,
             "algorithmic_bytes": alg_bytes,
+        },
+        # the bound that holds: FP64 VALU issue (counted FLOP per launch over
+        # the same live kernel time)
+        "fp64_rate": {
+            "achieved": round(tflops, 2) if tflops else None,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(tflops / FP64_PEAK_TFLOPS, 4) if tflops else None,
+            "flops_per_launch": flops,
         },
     }
 
