@@ -733,6 +733,19 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
                         cc[2] = a.coords[3 * lcol + 2];
                     }
                 }
+                // boundary row and source of this lane's boundary node (lane
+                // 0: CMB, the top lane: surface), fetched while every lane is
+                // active: a lane-dependent source lane keeps the fetch one LDS
+                // permute instead of VALU broadcasts and selects
+                T bt_l[3] = {T(0), T(0), T(0)}, u_l[3] = {T(0), T(0), T(0)};
+                if constexpr (R == 64) {
+                    const int off = rl == 0 ? 0 : 3;
+    #pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        bt_l[c] = lane_fetch(rec, REC_BTAB + off + c);
+                        u_l[c] = lane_fetch(rec, REC_UCMB + off + c);
+                    }
+                }
                 T pbn[3] = {T(0), T(0), T(0)};
                 if (pen_on) {
     #pragma clang fp contract(off)
@@ -765,12 +778,10 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
                         const int bcv = at_cmb ? a.bc_cmb : a.bc_surf;
                         T bt[3], u[3];
                         if constexpr (R == 64) {
-                            // lane 0 takes the CMB entries, the top lane the surface ones
-                            const int off = at_cmb ? 0 : 3;
     #pragma unroll
                             for (int c = 0; c < 3; ++c) {
-                                bt[c] = lane_fetch(rec, REC_BTAB + off + c);
-                                u[c] = lane_fetch(rec, REC_UCMB + off + c);
+                                bt[c] = bt_l[c];
+                                u[c] = u_l[c];
                             }
                         } else {
                             const T *btp = a.btab + (lcol * 2 + (at_cmb ? 0 : 1)) * 3;
