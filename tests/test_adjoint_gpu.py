@@ -4,7 +4,8 @@
   homogeneity and finite differences in tests/test_oracle_adjoint_pins.py).
 - dJ/dlog(eta) of J = 1/2 ||u - u_obs||^2 (OWNED velocity dofs) through the
   FP64 Stokes solve against central finite differences (h = 1e-6 in log eta,
-  solves converged to 1e-13): |g.d - FD| / |FD| <= 1e-6.
+  solves converged to 1e-13): |g.d - FD| / |FD| <= 1e-6, at level 4 (3
+  directions) and at level 5 with eta_true log-normal and 20 directions.
 """
 import numpy as np
 import pytest
@@ -79,6 +80,50 @@ def test_stokes_adjoint_finite_differences(gpu_ctx):
     gh = g.download()
     h = 1e-6
     for seed in (7, 8, 9):
+        d = consistent(of, np.random.default_rng(seed).normal(size=of.shape))
+        fd = (J(log_eta + h * d) - J(log_eta - h * d)) / (2 * h)
+        gd = float(np.sum((gh * d)[own]))
+        assert abs(gd - fd) <= 1e-6 * abs(fd), (seed, gd, fd)
+
+
+def test_adjoint_level5_twenty_directions(gpu_ctx):
+    """The adjoint case at its stated size: velocity level 5 / pressure
+    level 4, Dirichlet on both surfaces. u_obs is the FP64 solve at
+    eta_true = exp(N(0, 1)); the gradient of J with respect to nodal log eta
+    is taken at eta_true * exp(0.1 N(0, 1)) and checked against central finite
+    differences (h = 1e-6) along 20 seeded directions."""
+    of = O.Mesh(5)
+    own = of.ownership_mask == 1
+    S = tng.StokesSolver(2, 5, 0.55, 1.0, (D, D), tng.StokesParams.defaults(
+        krylov_restart=50, krylov_max_iterations=800, krylov_relative_tolerance=1e-13,
+        krylov_absolute_tolerance=1e-30))
+    dv, dp = S.domain_velocity, S.domain_pressure
+    rng = np.random.default_rng(50)
+    log_eta_true = consistent(of, rng.normal(size=of.shape))
+    log_eta = log_eta_true + 0.1 * consistent(of, rng.normal(size=of.shape))
+    fu, fp = vec(dv, 3, field(of, 3, 51)), vec(dp, 1)
+    S.enforce_rhs(fu)
+
+    def solve(le):
+        S.set_viscosity(vec(dv, 1, np.exp(le)))
+        u, p = vec(dv, 3), vec(dp, 1)
+        rep = S.solve(u, p, fu, fp)
+        assert rep["converged"] == 1, rep
+        return u
+
+    u_obs = solve(log_eta_true).download()
+
+    def J(le):
+        d = solve(le).download() - u_obs
+        return 0.5 * float(np.sum(d[:, own] ** 2))
+
+    u = solve(log_eta)
+    g = vec(dv, 1)
+    rep = S.adjoint_gradient(u, vec(dv, 3, u.download() - u_obs), g)
+    assert rep["converged"] == 1, rep
+    gh = g.download()
+    h = 1e-6
+    for seed in range(100, 120):
         d = consistent(of, np.random.default_rng(seed).normal(size=of.shape))
         fd = (J(log_eta + h * d) - J(log_eta - h * d)) / (2 * h)
         gd = float(np.sum((gh * d)[own]))
