@@ -5,7 +5,8 @@ oracle's operator applies at small levels.
   largest eigenvalue of D^{-1} A;
 - one Chebyshev sweep (linalg/solvers/chebyshev.hpp:102-165) vs the closed-form
   Chebyshev error polynomial T_p((theta - M)/delta) / T_p(theta/delta);
-- PCG (linalg/solvers/pcg.hpp:81-170) vs textbook CG on the dense matrix;
+- PCG (linalg/solvers/pcg.hpp:81-170) vs textbook CG on the dense matrix, and
+  at level 4 vs textbook CG with the oracle's apply;
 - the Stokes operator (epsilon_divdiv_stokes.hpp:55-84) vs the oracle blocks;
 - V-cycle contraction, and the FGMRES Stokes solve (fgmres.hpp:109-373 with the
   block-triangular preconditioner) vs a dense direct solve, in FP64 and with
@@ -165,6 +166,49 @@ def test_pcg_matches_textbook_cg(level1, gpu_ctx):
     it, rel = tng.pcg_solve(op, x, b, max_iterations=500, rtol=1e-12, atol=0.0)
     assert rel <= 1e-12
     assert rel_err(to_unique(om, x.download()), to_unique(om, xs)) < 1e-9
+
+
+def test_cg_level4_dirichlet(gpu_ctx):
+    """The CPU-sized configuration's solve: level-4 shell, eta = exp(z) with z
+    ~ N(0, 1) per node, Dirichlet on both surfaces, CG from x0 = 0. The first
+    20 iterates equal textbook CG in numpy on unique dofs with the oracle's
+    apply as the matvec; run to 1e-10, the true residual b - A x of the result
+    is 1e-10 too."""
+    om = O.Mesh(4)
+    dom = tng.DistributedDomain.create_uniform(4, 4, 0.55, 1.0)
+    k = np.exp(consistent(om, np.random.default_rng(60).normal(size=om.shape)))
+    xs = field(om, 3, 61)
+    op, kv = gpu_op(dom, k)
+    b = vec(dom, 3)
+    op.apply(vec(dom, 3, xs), b)
+    bu = to_unique(om, b.download())
+
+    def A(pu):
+        return to_unique(om, om.epsdivdiv(k, from_unique(om, pu, 3), D, D, path="fast"))
+
+    kmax = 20
+    xr, r = np.zeros_like(bu), bu.copy()
+    p = r.copy()
+    for _ in range(kmax):
+        ap = A(p)
+        a = (r @ r) / (p @ ap)
+        xr += a * p
+        rn = r - a * ap
+        p = rn + (rn @ rn) / (r @ r) * p
+        r = rn
+    x = vec(dom, 3)
+    it, rel = tng.pcg_solve(op, x, b, max_iterations=kmax, rtol=0.0, atol=0.0)
+    assert it == kmax
+    assert rel_err(to_unique(om, x.download()), xr) < 1e-9
+    assert abs(rel - np.linalg.norm(r) / np.linalg.norm(bu)) < 1e-8
+
+    x = vec(dom, 3)
+    it, rel = tng.pcg_solve(op, x, b, max_iterations=5000, rtol=1e-10, atol=0.0)
+    assert rel <= 1e-10, (it, rel)
+    ax = vec(dom, 3)
+    op.apply(x, ax)
+    true_rel = np.linalg.norm(bu - to_unique(om, ax.download())) / np.linalg.norm(bu)
+    assert true_rel < 1e-9, (it, true_rel)
 
 
 # ------------------------------------------------------------------ Stokes
