@@ -112,6 +112,21 @@ class Workload:
         t = (self.r_max - self.r) / (self.r_max - self.r_min)
         return np.exp(np.log(contrast) * t) * np.exp(sigma * normal(seed, 0, self.ids))
 
+    def eta_plumes(self, seed=105, jump=1e2, r_jump=0.85, n_plumes=6, anomaly=1e2, width=0.2, sigma=1.0):
+        """Viscosity of the largest single-GPU case: `jump` below r_jump,
+        divided by up to `anomaly` in n_plumes radial columns around seeded
+        axes (Gaussian in the angle from the nearest axis, `width` radians),
+        times log-normal(sigma) noise."""
+        ax = np.stack([normal(seed, 1 + c, np.arange(n_plumes, dtype=np.uint64)) for c in range(3)], axis=1)
+        ax /= np.linalg.norm(ax, axis=1, keepdims=True)
+        x = self.coords / np.linalg.norm(self.coords, axis=-1, keepdims=True)
+        shape = np.zeros(self.coords.shape[:3])
+        for a in ax:
+            ang = np.arccos(np.clip(x @ a, -1.0, 1.0))
+            shape = np.maximum(shape, np.exp(-0.5 * (ang / width) ** 2))
+        return (np.where(self.r < r_jump, jump, 1.0) * np.exp(-np.log(anomaly) * shape)[..., None]
+                * np.exp(sigma * normal(seed, 0, self.ids)))
+
     def temperature_harmonics(self, seed=104, max_degree=8):
         """sum_{l <= max_degree, |m| <= l} a_lm Y_lm(lat, lon) * sin(pi (r - r_min) / (r_max - r_min)),
         real orthonormal spherical harmonics, a_lm ~ N(0, 1) from the seed (component l*l + l + m)."""
