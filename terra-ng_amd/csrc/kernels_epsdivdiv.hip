@@ -736,9 +736,13 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
                 // boundary row and source of this lane's boundary node (lane
                 // 0: CMB, the top lane: surface), fetched while every lane is
                 // active: a lane-dependent source lane keeps the fetch one LDS
-                // permute instead of VALU broadcasts and selects
+                // permute instead of VALU broadcasts and selects (FP64 kernels
+                // of eight one-column waves without radial chunks or the fused
+                // step: elsewhere the extra registers spill or lower the
+                // occupancy)
+                constexpr bool hoist = R == 64 && !FUSE && sizeof(T) == 8 && NW == 8 && RG == 1;
                 T bt_l[3] = {T(0), T(0), T(0)}, u_l[3] = {T(0), T(0), T(0)};
-                if constexpr (R == 64) {
+                if constexpr (hoist) {
                     const int off = rl == 0 ? 0 : 3;
     #pragma unroll
                     for (int c = 0; c < 3; ++c) {
@@ -777,7 +781,15 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
                     if (at_cmb || at_surf) {
                         const int bcv = at_cmb ? a.bc_cmb : a.bc_surf;
                         T bt[3], u[3];
-                        if constexpr (R == 64) {
+                        if constexpr (R == 64 && !hoist) {
+                            // lane 0 takes the CMB entries, the top lane the surface ones
+                            const int off = at_cmb ? 0 : 3;
+    #pragma unroll
+                            for (int c = 0; c < 3; ++c) {
+                                bt[c] = lane_fetch(rec, REC_BTAB + off + c);
+                                u[c] = lane_fetch(rec, REC_UCMB + off + c);
+                            }
+                        } else if constexpr (R == 64) {
     #pragma unroll
                             for (int c = 0; c < 3; ++c) {
                                 bt[c] = bt_l[c];
