@@ -12,7 +12,9 @@ GDoF/s = 3 * unique nodes / seconds per apply (the reference's definition,
 apps/benchmarks/performance/benchmark_operators.cpp:213-214).
 
 Prints one JSON line.  Also reports the roofline of the apply kernel (HIP
-events on the library stream), the CPU oracle timed on this host, and the
+events on the library stream) against HBM ("roofline") and against the FP64
+vector peak ("fp64_rate", from the counted FP64 instructions per launch),
+the CPU oracle timed on this host, and the
 mixed-precision Stokes solve ("solve"): Q1-iso-Q2/Q1 on the level-7 velocity
 / level-6 pressure grids, no-slip both surfaces, multigrid levels 2..7,
 viscosity contrast 1e4 (radial profile) times log-normal(1) noise, buoyancy
