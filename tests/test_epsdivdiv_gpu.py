@@ -183,6 +183,38 @@ def test_config2_shape_level5_freeslip(gpu_ctx):
     assert rel_err(g, ref) < 1e-12
 
 
+def test_config2_level6_freeslip_timed_size(gpu_ctx):
+    """The timed apply workload itself: level 6 (10 x 64x64 x 64 cells),
+    free slip on both surfaces, default penalty 1e-7, layered 1e3 jump at
+    r = 0.85 times log-normal(0.5), u ~ U[-1, 1] (seeds 101/102).  This runs
+    the 64-lane FP64 kernel with one round of resident workgroups, each
+    sweeping a share of many node rows across strip boundaries."""
+    dom, om, w = setup(6)
+    k = w.eta_layered(102)
+    u = w.velocity(101)
+    g = gpu_apply(dom, k, u, (F, F))
+    ref = om.epsdivdiv(k, u, F, F, path="fast")
+    assert rel_err(g, ref) < 1e-12
+    own = om.ownership_mask == 1
+    assert rel_err(g, ref, own) < 1e-12
+
+
+@pytest.mark.parametrize("L,blocks", [(3, 5), (3, 7), (4, 3), (4, 37)])
+@pytest.mark.parametrize("bcs", [(F, F), (D, D), (F, D)])
+def test_balanced_shares_cross_strips(gpu_ctx, L, blocks, bcs, monkeypatch):
+    """Few workgroups: every share of node rows runs over several strip
+    boundaries (and subdomain boundaries inside a strip)."""
+    monkeypatch.setenv("TNG_APPLY_BLOCKS", str(blocks))
+    dom, om, w = setup(L)
+    k = w.eta_layered(102)
+    u = w.velocity(101)
+    d0 = w.velocity(103)
+    for mode in (tng.REPLACE, tng.ADD):
+        g = gpu_apply(dom, k, u, bcs, eps=1.0, apply_mode=mode, dst0=d0)
+        ref = om.epsdivdiv(k, u, *bcs, path="fast", penalty_eps=1.0, apply_mode=mode, dst=d0)
+        assert rel_err(g, ref) < 1e-12
+
+
 # FP32 rounding is amplified by the cancellation between lateral and radial
 # stencil terms, which grows with the cells' aspect ratio (lateral width over
 # radial height: about 1.4 at (4, 4), 40 at (2, 7), 80 at (2, 8)).
