@@ -16,12 +16,13 @@ events on the library stream) against HBM ("roofline") and against the FP64
 vector peak ("fp64_rate", from the counted FP64 instructions per launch),
 the CPU oracle timed on this host, and the
 mixed-precision Stokes solve ("solve"): Q1-iso-Q2/Q1 on the level-7 velocity
-/ level-6 pressure grids, no-slip both surfaces, multigrid levels 2..7,
+/ level-6 pressure grids, no-slip both surfaces, multigrid levels 2..7 with a
+Galerkin (GCA) coarse operator on level 2 formed top-down from level 7,
 viscosity contrast 1e4 (radial profile) times log-normal(1) noise, buoyancy
 RHS from a degree <= 8 spherical-harmonic temperature, FGMRES(30) to 1e-8
 relative with the FP32 V-cycle preconditioner and again with the FP64 one;
-seconds per solve, iterations, and the FP32 solution's error against the FP64
-solve.
+seconds per solve, iterations, and the FP32 solution's velocity and pressure
+errors against the FP64 solve.
 """
 import argparse
 import hashlib
@@ -60,16 +61,29 @@ def profiled_record(level, bcs, dtype):
 def solve_leg(args, tng, inputs, np):
     """Mixed-precision Stokes solve against the FP64-preconditioned solve, same inputs."""
     lv = args.solve_level
+    if args.solve_gca == 0:
+        coarse = f"re-discretised coarse operators (gca=0)"
+    else:
+        where = (f"Galerkin operator on the coarsest level {args.solve_min_level} only (formed top-down from level "
+                 f"{lv}; levels {args.solve_min_level + 1}..{lv - 1} matrix-free)" if args.solve_gca_coarsest_only
+                 else f"Galerkin operators on levels {args.solve_min_level}..{lv - 1}")
+        rule = "exact R A P" if args.solve_gca_exact else "the reference's top-face interpolation rule"
+        coarse = f"GCA (gca={args.solve_gca}): {where}, {rule}"
+    smoother = "damped Jacobi" if args.solve_smoother == 1 else "Chebyshev order 3"
+    orth = "CGS2 orthogonalisation" if args.solve_orth == 1 else "modified Gram-Schmidt"
     out = {"workload": f"Stokes solve, velocity level {lv} / pressure level {lv - 1}, multigrid levels "
                        f"{args.solve_min_level}..{lv}, no-slip both surfaces, eta = 1e4 radial contrast x "
                        f"lognormal(1), buoyancy RHS (T: spherical harmonics l<=8), FGMRES({args.solve_restart}) "
-                       f"to {args.solve_rtol:g} rel, Chebyshev order 3, gca={args.solve_gca}"}
+                       f"to {args.solve_rtol:g} rel, {orth}, {smoother}, {coarse}"}
     sols = {}
     for name, dt in (("fp64_mg", tng.F64), ("fp32_mg", tng.F32)):
         prm = tng.StokesParams.defaults(krylov_restart=args.solve_restart,
                                         krylov_max_iterations=args.solve_max_iterations,
                                         krylov_relative_tolerance=args.solve_rtol, krylov_absolute_tolerance=1e-30,
-                                        viscous_pc_chebyshev_order=3, viscous_pc_dtype=dt, gca=args.solve_gca)
+                                        viscous_pc_chebyshev_order=3, viscous_pc_dtype=dt, gca=args.solve_gca,
+                                        gca_coarsest_only=args.solve_gca_coarsest_only,
+                                        gca_exact=args.solve_gca_exact, viscous_pc_smoother=args.solve_smoother,
+                                        krylov_orthogonalization=args.solve_orth)
         print(f"[bench] solve {name}: building levels {args.solve_min_level}..{lv}", file=sys.stderr, flush=True)
         c0 = time.perf_counter()
         S = tng.StokesSolver(args.solve_min_level, lv, 0.55, 1.0, (tng.DIRICHLET, tng.DIRICHLET), prm)
@@ -93,6 +107,7 @@ def solve_leg(args, tng, inputs, np):
         sec = time.perf_counter() - t0
         own = dv.ownership_mask() == 1
         sols[name] = u.download()[:, own]
+        sols[name + "_p"] = p.download()[dp.ownership_mask() == 1]
         out[name] = {"seconds": round(sec, 4), "setup_seconds": round(setup_s, 3),
                      "iterations": rep["iterations"], "converged": bool(rep["converged"]),
                      "relative_residual": rep["relative_residual"], "coarse_iterations": rep["coarse_iterations"]}
@@ -101,8 +116,53 @@ def solve_leg(args, tng, inputs, np):
         del S, u, p, fu, fp
     ref = sols["fp64_mg"]
     out["fp32_mg_velocity_error_vs_fp64"] = float(np.linalg.norm(sols["fp32_mg"] - ref) / np.linalg.norm(ref))
+    # pressures have their mean over unique nodes removed by the solve
+    pref = sols["fp64_mg_p"]
+    out["fp32_mg_pressure_error_vs_fp64"] = float(np.linalg.norm(sols["fp32_mg_p"] - pref) / np.linalg.norm(pref))
     out["speedup_fp32_mg"] = round(out["fp64_mg"]["seconds"] / out["fp32_mg"]["seconds"], 3)
     return out
+
+
+def adjoint_leg(args, tng, inputs, np):
+    """Viscosity adjoint at the adjoint configuration: velocity level 5 / pressure level 4, no-slip, u_obs the
+    FP64 solve at eta_true = exp(N(0,1)), gradient of J = 1/2 |u - u_obs|^2 w.r.t. nodal log eta at
+    eta_true * exp(0.1 N(0,1)): forward solve + adjoint solve + coefficient-gradient kernel."""
+    lv = args.adjoint_level
+    prm = tng.StokesParams.defaults(krylov_restart=30, krylov_max_iterations=600,
+                                    krylov_relative_tolerance=args.adjoint_rtol, krylov_absolute_tolerance=1e-30,
+                                    viscous_pc_chebyshev_order=3)
+    S = tng.StokesSolver(2, lv, 0.55, 1.0, (tng.DIRICHLET, tng.DIRICHLET), prm)
+    dv, dp = S.domain_velocity, S.domain_pressure
+    w = inputs.Workload(dv.coords(), dv.radii(), lv, lv)
+    z = w.eta_lognormal(106, 1.0)
+    eta_true = z
+    eta = z * np.exp(0.1 * np.log(w.eta_lognormal(107, 1.0)))
+    fu = tng.VectorQ1Vec(dv)
+    S.buoyancy_rhs(tng.VectorQ1Scalar(dv).upload(w.temperature_harmonics(104, 8)), 1e5, fu)
+    fp = tng.VectorQ1Scalar(dp)
+    S.set_viscosity(tng.VectorQ1Scalar(dv).upload(eta_true))
+    u_obs, p0 = tng.VectorQ1Vec(dv), tng.VectorQ1Scalar(dp)
+    S.solve(u_obs, p0, fu, fp)
+    S.set_viscosity(tng.VectorQ1Scalar(dv).upload(eta))
+    dv.ctx.synchronize()
+    t0 = time.perf_counter()
+    u, p = tng.VectorQ1Vec(dv), tng.VectorQ1Scalar(dp)
+    frep = S.solve(u, p, fu, fp)
+    dv.ctx.synchronize()
+    t1 = time.perf_counter()
+    dJdu = tng.VectorQ1Vec(dv)
+    tng.lincomb(dJdu, [1.0, -1.0], [u, u_obs])
+    g = tng.VectorQ1Scalar(dv)
+    arep = S.adjoint_gradient(u, dJdu, g)
+    dv.ctx.synchronize()
+    t2 = time.perf_counter()
+    return {"workload": f"viscosity adjoint, velocity level {lv} / pressure level {lv - 1}, no-slip, eta_true = "
+                        f"lognormal(1), gradient at eta_true x exp(0.1 N(0,1)), FGMRES(30) to {args.adjoint_rtol:g}, "
+                        f"FP64 V-cycle",
+            "forward_seconds": round(t1 - t0, 4), "forward_iterations": frep["iterations"],
+            "adjoint_seconds": round(t2 - t1, 4), "adjoint_iterations": arep["iterations"],
+            "adjoint_relative_residual": arep["relative_residual"],
+            "gradient_norm": float(np.linalg.norm(g.download()))}
 
 
 def io_leg(args, tng, np):
@@ -154,10 +214,20 @@ def main():
     ap.add_argument("--no-solve", action="store_true", help="skip the mixed-precision Stokes solve")
     ap.add_argument("--solve-level", type=int, default=7, help="velocity level of the Stokes solve")
     ap.add_argument("--solve-min-level", type=int, default=2)
-    ap.add_argument("--solve-gca", type=int, default=0)
+    ap.add_argument("--solve-gca", type=int, default=1,
+                    help="0 re-discretised coarse operators, 1 Galerkin (GCA), 2 adaptive GCA")
+    ap.add_argument("--solve-gca-coarsest-only", type=int, default=1,
+                    help="1: the Galerkin operator is used on the coarsest level only")
+    ap.add_argument("--solve-gca-exact", type=int, default=0)
+    ap.add_argument("--solve-smoother", type=int, default=0, help="0 Chebyshev, 1 damped Jacobi")
+    ap.add_argument("--solve-orth", type=int, default=0, help="0 modified Gram-Schmidt, 1 CGS2")
+    ap.add_argument("--only-solve", action="store_true", help="run the Stokes solve leg only")
     ap.add_argument("--solve-restart", type=int, default=30)
     ap.add_argument("--solve-max-iterations", type=int, default=300)
     ap.add_argument("--solve-rtol", type=float, default=1e-8)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the viscosity adjoint timing")
+    ap.add_argument("--adjoint-level", type=int, default=5)
+    ap.add_argument("--adjoint-rtol", type=float, default=1e-8)
     ap.add_argument("--io", action="store_true", help="also time XDMF output of a field set")
     ap.add_argument("--io-level", type=int, default=7)
     ap.add_argument("--io-steps", type=int, default=3)
@@ -175,6 +245,9 @@ def main():
     import tng
     from tng import inputs
 
+    if args.only_solve:
+        print(json.dumps({"solve": solve_leg(args, tng, inputs, np)}))
+        return
     L = args.level
     bc = tng.FREESLIP if args.bcs == "freeslip" else tng.DIRICHLET
     dt = tng.F64 if args.dtype == "f64" else tng.F32
@@ -313,6 +386,12 @@ def main():
             result["solve"] = solve_leg(args, tng, inputs, np)
         except Exception as e:  # the apply measurement stands on its own
             result["solve"] = {"error": f"{type(e).__name__}: {e}"}
+    if not args.no_adjoint:
+        print(f"[bench] adjoint at level {args.adjoint_level}", file=sys.stderr, flush=True)
+        try:
+            result["adjoint"] = adjoint_leg(args, tng, inputs, np)
+        except Exception as e:  # the apply measurement stands on its own
+            result["adjoint"] = {"error": f"{type(e).__name__}: {e}"}
     if args.io:
         print(f"[bench] io: XDMF output at level {args.io_level}", file=sys.stderr, flush=True)
         result["io"] = io_leg(args, tng, np)

@@ -62,6 +62,19 @@ tng_status tng_ctx_create(int device_ordinal, tng_ctx *out);
  * destroyed with the context. */
 tng_status tng_ctx_create_with_stream(int device_ordinal, void *hip_stream, tng_ctx *out);
 tng_status tng_ctx_destroy(tng_ctx ctx);
+/* No counterpart in the reference (Kokkos picks its own launch shapes).
+ * Launch-shape settings of the EpsilonDivDiv apply on this context; every
+ * setting computes the same operator, the defaults are the measured best.
+ * name / default / meaning:
+ *   "apply_blocks"        0  FP64 applies: workgroups sharing the node rows (0: one resident round)
+ *   "apply_seg"           0  FP32 and fused applies: node rows per workgroup (0: chosen per mesh)
+ *   "apply_radial_split"  1  0: 128 and 256 layers as radial chunks of one workgroup
+ *   "apply_radial_chunks" 0  1: 128 and 256 layers as two or four cells per lane
+ *   "apply_waves"         8  4: four-wave workgroups at 64 layers
+ *   "cheb_fuse"           1  0: Chebyshev sub-steps as separate kernels (read when a
+ *                            Stokes solver is created)
+ * A negative value restores the default; an unknown name is refused. */
+tng_status tng_ctx_set_tuning(tng_ctx ctx, const char *name, int value);
 /* Kokkos::fence() */
 tng_status tng_ctx_synchronize(tng_ctx ctx);
 /* Message of the last failed call on this context ("" if none). */
@@ -165,6 +178,19 @@ tng_status tng_op_destroy(tng_op op);
 tng_status tng_op_get_local_matrices(tng_op op, double *host_out);
 tng_status tng_op_set_local_matrices(tng_op op, const double *host_in);
 tng_status tng_op_clear_local_matrices(tng_op op);
+/* set_stored_matrix_mode (epsilon_divdiv_kerngen.hpp:503-530) with
+ * OperatorStoredMatrixMode::Full (full = 1: zero matrices of the operator's
+ * precision, as LocalMatrixStorage allocates them) or Off (full = 0). */
+tng_status tng_op_set_stored_matrix_mode(tng_op op, int full);
+/* GCACapable per-element access (linalg/operator.hpp:80-112;
+ * epsilon_divdiv_kerngen.hpp:533-575): get_local_matrix(sd, x_cell, y_cell,
+ * r_cell, wedge) returns the stored matrix, or the Neumann matrix assembled
+ * from k without storage; set_local_matrix needs stored mode (the reference
+ * asserts it).  mat: 324 doubles, row-major 18 x 18, dof index node + 6 *
+ * dimension (kernel_helpers.hpp:617-651). */
+tng_status tng_op_get_local_matrix(tng_op op, int sd, int x_cell, int y_cell, int r_cell, int wedge, double *mat);
+tng_status tng_op_set_local_matrix(tng_op op, int sd, int x_cell, int y_cell, int r_cell, int wedge,
+                                   const double *mat);
 /* Copy (and convert to dst's precision) the stored FP64 matrices of src. */
 tng_status tng_op_copy_local_matrices(tng_op dst, tng_op src);
 /* linalg/solvers/gca/gca.hpp:41-531 TwoGridGCA(fine_op, coarse_op) with
@@ -173,6 +199,12 @@ tng_status tng_op_copy_local_matrices(tng_op dst, tng_op src);
  * A_fine is the fine operator's stored matrices, or its Neumann local
  * matrices assembled from its (FP64) k when it stores none. */
 tng_status tng_gca_assemble(tng_op fine, tng_op coarse);
+/* As tng_gca_assemble with the Galerkin product R A P: fine nodes on the top
+ * face of an interior coarse cell draw their weights from the coarse top
+ * layer into the coarse wedge's top slots.  The reference's rule
+ * (gca.hpp:206-208, r_idx_coarse_bot = r_fine / 2) places those weights in the
+ * bottom slots; this variant has no counterpart in the reference. */
+tng_status tng_gca_assemble_exact(tng_op fine, tng_op coarse);
 /* Adaptive GCA (gca = 2).  linalg/solvers/gca/gca_elements_collector.hpp:20-96
  * GCAElementsCollector(fine_domain, k, level_range, GCAElements): per fine
  * wedge, |grad k| in reference coordinates at the centroid; above 10 the
@@ -230,6 +262,40 @@ tng_status tng_vec_dot(tng_vec x, tng_vec y, double *out);
 tng_status tng_vec_max_abs(tng_vec x, double *out);
 /* linalg/vector.hpp:189-197 has_nan_or_inf (synchronises) */
 tng_status tng_vec_has_nan_or_inf(tng_vec x, int *out);
+/* norm_2 (linalg/vector.hpp:7-42 via dot_impl; vector_q1.hpp:245-250):
+ * sqrt of the OWNED-masked dot of x with itself. */
+tng_status tng_vec_norm2(tng_vec x, double *out);
+/* dot_impl with the result left on the device: *out_device (a device address,
+ * 8 bytes) receives the OWNED-masked dot, stream-ordered, without a host
+ * synchronisation. */
+tng_status tng_vec_dot_device(tng_vec x, tng_vec y, double *out_device);
+/* randomize_impl (linalg/vector.hpp:7-42; kernels/common/grid_operations.hpp
+ * :748-773 rand): uniform [0, 1) entries on every copy.  The reference draws
+ * from a Kokkos XorShift64 pool with seed 12345 (order not reproducible across
+ * backends); here every entry is SplitMix64 of (seed, component, linear index),
+ * reproducible and independent of the launch. */
+tng_status tng_vec_fill_seeded(tng_vec x, unsigned long long seed);
+
+/* fe/wedge/operators/shell/epsilon_divdiv_stokes.hpp:18-133 EpsDivDivStokes
+ * (domain_fine, domain_coarse, ..., k, bcs, diagonal): K = [A B^T; B 0] with
+ * A = EpsilonDivDivKerngen on the fine (velocity) mesh with coefficient k,
+ * B^T = GradientKerngen, B = DivergenceKerngen (coarse = pressure mesh), and
+ * the Zero block (zero.hpp).  The Stokes operator owns its blocks. */
+tng_status tng_stokes_create(tng_mesh fine, tng_mesh coarse, tng_vec k, tng_bc bc_cmb, tng_bc bc_surface,
+                             int diagonal, tng_op *out);
+/* EpsDivDivStokes::apply_impl (epsilon_divdiv_stokes.hpp:55-84):
+ * out_u = A u + B^T p, out_p = B u; A (Replace, SkipCommunication), B^T (Add,
+ * CommunicateAdditively), B (Replace, CommunicateAdditively), then every
+ * block is reset to Replace + CommunicateAdditively.  Diagonal operator:
+ * out_u = diag(A) u, out_p untouched. */
+tng_status tng_stokes_op_apply(tng_op op, tng_vec u, tng_vec p, tng_vec out_u, tng_vec out_p);
+/* block_11() / block_12() / block_21() / block_22() (epsilon_divdiv_stokes.hpp
+ * :86-133; Block2x2OperatorLike, linalg/operator.hpp:117-141): row, col in
+ * {1, 2}.  The handle is borrowed: it stays valid while the Stokes operator
+ * lives and tng_op_destroy refuses it.  Blocks 12 and 21 of a diagonal
+ * operator are refused, as the reference throws.  Block 22 is Zero: Replace
+ * writes 0, Add leaves dst. */
+tng_status tng_stokes_block(tng_op op, int row, int col, tng_op *block);
 
 /* ---------------------------------------------------------------- solvers */
 
@@ -246,6 +312,9 @@ tng_status tng_power_iteration(tng_op op, tng_vec inv_diag, int iterations, unsi
 /* linalg/solvers/chebyshev.hpp:102-165 with a given estimate of the largest
  * eigenvalue of D^{-1} A (lambda_max = 1.5 est, lambda_min = 0.1 est);
  * `iterations` sweeps of order `order` on A x = b. */
+/* linalg/solvers/jacobi.hpp:16-60 Jacobi(inverse_diagonal, iterations, tmp,
+ * omega).solve(A, x, b): `iterations` times x += omega D^{-1} (b - A x). */
+tng_status tng_jacobi_smooth(tng_op op, tng_vec inv_diag, int iterations, double omega, tng_vec x, tng_vec b);
 tng_status tng_chebyshev_smooth(tng_op op, tng_vec inv_diag, int order, int iterations, double lambda_max_estimate,
                                 tng_vec x, tng_vec b);
 
@@ -267,6 +336,23 @@ typedef struct {
     double coarse_relative_tolerance;           /* 1e-6 */
     double coarse_absolute_tolerance;           /* 1e-16 */
     unsigned long long seed;                    /* 12345 */
+    /* Options of this library (no counterpart in StokesSolverParameters): */
+    int gca_coarsest_only;   /* 0; 1: with gca = 1 or 2 the Galerkin products are formed top-down through
+                                every level, but only the coarsest level's operator uses them (the levels
+                                between keep re-discretised matrix-free operators; the intermediate
+                                matrices are released after the set-up) */
+    int gca_exact;           /* 0: the reference's interpolation rule (gca.hpp:206-208), which takes the
+                                weights of fine nodes on the top face of an interior coarse cell into the
+                                coarse wedge's bottom slots; 1: the Galerkin product R A P */
+    int viscous_pc_smoother; /* 0: Chebyshev (chebyshev.hpp); 1: damped Jacobi (jacobi.hpp:16-60) with
+                                omega = 2 / (1.3 lambda_max), lambda_max of D^{-1} A from the power
+                                iteration (the reference tests' choice, test_epsilon_divdiv_stokes.cpp:563) */
+    int krylov_orthogonalization; /* 0: modified Gram-Schmidt (fgmres.hpp:211-224); 1: classical
+                                     Gram-Schmidt with one re-orthogonalisation (CGS2), the projections of
+                                     a pass fused into one multi-dot */
+    int rotation_correction; /* 0; 1 (free slip on both surfaces): after the block preconditioner, a
+                                correction on the three rigid rotations, the near-null modes of the
+                                penalised free-slip operator: z += Z E^{-1} Z^T (v - K z), E = Z^T K Z */
 } tng_stokes_params;
 
 /* Outcome of a solve.  history (caller memory, may be NULL) receives the
@@ -391,6 +477,85 @@ tng_status tng_xdmf_destroy(tng_xdmf writer);
  * checkpoint must have the mesh's subdomain sizes.  TNG_IO_ERROR on a
  * missing or malformed file. */
 tng_status tng_xdmf_checkpoint_read(const char *directory, const char *label, int step, tng_vec out);
+
+/* ------------------------------------------------------------------------- */
+/* The caller around the Stokes solve: the implicit SUPG energy step and one  */
+/* step of the mantle-circulation time loop.                                  */
+/* ------------------------------------------------------------------------- */
+/* fe/wedge/operators/shell/unsteady_advection_diffusion_supg.hpp:128-206
+ * UnsteadyAdvectionDiffusionSUPG(domain, ..., velocity, diffusivity, dt,
+ * treat_boundary, diagonal, mass_scaling, lumped_mass): the scalar operator
+ * mass_scaling M + dt (K_diff + K_adv + tau K_streamline), 6-point rule,
+ * tau from supg_tau (:19-49); treat_boundary keeps only the diagonal entry of
+ * boundary rows and removes boundary columns; Replace + CommunicateAdditively
+ * by default (tng_op_set_modes applies).  The velocity is captured by
+ * reference (FP64, 3 components, same mesh). */
+tng_status tng_supg_create(tng_mesh mesh, tng_vec velocity, double diffusivity, double dt, int treat_boundary,
+                           int diagonal, double mass_scaling, int lumped_mass, tng_op *out);
+/* dt() (:206-207) and set_supg_enabled (:209-215) */
+tng_status tng_supg_set_dt(tng_op op, double dt);
+tng_status tng_supg_set_enabled(tng_op op, int on);
+/* fe/wedge/operators/shell/mass.hpp: dst = M src, consistent scalar mass,
+ * 6-point rule, Replace + CommunicateAdditively (FP64). */
+tng_status tng_mass_apply(tng_vec src, tng_vec dst);
+
+/* apps/mantlecirculation/src/parameters.hpp:186-201 EnergySolverParameters
+ * (Krylov fields, same defaults), PhysicsParameters::diffusivity (1),
+ * BoundaryConditionsParameters temperature_cmb (1) / temperature_surface (0),
+ * TimeSteppingParameters dt_scaling (0.5) and energy_substeps (1). */
+typedef struct {
+    int krylov_restart;               /* 5 */
+    int krylov_max_iterations;        /* 100 */
+    double krylov_relative_tolerance; /* 1e-6 */
+    double krylov_absolute_tolerance; /* 1e-12 */
+    double diffusivity;               /* 1 */
+    double temperature_cmb;           /* 1 */
+    double temperature_surface;       /* 0 */
+    double dt_scaling;                /* 0.5 */
+    int energy_substeps;              /* 1 */
+} tng_energy_params;
+typedef struct tng_energy_s *tng_energy;
+void tng_energy_params_default(tng_energy_params *params);
+/* apps/mantlecirculation/src/energy_solver.hpp:85-276 SUPGSolver on the Stokes
+ * solver's velocity mesh: advects with `velocity` (captured by reference) and
+ * updates `temperature` in place.  h = the smallest radial spacing. */
+tng_status tng_energy_create(tng_stokes_solver solver, tng_vec velocity, tng_vec temperature,
+                             const tng_energy_params *params, tng_energy *out);
+tng_status tng_energy_destroy(tng_energy energy);
+/* SUPGSolver::compute_dt (:170-182): dt_scaling * h / max |u| (1e-3 for a
+ * velocity below 1e-12). */
+tng_status tng_energy_compute_dt(tng_energy energy, double *dt);
+/* SUPGSolver::step (:194-245): inverse diagonal at dt, then per substep
+ * q = M T, Dirichlet elimination (strong_algebraic_dirichlet_enforcement.hpp
+ * :24-47 with the CMB / surface temperatures) and FGMRES with the diagonal
+ * preconditioner on (M + dt A) T = q, T as the initial guess.  report
+ * (may be NULL) sums the substeps' iterations. */
+tng_status tng_energy_step(tng_energy energy, double dt, tng_solve_report *report);
+/* StokesContext::update_viscosity (stokes_solver.hpp:623-638) with the
+ * Frank-Kamenetskii law (interpolators.hpp:219-244): eta = rmu^(0.5 - T) on
+ * the velocity level only; as in the reference, the coarse levels, inverse
+ * diagonals and smoother estimates keep their values. */
+tng_status tng_stokes_solver_update_viscosity(tng_stokes_solver solver, tng_vec temperature, double rmu);
+/* One step of the time loop (apps/mantlecirculation/mantlecirculation.cpp
+ * :413-445): snapshot T, dt from the current velocity, then per Picard
+ * iteration (T restored after the first) the Stokes solve with the buoyancy of
+ * T (Ra T r/|r|, from the current (u, p)), the energy step, and (law 1) the
+ * viscosity update.  u must be the energy solver's velocity. */
+typedef struct {
+    int picard_iterations;  /* TimeSteppingParameters::picard_iterations (1) */
+    int viscosity_law;      /* 0 constant, 1 Frank-Kamenetskii */
+    double rmu;             /* ViscosityParameters::rmu (1) */
+    double rayleigh_number; /* PhysicsParameters::rayleigh_number (1e5) */
+    /* 0: as the reference, the new viscosity reaches the velocity level only;
+     * 1: then also the coarse levels (injection) with the inverse diagonals,
+     * Galerkin coarse operators, Schur scaling and smoother estimates rebuilt,
+     * as tng_stokes_solver_set_viscosity does (no counterpart in the reference:
+     * for large viscosity contrasts, where the stale hierarchy stalls FGMRES). */
+    int refresh_hierarchy;
+} tng_timestep_params;
+tng_status tng_timestep(tng_stokes_solver solver, tng_energy energy, tng_vec u, tng_vec p,
+                        const tng_timestep_params *params, double *dt, tng_solve_report *stokes_report,
+                        tng_solve_report *energy_report);
 
 #ifdef __cplusplus
 }

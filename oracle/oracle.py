@@ -62,6 +62,10 @@ def lib():
         L.orc_gca.argtypes = [vp, vp, dp, dp, ip]
         L.orc_epsdivdiv_log_k_gradient.argtypes = [vp, dp, ip, ip, dp, dp, dp]
         L.orc_vector_mass_apply.argtypes = [vp, dp, dp]
+        L.orc_supg_tau.restype = C.c_double
+        L.orc_supg_tau.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double]
+        L.orc_supg_apply.argtypes = [vp, dp, C.c_double, C.c_double, C.c_double, ip, ip, ip, ip, ip, ip, dp, dp]
+        L.orc_scalar_mass_apply.argtypes = [vp, dp, dp]
         L.orc_num_threads.restype = ip
         _lib = L
     return _lib
@@ -267,6 +271,29 @@ def vector_mass(mesh, u):
     u = _c(u)
     out = np.zeros((3,) + mesh.shape)
     lib().orc_vector_mass_apply(mesh.h, _dp(u), _dp(out))
+    return out
+
+
+def supg_tau(vel_norm, kappa, h, pe_tol=1e-8):
+    """supg_tau (unsteady_advection_diffusion_supg.hpp:19-49)."""
+    return lib().orc_supg_tau(float(vel_norm), float(kappa), float(h), float(pe_tol))
+
+
+def supg_apply(mesh, vel, src, kappa=1.0, dt=1.0, mass_scaling=1.0, treat_boundary=True, diagonal=False,
+               lumped=False, supg=True, apply_mode=REPLACE, comm=COMMUNICATE, dst=None):
+    """UnsteadyAdvectionDiffusionSUPG apply (unsteady_advection_diffusion_supg.hpp:214-433)."""
+    vel, src = _c(vel), _c(src)
+    out = np.zeros(mesh.shape) if dst is None else np.array(dst, dtype=np.float64, copy=True)
+    lib().orc_supg_apply(mesh.h, _dp(vel), float(kappa), float(dt), float(mass_scaling), int(treat_boundary),
+                         int(diagonal), int(lumped), int(supg), int(apply_mode), int(comm), _dp(src), _dp(out))
+    return out
+
+
+def scalar_mass(mesh, x):
+    """Mass (mass.hpp): consistent scalar mass, additive exchange."""
+    x = _c(x)
+    out = np.zeros(mesh.shape)
+    lib().orc_scalar_mass_apply(mesh.h, _dp(x), _dp(out))
     return out
 
 

@@ -1572,3 +1572,152 @@ void orc_vector_mass_apply(const orc_mesh *m, const double *src, double *dst)
                 }
     orc_halo_sum(m, dst, 3);
 }
+
+/* ------------------------------------------------------------------------- */
+/* Energy equation (the caller after the Stokes solve).                      */
+/* supg_tau: unsteady_advection_diffusion_supg.hpp:19-49.                     */
+/* ------------------------------------------------------------------------- */
+double orc_supg_tau(double vel_norm, double kappa, double h, double Pe_tol)
+{
+    const double kappa_min = 1e-8;
+    const double kappa_virtual = kappa > kappa_min ? kappa : kappa_min;
+    const double eps_vel = 1e-12;
+    if (vel_norm <= eps_vel) return 0.0;
+    const double Pe = vel_norm * h / (2.0 * kappa_virtual);
+    if (Pe <= Pe_tol) return 0.0;
+    const double tau_max = 10.0 * h / (vel_norm > eps_vel ? vel_norm : eps_vel);
+    const double coth_term = 1.0 / tanh(Pe) - 1.0 / Pe;
+    const double tau = (h / (2.0 * vel_norm)) * coth_term;
+    return tau > tau_max ? tau_max : tau;
+}
+
+/* UnsteadyAdvectionDiffusionSUPG::operator() (unsteady_advection_diffusion_supg.hpp
+ * :240-433): per hex cell and wedge, velocity interpolated to the 6 Felippa
+ * points, tau averaged over the points with the quadrature weights (h = r_2 -
+ * r_1), local 6 x 6 matrices
+ *   M += w m phi_i phi_j |det J|,
+ *   A += w dt (kappa grad_i . grad_j + (u . grad_j) phi_i + tau (u . grad_j)(u . grad_i)) |det J|,
+ * optional lumping of M, the Dirichlet Hadamard mask (cells whose origin node
+ * is on the CMB / whose top node is on the surface), optional diagonal, then
+ * dst += (M + A) src per wedge; Replace zeroes dst first (apply_impl :214-238);
+ * additive exchange. */
+void orc_supg_apply(const orc_mesh *m, const double *vel, double kappa, double dt, double mass_scaling,
+                    int treat_boundary, int diagonal, int lumped, int supg_enabled, int apply_mode, int comm,
+                    const double *src, double *dst)
+{
+    static const double QP[6][3] = {{0.6666666666666666, 0.1666666666666667, -0.5773502691896257},
+                                    {0.1666666666666667, 0.6666666666666666, -0.5773502691896257},
+                                    {0.1666666666666667, 0.1666666666666667, -0.5773502691896257},
+                                    {0.6666666666666666, 0.1666666666666667, 0.5773502691896257},
+                                    {0.1666666666666667, 0.6666666666666666, 0.5773502691896257},
+                                    {0.1666666666666667, 0.1666666666666667, 0.5773502691896257}};
+    static const double QW[6] = {0.1666666666666667, 0.1666666666666667, 0.1666666666666667,
+                                 0.1666666666666667, 0.1666666666666667, 0.1666666666666667};
+    const int64_t N = m->nalloc;
+    if (apply_mode == 0) memset(dst, 0, sizeof(double) * N);
+    const int nx = m->nx;
+    for (int sd = 0; sd < m->nsd; ++sd)
+        for (int xc = 0; xc < nx - 1; ++xc)
+            for (int yc = 0; yc < nx - 1; ++yc)
+                for (int rc = 0; rc < m->nr - 1; ++rc) {
+                    const double r_1 = m->radii[(size_t)sd * m->nr + rc], r_2 = m->radii[(size_t)sd * m->nr + rc + 1];
+                    const double h = r_2 - r_1;
+                    const int at_cmb = has_flag(m->bmask[lin(m, sd, xc, yc, rc)], SBF_CMB);
+                    const int at_surf = has_flag(m->bmask[lin(m, sd, xc, yc, rc + 1)], SBF_SURFACE);
+                    for (int w = 0; w < 2; ++w) {
+                        vec3 P[3];
+                        int64_t idx[6];
+                        for (int n = 0; n < 6; ++n)
+                            idx[n] = lin(m, sd, xc + WOFF[w][n][0], yc + WOFF[w][n][1], rc + WOFF[w][n][2]);
+                        for (int n = 0; n < 3; ++n) {
+                            const double *c =
+                                m->coords + 3 * (((size_t)sd * nx + xc + WOFF[w][n][0]) * nx + yc + WOFF[w][n][1]);
+                            P[n] = v3(c[0], c[1], c[2]);
+                        }
+                        /* velocity at the quadrature points */
+                        vec3 uq[6];
+                        for (int q = 0; q < 6; ++q) {
+                            uq[q] = v3(0, 0, 0);
+                            for (int i = 0; i < 6; ++i) {
+                                const double sh = shape_w(i, QP[q][0], QP[q][1], QP[q][2]);
+                                for (int d = 0; d < 3; ++d) uq[q].v[d] += vel[(size_t)d * N + idx[i]] * sh;
+                            }
+                        }
+                        double tau_accum = 0.0, waccum = 0.0;
+                        for (int q = 0; q < 6; ++q) {
+                            const double tq = orc_supg_tau(vnorm(uq[q]), kappa, h, 1e-08);
+                            tau_accum += tq * QW[q];
+                            waccum += QW[q];
+                        }
+                        const double tau = supg_enabled ? (waccum > 0.0 ? tau_accum / waccum : 0.0) : 0.0;
+                        double M[6][6], A[6][6];
+                        memset(M, 0, sizeof(M));
+                        memset(A, 0, sizeof(A));
+                        for (int q = 0; q < 6; ++q) {
+                            const double xi = QP[q][0], eta = QP[q][1], zeta = QP[q][2];
+                            const mat3 J = jac_w(P[0], P[1], P[2], r_1, r_2, xi, eta, zeta);
+                            const double dJ = det3(&J);
+                            const double det = fabs(dJ);
+                            const mat3 JiT = inv_transposed3(&J, dJ);
+                            vec3 g[6];
+                            double sh[6];
+                            for (int i = 0; i < 6; ++i) {
+                                const vec3 gr = grad_shape_w(i, xi, eta, zeta);
+                                sh[i] = shape_w(i, xi, eta, zeta);
+                                for (int a = 0; a < 3; ++a)
+                                    g[i].v[a] = JiT.a[a][0] * gr.v[0] + JiT.a[a][1] * gr.v[1] + JiT.a[a][2] * gr.v[2];
+                            }
+                            for (int i = 0; i < 6; ++i)
+                                for (int j = 0; j < 6; ++j) {
+                                    const double mass = sh[i] * sh[j];
+                                    const double diffusion =
+                                        kappa * (g[i].v[0] * g[j].v[0] + g[i].v[1] * g[j].v[1] + g[i].v[2] * g[j].v[2]);
+                                    const double ugj = uq[q].v[0] * g[j].v[0] + uq[q].v[1] * g[j].v[1] + uq[q].v[2] * g[j].v[2];
+                                    const double ugi = uq[q].v[0] * g[i].v[0] + uq[q].v[1] * g[i].v[1] + uq[q].v[2] * g[i].v[2];
+                                    const double advection = ugj * sh[i];
+                                    const double streamline = tau * ugj * ugi;
+                                    M[i][j] += QW[q] * mass_scaling * mass * det;
+                                    A[i][j] += QW[q] * dt * (diffusion + advection + streamline) * det;
+                                }
+                        }
+                        if (lumped)
+                            for (int i = 0; i < 6; ++i) {
+                                double s = 0.0;
+                                for (int j = 0; j < 6; ++j) s += M[i][j] * 1.0;
+                                for (int j = 0; j < 6; ++j) M[i][j] = i == j ? s : 0.0;
+                            }
+                        if (treat_boundary)
+                            for (int i = 0; i < 6; ++i)
+                                for (int j = 0; j < 6; ++j) {
+                                    const int off = i != j;
+                                    if ((at_cmb && off && (i < 3 || j < 3)) || (at_surf && off && (i >= 3 || j >= 3))) {
+                                        M[i][j] *= 0.0;
+                                        A[i][j] *= 0.0;
+                                    }
+                                }
+                        if (diagonal)
+                            for (int i = 0; i < 6; ++i)
+                                for (int j = 0; j < 6; ++j)
+                                    if (i != j) M[i][j] = A[i][j] = 0.0;
+                        for (int i = 0; i < 6; ++i) {
+                            double acc = 0.0;
+                            for (int j = 0; j < 6; ++j) acc += (M[i][j] + A[i][j]) * src[idx[j]];
+                            dst[idx[i]] += acc;
+                        }
+                    }
+                }
+    if (comm) orc_halo_sum(m, dst, 1);
+}
+
+/* Mass (fe/wedge/operators/shell/mass.hpp): consistent scalar mass with the
+ * 6-point rule, additive exchange (the RHS M T^n of the energy step). */
+void orc_scalar_mass_apply(const orc_mesh *m, const double *src, double *dst)
+{
+    double *tmp = (double *)calloc((size_t)3 * m->nalloc, sizeof(double));
+    double *out = (double *)calloc((size_t)3 * m->nalloc, sizeof(double));
+    memcpy(tmp, src, sizeof(double) * m->nalloc);
+    orc_vector_mass_apply(m, tmp, out);
+    memcpy(dst, out, sizeof(double) * m->nalloc);
+    free(tmp);
+    free(out);
+}

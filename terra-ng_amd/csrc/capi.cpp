@@ -1,5 +1,6 @@
 // C ABI entry points (include/tng.h).  Every call converts library errors
 // into a tng_status and records the message on the context.
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -110,6 +111,31 @@ tng_status tng_ctx_synchronize(tng_ctx ctx)
 {
     if (!ctx) return TNG_INVALID_ARGUMENT;
     return guarded(ctx, [&] { TNG_HIP_CHECK(hipStreamSynchronize(ctx->stream)); });
+}
+
+tng_status tng_ctx_set_tuning(tng_ctx ctx, const char *name, int value)
+{
+    if (!ctx || !name) return TNG_INVALID_ARGUMENT;
+    const tng_tuning dflt;
+    struct Entry {
+        const char *name;
+        int tng_tuning::*field;
+    };
+    static const Entry entries[] = {
+        {"apply_blocks", &tng_tuning::apply_blocks},
+        {"apply_seg", &tng_tuning::apply_seg},
+        {"apply_radial_split", &tng_tuning::apply_radial_split},
+        {"apply_radial_chunks", &tng_tuning::apply_radial_chunks},
+        {"apply_waves", &tng_tuning::apply_waves},
+        {"cheb_fuse", &tng_tuning::cheb_fuse},
+    };
+    for (const Entry &e : entries)
+        if (std::strcmp(e.name, name) == 0) {
+            ctx->tuning.*e.field = value < 0 ? dflt.*e.field : value;
+            return TNG_OK;
+        }
+    ctx->last_error = std::string("unknown tuning setting: ") + name;
+    return TNG_INVALID_ARGUMENT;
 }
 
 tng_status tng_ctx_set_profiling(tng_ctx ctx, int on)
@@ -349,11 +375,20 @@ tng_status tng_op_apply(tng_op op, tng_vec src, tng_vec dst)
 tng_status tng_op_destroy(tng_op op)
 {
     if (!op) return TNG_INVALID_ARGUMENT;
+    if (op->parent) {
+        op->mesh->ctx->last_error = "destroy: a block of a Stokes operator is released with the Stokes operator";
+        return TNG_INVALID_ARGUMENT;
+    }
     delete op;
     return TNG_OK;
 }
 
 }  // extern "C"
+
+tng_operator::~tng_operator()
+{
+    for (tng_operator *b : blocks) delete b;
+}
 
 hipEvent_t *tng_context::prof_next_set()
 {
@@ -463,13 +498,47 @@ void op_apply(tng_operator &op, const tng_vector &src, tng_vector &dst)
     case OP_RESTRICTION: in = op.mesh; out = op.mesh2; nin = nout = src.ncomp; break;
     case OP_KMASS_LUMPED: in = out = op.mesh; nin = nout = 1; break;
     case OP_VECTOR_MASS: in = out = op.mesh; break;
+    case OP_ZERO: in = out = op.mesh; nin = nout = 1; break;
+    case OP_SUPG: in = out = op.mesh; nin = nout = 1; break;
+    case OP_STOKES: throw Error(TNG_INVALID_ARGUMENT, "apply: a Stokes operator takes (u, p) pairs: use tng_stokes_op_apply");
     }
     TNG_REQUIRE(src.mesh == in && dst.mesh == out, "apply: vectors do not live on the operator's meshes");
     TNG_REQUIRE(src.ncomp == nin && dst.ncomp == nout, "apply: wrong number of vector components");
     if (op.k) TNG_REQUIRE(op.k->dtype == src.dtype, "apply: coefficient and vectors must share one dtype");
     hipStream_t s = op.mesh->ctx->stream;
     if (op.kind == OP_EPSDIVDIV) epsdivdiv_apply(op, src, dst, s);
-    else stokes_block_apply(op, src, dst, s);
+    else if (op.kind == OP_SUPG) supg_apply(op, src, dst, s);
+    else if (op.kind == OP_ZERO) {
+        if (op.apply_mode == TNG_REPLACE) launch_fill(dst, 0.0, s);
+    } else stokes_block_apply(op, src, dst, s);
+}
+
+// EpsDivDivStokes::apply_impl (epsilon_divdiv_stokes.hpp:55-84): A (Replace,
+// SkipCommunication) -> B^T (Add, Communicate) -> B (Replace, Communicate),
+// B^T and B skipped for the diagonal operator; every block is left in
+// Replace + Communicate.
+void stokes_op_apply(tng_operator &op, const tng_vector &u, const tng_vector &p, tng_vector &out_u, tng_vector &out_p)
+{
+    tng_operator &A = *op.blocks[0], &Bt = *op.blocks[1], &B = *op.blocks[2];
+    TNG_REQUIRE(u.mesh == op.mesh && out_u.mesh == op.mesh && p.mesh == op.mesh2 && out_p.mesh == op.mesh2,
+                "Stokes apply: velocity on the fine mesh, pressure on the coarse mesh");
+    if (!op.diagonal) {
+        A.apply_mode = TNG_REPLACE;
+        A.comm_mode = TNG_SKIP_COMMUNICATION;
+    }
+    Bt.apply_mode = TNG_ADD;
+    Bt.comm_mode = TNG_COMMUNICATE_ADDITIVELY;
+    B.apply_mode = TNG_REPLACE;
+    B.comm_mode = TNG_COMMUNICATE_ADDITIVELY;
+    op_apply(A, u, out_u);
+    if (!op.diagonal) {
+        op_apply(Bt, p, out_u);
+        op_apply(B, u, out_p);
+    }
+    for (tng_operator *b : {&A, &Bt, &B}) {
+        b->apply_mode = TNG_REPLACE;
+        b->comm_mode = TNG_COMMUNICATE_ADDITIVELY;
+    }
 }
 
 }  // namespace tng
@@ -585,6 +654,180 @@ tng_status tng_vec_has_nan_or_inf(tng_vec x, int *out)
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
+// EpsDivDivStokes block operator
+// ---------------------------------------------------------------------------
+extern "C" {
+
+tng_status tng_stokes_create(tng_mesh fine, tng_mesh coarse, tng_vec k, tng_bc bc_cmb, tng_bc bc_surface,
+                             int diagonal, tng_op *out)
+{
+    if (!fine || !coarse || !k || !out || !fine->ctx) return TNG_INVALID_ARGUMENT;
+    *out = nullptr;
+    return guarded(fine->ctx, [&] {
+        set_device(fine->ctx);
+        std::unique_ptr<tng_operator> op(new tng_operator());
+        op->kind = OP_STOKES;
+        op->mesh = fine;
+        op->mesh2 = coarse;
+        op->bc_cmb = bc_cmb;
+        op->bc_surf = bc_surface;
+        op->diagonal = diagonal != 0;
+        op->blocks[1] = new_two_level(OP_GRADIENT, fine, coarse, bc_cmb, bc_surface);
+        op->blocks[2] = new_two_level(OP_DIVERGENCE, fine, coarse, bc_cmb, bc_surface);
+        op->blocks[0] = new_epsdivdiv(fine, k, bc_cmb, bc_surface, diagonal != 0);
+        std::unique_ptr<tng_operator> zero(new tng_operator());
+        zero->kind = OP_ZERO;
+        zero->mesh = coarse;
+        op->blocks[3] = zero.release();
+        for (tng_operator *b : op->blocks) b->parent = op.get();
+        *out = op.release();
+    });
+}
+
+tng_status tng_stokes_op_apply(tng_op op, tng_vec u, tng_vec p, tng_vec out_u, tng_vec out_p)
+{
+    if (!op || !u || !p || !out_u || !out_p) return TNG_INVALID_ARGUMENT;
+    if (op->kind != OP_STOKES) {
+        op->mesh->ctx->last_error = "tng_stokes_op_apply: not a Stokes operator";
+        return TNG_INVALID_ARGUMENT;
+    }
+    return guarded(op->mesh->ctx, [&] {
+        set_device(op->mesh->ctx);
+        stokes_op_apply(*op, *u, *p, *out_u, *out_p);
+    });
+}
+
+tng_status tng_stokes_block(tng_op op, int row, int col, tng_op *block)
+{
+    if (!op || !block || row < 1 || row > 2 || col < 1 || col > 2) return TNG_INVALID_ARGUMENT;
+    *block = nullptr;
+    if (op->kind != OP_STOKES) {
+        op->mesh->ctx->last_error = "tng_stokes_block: not a Stokes operator";
+        return TNG_INVALID_ARGUMENT;
+    }
+    if (op->diagonal && row != col) {
+        op->mesh->ctx->last_error = "block_12() / block_21() are not available for a diagonal Stokes operator";
+        return TNG_INVALID_ARGUMENT;
+    }
+    *block = op->blocks[(row - 1) * 2 + (col - 1)];
+    return TNG_OK;
+}
+
+tng_status tng_vec_norm2(tng_vec x, double *out)
+{
+    if (!x || !out) return TNG_INVALID_ARGUMENT;
+    return guarded(x->mesh->ctx, [&] {
+        set_device(x->mesh->ctx);
+        *out = std::sqrt(reduce_to_host(REDUCE_DOT, *x, x, x->mesh->ctx->stream));
+    });
+}
+
+tng_status tng_vec_dot_device(tng_vec x, tng_vec y, double *out_device)
+{
+    if (!x || !y || !out_device) return TNG_INVALID_ARGUMENT;
+    return guarded(x->mesh->ctx, [&] {
+        set_device(x->mesh->ctx);
+        hipStream_t s = x->mesh->ctx->stream;
+        launch_reduce(REDUCE_DOT, *x, y, 0, s);
+        TNG_HIP_CHECK(hipMemcpyAsync(out_device, x->mesh->ctx->red_result.p, sizeof(double),
+                                     hipMemcpyDeviceToDevice, s));
+    });
+}
+
+tng_status tng_vec_fill_seeded(tng_vec x, unsigned long long seed)
+{
+    if (!x) return TNG_INVALID_ARGUMENT;
+    return guarded(x->mesh->ctx, [&] {
+        set_device(x->mesh->ctx);
+        launch_randomize(*x, seed, x->mesh->ctx->stream);
+    });
+}
+
+tng_status tng_op_set_stored_matrix_mode(tng_op op, int full)
+{
+    if (!op || op->kind != OP_EPSDIVDIV || (full != 0 && full != 1)) return TNG_INVALID_ARGUMENT;
+    return guarded(op->mesh->ctx, [&] {
+        set_device(op->mesh->ctx);
+        hipStream_t s = op->mesh->ctx->stream;
+        TNG_HIP_CHECK(hipStreamSynchronize(s));
+        if (!full) {
+            op->stored = false;
+            op->d_lmat.release();
+            return;
+        }
+        TNG_REQUIRE(op->mesh->nr >= 3, "stored matrices need at least two radial layers");
+        const size_t es = op->k->dtype == TNG_F64 ? 8 : 4;
+        const size_t bytes = stored_matrix_count(*op->mesh) * 324 * es;
+        op->d_lmat.alloc(bytes);
+        TNG_HIP_CHECK(hipMemsetAsync(op->d_lmat.p, 0, bytes, s));
+        op->stored = true;
+        op->lmat_dtype = op->k->dtype;
+    });
+}
+
+static size_t local_matrix_offset(const tng_domain &m, int sd, int x, int y, int r, int wedge)
+{
+    TNG_REQUIRE(sd >= 0 && sd < m.nsd && x >= 0 && x < m.nx - 1 && y >= 0 && y < m.nx - 1 && r >= 0 &&
+                    r < m.nr - 1 && (wedge == 0 || wedge == 1),
+                "local matrix: cell index out of range");
+    return ((((size_t)sd * (m.nx - 1) + x) * (m.nx - 1) + y) * (m.nr - 1) + r) * 2 + wedge;
+}
+
+tng_status tng_op_get_local_matrix(tng_op op, int sd, int x, int y, int r, int wedge, double *mat)
+{
+    if (!op || !mat || op->kind != OP_EPSDIVDIV) return TNG_INVALID_ARGUMENT;
+    return guarded(op->mesh->ctx, [&] {
+        set_device(op->mesh->ctx);
+        hipStream_t s = op->mesh->ctx->stream;
+        const size_t w = local_matrix_offset(*op->mesh, sd, x, y, r, wedge);
+        if (op->stored) {
+            if (op->lmat_dtype == TNG_F64) {
+                TNG_HIP_CHECK(hipMemcpyAsync(mat, op->d_lmat.as<double>() + w * 324, 324 * sizeof(double),
+                                             hipMemcpyDeviceToHost, s));
+                TNG_HIP_CHECK(hipStreamSynchronize(s));
+            } else {
+                float tmp[324];
+                TNG_HIP_CHECK(hipMemcpyAsync(tmp, op->d_lmat.as<float>() + w * 324, sizeof(tmp),
+                                             hipMemcpyDeviceToHost, s));
+                TNG_HIP_CHECK(hipStreamSynchronize(s));
+                for (int i = 0; i < 324; ++i) mat[i] = tmp[i];
+            }
+            return;
+        }
+        // no storage: the Neumann matrix assembled from k (assemble_local_matrix)
+        DevBuf all;
+        epsdivdiv_local_matrices(*op, all, s);
+        TNG_HIP_CHECK(hipMemcpyAsync(mat, all.as<double>() + w * 324, 324 * sizeof(double), hipMemcpyDeviceToHost,
+                                     s));
+        TNG_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+tng_status tng_op_set_local_matrix(tng_op op, int sd, int x, int y, int r, int wedge, const double *mat)
+{
+    if (!op || !mat || op->kind != OP_EPSDIVDIV) return TNG_INVALID_ARGUMENT;
+    return guarded(op->mesh->ctx, [&] {
+        set_device(op->mesh->ctx);
+        TNG_REQUIRE(op->stored, "set_local_matrix: the operator stores no matrices (tng_op_set_stored_matrix_mode)");
+        hipStream_t s = op->mesh->ctx->stream;
+        const size_t w = local_matrix_offset(*op->mesh, sd, x, y, r, wedge);
+        if (op->lmat_dtype == TNG_F64) {
+            TNG_HIP_CHECK(hipMemcpyAsync(op->d_lmat.as<double>() + w * 324, mat, 324 * sizeof(double),
+                                         hipMemcpyHostToDevice, s));
+            TNG_HIP_CHECK(hipStreamSynchronize(s));
+        } else {
+            float tmp[324];
+            for (int i = 0; i < 324; ++i) tmp[i] = (float)mat[i];
+            TNG_HIP_CHECK(hipMemcpyAsync(op->d_lmat.as<float>() + w * 324, tmp, sizeof(tmp), hipMemcpyHostToDevice,
+                                         s));
+            TNG_HIP_CHECK(hipStreamSynchronize(s));
+        }
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
 // stored local matrices and GCA
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -653,6 +896,16 @@ tng_status tng_gca_assemble(tng_op fine, tng_op coarse)
         set_device(coarse->mesh->ctx);
         TNG_REQUIRE(coarse->k->dtype == TNG_F64, "GCA assembles FP64 matrices: the coarse operator must be FP64");
         gca_assemble(*fine, *coarse, coarse->mesh->ctx->stream);
+    });
+}
+
+tng_status tng_gca_assemble_exact(tng_op fine, tng_op coarse)
+{
+    if (!fine || !coarse || fine->kind != OP_EPSDIVDIV || coarse->kind != OP_EPSDIVDIV) return TNG_INVALID_ARGUMENT;
+    return guarded(coarse->mesh->ctx, [&] {
+        set_device(coarse->mesh->ctx);
+        TNG_REQUIRE(coarse->k->dtype == TNG_F64, "GCA assembles FP64 matrices: the coarse operator must be FP64");
+        gca_assemble(*fine, *coarse, coarse->mesh->ctx->stream, true);
     });
 }
 

@@ -489,7 +489,7 @@ void xdmf_checkpoint_read(const std::string &dir, const std::string &label, int 
     const std::vector<char> meta = read_file(dir + "/checkpoint_metadata.bin");
     size_t pos = 0;
     auto need = [&](size_t n) {
-        if (pos + n > meta.size()) io_error("checkpoint metadata: unexpected end of file");
+        if (n > meta.size() - pos) io_error("checkpoint metadata: unexpected end of file");
     };
     auto i32 = [&]() { need(4); int32_t x; std::memcpy(&x, meta.data() + pos, 4); pos += 4; return x; };
     auto i64 = [&]() { need(8); int64_t x; std::memcpy(&x, meta.data() + pos, 8); pos += 8; return x; };
@@ -498,6 +498,12 @@ void xdmf_checkpoint_read(const std::string &dir, const std::string &label, int 
     if (version != 0 && version != 1)
         io_error("supported checkpoint versions: 0, 1; this checkpoint has version " + std::to_string(version));
     const int32_t S = i32(), Sr = i32(), sx = i32(), sy = i32(), sr = i32();
+    if (S <= 0 || Sr <= 0 || sr < 2 || sx < 2 || sy < 2)
+        io_error("checkpoint metadata: invalid subdomain counts or extents");
+    if (S != v.mesh->S || Sr != v.mesh->Sr)
+        io_error("checkpoint metadata: the subdomain decomposition does not match the mesh");
+    if ((int64_t)Sr * (sr - 1) + 1 > (int64_t)(meta.size() - pos) / 8)
+        io_error("checkpoint metadata: unexpected end of file");
     double prev = 0.0;
     for (int i = 0; i < Sr * (sr - 1) + 1; ++i) {
         const double r = f64();
@@ -509,6 +515,7 @@ void xdmf_checkpoint_read(const std::string &dir, const std::string &label, int 
     int32_t dtype = -1, bytes = 0, vec_dim = 0;
     for (int f = 0; f < nfiles; ++f) {
         const int32_t len = i32();
+        if (len < 0) io_error("checkpoint metadata: negative name length");
         need((size_t)len);
         const std::string name(meta.data() + pos, (size_t)len);
         pos += (size_t)len;
@@ -524,6 +531,8 @@ void xdmf_checkpoint_read(const std::string &dir, const std::string &label, int 
     if (sx != m.nx || sy != m.nx || sr != m.nr) io_error("grid data extents do not match metadata");
     if (i32() != 0) io_error("checkpoint ordering type is not 0; not supported");
     const int64_t nglobal = (int64_t)10 * S * S * Sr;
+    if (nglobal < m.nsd) io_error("checkpoint metadata: fewer subdomains than the mesh has");
+    need((size_t)nglobal * 8);
     std::vector<int64_t> ids((size_t)nglobal);
     for (auto &id : ids) id = i64();
     if (dtype != 2 || (bytes != 4 && bytes != 8)) io_error("unsupported data type in checkpoint");

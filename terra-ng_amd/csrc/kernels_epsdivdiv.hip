@@ -38,6 +38,8 @@
 // atomics, no zeroing pass, deterministic results.
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
+#include <string>
 
 #include "tng_internal.hpp"
 
@@ -353,12 +355,14 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
     constexpr int VX = NW / RG * VPW;
     constexpr int NV = VX * RG;
     constexpr int KL = K + 1;
-    // left-corner node values of rows yc and yc+1, radially combined, per
-    // virtual column; the top level of the last cell (overflow); the top
-    // level of a lower chunk (its own cells' sum, before the lateral exchange)
-    __shared__ T xch[2][NV][2][K][3][R];
-    __shared__ T xovf[2][NV][2][3];
-    __shared__ T xtop[2][RG > 1 ? NV : 1][3];
+    // left-corner contributions of every cell for rows yc (corner 00) and
+    // yc+1 (corner 01), lateral part L and radial part C apart, per virtual
+    // column, double-buffered by step parity (one cell per lane; more cells
+    // per lane use one buffer and a second barrier); the last cell's own L
+    // and C of a lower chunk's top lane (before the lateral exchange)
+    constexpr int NB = K == 1 ? 2 : 1;
+    __shared__ T xch[NB][NV][2][2][K][3][R];
+    __shared__ T xtop[NB][RG > 1 ? NV : 1][2][3];
 
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -548,18 +552,24 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
     #pragma unroll
         for (int p = 0; p < KL; ++p) radl[p] = a.radii[(int64_t)sd * nr + min(c0 + rl * K + p, nr - 1)];
 
-        T accA[KL][3], accB[KL][3];
+        // Accumulators per cell of the lane: lateral part L and radial part C of
+        // every contribution to the cell's corner column of the output row.
+        // Node level l of the column receives L - C from cell l (as its bottom)
+        // and L + C from cell l - 1 (as its top); the store forms that sum once.
+        T aLA[K][3], aCA[K][3], aLB[K][3], aCB[K][3];
     #pragma unroll
-        for (int i = 0; i < KL; ++i)
+        for (int i = 0; i < K; ++i)
     #pragma unroll
-            for (int c = 0; c < 3; ++c) accA[i][c] = accB[i][c] = T(0);
+            for (int c = 0; c < 3; ++c) aLA[i][c] = aCA[i][c] = aLB[i][c] = aCB[i][c] = T(0);
 
         // One y-step: bottom rows B0/B1 (processed), the prefetched raw row yc+1
-        // becomes the top rows T0/T1; acc_cur collects node row yc, acc_next row
-        // yc+1.  The caller alternates the roles of the two row pairs and the two
-        // accumulators, so nothing is copied between steps.
+        // becomes the top rows T0/T1; (aL_cur, aC_cur) collect node row yc,
+        // (aL_next, aC_next) row yc+1.  The caller alternates the roles of the
+        // two row pairs and the two accumulator pairs, so nothing is copied
+        // between steps.
         auto do_step = [&](int step, NodeCol<T, K> &B0, NodeCol<T, K> &B1, NodeCol<T, K> &T0, NodeCol<T, K> &T1,
-                           T (&acc_cur)[KL][3], T (&acc_next)[KL][3], T &rec, T &rec_next) {
+                           T (&aL_cur)[K][3], T (&aC_cur)[K][3], T (&aL_next)[K][3], T (&aC_next)[K][3], T &rec,
+                           T &rec_next) {
             const int yc = Y0 - 1 + step;
             // one cell per lane and column per wave: the cell's geometry is
             // wave-uniform; scalar loads issued here have the processing of the
@@ -609,8 +619,7 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
             load_raw<T, K>(a, colbase(xc, yc + 2) * nr, lvl0, P0);
             load_raw<T, K>(a, colbase(xc + 1, yc + 2) * nr, lvl0, P1);
 
-            T s00[KL][3], s01[KL][3];  // written by the corner loop below
-
+            const int buf = NB == 2 ? (step & 1) : 0;
             const bool cell_ok = xc >= 0 && xc < nc && yc >= 0 && yc < nc;
             {
                 T g0[10], g1[10];
@@ -645,77 +654,72 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
                     // wedge 1: (c11, c01, c10)
                     wedge_stress<T>(g1, T1.su[p], T0.su[p], B1.su[p], T1.du[p], T0.du[p], B1.du[p],
                                     T1.kk[p] + T0.kk[p] + B1.kk[p], qq[p], cvp, E0, E1, Cd);
-                    // corners: c00 and c11 get one wedge each, c10 and c01 both;
-                    // lateral parts L and radial parts C enter the bottom node as
-                    // L - C and the top node as L + C
+                    // corners: c00 and c11 get one wedge each, c10 and c01 both.
+                    // The own corners (c10: row yc, c11: row yc+1) go to the
+                    // accumulators, the left corners (c00, c01) to the neighbour
+                    // v-1 through LDS, L and C apart.
     #pragma unroll
                     for (int c = 0; c < 3; ++c) {
                         const T x00 = -(A0[c] + A1[c]), x11 = -(E0[c] + E1[c]);
                         const T l10 = A0[c] + E1[c], l01 = A1[c] + E0[c], c2 = Cc[c] + Cd[c];
-                        if (p == 0) {
-                            s00[p][c] = x00 - Cc[c];
-                            s01[p][c] = l01 - c2;
-                        } else {
-                            s00[p][c] += x00 - Cc[c];
-                            s01[p][c] += l01 - c2;
-                        }
-                        s00[p + 1][c] = x00 + Cc[c];
-                        s01[p + 1][c] = l01 + c2;
-                        acc_cur[p][c] += l10 - c2;
-                        acc_cur[p + 1][c] += l10 + c2;
-                        acc_next[p][c] += x11 - Cd[c];
-                        acc_next[p + 1][c] += x11 + Cd[c];
+                        aL_cur[p][c] += l10;
+                        aC_cur[p][c] += c2;
+                        aL_next[p][c] += x11;
+                        aC_next[p][c] += Cd[c];
+                        xch[buf][vi][0][0][p][c][rl] = x00;
+                        xch[buf][vi][0][1][p][c][rl] = Cc[c];
+                        xch[buf][vi][1][0][p][c][rl] = l01;
+                        xch[buf][vi][1][1][p][c][rl] = c2;
                     }
                 }
             }
-
-            // left corners -> neighbour v-1, combined radially
-            const int buf = step & 1;
+            if constexpr (RG > 1) {
+                // the lower chunk's top lane: its last cell's own L and C
+                if (ovf_lane && !top_chunk) {
     #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const T b00 = wave_shr1(s00[K][c]), b01 = wave_shr1(s01[K][c]);
-    #pragma unroll
-                for (int p = 0; p < K; ++p) {
-                    // one column per wave: the bound-controlled shift already
-                    // gives lane 0 a zero
-                    xch[buf][vi][0][p][c][rl] = s00[p][c] + (p == 0 && (R == 64 || rl > 0) ? b00 : T(0));
-                    xch[buf][vi][1][p][c][rl] = s01[p][c] + (p == 0 && (R == 64 || rl > 0) ? b01 : T(0));
-                }
-                if (ovf_lane) {
-                    xovf[buf][vi][0][c] = s00[K][c];
-                    xovf[buf][vi][1][c] = s01[K][c];
-                    if constexpr (RG > 1) {
-                        if (!top_chunk) xtop[buf][vi][c] = acc_cur[K][c];
+                    for (int c = 0; c < 3; ++c) {
+                        xtop[buf][vi][0][c] = aL_cur[K - 1][c];
+                        xtop[buf][vi][1][c] = aC_cur[K - 1][c];
                     }
                 }
             }
             __syncthreads();
+            T below[3] = {T(0), T(0), T(0)};
+            if constexpr (RG > 1) {
+                // lane 0 of an upper chunk: the top level of the lower chunk's
+                // last cell, with the neighbour's part added as that lane adds it
+                if (rl == 0 && !bot_chunk && v + 1 < VX) {
+    #pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        below[c] = (xtop[buf][vi - VX][0][c] + xch[buf][vi - VX + 1][0][0][K - 1][c][R - 1]) +
+                                   (xtop[buf][vi - VX][1][c] + xch[buf][vi - VX + 1][0][1][K - 1][c][R - 1]);
+                }
+            }
             if (v + 1 < VX) {
     #pragma unroll
                 for (int c = 0; c < 3; ++c) {
     #pragma unroll
                     for (int p = 0; p < K; ++p) {
-                        acc_cur[p][c] += xch[buf][vi + 1][0][p][c][rl];
-                        acc_next[p][c] += xch[buf][vi + 1][1][p][c][rl];
-                    }
-                    if (ovf_lane) {
-                        acc_cur[K][c] += xovf[buf][vi + 1][0][c];
-                        acc_next[K][c] += xovf[buf][vi + 1][1][c];
+                        aL_cur[p][c] += xch[buf][vi + 1][0][0][p][c][rl];
+                        aC_cur[p][c] += xch[buf][vi + 1][0][1][p][c][rl];
+                        aL_next[p][c] += xch[buf][vi + 1][1][0][p][c][rl];
+                        aC_next[p][c] += xch[buf][vi + 1][1][1][p][c][rl];
                     }
                 }
             }
+            if constexpr (NB == 1) __syncthreads();  // the buffer is written again in the next step
 
-            // node row yc is complete: combine radially and store
-            T below[3];
+            // node row yc is complete: the top level of every cell, L + C, goes
+            // to the node above; the first level of the lane takes the lane below's
+            T topv[K][3];
+    #pragma unroll
+            for (int p = 0; p < K; ++p)
+    #pragma unroll
+                for (int c = 0; c < 3; ++c) topv[p][c] = aL_cur[p][c] + aC_cur[p][c];
     #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const T up = wave_shr1(acc_cur[K][c]);
-                below[c] = (R == 64 || rl > 0) ? up : T(0);
-                if constexpr (RG > 1) {
-                    // lane 0 of an upper chunk: the lower chunk's top level
-                    if (rl == 0 && !bot_chunk && v + 1 < VX)
-                        below[c] = xtop[buf][vi - VX][c] + xovf[buf][vi - VX + 1][0][c];
-                }
+                const T up = wave_shr1(topv[K - 1][c]);
+                if (RG == 1 || rl > 0 || bot_chunk) below[c] = (R == 64 || rl > 0) ? up : T(0);
             }
             const int Y = yc;
             if (Y >= Y0 && v < VX - 1 && X < nx && col_ok && lane_has) {
@@ -763,7 +767,10 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
                     const int l = c0 + rl * K + p;
                     T val[3];
     #pragma unroll
-                    for (int c = 0; c < 3; ++c) val[c] = acc_cur[p][c] + (p == 0 ? below[c] : T(0));
+                    for (int c = 0; c < 3; ++c) {
+                        if (p == K) val[c] = topv[K - 1][c];
+                        else val[c] = (aL_cur[p][c] - aC_cur[p][c]) + (p == 0 ? below[c] : topv[p > 0 ? p - 1 : 0][c]);
+                    }
                     // level shared with the radial part above (lower partial)
                     // or below (upper partial): completed by the interface kernel
                     const bool if_lo = SPLIT && p == K && part < rsplit - 1;
@@ -782,12 +789,15 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
                         const int bcv = at_cmb ? a.bc_cmb : a.bc_surf;
                         T bt[3], u[3];
                         if constexpr (R == 64 && !hoist) {
-                            // lane 0 takes the CMB entries, the top lane the surface ones
-                            const int off = at_cmb ? 0 : 3;
+                            // lane 0 takes the CMB entries, the top lane the surface ones:
+                            // wave-uniform reads of the record lanes (a lane permute here
+                            // would read lanes that the branch has switched off)
     #pragma unroll
                             for (int c = 0; c < 3; ++c) {
-                                bt[c] = lane_fetch(rec, REC_BTAB + off + c);
-                                u[c] = lane_fetch(rec, REC_UCMB + off + c);
+                                const T btc = lane_value(rec, REC_BTAB + c), bts = lane_value(rec, REC_BTAB + 3 + c);
+                                const T uc = lane_value(rec, REC_UCMB + c), us = lane_value(rec, REC_USURF + c);
+                                bt[c] = at_cmb ? btc : bts;
+                                u[c] = at_cmb ? uc : us;
                             }
                         } else if constexpr (R == 64) {
     #pragma unroll
@@ -861,9 +871,9 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
                 }
             }
     #pragma unroll
-            for (int i = 0; i < KL; ++i)
+            for (int i = 0; i < K; ++i)
     #pragma unroll
-                for (int c = 0; c < 3; ++c) acc_cur[i][c] = T(0);
+                for (int c = 0; c < 3; ++c) aL_cur[i][c] = aC_cur[i][c] = T(0);
         };
 
         const int nsteps = Yend - (Y0 - 1);
@@ -872,21 +882,22 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
         if constexpr (R == 64 && K == 1) {
             // two steps per iteration with alternating roles (no register copies)
             for (int step = 0; step < nsteps; step += 2) {
-                do_step(step, B0, B1, C0, C1, accA, accB, rec, rec2);
-                if (step + 1 < nsteps) do_step(step + 1, C0, C1, B0, B1, accB, accA, rec2, rec);
+                do_step(step, B0, B1, C0, C1, aLA, aCA, aLB, aCB, rec, rec2);
+                if (step + 1 < nsteps) do_step(step + 1, C0, C1, B0, B1, aLB, aCB, aLA, aCA, rec2, rec);
             }
         } else {
             for (int step = 0; step < nsteps; ++step) {
-                do_step(step, B0, B1, C0, C1, accA, accB, rec, rec2);
+                do_step(step, B0, B1, C0, C1, aLA, aCA, aLB, aCB, rec, rec2);
                 B0 = C0;
                 B1 = C1;
                 rec = rec2;
     #pragma unroll
-                for (int i = 0; i < KL; ++i)
+                for (int i = 0; i < K; ++i)
     #pragma unroll
                     for (int c = 0; c < 3; ++c) {
-                        accA[i][c] = accB[i][c];
-                        accB[i][c] = T(0);
+                        aLA[i][c] = aLB[i][c];
+                        aCA[i][c] = aCB[i][c];
+                        aLB[i][c] = aCB[i][c] = T(0);
                     }
             }
         }
@@ -1175,12 +1186,6 @@ template <> MeshPtrs<float> mesh_ptrs<float>(const tng_domain &m)
     return {m.d_coords_f.as<float>(), m.d_radii_f.as<float>(), m.d_latgeom_f.as<float>()};
 }
 
-int env_int(const char *name, int dflt)
-{
-    const char *v = std::getenv(name);
-    return v ? std::atoi(v) : dflt;
-}
-
 // Workgroups resident at once on the current device for one kernel.
 int resident_groups(const void *kernel, int threads)
 {
@@ -1212,7 +1217,7 @@ int pick_segment(int nx, int64_t lateral_groups, int resident)
 }
 
 template <typename T, int R, int K, int NW, int RG, int BCM = BCM_RUNTIME, bool FUSE = false, bool SPLIT = false>
-void launch_apply_rk(ApplyArgs<T> a, int64_t lateral_groups, hipStream_t s)
+void launch_apply_rk(ApplyArgs<T> a, int64_t lateral_groups, hipStream_t s, const tng_tuning &tu)
 {
     const void *kern = reinterpret_cast<const void *>(&epsdivdiv_apply_kernel<T, R, K, NW, RG, BCM, FUSE, SPLIT>);
     static int cached_dev = -1, cached_resident = 0;
@@ -1226,13 +1231,13 @@ void launch_apply_rk(ApplyArgs<T> a, int64_t lateral_groups, hipStream_t s)
     int blocks;
     if (sizeof(T) == 8 && !FUSE) {
         // one round of resident workgroups, each sweeping an equal share of
-        // the node rows (TNG_APPLY_BLOCKS overrides the count)
+        // the node rows (the apply_blocks setting overrides the count)
         const int64_t rows = lateral_groups * a.nx;
-        int64_t want = env_int("TNG_APPLY_BLOCKS", 0);
+        int64_t want = tu.apply_blocks;
         if (want <= 0) want = cached_resident;
         blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, rows));
     } else {
-        int seg = env_int("TNG_APPLY_SEG", 0);
+        int seg = tu.apply_seg;
         if (seg <= 0) seg = pick_segment(a.nx, lateral_groups, cached_resident);
         a.seg_len = seg;
         a.segs = (a.nx + seg - 1) / seg;
@@ -1247,11 +1252,11 @@ struct ApplyShape {
     int R, K, RG, NW, S;
 };
 
-ApplyShape apply_shape(int ncells, bool fused = false)
+ApplyShape apply_shape(int ncells, const tng_tuning &tu, bool fused = false)
 {
     ApplyShape sh{64, 1, 1, 8, 1};
-    const int chunks = env_int("TNG_APPLY_RADIAL_CHUNKS", 0);
-    const bool split = env_int("TNG_APPLY_RADIAL_SPLIT", 1) != 0;
+    const int chunks = tu.apply_radial_chunks;
+    const bool split = tu.apply_radial_split != 0;
     if (ncells >= 128 && chunks == 0 && split && !fused) {
         // radial parts of 64 cells swept like separate subdomains (eight
         // one-column waves per workgroup, seven new columns each), their
@@ -1262,24 +1267,24 @@ ApplyShape apply_shape(int ncells, bool fused = false)
         while (sh.R < ncells) sh.R *= 2;
     } else if ((ncells == 128 || ncells == 256) && chunks != 1) {
         // one wave per radial chunk of 64 cells; the K-cells-per-lane kernels
-        // (TNG_APPLY_RADIAL_CHUNKS=1) exceed the register file at K = 4
+        // (apply_radial_chunks = 1) exceed the register file at K = 4
         sh.RG = ncells / 64;
     } else {
         sh.K = ncells / 64;
         sh.NW = 4;
     }
-    if (sh.K == 1 && sh.R == 64 && sh.RG == 1 && sh.S == 1 && env_int("TNG_APPLY_WAVES", 8) == 4) sh.NW = 4;
+    if (sh.K == 1 && sh.R == 64 && sh.RG == 1 && sh.S == 1 && tu.apply_waves == 4) sh.NW = 4;
     return sh;
 }
 
-template <typename T, bool FUSE = false> void launch_apply(ApplyArgs<T> a, hipStream_t s)
+template <typename T, bool FUSE = false> void launch_apply(ApplyArgs<T> a, hipStream_t s, const tng_tuning &tu)
 {
     TNG_REQUIRE(a.ncells >= 1 && (a.ncells & (a.ncells - 1)) == 0, "radial layers per subdomain must be a power of 2");
     TNG_REQUIRE((int64_t)a.nsd * a.nx * a.nx * a.nr < ((int64_t)1 << 31), "node copies exceed 32-bit indexing");
     // shapes: up to 64 radial cells one cell per lane (R lanes per column);
     // more as radial parts of 64 cells, or (128 and 256 cells) radial chunks
     // of one wave each or K cells per lane
-    const ApplyShape sh = apply_shape(a.ncells, FUSE);
+    const ApplyShape sh = apply_shape(a.ncells, tu, FUSE);
     const int R = sh.R, K = sh.K, RG = sh.RG, NW = sh.NW;
     TNG_REQUIRE(K <= 4, "more than 256 radial layers per subdomain need the radial parts");
     a.rsplit = sh.S;
@@ -1299,10 +1304,10 @@ template <typename T, bool FUSE = false> void launch_apply(ApplyArgs<T> a, hipSt
             // unreachable: rejected above
         } else {
             switch (bcm) {
-            case 0: launch_apply_rk<T, 64, 1, 8, 1, 0, false, true>(a, lg, s); break;
-            case 1: launch_apply_rk<T, 64, 1, 8, 1, 1, false, true>(a, lg, s); break;
-            case 2: launch_apply_rk<T, 64, 1, 8, 1, 2, false, true>(a, lg, s); break;
-            default: launch_apply_rk<T, 64, 1, 8, 1, 3, false, true>(a, lg, s); break;
+            case 0: launch_apply_rk<T, 64, 1, 8, 1, 0, false, true>(a, lg, s, tu); break;
+            case 1: launch_apply_rk<T, 64, 1, 8, 1, 1, false, true>(a, lg, s, tu); break;
+            case 2: launch_apply_rk<T, 64, 1, 8, 1, 2, false, true>(a, lg, s, tu); break;
+            default: launch_apply_rk<T, 64, 1, 8, 1, 3, false, true>(a, lg, s, tu); break;
             }
         }
     } else if (RG == 4) {
@@ -1310,40 +1315,40 @@ template <typename T, bool FUSE = false> void launch_apply(ApplyArgs<T> a, hipSt
             // unreachable: the fused step is limited to one or two chunks
         } else {
             switch (bcm) {
-            case 0: launch_apply_rk<T, 64, 1, 8, 4, 0>(a, lg, s); break;
-            case 1: launch_apply_rk<T, 64, 1, 8, 4, 1>(a, lg, s); break;
-            case 2: launch_apply_rk<T, 64, 1, 8, 4, 2>(a, lg, s); break;
-            default: launch_apply_rk<T, 64, 1, 8, 4, 3>(a, lg, s); break;
+            case 0: launch_apply_rk<T, 64, 1, 8, 4, 0>(a, lg, s, tu); break;
+            case 1: launch_apply_rk<T, 64, 1, 8, 4, 1>(a, lg, s, tu); break;
+            case 2: launch_apply_rk<T, 64, 1, 8, 4, 2>(a, lg, s, tu); break;
+            default: launch_apply_rk<T, 64, 1, 8, 4, 3>(a, lg, s, tu); break;
             }
         }
     } else if (RG == 2) {
         switch (bcm) {
-        case 0: launch_apply_rk<T, 64, 1, 8, 2, 0, FUSE>(a, lg, s); break;
-        case 1: launch_apply_rk<T, 64, 1, 8, 2, 1, FUSE>(a, lg, s); break;
-        case 2: launch_apply_rk<T, 64, 1, 8, 2, 2, FUSE>(a, lg, s); break;
-        default: launch_apply_rk<T, 64, 1, 8, 2, 3, FUSE>(a, lg, s); break;
+        case 0: launch_apply_rk<T, 64, 1, 8, 2, 0, FUSE>(a, lg, s, tu); break;
+        case 1: launch_apply_rk<T, 64, 1, 8, 2, 1, FUSE>(a, lg, s, tu); break;
+        case 2: launch_apply_rk<T, 64, 1, 8, 2, 2, FUSE>(a, lg, s, tu); break;
+        default: launch_apply_rk<T, 64, 1, 8, 2, 3, FUSE>(a, lg, s, tu); break;
         }
     } else if (K == 1 && R == 64 && NW == 8) {
         switch (bcm) {
-        case 0: launch_apply_rk<T, 64, 1, 8, 1, 0, FUSE>(a, lg, s); break;
-        case 1: launch_apply_rk<T, 64, 1, 8, 1, 1, FUSE>(a, lg, s); break;
-        case 2: launch_apply_rk<T, 64, 1, 8, 1, 2, FUSE>(a, lg, s); break;
-        default: launch_apply_rk<T, 64, 1, 8, 1, 3, FUSE>(a, lg, s); break;
+        case 0: launch_apply_rk<T, 64, 1, 8, 1, 0, FUSE>(a, lg, s, tu); break;
+        case 1: launch_apply_rk<T, 64, 1, 8, 1, 1, FUSE>(a, lg, s, tu); break;
+        case 2: launch_apply_rk<T, 64, 1, 8, 1, 2, FUSE>(a, lg, s, tu); break;
+        default: launch_apply_rk<T, 64, 1, 8, 1, 3, FUSE>(a, lg, s, tu); break;
         }
     } else if constexpr (FUSE) {
         // unreachable: rejected above
     } else if (K == 1) {
         switch (R) {
-        case 4: launch_apply_rk<T, 4, 1, 8, 1>(a, lg, s); break;
-        case 8: launch_apply_rk<T, 8, 1, 8, 1>(a, lg, s); break;
-        case 16: launch_apply_rk<T, 16, 1, 8, 1>(a, lg, s); break;
-        case 32: launch_apply_rk<T, 32, 1, 8, 1>(a, lg, s); break;
-        default: launch_apply_rk<T, 64, 1, 4, 1>(a, lg, s); break;
+        case 4: launch_apply_rk<T, 4, 1, 8, 1>(a, lg, s, tu); break;
+        case 8: launch_apply_rk<T, 8, 1, 8, 1>(a, lg, s, tu); break;
+        case 16: launch_apply_rk<T, 16, 1, 8, 1>(a, lg, s, tu); break;
+        case 32: launch_apply_rk<T, 32, 1, 8, 1>(a, lg, s, tu); break;
+        default: launch_apply_rk<T, 64, 1, 4, 1>(a, lg, s, tu); break;
         }
     } else if (K == 2) {
-        launch_apply_rk<T, 64, 2, 4, 1>(a, lg, s);
+        launch_apply_rk<T, 64, 2, 4, 1>(a, lg, s, tu);
     } else {
-        launch_apply_rk<T, 64, 4, 4, 1>(a, lg, s);
+        launch_apply_rk<T, 64, 4, 4, 1>(a, lg, s, tu);
     }
     if (a.rsplit > 1) {
         const int64_t n = (int64_t)a.nsd * a.nx * a.nx * (a.rsplit - 1);
@@ -1381,9 +1386,10 @@ template <typename T> PenArgs<T> pen_args(tng_operator &op, const tng_vector &sr
 
 // fs/fs penalty: the per-block rotation moments of src into the context's
 // partials (kPenBlocks x 3 doubles)
-template <typename T> void penalty_partials(tng_operator &op, const tng_vector &src, hipStream_t s)
+template <typename T> void penalty_partials(tng_operator &op, const tng_vector &src, hipStream_t s, double *partials)
 {
-    const PenArgs<T> p = pen_args<T>(op, src);
+    PenArgs<T> p = pen_args<T>(op, src);
+    p.partials = partials;
     hipLaunchKernelGGL(penalty_partial_kernel<T>, dim3(kPenBlocks), dim3(kPenThreads), 0, s, p);
     TNG_HIP_CHECK(hipGetLastError());
 }
@@ -1407,9 +1413,10 @@ template <typename T> PenArgs<T> pen_args(tng_operator &op, const tng_vector &sr
 }
 
 // fs/fs penalty: the rotation moment b of src into op.d_penalty_b
-template <typename T> void penalty_moment(tng_operator &op, const tng_vector &src, hipStream_t s)
+template <typename T> void penalty_moment(tng_operator &op, const tng_vector &src, hipStream_t s, double *partials)
 {
-    const PenArgs<T> p = pen_args<T>(op, src);
+    PenArgs<T> p = pen_args<T>(op, src);
+    p.partials = partials;
     hipLaunchKernelGGL(penalty_partial_kernel<T>, dim3(kPenBlocks), dim3(kPenThreads), 0, s, p);
     hipLaunchKernelGGL(penalty_final_kernel<T>, dim3(1), dim3(256), 0, s, p, kPenBlocks);
     TNG_HIP_CHECK(hipGetLastError());
@@ -1440,7 +1447,8 @@ template <typename T> void apply_impl(tng_operator &op, const tng_vector &src, t
     const bool add = op.apply_mode == TNG_ADD;
     hipEvent_t *ev = ctx.profiling ? ctx.prof_next_set() : nullptr;
     if (ev) TNG_HIP_CHECK(hipEventRecord(ev[0], s));
-    auto penalty_moment = [&]() { tng::penalty_moment<T>(op, src, s); };
+    double *partials = m.ctx->red_partials.as<double>();
+    auto penalty_moment = [&]() { tng::penalty_moment<T>(op, src, s, partials); };
     if (op.stored) {
         // slow path on stored matrices (Dirichlet/free-slip handled per row)
         const bool pen = op.penalty_active && !op.diagonal;
@@ -1457,7 +1465,7 @@ template <typename T> void apply_impl(tng_operator &op, const tng_vector &src, t
         // penalty: the per-block moments only; every apply workgroup combines them
         const bool pen = op.penalty_active;
         ApplyArgs<T> a = apply_args<T>(op, src, dst, false, add);
-        const ApplyShape sh = apply_shape(m.nr - 1);
+        const ApplyShape sh = apply_shape(m.nr - 1, m.ctx->tuning);
         if (sh.S > 1) {
             // partial sums of the levels between radial parts
             const size_t bytes = sizeof(T) * 6 * (sh.S - 1) * (size_t)m.nsd * m.nx * m.nx;
@@ -1465,14 +1473,14 @@ template <typename T> void apply_impl(tng_operator &op, const tng_vector &src, t
             a.iface = op.d_iface.as<T>();
         }
         if (pen) {
-            penalty_partials<T>(op, src, s);
-            a.pen_part = m.ctx->red_partials.as<double>();
+            penalty_partials<T>(op, src, s, partials);
+            a.pen_part = partials;
             a.pen_nb = kPenBlocks;
             a.pen_eps = op.penalty_eps;
             for (int i = 0; i < 9; ++i) a.pen_ginv[i] = m.rot_gram_inv[i];
         }
         if (ev) TNG_HIP_CHECK(hipEventRecord(ev[1], s));
-        launch_apply<T>(a, s);
+        launch_apply<T>(a, s, m.ctx->tuning);
         if (ev) TNG_HIP_CHECK(hipEventRecord(ev[2], s));
     }
     if (op.comm_mode == TNG_COMMUNICATE_ADDITIVELY) launch_halo_sum(m, dst, s);
@@ -1492,14 +1500,14 @@ bool apply_cheb_impl(tng_operator &op, const tng_vector &x, tng_vector &ax, tng_
     if (op.stored || op.diagonal || op.apply_mode != TNG_REPLACE || op.comm_mode != TNG_COMMUNICATE_ADDITIVELY ||
         m.rad_sdr != 0)
         return false;
-    const ApplyShape sh = apply_shape(m.nr - 1, true);
+    const ApplyShape sh = apply_shape(m.nr - 1, m.ctx->tuning, true);
     if (!(sh.R == 64 && sh.K == 1 && sh.NW == 8 && sh.RG <= 2 && sh.S == 1)) return false;
     tng_context &ctx = *m.ctx;
     hipEvent_t *ev = ctx.profiling ? ctx.prof_next_set() : nullptr;
     if (ev) TNG_HIP_CHECK(hipEventRecord(ev[0], s));
     if (!op.btab_valid) epsdivdiv_update_boundary_table(op, s);
     const bool pen = op.penalty_active;
-    if (pen) penalty_moment<T>(op, x, s);
+    if (pen) penalty_moment<T>(op, x, s, m.ctx->red_partials.as<double>());
     ApplyArgs<T> a = apply_args<T>(op, x, ax, pen, false);
     a.cb0 = (const T *)b.comp(0); a.cb1 = (const T *)b.comp(1); a.cb2 = (const T *)b.comp(2);
     a.ci0 = (const T *)invd.comp(0); a.ci1 = (const T *)invd.comp(1); a.ci2 = (const T *)invd.comp(2);
@@ -1509,7 +1517,7 @@ bool apply_cheb_impl(tng_operator &op, const tng_vector &x, tng_vector &ax, tng_
     a.cad = (T)ad;
     a.cread_d = read_d ? 1 : 0;
     if (ev) TNG_HIP_CHECK(hipEventRecord(ev[1], s));
-    launch_apply<T, true>(a, s);
+    launch_apply<T, true>(a, s, m.ctx->tuning);
     if (ev) TNG_HIP_CHECK(hipEventRecord(ev[2], s));
     launch_halo_cheb(m, ax, b, invd, d, x, x_out, az, ad, read_d, s);
     if (ev) TNG_HIP_CHECK(hipEventRecord(ev[3], s));

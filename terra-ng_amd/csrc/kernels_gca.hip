@@ -6,10 +6,9 @@
 //    A(i + 6 di, j + 6 dj) = k |det J| [ (g_i . g_j) d_{di,dj} + g_i[dj] g_j[di]
 //                                        - 2/3 g_i[di] g_j[dj] ]
 //    with g_n = J^{-T} grad N_n at the centroid and k the mean nodal k.
-//  - gca_kernel: per coarse cell, sum over its 8 fine cells x 2 wedges of
-//    P^T A_fine P (linalg/solvers/gca/gca.hpp:153-397), A_fine stored or
-//    assembled on the fly; one workgroup per coarse cell, one thread per
-//    matrix entry.
+//  - gca_onthefly_kernel / gca_stored_kernel: per coarse cell, sum over its
+//    8 fine cells x 2 wedges of P^T A_fine P (linalg/solvers/gca/gca.hpp
+//    :153-397), A_fine assembled on the fly from k or read from storage.
 //  - stored_apply_kernel: the slow path on stored matrices
 //    (epsilon_divdiv_kerngen.hpp:1013-1242) as a per-node gather: a node
 //    collects rows of the <= 12 wedges around it, so every matrix row is read
@@ -123,101 +122,175 @@ __device__ __forceinline__ double pc_weight(int xf, int yf, int rf, int xc, int 
     return 0.0;
 }
 
+// Row n of the constant interpolation P (6 x 6) from the parent coarse wedge's
+// nodes to node n of fine wedge w of the fine hex h of a coarse cell
+// (gca.hpp:190-361).  The fine node sits at offsets (ox, oy, orr) in 0..2 from
+// the coarse cell's first node, so P depends only on (h, w, n), on whether the
+// coarse cell is the top cell of its subdomain, and on the top-face rule.
+// The reference takes the coarse bottom layer of a fine node as r_fine / 2
+// (gca.hpp:206-208), except at the subdomain's top node level, where it takes
+// r_fine / 2 - 1: fine nodes on the top face of an interior coarse cell at odd
+// lateral positions then draw their weights from the coarse top layer but
+// store them in the bottom slots.  exact = true uses the cell's own bottom
+// layer everywhere, which gives the Galerkin product R A P.
+__device__ __forceinline__ void gca_prow(int h, int w, int n, bool top_cell, bool exact, double prow[6])
+{
+    const int ox = (h & 1) + kWN[w][n % 3][0], oy = ((h >> 1) & 1) + kWN[w][n % 3][1], orr = (h >> 2) + n / 3;
+    for (int c = 0; c < 6; ++c) prow[c] = 0.0;
+    if (ox % 2 == 0 && oy % 2 == 0 && orr % 2 == 0) {
+        prow[n] = 1.0;
+        return;
+    }
+    if (ox % 2 == 0 && oy % 2 == 0) {
+        prow[n % 3] = 0.5;
+        prow[n % 3 + 3] = 0.5;
+        return;
+    }
+    const int rb = (orr == 2 && (top_cell || exact)) ? 0 : orr / 2;
+    int x0, y0, x1, y1, l0, l1, l2, l3;
+    if (ox % 2 == 0) {
+        x0 = ox / 2; x1 = ox / 2; y0 = oy / 2; y1 = oy / 2 + 1; l0 = 0; l1 = 2; l2 = 3; l3 = 5;
+    } else if (oy % 2 == 0) {
+        x0 = ox / 2; x1 = ox / 2 + 1; y0 = oy / 2; y1 = oy / 2; l0 = 0; l1 = 1; l2 = 3; l3 = 4;
+    } else {
+        x0 = ox / 2 + 1; x1 = ox / 2; y0 = oy / 2; y1 = oy / 2 + 1; l0 = 1; l1 = 2; l2 = 4; l3 = 5;
+    }
+    prow[l0] = pc_weight(ox, oy, orr, x0, y0, rb);
+    prow[l1] = pc_weight(ox, oy, orr, x1, y1, rb);
+    prow[l2] = pc_weight(ox, oy, orr, x0, y0, rb + 1);
+    prow[l3] = pc_weight(ox, oy, orr, x1, y1, rb + 1);
+}
+
+// The 8 fine wedges (2 h + w) that each coarse wedge collects, in the order
+// h-major, w-minor (gca.hpp:374-391).
+__constant__ int kGcaFrom[2][8] = {{0, 1, 2, 4, 8, 9, 10, 12}, {3, 5, 6, 7, 11, 13, 14, 15}};
+
 struct GcaArgs {
     const double *lf;                   // stored fine matrices, or nullptr: assemble from k
     const double *coords, *radii, *k;   // fine mesh, fine k
     double *lc;                         // coarse matrices
     int nsd, nxf, nrf, nxc, nrc;
+    int exact;                          // 1: Galerkin product R A P on the top faces too
+    int64_t ncell;                      // coarse cells
 };
 
-// one workgroup per coarse cell; thread e < 324 owns entry e of both coarse wedges
-__global__ __launch_bounds__(384) void gca_kernel(const GcaArgs a)
+constexpr int kGcaCells = 16;  // coarse cells per workgroup of the on-the-fly kernel
+
+// Fine operator assembled on the fly (Neumann local matrices from k).  Every
+// fine matrix entry is a bilinear form of two nodal gradients,
+//   A_f((i,di),(j,dj)) = jkw [ (g_i . g_j) d_{di,dj} + g_i[dj] g_j[di] - 2/3 g_i[di] g_j[dj] ],
+// so P^T A_f P is the same form of the combined gradients G_c = sum_n P[n][c] g_n.
+// Phase 1: one thread per fine wedge of 16 coarse cells computes its geometry
+// and G (6 x 3) into LDS.  Phase 2: the 16 x 648 coarse entries, each a sum
+// over the 8 fine wedges of its coarse wedge, written contiguously.
+__global__ __launch_bounds__(256) void gca_onthefly_kernel(const GcaArgs a)
 {
-    __shared__ double Af[324];
-    __shared__ double Pm[6][6];
-    __shared__ double gsh[6][3];
-    __shared__ double jkw_sh;
+    __shared__ double sG[kGcaCells][16][6][3];
+    __shared__ double sJ[kGcaCells][16];
+    const int ncc = a.nxc - 1, ncrc = a.nrc - 1;
+    const int64_t cell0 = (int64_t)blockIdx.x * kGcaCells;
+    const int ncl = (int)min<int64_t>(kGcaCells, a.ncell - cell0);
+    {
+        const int cl = threadIdx.x >> 4, fw = threadIdx.x & 15;
+        if (cl < ncl) {
+            int64_t t = cell0 + cl;
+            const int Rr = (int)(t % ncrc); t /= ncrc;
+            const int Y = (int)(t % ncc); t /= ncc;
+            const int X = (int)(t % ncc);
+            const int sd = (int)(t / ncc);
+            const int h = fw >> 1, w = fw & 1;
+            const int fx = 2 * X + (h & 1), fy = 2 * Y + ((h >> 1) & 1), fr = 2 * Rr + (h >> 2);
+            double g[6][3], jkw;
+            wedge_gradients(a.coords, a.radii, a.k, a.nxf, a.nrf, sd, fx, fy, fr, w, g, &jkw);
+            double G[6][3];
+            for (int c = 0; c < 6; ++c)
+                for (int d = 0; d < 3; ++d) G[c][d] = 0.0;
+            const bool top = Rr == ncrc - 1;
+            for (int n = 0; n < 6; ++n) {
+                double prow[6];
+                gca_prow(h, w, n, top, a.exact != 0, prow);
+                for (int c = 0; c < 6; ++c)
+                    if (prow[c] != 0.0)
+                        for (int d = 0; d < 3; ++d) G[c][d] = fma(prow[c], g[n][d], G[c][d]);
+            }
+            for (int c = 0; c < 6; ++c)
+                for (int d = 0; d < 3; ++d) sG[cl][fw][c][d] = G[c][d];
+            sJ[cl][fw] = jkw;
+        }
+    }
+    __syncthreads();
+    double *out = a.lc + cell0 * 648;
+    for (int o = threadIdx.x; o < ncl * 648; o += 256) {
+        const int cl = o / 648, rem = o - cl * 648;
+        const int tw = rem >= 324 ? 1 : 0, e = rem - tw * 324;
+        const int row = e / 18, col = e % 18;
+        const int di = row / 6, ii = row % 6, dj = col / 6, jj = col % 6;
+        double s = 0.0;
+        for (int q = 0; q < 8; ++q) {
+            const int fw = kGcaFrom[tw][q];
+            const double *Gi = sG[cl][fw][ii], *Gj = sG[cl][fw][jj];
+            const double gg = Gi[0] * Gj[0] + Gi[1] * Gj[1] + Gi[2] * Gj[2];
+            s += sJ[cl][fw] * ((di == dj ? gg : 0.0) + Gi[dj] * Gj[di] - (2.0 / 3.0) * Gi[di] * Gj[dj]);
+        }
+        out[o] = s;
+    }
+}
+
+// Fine operator from stored matrices: one workgroup per coarse cell loads the
+// 16 fine matrices (41 KB) into LDS with every load issued before the first
+// use, the 16 P matrices beside them, then forms the 648 coarse entries
+// sum_{a,b} P[a][i] A_f((a,di),(b,dj)) P[b][j] over the 8 fine wedges of each
+// coarse wedge, skipping zero weights.
+__global__ __launch_bounds__(256) void gca_stored_kernel(const GcaArgs a)
+{
+    __shared__ double sA[16][324];
+    __shared__ double sP[16][6][6];
     const int ncc = a.nxc - 1, ncrc = a.nrc - 1, ncf = a.nxf - 1, ncrf = a.nrf - 1;
     int64_t t = blockIdx.x;
     const int Rr = (int)(t % ncrc); t /= ncrc;
     const int Y = (int)(t % ncc); t /= ncc;
     const int X = (int)(t % ncc);
     const int sd = (int)(t / ncc);
-    const int e = threadIdx.x;
-    const int row = e / 18, col = e % 18;
-    const int di = row / 6, ii = row % 6, dj = col / 6, jj = col % 6;
-    double acc0 = 0.0, acc1 = 0.0;
-    for (int h = 0; h < 8; ++h) {
-        const int fx = 2 * X + (h & 1), fy = 2 * Y + ((h >> 1) & 1), fr = 2 * Rr + (h >> 2);
-        for (int w = 0; w < 2; ++w) {
-            if (a.lf == nullptr && e == 0) {
-                double g[6][3], jkw;
-                wedge_gradients(a.coords, a.radii, a.k, a.nxf, a.nrf, sd, fx, fy, fr, w, g, &jkw);
-                for (int n = 0; n < 6; ++n)
-                    for (int c = 0; c < 3; ++c) gsh[n][c] = g[n][c];
-                jkw_sh = jkw;
-            }
-            if (e < 6) {
-                // row e of P: fine wedge node e from the parent coarse wedge's nodes
-                const int n = e;
-                double prow[6] = {0, 0, 0, 0, 0, 0};
-                const int xf = fx + kWN[w][n % 3][0], yf = fy + kWN[w][n % 3][1], rf = fr + n / 3;
-                if (xf % 2 == 0 && yf % 2 == 0 && rf % 2 == 0) {
-                    prow[n] = 1.0;
-                } else if (xf % 2 == 0 && yf % 2 == 0) {
-                    prow[n % 3] = 0.5;
-                    prow[n % 3 + 3] = 0.5;
-                } else {
-                    // the reference's coarse bottom layer (gca.hpp:206-208)
-                    const int rb = rf < a.nrf - 1 ? rf / 2 : rf / 2 - 1;
-                    int x0, y0, x1, y1, l0, l1, l2, l3;
-                    if (xf % 2 == 0) {
-                        x0 = xf / 2; x1 = xf / 2; y0 = yf / 2; y1 = yf / 2 + 1; l0 = 0; l1 = 2; l2 = 3; l3 = 5;
-                    } else if (yf % 2 == 0) {
-                        x0 = xf / 2; x1 = xf / 2 + 1; y0 = yf / 2; y1 = yf / 2; l0 = 0; l1 = 1; l2 = 3; l3 = 4;
-                    } else {
-                        x0 = xf / 2 + 1; x1 = xf / 2; y0 = yf / 2; y1 = yf / 2 + 1; l0 = 1; l1 = 2; l2 = 4; l3 = 5;
-                    }
-                    prow[l0] = pc_weight(xf, yf, rf, x0, y0, rb);
-                    prow[l1] = pc_weight(xf, yf, rf, x1, y1, rb);
-                    prow[l2] = pc_weight(xf, yf, rf, x0, y0, rb + 1);
-                    prow[l3] = pc_weight(xf, yf, rf, x1, y1, rb + 1);
-                }
-                for (int c = 0; c < 6; ++c) Pm[n][c] = prow[c];
-            }
-            __syncthreads();
-            if (e < 324) {
-                if (a.lf) {
-                    Af[e] = a.lf[lmat_index(ncf, ncrf, sd, fx, fy, fr, w) + e];
-                } else {
-                    double g[6][3];
-                    for (int n = 0; n < 6; ++n)
-                        for (int c = 0; c < 3; ++c) g[n][c] = gsh[n][c];
-                    Af[e] = local_entry(g, jkw_sh, row, col);
-                }
-            }
-            __syncthreads();
-            if (e < 324) {
-                double s = 0.0;
-                for (int aa = 0; aa < 6; ++aa) {
-                    const double pa = Pm[aa][ii];
-                    if (pa == 0.0) continue;
-                    double r = 0.0;
-                    for (int b = 0; b < 6; ++b) r += Af[(aa + 6 * di) * 18 + b + 6 * dj] * Pm[b][jj];
-                    s += pa * r;
-                }
-                // target coarse wedge (gca.hpp:374-391)
-                const bool to0 = (w == 0 && (h == 0 || h == 1 || h == 2 || h == 4 || h == 5 || h == 6)) ||
-                                 (w == 1 && (h == 0 || h == 4));
-                if (to0) acc0 += s;
-                else acc1 += s;
-            }
-            __syncthreads();
-        }
+    // the fine matrices of hexes h and h + 4 (radial neighbours), both
+    // wedges, are 4 x 324 contiguous doubles
+    for (int i = threadIdx.x; i < 16 * 324; i += 256) {
+        const int blk = i / 1296, off = i - blk * 1296;  // blk = h & 3
+        const int fx = 2 * X + (blk & 1), fy = 2 * Y + (blk >> 1);
+        const double v = a.lf[lmat_index(ncf, ncrf, sd, fx, fy, 2 * Rr, 0) + off];
+        // off = (dr * 2 + w) * 324 + e  ->  fine wedge fw = 2 (blk + 4 dr) + w
+        const int dr = off / 648, w = (off / 324) & 1, e = off % 324;
+        sA[2 * (blk + 4 * dr) + w][e] = v;
     }
-    if (e < 324) {
-        a.lc[lmat_index(ncc, ncrc, sd, X, Y, Rr, 0) + e] = acc0;
-        a.lc[lmat_index(ncc, ncrc, sd, X, Y, Rr, 1) + e] = acc1;
+    if (threadIdx.x < 96) {
+        const int fw = threadIdx.x / 6, n = threadIdx.x % 6;
+        double prow[6];
+        gca_prow(fw >> 1, fw & 1, n, Rr == ncrc - 1, a.exact != 0, prow);
+        for (int c = 0; c < 6; ++c) sP[fw][n][c] = prow[c];
+    }
+    __syncthreads();
+    double *out = a.lc + (int64_t)blockIdx.x * 648;
+    for (int o = threadIdx.x; o < 648; o += 256) {
+        const int tw = o >= 324 ? 1 : 0, e = o - tw * 324;
+        const int row = e / 18, col = e % 18;
+        const int di = row / 6, ii = row % 6, dj = col / 6, jj = col % 6;
+        double s = 0.0;
+        for (int q = 0; q < 8; ++q) {
+            const int fw = kGcaFrom[tw][q];
+            const double *A = sA[fw];
+            double sw = 0.0;
+            for (int aa = 0; aa < 6; ++aa) {
+                const double pa = sP[fw][aa][ii];
+                if (pa == 0.0) continue;
+                double r = 0.0;
+                for (int b = 0; b < 6; ++b) {
+                    const double pb = sP[fw][b][jj];
+                    if (pb != 0.0) r += A[(aa + 6 * di) * 18 + b + 6 * dj] * pb;
+                }
+                sw += pa * r;
+            }
+            s += sw;
+        }
+        out[o] = s;
     }
 }
 
@@ -474,7 +547,7 @@ void epsdivdiv_local_matrices(const tng_operator &op, DevBuf &out, hipStream_t s
 
 size_t stored_matrix_count(const tng_domain &m) { return (size_t)m.nsd * (m.nx - 1) * (m.nx - 1) * (m.nr - 1) * 2; }
 
-void gca_assemble(const tng_operator &fine, tng_operator &coarse, hipStream_t s)
+void gca_assemble(const tng_operator &fine, tng_operator &coarse, hipStream_t s, bool exact)
 {
     const tng_domain &f = *fine.mesh, &c = *coarse.mesh;
     TNG_REQUIRE(f.nsd == c.nsd && f.nx - 1 == 2 * (c.nx - 1) && f.nr - 1 == 2 * (c.nr - 1),
@@ -489,10 +562,14 @@ void gca_assemble(const tng_operator &fine, tng_operator &coarse, hipStream_t s)
     }
     const size_t bytes = stored_matrix_count(c) * 324 * sizeof(double);
     if (coarse.d_lmat.bytes != bytes) coarse.d_lmat.alloc(bytes);
-    GcaArgs a{lf, f.d_coords.as<double>(), f.d_radii.as<double>(), fine.k ? (const double *)fine.k->comp(0) : nullptr,
-              coarse.d_lmat.as<double>(), f.nsd, f.nx, f.nr, c.nx, c.nr};
     const int64_t ncell = (int64_t)c.nsd * (c.nx - 1) * (c.nx - 1) * (c.nr - 1);
-    hipLaunchKernelGGL(gca_kernel, dim3((unsigned)ncell), dim3(384), 0, s, a);
+    GcaArgs a{lf, f.d_coords.as<double>(), f.d_radii.as<double>(), fine.k ? (const double *)fine.k->comp(0) : nullptr,
+              coarse.d_lmat.as<double>(), f.nsd, f.nx, f.nr, c.nx, c.nr, exact ? 1 : 0, ncell};
+    if (lf)
+        hipLaunchKernelGGL(gca_stored_kernel, dim3((unsigned)ncell), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(gca_onthefly_kernel, dim3((unsigned)((ncell + kGcaCells - 1) / kGcaCells)), dim3(256), 0,
+                           s, a);
     TNG_HIP_CHECK(hipGetLastError());
     coarse.stored = true;
     coarse.lmat_dtype = TNG_F64;
@@ -516,7 +593,7 @@ void gca_collect(const tng_vector &k_fine, int level_range, const tng_domain &co
 }
 
 void gca_assemble_selective(const tng_operator &fine, tng_operator &coarse, int level_range,
-                            const tng_domain &coarsest, const DevBuf &flags, hipStream_t s)
+                            const tng_domain &coarsest, const DevBuf &flags, hipStream_t s, bool exact)
 {
     const tng_domain &c = *coarse.mesh;
     const int ncx0 = coarsest.nx - 1, ncr0 = coarsest.nr - 1;
@@ -524,7 +601,7 @@ void gca_assemble_selective(const tng_operator &fine, tng_operator &coarse, int 
     TNG_REQUIRE(level_range >= 0 && c.nsd == coarsest.nsd && (c.nx - 1) == (ncx0 << level_range) &&
                     (c.nr - 1) == (ncr0 << level_range),
                 "GCA: the coarse operator's mesh must be level_range refinements above the coarsest mesh");
-    gca_assemble(fine, coarse, s);
+    gca_assemble(fine, coarse, s, exact);
     DevBuf local;
     epsdivdiv_local_matrices(coarse, local, s);
     const int64_t n = (int64_t)stored_matrix_count(c) * 324;

@@ -39,19 +39,70 @@ __host__ __device__ __forceinline__ int fine_lat_idx(int x, int y, int w)
                   : ((y & 1) ? ((x & 1) ? 0 : 1) : ((x & 1) ? 2 : 3));
 }
 
+// DPP wave_shr:1 -- lane i receives lane i-1; lane 0 receives 0 (bound control).
+__device__ __forceinline__ double wave_shr1_t(double v)
+{
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x138, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x138, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float wave_shr1_t(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+// A wave-uniform value moved to scalar registers.
+__device__ __forceinline__ double uniform_value(double v)
+{
+    return __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(v)),
+                            __builtin_amdgcn_readfirstlane(__double2loint(v)));
+}
+__device__ __forceinline__ float uniform_value(float v)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+// DPP wave_shl:1 -- lane i receives lane i+1; lane 63 receives 0.
+__device__ __forceinline__ double wave_shl1_t(double v)
+{
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x130, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x130, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float wave_shl1_t(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+// DPP quad_perm [1, 0, 3, 2] -- lanes 2i and 2i+1 exchange values.
+__device__ __forceinline__ double pair_swap_t(double v)
+{
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0xb1, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0xb1, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float pair_swap_t(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xb1, 0xf, 0xf, true));
+}
+
+// Lateral part of coarse shape function k (0..2) at the centroid (1/3, 1/3)
+// of fine lateral wedge lat (0..3: the three corner triangles, then the
+// inverted centre one).
+template <typename T> __device__ __forceinline__ T shape_lat_c(int k, int lat)
+{
+    const T third = T(1.0 / 3.0);
+    switch (k % 3) {
+    case 0: return lat == 0 ? T(-0.5) * third - T(0.5) * third + T(1) : (lat == 3 ? T(0.5) * third + T(0.5) * third
+                                                                                   : T(-0.5) * third - T(0.5) * third + T(0.5));
+    case 1: return lat == 1 ? T(0.5) * third + T(0.5) : (lat == 3 ? T(0.5) - T(0.5) * third : T(0.5) * third);
+    default: return lat == 2 ? T(0.5) * third + T(0.5) : (lat == 3 ? T(0.5) - T(0.5) * third : T(0.5) * third);
+    }
+}
+
 // Coarse shape function i (0..5) at the fine centroid (1/3, 1/3, 0).
 template <typename T> __device__ __forceinline__ T shape_coarse_c(int i, int lat, int rad)
 {
-    const T third = T(1.0 / 3.0);
-    T sl;
-    switch (i % 3) {
-    case 0: sl = lat == 0 ? T(-0.5) * third - T(0.5) * third + T(1) : (lat == 3 ? T(0.5) * third + T(0.5) * third
-                                                                                 : T(-0.5) * third - T(0.5) * third + T(0.5)); break;
-    case 1: sl = lat == 1 ? T(0.5) * third + T(0.5) : (lat == 3 ? T(0.5) - T(0.5) * third : T(0.5) * third); break;
-    default: sl = lat == 2 ? T(0.5) * third + T(0.5) : (lat == 3 ? T(0.5) - T(0.5) * third : T(0.5) * third); break;
-    }
     const T sr = (i / 3 == 0) ? (rad == 0 ? T(0.75) : T(0.25)) : (rad == 0 ? T(0.25) : T(0.75));
-    return sl * sr;
+    return shape_lat_c<T>(i, lat) * sr;
 }
 
 // wedge corner (dx, dy) per wedge and local lateral index
@@ -180,6 +231,190 @@ __global__ __launch_bounds__(256) void div_cell_kernel(const DivArgs<T> a, T *cd
             }
         }
         cdiv[(size_t)gid * 2 + w] = -(g[9] * vol) * div;
+    }
+}
+
+// B in one pass.  A workgroup covers CW - 1 adjacent coarse node columns
+// X0 .. X0 + CW - 2 of a subdomain over a chunk of cy coarse rows; its waves
+// are the coarse cell columns Xc = X0 - 1 .. X0 + CW - 2 (the first one a
+// halo, evaluated again by the workgroup to the left) times RW radial parts
+// of 64 fine cells (lane = fine cell).  Sweeping the coarse rows Y, a wave
+// evaluates the four fine cells of coarse cell (Xc, Y) -- per fine wedge
+// f = -|det J| vol div u, with
+//   div u = (M0 . (U_1 - U_0) + M1 . (U_2 - U_0)) / (2 r_mid) + M2 . sum_k D_k / (6h)
+// (U_k = bottom + top and D_k = top - bottom of the wedge's corner column k)
+// -- and adds f times the lateral coarse shape value of each corner to the
+// corner's node: (Xc, Y), (Xc + 1, Y), or carried to the next row (Xc, Y + 1),
+// (Xc + 1, Y + 1).  The sums for node column Xc + 1 pass to the next wave
+// through LDS.  The radial coarse weights (3/4, 1/4) split a lane's sum
+// between the bottom and top coarse node of its coarse cell, a lane pair
+// forms one coarse cell, and the cell below's top part arrives by a DPP
+// shift (across radial parts from the lanes 62, 63 sums in LDS).  The fine
+// node columns are swept along x, three rows at a time, the next x-column's
+// loads in flight while the current one is used; nothing is staged in HBM.
+// Trial-side boundary values are projected on loading (node level 0 belongs
+// only to the CMB cell, level ncr only to the surface cell).
+constexpr int kDivMaxRows = 8;  // coarse rows per workgroup (cy) at most
+template <typename T, int RW, int CW>
+__global__ __launch_bounds__(64 * RW * CW) void div_column_kernel(const DivArgs<T> a, T *dst, int add, int cy)
+{
+    __shared__ T xr[2][CW][RW][64];  // node column Xc + 1 sums of each wave, by row parity
+    __shared__ T xl[2][CW][RW][2];   // lanes 62, 63 of the node column Xc sums
+    // lateral geometry of the workgroup's fine cells: 2 CW columns x 2 (cy + 1) rows
+    __shared__ T geo[2 * CW][2 * (kDivMaxRows + 1)][20];
+    const int nx = a.nx, nr = a.nr, nc = nx - 1, nxc = a.nxc, nrc = a.nrc;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int part = wv % RW, xs = wv / RW;
+    const int nxw = (nxc + CW - 2) / (CW - 1);
+    const int nchunk = (nxc + cy - 1) / cy;
+    const int64_t wg = xcd_block();
+    const int wx = (int)(wg % nxw);
+    const int chunk = (int)((wg / nxw) % nchunk);
+    const int sd = (int)(wg / ((int64_t)nxw * nchunk));
+    const int Xc = wx * (CW - 1) - 1 + xs;
+    const bool cells = Xc >= 0 && Xc <= nxc - 2;
+    const bool out = xs >= 1 && Xc <= nxc - 1;
+    const int c = part * 64 + lane;  // fine radial cell of the lane
+    const int bcb = part == 0 && a.sd_cmb[sd] != 0 ? a.bc_cmb : TNG_NEUMANN;
+    const int bct = part == RW - 1 && a.sd_surf[sd] != 0 ? a.bc_surf : TNG_NEUMANN;
+    T hs, tz, vol;
+    {
+        const T r0 = a.radii[(int64_t)sd * nr + c], r1 = a.radii[(int64_t)sd * nr + c + 1];
+        const T rmid = T(0.5) * (r0 + r1), h = T(0.5) * (r1 - r0);
+        hs = T(0.5) / rmid;
+        tz = T(1) / (T(6) * h);
+        vol = rmid * rmid * h;
+    }
+    const bool odd = lane & 1;
+    const T srb = odd ? T(0.25) : T(0.75), srt = odd ? T(0.75) : T(0.25);
+    const int Y0 = chunk * cy, Y1 = min(Y0 + cy, nxc), Ys = max(Y0 - 1, 0);
+    {
+        // fine cells x = 2 (X0 - 1) + i, y = 2 Ys + j of existing coarse cells
+        const int xf0 = 2 * (wx * (CW - 1) - 1), yf0 = 2 * Ys;
+        const int nyf = 2 * (min(Y1 - 1, nxc - 2) - Ys + 1);
+        for (int e = threadIdx.x; e < 2 * CW * 2 * (kDivMaxRows + 1) * 20; e += 64 * RW * CW) {
+            const int v = e % 20, j = (e / 20) % (2 * (kDivMaxRows + 1)), i = e / (20 * 2 * (kDivMaxRows + 1));
+            const int xf = xf0 + i, yf = yf0 + j;
+            if (xf >= 0 && xf < nc && j < nyf) geo[i][j][v] = a.latgeom[(((int64_t)sd * nc + xf) * nc + yf) * 20 + v];
+        }
+        __syncthreads();
+    }
+    const T *src[3] = {a.u0, a.u1, a.u2};
+    // node copies are indexed in 32 bits (checked on the host)
+    const uint32_t xstride = (uint32_t)nx * nr, ystride = (uint32_t)nr;
+    const uint32_t sdbase = (uint32_t)sd * nx * xstride;
+    // raw values of the node column x at rows y0, y0 + 1, y0 + 2: levels c and
+    // c + 1 of every lane; the next x-column's loads are in flight while one is used
+    T pr[3][3], pt[3][3];
+    auto issue = [&](int x, int y0) {
+        const uint32_t cb = sdbase + (uint32_t)x * xstride + (uint32_t)y0 * ystride;
+#pragma unroll
+        for (int row = 0; row < 3; ++row) {
+            const uint32_t b = cb + row * ystride + c;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                pr[row][d] = src[d][b];
+                pt[row][d] = src[d][b + 1];
+            }
+        }
+    };
+    if (cells && Ys <= nxc - 2) issue(2 * Xc, 2 * Ys);
+    T Lc = T(0), Ln = T(0), Rc = T(0), Rn = T(0);
+    T U[2][3][3], Dv[2][3][3];  // [x slot][row][component]
+    for (int Y = Ys; Y < Y1; ++Y) {
+        Lc = Ln;
+        Rc = Rn;
+        Ln = Rn = T(0);
+        if (cells && Y <= nxc - 2) {
+#pragma unroll
+            for (int xi = 0; xi < 3; ++xi) {
+                const int x = 2 * Xc + xi;
+                const int s = xi & 1;
+                // the column's sums and differences, then the next column's loads
+#pragma unroll
+                for (int row = 0; row < 3; ++row) {
+                    T lo[3] = {pr[row][0], pr[row][1], pr[row][2]};
+                    T hi[3] = {pt[row][0], pt[row][1], pt[row][2]};
+                    const int64_t lcol = ((int64_t)sd * nx + x) * nx + 2 * Y + row;
+                    if (bcb != TNG_NEUMANN && lane == 0) bc_project(a.coords, lcol, bcb, lo);
+                    if (bct != TNG_NEUMANN && lane == 63) bc_project(a.coords, lcol, bct, hi);
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        U[s][row][d] = lo[d] + hi[d];
+                        Dv[s][row][d] = hi[d] - lo[d];
+                    }
+                }
+                if (xi < 2)
+                    issue(x + 1, 2 * Y);
+                else if (Y + 1 < Y1 && Y + 1 <= nxc - 2)
+                    issue(2 * Xc, 2 * Y + 2);
+                if (xi == 0) continue;
+                // the two fine cells (x - 1, 2Y + fy)
+                const int fx = xi - 1, sL = (xi - 1) & 1, sR = s;
+#pragma unroll
+                for (int fy = 0; fy < 2; ++fy) {
+                    const T *gw = geo[2 * xs + fx][2 * (Y - Ys) + fy];
+#pragma unroll
+                    for (int w = 0; w < 2; ++w) {
+                        T g[10];
+#pragma unroll
+                        for (int v = 0; v < 10; ++v) g[v] = uniform_value(gw[w * 10 + v]);
+                        // corners k = 0, 1, 2: (0,0), (1,0), (0,1) / (1,1), (0,1), (1,0)
+                        const int s0 = w == 0 ? sL : sR, r0 = fy + (w == 0 ? 0 : 1);
+                        const int s1 = w == 0 ? sR : sL, r1 = fy + (w == 0 ? 0 : 1);
+                        const int s2 = w == 0 ? sL : sR, r2 = fy + (w == 0 ? 1 : 0);
+                        const int lat = fine_lat_idx(fx, fy, w);
+                        T lat1 = T(0), rad = T(0);
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) {
+                            const T u0 = U[s0][r0][d];
+                            lat1 += g[d] * (U[s1][r1][d] - u0) + g[3 + d] * (U[s2][r2][d] - u0);
+                            rad += g[6 + d] * (Dv[s0][r0][d] + Dv[s1][r1][d] + Dv[s2][r2][d]);
+                        }
+                        const T f = (-g[9] * vol) * (hs * lat1 + tz * rad);
+                        // coarse wedge w's corners: w = 0 -> (Xc, Y), (Xc+1, Y), (Xc, Y+1);
+                        // w = 1 -> (Xc+1, Y+1), (Xc, Y+1), (Xc+1, Y)
+                        if (w == 0) {
+                            Lc += shape_lat_c<T>(0, lat) * f;
+                            Rc += shape_lat_c<T>(1, lat) * f;
+                            Ln += shape_lat_c<T>(2, lat) * f;
+                        } else {
+                            Rn += shape_lat_c<T>(0, lat) * f;
+                            Ln += shape_lat_c<T>(1, lat) * f;
+                            Rc += shape_lat_c<T>(2, lat) * f;
+                        }
+                    }
+                }
+            }
+        }
+        if (Y < Y0) continue;
+        // node (Xc, Y) = this wave's left sums + the left neighbour's right sums
+        const int pb = Y & 1;
+        xr[pb][xs][part][lane] = Rc;
+        if (lane >= 62) xl[pb][xs][part][lane - 62] = Lc;
+        __syncthreads();
+        if (!out) continue;
+        const T tot = Lc + xr[pb][xs - 1][part][lane];
+        // bottom parts of the coarse cell above each level, top parts of the one below
+        const T b = tot * srb, t = tot * srt;
+        const T bs = b + pair_swap_t(b), ts = t + pair_swap_t(t);
+        T below = wave_shr1_t(ts);
+        if (part > 0 && lane == 0) {
+            const T t62 = (xl[pb][xs][part - 1][0] + xr[pb][xs - 1][part - 1][62]) * T(0.25);
+            const T t63 = (xl[pb][xs][part - 1][1] + xr[pb][xs - 1][part - 1][63]) * T(0.75);
+            below = t63 + t62;
+        }
+        const int64_t ob = (((int64_t)sd * nxc + Xc) * nxc + Y) * nrc + part * 32;
+        if (!odd) {
+            const T v = bs + below;
+            const int64_t i = ob + (lane >> 1);
+            dst[i] = add ? dst[i] + v : v;
+        }
+        if (part == RW - 1 && lane == 63) {
+            const int64_t i = ob + 32;
+            dst[i] = add ? dst[i] + ts : ts;
+        }
     }
 }
 
@@ -426,6 +661,153 @@ __global__ __launch_bounds__(256) void gradient_kernel(const GradArgs<T> a, cons
     a.d0[gid] = o[0]; a.d1[gid] = o[1]; a.d2[gid] = o[2];
 }
 
+// B^T as one wave per fine node column (X, Y) with K cells per lane (64 K
+// radial cells; the top node level is the last lane's extra output).  Per
+// lane and cell c (the cell above node level c) and per wedge w that contains
+// the column (corner k): f = -|det J| p(centroid), a = f / r_mid, b = f / (6h);
+// node level c then receives, per wedge,
+//   (a_{c-1} + a_c) g_k + (b_{c-1} - b_c) M2,   g_k = l0 M0 + l1 M1,
+// the cell below's a, b coming from the lane below by a DPP shift.  Each lane
+// loads the coarse pressure levels of its cells from the 3 x 3 coarse node
+// columns around the column, all at once; the lateral geometry is
+// wave-uniform.  The parities (PX, PY) of the column fix which coarse columns
+// and lateral shape values each wedge uses, so they are compile-time.
+// Boundary nodes: Dirichlet 0, free slip the tangential part.
+template <typename T, int K, int PX, int PY>
+__device__ __forceinline__ void gradient_column(const GradArgs<T> &a, int lane, int X, int Y, int sd)
+{
+    constexpr int NP = K == 1 ? 2 : K / 2 + 1;  // coarse levels a lane reads per column
+    const int nx = a.nx, nr = a.nr, nc = nx - 1, ncr = nr - 1, nxc = a.nxc;
+    // coarse columns cx0 + i, cy0 + j (i, j < 3) with cx0 = floor((X - 1) / 2);
+    // the clamped ones are never used
+    const int cx0 = (X - 1) >> 1, cy0 = (Y - 1) >> 1;
+    const int cb = K == 1 ? lane / 2 : lane * (K / 2);  // first coarse level of the lane
+    T pc[3][3][NP];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int cx = min(max(cx0 + i, 0), nxc - 1), cy = min(max(cy0 + j, 0), nxc - 1);
+            const T *pp = a.p + (((int64_t)sd * nxc + cx) * nxc + cy) * a.nrc + cb;
+#pragma unroll
+            for (int q = 0; q < NP; ++q) pc[i][j][q] = pp[q];
+        }
+    const bool has_cmb = a.sd_cmb[sd] != 0, has_surf = a.sd_surf[sd] != 0;
+    constexpr int corner_of[2][2][2] = {{{0, 2}, {1, -1}}, {{-1, 1}, {2, 0}}};
+    T sfac[K], tz[K], vol[K];
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+        const int c = lane * K + p;
+        const T r0 = a.radii[(int64_t)sd * nr + c], r1 = a.radii[(int64_t)sd * nr + c + 1];
+        const T rmid = T(0.5) * (r0 + r1), h = T(0.5) * (r1 - r0);
+        sfac[p] = T(1) / rmid;
+        tz[p] = T(1) / (T(6) * h);
+        vol[p] = rmid * rmid * h;
+    }
+    T o[K + 1][3];
+#pragma unroll
+    for (int p = 0; p <= K; ++p)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) o[p][d] = T(0);
+    using CT = const T __attribute__((address_space(4)));
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int xc = X - 1 + i;
+        if (xc < 0 || xc >= nc) continue;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int yc = Y - 1 + j;
+            if (yc < 0 || yc >= nc) continue;
+            const CT *gcell = (const CT *)a.latgeom + (((int64_t)sd * nc + xc) * nc + yc) * 20;
+            // coarse column of the cell relative to (cx0, cy0); parities of the cell
+            const int cxl = i == 0 ? 0 : (PX ? 0 : 1), cyl = j == 0 ? 0 : (PY ? 0 : 1);
+            const int pxc = PX ^ 1 ^ i, pyc = PY ^ 1 ^ j;
+#pragma unroll
+            for (int w = 0; w < 2; ++w) {
+                const int k = corner_of[w][1 - i][1 - j];
+                if (k < 0) continue;
+                T g[10];
+#pragma unroll
+                for (int v = 0; v < 10; ++v) g[v] = gcell[w * 10 + v];
+                const T l0 = k == 0 ? T(-0.5) : (k == 1 ? T(0.5) : T(0));
+                const T l1 = k == 0 ? T(-0.5) : (k == 1 ? T(0) : T(0.5));
+                const T gl[3] = {l0 * g[0] + l1 * g[3], l0 * g[1] + l1 * g[4], l0 * g[2] + l1 * g[5]};
+                const int lat = fine_lat_idx(pxc, pyc, w);
+                const T sl0 = shape_lat_c<T>(0, lat), sl1 = shape_lat_c<T>(1, lat), sl2 = shape_lat_c<T>(2, lat);
+                // coarse columns of the coarse wedge's corners 0, 1, 2
+                const int ax = cxl + (w == 0 ? 0 : 1), ay = cyl + (w == 0 ? 0 : 1);
+                const int bx = cxl + (w == 0 ? 1 : 0), by = cyl + (w == 0 ? 0 : 1);
+                const int dx = cxl + (w == 0 ? 0 : 1), dy = cyl + (w == 0 ? 1 : 0);
+                T av[K], bv[K];
+#pragma unroll
+                for (int p = 0; p < K; ++p) {
+                    const int q = K == 1 ? 0 : p / 2;  // coarse level of the cell within the lane
+                    const bool top = K == 1 ? (lane & 1) != 0 : (p & 1) != 0;
+                    const T wb = top ? T(0.25) : T(0.75), wt = top ? T(0.75) : T(0.25);
+                    const T pi = sl0 * (wb * pc[ax][ay][q] + wt * pc[ax][ay][q + 1]) +
+                                 sl1 * (wb * pc[bx][by][q] + wt * pc[bx][by][q + 1]) +
+                                 sl2 * (wb * pc[dx][dy][q] + wt * pc[dx][dy][q + 1]);
+                    const T f = -(g[9] * vol[p]) * pi;
+                    av[p] = f * sfac[p];
+                    bv[p] = f * tz[p];
+                }
+                // the cell below the lane's first node level: the lane below's last cell
+                const T ab = wave_shr1_t(av[K - 1]), bb = wave_shr1_t(bv[K - 1]);
+#pragma unroll
+                for (int p = 0; p < K; ++p) {
+                    const T A = (p == 0 ? ab : av[p - 1]) + av[p];
+                    const T Bv = (p == 0 ? bb : bv[p - 1]) - bv[p];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) o[p][d] += A * gl[d] + Bv * g[6 + d];
+                }
+                // the top node level (last lane): the last cell only
+#pragma unroll
+                for (int d = 0; d < 3; ++d) o[K][d] += av[K - 1] * gl[d] + bv[K - 1] * g[6 + d];
+            }
+        }
+    }
+    const int64_t lcol = ((int64_t)sd * nx + X) * nx + Y;
+    const T *cn = a.coords + 3 * lcol;
+    T *dst[3] = {a.d0, a.d1, a.d2};
+#pragma unroll
+    for (int p = 0; p <= K; ++p) {
+        if (p == K && lane != 63) continue;
+        const int l = p == K ? ncr : lane * K + p;
+        T v[3] = {o[p][0], o[p][1], o[p][2]};
+        const bool at_cmb = l == 0 && has_cmb, at_surf = l == ncr && has_surf;
+        const int bc = at_cmb ? a.bc_cmb : (at_surf ? a.bc_surf : TNG_NEUMANN);
+        if (bc == TNG_DIRICHLET) {
+            v[0] = v[1] = v[2] = T(0);
+        } else if (bc == TNG_FREESLIP) {
+            const T dd = cn[0] * v[0] + cn[1] * v[1] + cn[2] * v[2];
+            v[0] -= dd * cn[0]; v[1] -= dd * cn[1]; v[2] -= dd * cn[2];
+        }
+        const int64_t i = lcol * nr + l;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) dst[d][i] = a.add ? dst[d][i] + v[d] : v[d];
+    }
+}
+
+constexpr int kGcolWaves = 4;
+template <typename T, int K>
+__global__ __launch_bounds__(64 * kGcolWaves) void gradient_column_kernel(const GradArgs<T> a)
+{
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int64_t ncol = (int64_t)a.nsd * a.nx * a.nx;
+    const int64_t col = (int64_t)xcd_block() * kGcolWaves + wv;
+    if (col >= ncol) return;
+    const int Y = (int)(col % a.nx);
+    const int X = (int)((col / a.nx) % a.nx);
+    const int sd = (int)(col / ((int64_t)a.nx * a.nx));
+    switch ((X & 1) * 2 + (Y & 1)) {
+    case 0: gradient_column<T, K, 0, 0>(a, lane, X, Y, sd); break;
+    case 1: gradient_column<T, K, 0, 1>(a, lane, X, Y, sd); break;
+    case 2: gradient_column<T, K, 1, 0>(a, lane, X, Y, sd); break;
+    default: gradient_column<T, K, 1, 1>(a, lane, X, Y, sd); break;
+    }
+}
+
 // Constant prolongation (fe/wedge/shell/grid_transfer_constant.hpp:9-90):
 // P1 interpolation per fine node from the coarse nodes of its position.
 template <typename T>
@@ -553,7 +935,7 @@ __global__ __launch_bounds__(256) void kmass_lumped_kernel(const T *k, const T *
 // sum_q w_q N_i(q) u_h(q) r(q)^2 (dr/2) |det L|.
 template <typename T>
 __global__ __launch_bounds__(256) void vector_mass_kernel(const Comp3<const T> src, const Comp3<T> dst, const T *radii,
-                                                          const T *latgeom, int nsd, int nx, int nr)
+                                                          const T *latgeom, int nsd, int nx, int nr, int ncomp)
 {
     const int64_t n = (int64_t)nsd * nx * nx * nr;
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -589,7 +971,7 @@ __global__ __launch_bounds__(256) void vector_mass_kernel(const Comp3<const T> s
                     for (int j = 0; j < 6; ++j) {
                         const int jx = xc + kWC[w][j % 3][0], jy = yc + kWC[w][j % 3][1];
                         const int64_t ji = (((int64_t)sd * nx + jx) * nx + jy) * nr + rc + j / 3;
-                        for (int c = 0; c < 3; ++c) uv[j][c] = src.p[c][ji];
+                        for (int c = 0; c < 3; ++c) uv[j][c] = c < ncomp ? src.p[c][ji] : T(0);
                     }
                     for (int q = 0; q < 6; ++q) {
                         const T xi = qx[q % 3], eta = qy[q % 3], zeta = qz[q / 3];
@@ -597,7 +979,7 @@ __global__ __launch_bounds__(256) void vector_mass_kernel(const Comp3<const T> s
                         const T Nr[2] = {T(0.5) * (T(1) - zeta), T(0.5) * (T(1) + zeta)};
                         const T r = r1 + T(0.5) * (r2 - r1) * (T(1) + zeta);
                         const T f = qw * Nl[kc] * Nr[top] * r * r * gr * detl;
-                        for (int c = 0; c < 3; ++c) {
+                        for (int c = 0; c < ncomp; ++c) {
                             T uh = T(0);
                             for (int j = 0; j < 6; ++j) uh += Nl[j % 3] * Nr[j / 3] * uv[j][c];
                             acc[c] += f * uh;
@@ -607,7 +989,7 @@ __global__ __launch_bounds__(256) void vector_mass_kernel(const Comp3<const T> s
             }
         }
     }
-    for (int c = 0; c < 3; ++c) dst.p[c][gid] = acc[c];
+    for (int c = 0; c < ncomp; ++c) dst.p[c][gid] = acc[c];
 }
 
 // Buoyancy of the app's RHS interpolator (apps/mantlecirculation/src/interpolators.hpp:99-133):
@@ -650,18 +1032,39 @@ template <typename T> void div_impl(tng_operator &op, const tng_vector &src, tng
     const int64_t ncell_c = (int64_t)c.nsd * (c.nx - 1) * (c.nx - 1) * (c.nr - 1);
     const int64_t ncell_f = (int64_t)f.nsd * (f.nx - 1) * (f.nx - 1) * (f.nr - 1);
     // coarse-wedge partials (12 per coarse cell), then the fine-cell values (2 per fine cell)
-    const size_t bytes = (size_t)(ncell_c * 12 + ncell_f * 2) * sizeof(T);
-    if (op.scratch.bytes < bytes) op.scratch.alloc(bytes);
     DivArgs<T> a{};
     const MeshP<T> m = mp<T>(f);
     a.u0 = (const T *)src.comp(0); a.u1 = (const T *)src.comp(1); a.u2 = (const T *)src.comp(2);
     a.coords = m.coords; a.radii = m.radii; a.latgeom = m.latgeom;
     a.sd_cmb = f.d_sd_cmb.as<uint8_t>(); a.sd_surf = f.d_sd_surf.as<uint8_t>();
-    a.partial = op.scratch.as<T>();
     a.nsd = f.nsd; a.nx = f.nx; a.nr = f.nr; a.nxc = c.nx; a.nrc = c.nr;
     a.bc_cmb = op.bc_cmb; a.bc_surf = op.bc_surf;
     a.cidx = Index4(c.nx - 1, c.nr - 1);
     a.fidx = Index4(f.nx - 1, f.nr - 1);
+    // one pass for 128 radial cells, and for 64 in FP32: workgroups of 3
+    // coarse node columns (4 cell-column waves, times 2 radial parts for 128
+    // cells) and a chunk of coarse rows.  Level 7 FP64: 0.60 ms against 0.63 ms
+    // for the three phases below; level 6 FP64: 0.118 against 0.090 ms, so the
+    // phases stay there.
+    const int ncr = f.nr - 1;
+    if ((ncr == 128 || (ncr == 64 && sizeof(T) == 4)) && f.nalloc < ((int64_t)1 << 31)) {
+        constexpr int CW = 4;
+        const int64_t nxw = (c.nx + CW - 2) / (CW - 1);
+        int cy = 8;
+        while (cy > 2 && f.nsd * nxw * ((c.nx + cy - 1) / cy) * CW < 8192) cy /= 2;
+        const unsigned g = (unsigned)(f.nsd * nxw * ((c.nx + cy - 1) / cy));
+        const int add = op.apply_mode == TNG_ADD ? 1 : 0;
+        if (ncr == 64)
+            hipLaunchKernelGGL((div_column_kernel<T, 1, CW>), dim3(g), dim3(64 * CW), 0, s, a, (T *)dst.comp(0), add, cy);
+        else
+            hipLaunchKernelGGL((div_column_kernel<T, 2, CW>), dim3(g), dim3(128 * CW), 0, s, a, (T *)dst.comp(0), add,
+                               cy);
+        TNG_HIP_CHECK(hipGetLastError());
+        return;
+    }
+    const size_t bytes = (size_t)(ncell_c * 12 + ncell_f * 2) * sizeof(T);
+    if (op.scratch.bytes < bytes) op.scratch.alloc(bytes);
+    a.partial = op.scratch.as<T>();
     T *cdiv = op.scratch.as<T>() + ncell_c * 12;
     if ((f.nr - 1) % 64 == 0)
         hipLaunchKernelGGL((div_cell_kernel<T, true>), dim3(blocks_for(ncell_f)), dim3(256), 0, s, a, cdiv);
@@ -688,6 +1091,20 @@ template <typename T> void grad_impl(tng_operator &op, const tng_vector &src, tn
     a.bc_cmb = op.bc_cmb; a.bc_surf = op.bc_surf; a.add = op.apply_mode == TNG_ADD ? 1 : 0;
     a.fcell = Index4(f.nx - 1, f.nr - 1);
     a.fnode = Index4(f.nx, f.nr);
+    // one wave per node column: 64, 128 or 256 radial cells
+    const int ncr = f.nr - 1;
+    if (ncr == 64 || ncr == 128 || ncr == 256) {
+        const int64_t ncol = (int64_t)f.nsd * f.nx * f.nx;
+        const unsigned g = (unsigned)((ncol + kGcolWaves - 1) / kGcolWaves);
+        if (ncr == 64)
+            hipLaunchKernelGGL((gradient_column_kernel<T, 1>), dim3(g), dim3(64 * kGcolWaves), 0, s, a);
+        else if (ncr == 128)
+            hipLaunchKernelGGL((gradient_column_kernel<T, 2>), dim3(g), dim3(64 * kGcolWaves), 0, s, a);
+        else
+            hipLaunchKernelGGL((gradient_column_kernel<T, 4>), dim3(g), dim3(64 * kGcolWaves), 0, s, a);
+        TNG_HIP_CHECK(hipGetLastError());
+        return;
+    }
     const int64_t nfw = (int64_t)f.nsd * (f.nx - 1) * (f.nx - 1) * (f.nr - 1) * 2;
     const size_t bytes = (size_t)nfw * sizeof(T);
     if (op.scratch.bytes < bytes) op.scratch.alloc(bytes);
@@ -760,12 +1177,14 @@ void stokes_block_apply(tng_operator &op, const tng_vector &src, tng_vector &dst
             hipLaunchKernelGGL(vector_mass_kernel<double>, dim3(blocks_for(m.nalloc)), dim3(256), 0, s,
                                Comp3<const double>{{(const double *)src.comp(0), (const double *)src.comp(1),
                                                     (const double *)src.comp(2)}},
-                               comps<double>(dst), m.d_radii.as<double>(), m.d_latgeom.as<double>(), m.nsd, m.nx, m.nr);
+                               comps<double>(dst), m.d_radii.as<double>(), m.d_latgeom.as<double>(), m.nsd, m.nx, m.nr,
+                               3);
         else
             hipLaunchKernelGGL(vector_mass_kernel<float>, dim3(blocks_for(m.nalloc)), dim3(256), 0, s,
                                Comp3<const float>{{(const float *)src.comp(0), (const float *)src.comp(1),
                                                    (const float *)src.comp(2)}},
-                               comps<float>(dst), m.d_radii_f.as<float>(), m.d_latgeom_f.as<float>(), m.nsd, m.nx, m.nr);
+                               comps<float>(dst), m.d_radii_f.as<float>(), m.d_latgeom_f.as<float>(), m.nsd, m.nx, m.nr,
+                               3);
         TNG_HIP_CHECK(hipGetLastError());
         if (op.comm_mode == TNG_COMMUNICATE_ADDITIVELY) launch_halo_sum(*op.mesh, dst, s);
         break;
@@ -773,6 +1192,30 @@ void stokes_block_apply(tng_operator &op, const tng_vector &src, tng_vector &dst
     default:
         throw Error(TNG_INVALID_ARGUMENT, "stokes_block_apply: wrong operator kind");
     }
+}
+
+// Consistent scalar mass (fe/wedge/operators/shell/mass.hpp), FP64, additive exchange.
+void launch_scalar_mass(const tng_vector &src, tng_vector &dst, hipStream_t s)
+{
+    const tng_domain &m = *src.mesh;
+    TNG_REQUIRE(src.ncomp == 1 && dst.ncomp == 1 && src.dtype == TNG_F64 && dst.dtype == TNG_F64 && dst.mesh == &m,
+                "scalar mass: FP64 scalars on one mesh");
+    const double *sp = (const double *)src.comp(0);
+    double *dp = (double *)dst.comp(0);
+    hipLaunchKernelGGL(vector_mass_kernel<double>, dim3(blocks_for(m.nalloc)), dim3(256), 0, s,
+                       Comp3<const double>{{sp, sp, sp}}, Comp3<double>{{dp, dp, dp}}, m.d_radii.as<double>(),
+                       m.d_latgeom.as<double>(), m.nsd, m.nx, m.nr, 1);
+    TNG_HIP_CHECK(hipGetLastError());
+    launch_halo_sum(m, dst, s);
+}
+
+// Node sums of per-cell wedge partials (cell, wedge, 6 wedge nodes), FP64 scalar.
+void launch_wedge_gather(const double *partial, tng_vector &dst, bool add, hipStream_t s)
+{
+    const tng_domain &m = *dst.mesh;
+    hipLaunchKernelGGL(wedge_gather_kernel<double>, dim3(blocks_for(m.nalloc)), dim3(256), 0, s, partial,
+                       (double *)dst.comp(0), add ? 1 : 0, m.nsd, m.nx, m.nr, Index4(m.nx, m.nr));
+    TNG_HIP_CHECK(hipGetLastError());
 }
 
 }  // namespace tng

@@ -305,6 +305,7 @@ void mesh_upload(tng_domain &m)
     up(m.d_coords, m.coords.data(), m.coords.size() * sizeof(double));
     up(m.d_radii, m.radii.data(), m.radii.size() * sizeof(double));
     up(m.d_omask, m.omask.data(), m.omask.size());
+    up(m.d_bmask, m.bmask.data(), m.bmask.size());
     up(m.d_sd_cmb, m.sd_has_cmb.data(), m.sd_has_cmb.size());
     up(m.d_sd_surf, m.sd_has_surf.data(), m.sd_has_surf.size());
     std::vector<float> cf(m.coords.begin(), m.coords.end()), rf(m.radii.begin(), m.radii.end());

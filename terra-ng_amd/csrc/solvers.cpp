@@ -22,6 +22,7 @@
 #include <memory>
 #include <vector>
 
+#include "tng_gs.hpp"
 #include "tng_internal.hpp"
 #include "tng_solvers.hpp"
 
@@ -108,11 +109,26 @@ void Chebyshev::solve(tng_operator &A, tng_vector &x, const tng_vector &b, bool 
         need_estimate = false;
     }
     if (!std::isfinite(lmax_estimate) || lmax_estimate <= 0.0) lmax_estimate = 1.0;
+    hipStream_t s = stream_of(x);
+    if (jacobi) {
+        // x += omega D^{-1} (b - A x), by default omega = 2 / (1.3 lambda_max)
+        const double omega = this->omega > 0.0 ? this->omega : 2.0 / (1.3 * lmax_estimate);
+        for (int iter = 0; iter < iterations; ++iter) {
+            if (x_zero && iter == 0)
+                launch_cheb_update(x, *d, b, nullptr, *inv_diag, omega, 0.0, false, true, s);
+            else if (xb && x.swappable && epsdivdiv_apply_cheb(A, x, *z, *xb, *d, b, *inv_diag, omega, 0.0, false, s))
+                swap_storage(x, *xb);
+            else {
+                apply(A, x, *z);
+                launch_cheb_update(x, *d, b, z, *inv_diag, omega, 0.0, false, false, s);
+            }
+        }
+        return;
+    }
     const double lambda_max = 1.5 * lmax_estimate, lambda_min = 0.1 * lmax_estimate;
     const double theta = 0.5 * (lambda_max + lambda_min), delta = 0.5 * (lambda_max - lambda_min);
     const double sigma = theta / delta;
     double rho = 1.0 / sigma;  // carried across iterations, as in the reference
-    hipStream_t s = stream_of(x);
     // d = az D^{-1}(b - A x) [+ ad d]; x += d.  With a spare vector the update
     // runs inside the apply and writes x + d to the spare, whose storage then
     // becomes x's (the apply still reads the old x).
@@ -232,6 +248,15 @@ StokesSolver::StokesSolver(tng_context *ctx, int min_level, int max_level, int r
     TNG_REQUIRE(prm.viscous_pc_dtype == TNG_F64 || prm.viscous_pc_dtype == TNG_F32, "invalid multigrid precision");
     TNG_REQUIRE(prm.gca >= 0 && prm.gca <= 2,
                 "gca must be 0 (re-discretised), 1 (Galerkin on all coarse levels) or 2 (adaptive Galerkin)");
+    TNG_REQUIRE(prm.viscous_pc_smoother == 0 || prm.viscous_pc_smoother == 1,
+                "viscous_pc_smoother must be 0 (Chebyshev) or 1 (damped Jacobi)");
+    TNG_REQUIRE(prm.krylov_orthogonalization == 0 || prm.krylov_orthogonalization == 1,
+                "krylov_orthogonalization must be 0 (modified Gram-Schmidt) or 1 (CGS2)");
+    TNG_REQUIRE(prm.rotation_correction == 0 ||
+                    (prm.rotation_correction == 1 && bc_cmb_ == TNG_FREESLIP && bc_surf_ == TNG_FREESLIP),
+                "rotation_correction needs free slip on both surfaces");
+    TNG_REQUIRE(prm.rotation_correction == 0 || prm.krylov_restart <= 64,
+                "rotation_correction supports krylov_restart <= 64");
     for (int l = min_level; l <= max_level; ++l) {
         meshes.emplace_back(new_mesh(ctx, l, l + radial_extra, r_min, r_max, lat_sdr, rad_sdr));
         meshes.back()->solver_owned = true;
@@ -266,7 +291,7 @@ StokesSolver::StokesSolver(tng_context *ctx, int min_level, int max_level, int r
         tng_domain *m = meshes[l].get();
         if (l == nl - 1 && mgt == TNG_F64) {
             L.A = A.get();
-        } else if (l < nl - 1 && prm.gca >= 1 && mgt == TNG_F64) {
+        } else if (galerkin_level(l) && mgt == TNG_F64) {
             L.A = gca64[l].get();
         } else {
             mg_ops.emplace_back(new_epsdivdiv(m, eta_mg[l], bc_cmb, bc_surf, false));
@@ -289,9 +314,8 @@ StokesSolver::StokesSolver(tng_context *ctx, int min_level, int max_level, int r
     mg.pre.resize(nl);
     mg.post.resize(nl);
     // spare solution vectors for the fused Chebyshev sub-steps (one-column
-    // apply kernels: 64 or 128 radial cells); TNG_CHEB_FUSE=0 turns them off
-    const char *fuse_env = std::getenv("TNG_CHEB_FUSE");
-    const bool fuse = !fuse_env || std::atoi(fuse_env) != 0;
+    // apply kernels: 64 or 128 radial cells); the cheb_fuse setting 0 turns them off
+    const bool fuse = vel->ctx->tuning.cheb_fuse != 0;
     for (int l = 0; l < nl; ++l) {
         const int ncells = meshes[l]->nr - 1;
         if (fuse && (ncells == 64 || ncells == 128) && meshes[l]->rad_sdr == 0)
@@ -299,6 +323,7 @@ StokesSolver::StokesSolver(tng_context *ctx, int min_level, int max_level, int r
     }
     for (int l = 0; l < nl; ++l)
         for (Chebyshev *s : {&mg.pre[l], &mg.post[l]}) {
+            s->jacobi = prm.viscous_pc_smoother == 1;
             s->order = prm.viscous_pc_chebyshev_order;
             s->iterations = prm.viscous_pc_num_smoothing_steps_prepost;
             s->power_iterations = prm.viscous_pc_num_power_iterations;
@@ -350,13 +375,21 @@ void StokesSolver::set_viscosity(const tng_vector &eta_fine)
     TNG_REQUIRE(eta_fine.mesh == vel && eta_fine.ncomp == 1, "viscosity must be a scalar field on the velocity mesh");
     const int nl = (int)meshes.size();
     assign(*eta[nl - 1], eta_fine);
-    // coarse levels: injection of the fine viscosity (nested nodes)
+    refresh_viscosity_hierarchy();
+}
+
+// coarse levels: injection of the velocity-level viscosity (nested nodes), then
+// every derived quantity rebuilt
+void StokesSolver::refresh_viscosity_hierarchy()
+{
+    const int nl = (int)meshes.size();
     for (int l = nl - 2; l >= 0; --l) launch_inject(*eta[l], *eta[l + 1], stream_of(*eta[l]));
     viscosity_updated();
 }
 
 void StokesSolver::viscosity_updated()
 {
+    rc_dirty = true;
     const int nl = (int)meshes.size();
     for (int l = 0; l < nl; ++l)
         if (eta_mg[l] != eta[l]) assign(*eta_mg[l], *eta[l]);
@@ -372,16 +405,24 @@ void StokesSolver::viscosity_updated()
         // matrices of all their descendants, the rest the re-discretised ones.
         hipStream_t s = vel->ctx->stream;
         if (params.gca == 2) gca_collect(*eta[nl - 1], nl - 1, *meshes[0], gca_flags, s);
+        const bool exact = params.gca_exact != 0;
         for (int l = nl - 2; l >= 0; --l) {
             const tng_operator &fine = l == nl - 2 ? *A : *gca64[l + 1];
             if (params.gca == 2)
-                gca_assemble_selective(fine, *gca64[l], l, *meshes[0], gca_flags, s);
+                gca_assemble_selective(fine, *gca64[l], l, *meshes[0], gca_flags, s, exact);
             else
-                gca_assemble(fine, *gca64[l], s);
+                gca_assemble(fine, *gca64[l], s, exact);
+            // coarsest-only: the finer level's matrices were an intermediate
+            if (params.gca_coarsest_only && l + 1 < nl - 1) {
+                TNG_HIP_CHECK(hipStreamSynchronize(s));
+                gca64[l + 1]->d_lmat.release();
+                gca64[l + 1]->stored = false;
+            }
         }
         if (params.viscous_pc_dtype != TNG_F64)
             for (int l = 0; l < nl - 1; ++l)
-                copy_stored_matrices(*mg.levels[l].A, *gca64[l], (tng_dtype)params.viscous_pc_dtype, s);
+                if (galerkin_level(l))
+                    copy_stored_matrices(*mg.levels[l].A, *gca64[l], (tng_dtype)params.viscous_pc_dtype, s);
     }
     // inverse diagonals (stokes_solver.hpp:390-417)
     for (int l = 0; l < nl; ++l) {
@@ -403,6 +444,19 @@ void StokesSolver::viscosity_updated()
     apply(*pmass, *ones_p, *inv_lumped_pmass);
     launch_invert(*inv_lumped_pmass, stream_of(*inv_lumped_pmass));
 }
+
+void StokesSolver::update_viscosity_velocity_level(const tng_vector &temp, double rmu)
+{
+    const int nl = (int)meshes.size();
+    hipStream_t s = vel->ctx->stream;
+    launch_viscosity_fk(*eta[nl - 1], temp, rmu, s);
+    A->btab_valid = false;
+    if (eta_mg[nl - 1] != eta[nl - 1]) assign(*eta_mg[nl - 1], *eta[nl - 1]);
+    mg.levels[nl - 1].A->btab_valid = false;
+    rc_dirty = true;
+}
+
+tng_vector *StokesSolver::make_vec(tng_domain *m, int ncomp) { return vec(pool, m, TNG_F64, ncomp); }
 
 // K = [A B^T; B 0]: A (Replace, Skip) -> B^T (Add, Communicate) -> B (Replace, Communicate)
 void StokesSolver::apply_stokes(const SVec &src, SVec &dst)
@@ -476,6 +530,112 @@ void s_assign(StokesSolver::SVec &y, const StokesSolver::SVec &x)
 bool s_nonfinite(const StokesSolver::SVec &x) { return has_nan_or_inf(*x.u) || has_nan_or_inf(*x.p); }
 }  // namespace
 
+// Rotation correction (free slip on both surfaces).  The three rigid
+// rotations m_i = e_i x X are the near-null modes of the penalised viscous
+// block (A m_i is of the order of the penalty eps, 1e-7), which the V-cycle
+// does not resolve; left to FGMRES they stall the restarted iteration.  After
+// the block preconditioner z = M v, with w = K z already formed:
+//   c = E^{-1} Z^T (v - w),  z += Z c,  w += (K Z) c,   E = Z^T K Z (3 x 3),
+// Z = [m_i / |m_i|; 0].  The operator and the stopping test are unchanged.
+void StokesSolver::rotation_setup()
+{
+    hipStream_t s = vel->ctx->stream;
+    const int m = params.krylov_restart;
+    if (rc_z.empty()) {
+        for (int i = 0; i < 3; ++i) {
+            rc_z.push_back({vec(pool, vel, TNG_F64, 3), vec(pool, pre_mesh, TNG_F64, 1)});
+            rc_kz.push_back({vec(pool, vel, TNG_F64, 3), vec(pool, pre_mesh, TNG_F64, 1)});
+        }
+        // device pointer tables: per basis slot j: [Z0, Z1, Z2, V_j]; [KZ0..2]; per j: [Z_j]; [w]
+        const int offV = 1, offZ = offV + m + 1, idxW = offZ + m;
+        auto put = [](std::vector<double *> &t, const SVec &v) {
+            for (int c = 0; c < 3; ++c) t.push_back((double *)v.u->comp(c));
+            t.push_back((double *)v.p->comp(0));
+        };
+        std::vector<double *> tab;
+        for (int j = 0; j < m; ++j) {
+            for (int i = 0; i < 3; ++i) put(tab, rc_z[i]);
+            put(tab, fg[offV + j]);
+        }
+        for (int i = 0; i < 3; ++i) put(tab, rc_kz[i]);
+        for (int j = 0; j < m; ++j) put(tab, fg[offZ + j]);
+        put(tab, fg[idxW]);
+        rc_ptrs.alloc(tab.size() * sizeof(double *));
+        TNG_HIP_CHECK(hipMemcpyAsync(rc_ptrs.p, tab.data(), tab.size() * sizeof(double *), hipMemcpyHostToDevice, s));
+        TNG_HIP_CHECK(hipStreamSynchronize(s));
+        const size_t np = gs_partials_doubles();
+        if (gs_part.bytes < np * sizeof(double)) gs_part.alloc(np * sizeof(double));
+        if (gs_out.bytes < 128 * sizeof(double)) gs_out.alloc(128 * sizeof(double));
+    }
+    for (int i = 0; i < 3; ++i) {
+        launch_rotation_field(*rc_z[i].u, i, s);
+        assign(*rc_z[i].p, 0.0);
+        const double nrm = std::sqrt(dot(*rc_z[i].u, *rc_z[i].u));
+        lincomb(*rc_z[i].u, {1.0 / nrm}, {rc_z[i].u});
+        apply_stokes(rc_z[i], rc_kz[i]);
+    }
+    double E[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) E[3 * i + j] = s_dot(rc_z[i], rc_kz[j]);
+    // LU with partial pivoting
+    for (int i = 0; i < 9; ++i) rc_lu[i] = E[i];
+    for (int k = 0; k < 3; ++k) {
+        int p = k;
+        for (int i = k + 1; i < 3; ++i)
+            if (std::abs(rc_lu[3 * i + k]) > std::abs(rc_lu[3 * p + k])) p = i;
+        rc_piv[k] = p;
+        if (p != k)
+            for (int c = 0; c < 3; ++c) std::swap(rc_lu[3 * k + c], rc_lu[3 * p + c]);
+        TNG_REQUIRE(rc_lu[3 * k + k] != 0.0 && std::isfinite(rc_lu[3 * k + k]),
+                    "rotation correction: Z^T K Z is singular");
+        for (int i = k + 1; i < 3; ++i) {
+            rc_lu[3 * i + k] /= rc_lu[3 * k + k];
+            for (int c = k + 1; c < 3; ++c) rc_lu[3 * i + c] -= rc_lu[3 * i + k] * rc_lu[3 * k + c];
+        }
+    }
+    rc_dirty = false;
+}
+
+void StokesSolver::rotation_correct(int j, SVec &v, SVec &z, SVec &w)
+{
+    (void)v;
+    (void)z;
+    hipStream_t s = stream_of(*w.u);
+    const int m = params.krylov_restart;
+    double *const *tab = rc_ptrs.as<double *const>();
+    double *const *t_dots = tab + 16 * j;           // [Z0, Z1, Z2, V_j]
+    double *const *t_kz = tab + 16 * m;              // [KZ0, KZ1, KZ2]
+    double *const *t_zj = tab + 16 * m + 12 + 4 * j; // [Z_j]
+    GsArgs ga{};
+    ga.V = t_dots;
+    for (int c = 0; c < 3; ++c) ga.w[c] = (const double *)w.u->comp(c);
+    ga.w[3] = (const double *)w.p->comp(0);
+    ga.mask_v = vel->d_omask.as<uint8_t>();
+    ga.mask_p = pre_mesh->d_omask.as<uint8_t>();
+    ga.nv = vel->nalloc;
+    ga.np = pre_mesh->nalloc;
+    ga.j = 3;
+    gs_dots(ga, gs_part.as<double>(), gs_out.as<double>(), s);
+    const int B = gs_bucket(3);
+    double d[2 * 8 + 3];
+    TNG_HIP_CHECK(hipMemcpyAsync(d, gs_out.p, sizeof(double) * (2 * B + 3), hipMemcpyDeviceToHost, s));
+    TNG_HIP_CHECK(hipStreamSynchronize(s));
+    // t = Z^T v - Z^T w, c = E^{-1} t
+    double c[3];
+    for (int i = 0; i < 3; ++i) c[i] = d[i] - d[B + i];
+    for (int k = 0; k < 3; ++k) std::swap(c[k], c[rc_piv[k]]);
+    for (int i = 1; i < 3; ++i)
+        for (int k = 0; k < i; ++k) c[i] -= rc_lu[3 * i + k] * c[k];
+    for (int i = 2; i >= 0; --i) {
+        for (int k = i + 1; k < 3; ++k) c[i] -= rc_lu[3 * i + k] * c[k];
+        c[i] /= rc_lu[3 * i + i];
+    }
+    GsAxpy cf{};
+    for (int i = 0; i < 3; ++i) cf.c[i] = c[i];
+    gs_axpy(t_dots, 3, cf, t_zj, ga.nv, ga.np, s);
+    gs_axpy(t_kz, 3, cf, tab + 16 * m + 12 + 4 * m, ga.nv, ga.np, s);
+}
+
 // FGMRES(m) (fgmres.hpp:109-373)
 void StokesSolver::fgmres(SVec &x, const SVec &b, tng_solve_report &rep)
 {
@@ -520,6 +680,7 @@ void StokesSolver::fgmres(SVec &x, const SVec &b, tng_solve_report &rep)
             }
             SVec &w = fg[idxW];
             apply_stokes(zj, w);
+            if (params.rotation_correction) rotation_correct(j, vj, zj, w);
             // modified Gram-Schmidt: h_ij = w . v_i, then w -= h_ij v_i, one
             // fused pass per i with the h values kept on the device; one copy
             // to the host per Arnoldi step
@@ -604,6 +765,225 @@ void StokesSolver::fgmres(SVec &x, const SVec &b, tng_solve_report &rep)
     }
 }
 
+// FGMRES(m) with the two-pass Gram-Schmidt of kernels_gs.hip (classical
+// Gram-Schmidt, each basis vector re-orthogonalised one step later).  Step j:
+// z_j = M u_j (u_j the tentative basis vector), w = K z_j, one multi-dot
+// gives s = Q^T u_j, u_j.u_j, c = Q^T w, u_j.w, w.w; then
+//   alpha = |u_j - Q s|,  q_j = (u_j - Q s) / alpha,
+//   rho = q_j.w = (u_j.w - s.c) / alpha,
+//   nu = |w - Q c - rho q_j| = sqrt(w.w - c.c - rho^2),
+//   u_{j+1} = (w - Q c - rho q_j) / nu   (second pass).
+// Arnoldi: K z_j = Q c + rho q_j + nu u_{j+1}; when u_{j+1} = Q' s' + alpha'
+// q_{j+1} is known one step later, column j of H becomes (c + nu s', rho +
+// nu s'_j, nu alpha').  The residual estimate of step j uses column j before
+// that correction (which is of the order of the loss of orthogonality); the
+// Givens factorisation is redone from the corrected H every step.
+void StokesSolver::fgmres_cgs2(SVec &x, const SVec &b, tng_solve_report &rep)
+{
+    const int m = params.krylov_restart;
+    const double rtol = params.krylov_relative_tolerance, atol = params.krylov_absolute_tolerance;
+    TNG_REQUIRE(m <= 32, "CGS2 orthogonalisation supports krylov_restart <= 32");
+    hipStream_t s = stream_of(*x.u);
+    SVec &r = fg[0];
+    apply_stokes(x, r);
+    s_lincomb(r, {1.0, -1.0}, {&b, &r});
+    double beta0 = std::sqrt(s_dot(r, r));
+    const double initial = beta0;
+    rep.initial_residual = initial;
+    rep.final_residual = beta0;
+    rep.relative_residual = 1.0;
+    record(rep, beta0);
+    if (beta0 <= atol) {
+        rep.converged = 1;
+        return;
+    }
+    const int offV = 1, offZ = offV + m + 1, idxW = offZ + m, idxAcc = idxW + 1;
+    // device table of the basis slots' component pointers
+    {
+        std::vector<double *> tab(4 * (size_t)(m + 1));
+        for (int i = 0; i <= m; ++i) {
+            const SVec &v = fg[offV + i];
+            for (int c = 0; c < 3; ++c) tab[4 * i + c] = (double *)v.u->comp(c);
+            tab[4 * i + 3] = (double *)v.p->comp(0);
+        }
+        if (gs_table.bytes < tab.size() * sizeof(double *)) gs_table.alloc(tab.size() * sizeof(double *));
+        TNG_HIP_CHECK(hipMemcpyAsync(gs_table.p, tab.data(), tab.size() * sizeof(double *), hipMemcpyHostToDevice, s));
+        const size_t np = gs_partials_doubles();
+        if (gs_part.bytes < np * sizeof(double)) gs_part.alloc(np * sizeof(double));
+        if (gs_out.bytes < 128 * sizeof(double)) gs_out.alloc(128 * sizeof(double));
+    }
+    GsArgs ga{};
+    ga.V = gs_table.as<double *const>();
+    {
+        const SVec &w = fg[idxW];
+        for (int c = 0; c < 3; ++c) ga.w[c] = (const double *)w.u->comp(c);
+        ga.w[3] = (const double *)w.p->comp(0);
+    }
+    ga.mask_v = vel->d_omask.as<uint8_t>();
+    ga.mask_p = pre_mesh->d_omask.as<uint8_t>();
+    ga.nv = vel->nalloc;
+    ga.np = pre_mesh->nalloc;
+    std::vector<double> H((size_t)(m + 1) * m), R, g(m + 1), cs(m + 1), sn(m + 1), y, dots(128);
+    auto h = [&](int i, int j) -> double & { return H[(size_t)i * m + j]; };
+    // Givens QR of columns 0..ncol-1 of H into R, g from g0 e_1; returns false
+    // on a vanishing diagonal (stagnation)
+    auto factor = [&](int ncol, double g0, int *bad) -> bool {
+        R = H;
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = g0;
+        auto rr = [&](int i, int j) -> double & { return R[(size_t)i * m + j]; };
+        for (int j = 0; j < ncol; ++j) {
+            for (int i = 0; i < j; ++i) {
+                const double t = cs[i] * rr(i, j) + sn[i] * rr(i + 1, j);
+                rr(i + 1, j) = -sn[i] * rr(i, j) + cs[i] * rr(i + 1, j);
+                rr(i, j) = t;
+            }
+            const double a = rr(j, j), bb = rr(j + 1, j);
+            const double rab = std::hypot(a, bb);
+            cs[j] = rab == 0.0 ? 1.0 : a / rab;
+            sn[j] = rab == 0.0 ? 0.0 : bb / rab;
+            rr(j, j) = rab;
+            rr(j + 1, j) = 0;
+            if (std::abs(rr(j, j)) < std::numeric_limits<double>::epsilon()) {
+                *bad = j;
+                return false;
+            }
+            const double gj = g[j], gj1 = g[j + 1];
+            g[j] = cs[j] * gj + sn[j] * gj1;
+            g[j + 1] = -sn[j] * gj + cs[j] * gj1;
+        }
+        return true;
+    };
+    int total = 0;
+    while (total < params.krylov_max_iterations) {
+        SVec &V0 = fg[offV];
+        s_lincomb(V0, {1.0 / beta0}, {&r});
+        std::fill(H.begin(), H.end(), 0.0);
+        double g0 = beta0;
+        double nu_prev = 0.0;
+        int inner = 0;
+        for (int j = 0; j < m && total < params.krylov_max_iterations; ++j, ++total) {
+            SVec &uj = fg[offV + j], &zj = fg[offZ + j];
+            rep.iterations += 1;
+            s_assign(zj, 0.0);
+            precondition(zj, uj, true);
+            if (s_nonfinite(zj)) {
+                rep.nonfinite_preconditioner += 1;
+                s_assign(zj, uj);
+            }
+            apply_stokes(zj, fg[idxW]);
+            if (params.rotation_correction) rotation_correct(j, uj, zj, fg[idxW]);
+            ga.j = j;
+            gs_dots(ga, gs_part.as<double>(), gs_out.as<double>(), s);
+            const int B = gs_bucket(j);
+            TNG_HIP_CHECK(hipMemcpyAsync(dots.data(), gs_out.p, sizeof(double) * (2 * B + 3), hipMemcpyDeviceToHost, s));
+            TNG_HIP_CHECK(hipStreamSynchronize(s));
+            const double *sv = dots.data(), *cv = dots.data() + B;
+            const double uu = dots[2 * B], uw = dots[2 * B + 1], ww = dots[2 * B + 2];
+            double ss = 0.0, sc = 0.0, cc = 0.0;
+            for (int i = 0; i < j; ++i) {
+                ss += sv[i] * sv[i];
+                sc += sv[i] * cv[i];
+                cc += cv[i] * cv[i];
+            }
+            const double alpha = std::sqrt(std::max(uu - ss, 0.0));
+            if (!(alpha > 0.0) || !std::isfinite(alpha)) {
+                inner = j;
+                rep.breakdowns += 1;
+                break;
+            }
+            // delayed correction of column j - 1 (u_j = Q s + alpha q_j)
+            if (j == 0) {
+                g0 = beta0 * alpha;  // r = beta0 u_0 = beta0 alpha q_0
+            } else {
+                for (int i = 0; i < j; ++i) h(i, j - 1) += nu_prev * sv[i];
+                h(j, j - 1) = nu_prev * alpha;
+            }
+            const double rho = (uw - sc) / alpha;
+            const double nu2 = ww - cc - rho * rho;
+            GsCoef cf{};
+            for (int i = 0; i < j; ++i) {
+                cf.s[i] = sv[i];
+                cf.c[i] = cv[i] - (rho / alpha) * sv[i];
+            }
+            cf.inv_alpha = 1.0 / alpha;
+            cf.r = rho / alpha;
+            double nu;
+            if (nu2 > 1e-8 * ww) {
+                nu = std::sqrt(nu2);
+                cf.inv_nu = 1.0 / nu;
+                gs_update(ga, cf, s);
+            } else {
+                // cancellation in w.w - c.c - rho^2: form the projection, then its norm
+                cf.inv_nu = 1.0;
+                gs_update(ga, cf, s);
+                SVec &un = fg[offV + j + 1];
+                nu = std::sqrt(std::max(s_dot(un, un), 0.0));
+                if (nu > 0.0) s_lincomb(un, {1.0 / nu}, {&un});
+            }
+            for (int i = 0; i < j; ++i) h(i, j) = cv[i];
+            h(j, j) = rho;
+            h(j + 1, j) = nu;
+            nu_prev = nu;
+            if (nu < std::numeric_limits<double>::epsilon() * initial) {
+                h(j + 1, j) = 0;
+                inner = j + 1;
+                rep.breakdowns += 1;
+                int bad = -1;
+                if (!factor(inner, g0, &bad)) inner = bad;
+                break;
+            }
+            int bad = -1;
+            if (!factor(j + 1, g0, &bad)) {
+                inner = bad;
+                rep.breakdowns += 1;
+                break;
+            }
+            const double abs_res = std::abs(g[j + 1]);
+            const double rel = abs_res / initial;
+            record(rep, abs_res);
+            if (!std::isfinite(abs_res)) {
+                inner = j;
+                ++total;
+                rep.nonfinite_residual += 1;
+                break;
+            }
+            inner = j + 1;
+            if (rel <= rtol || abs_res < atol) break;
+        }
+        // back substitution on the factor of the last (uncorrected) column set
+        if (inner > 0) {
+            int bad = -1;
+            if (!factor(inner, g0, &bad)) inner = bad;
+        }
+        auto rr = [&](int i, int j) -> double { return R[(size_t)i * m + j]; };
+        y.assign(inner, 0.0);
+        for (int row = inner - 1; row >= 0; --row) {
+            double sum = g[row];
+            for (int col = row + 1; col < inner; ++col) sum -= rr(row, col) * y[col];
+            y[row] = sum / rr(row, row);
+        }
+        SVec &acc = fg[idxAcc];
+        s_assign(acc, 0.0);
+        for (int i = 0; i < inner; ++i) s_lincomb(acc, {1.0, y[i]}, {&acc, &fg[offZ + i]});
+        s_lincomb(x, {1.0, 1.0}, {&x, &acc});
+        rep.restarts += 1;
+        apply_stokes(x, r);
+        s_lincomb(r, {1.0, -1.0}, {&b, &r});
+        beta0 = std::sqrt(s_dot(r, r));
+        rep.final_residual = beta0;
+        rep.relative_residual = beta0 / initial;
+        if (!std::isfinite(beta0)) {
+            rep.nonfinite_residual += 1;
+            return;
+        }
+        if (beta0 <= atol || beta0 / initial <= rtol) {
+            rep.converged = 1;
+            return;
+        }
+    }
+}
+
 void StokesSolver::record(tng_solve_report &rep, double abs_res)
 {
     if (rep.history && rep.history_len < rep.history_capacity) rep.history[rep.history_len] = abs_res;
@@ -622,7 +1002,9 @@ void StokesSolver::solve(SVec &x, const SVec &b, tng_solve_report &rep)
     rep.history = hist;
     rep.history_capacity = cap;
     mg.coarse_iterations = 0;
-    fgmres(x, b, rep);
+    if (params.rotation_correction && rc_dirty) rotation_setup();
+    if (params.krylov_orthogonalization == 1) fgmres_cgs2(x, b, rep);
+    else fgmres(x, b, rep);
     rep.coarse_iterations = mg.coarse_iterations;
     // subtract the pressure mean over the unique pressure nodes (stokes_solver.hpp:680-689)
     const double avg = dot(*x.p, *ones_p) / (double)num_dofs_pressure;
@@ -768,6 +1150,27 @@ tng_status tng_chebyshev_smooth(tng_op op, tng_vec inv_diag, int order, int iter
     });
 }
 
+tng_status tng_jacobi_smooth(tng_op op, tng_vec inv_diag, int iterations, double omega, tng_vec x, tng_vec b)
+{
+    if (!op || !inv_diag || !x || !b || iterations < 0 || !(omega > 0.0)) return TNG_INVALID_ARGUMENT;
+    return sguard(op->mesh->ctx, [&] {
+        check_square(*op, *x, *b);
+        TNG_REQUIRE(inv_diag->mesh == x->mesh && inv_diag->ncomp == x->ncomp && inv_diag->dtype == x->dtype,
+                    "inverse diagonal must match the vectors");
+        Scratch s;
+        tng::Chebyshev c;
+        c.jacobi = true;
+        c.omega = omega;
+        c.iterations = iterations;
+        c.inv_diag = inv_diag;
+        c.d = s.like(*x);
+        c.z = s.like(*x);
+        c.need_estimate = false;
+        c.lmax_estimate = 1.0;
+        c.solve(*op, *x, *b);
+    });
+}
+
 void tng_stokes_params_default(tng_stokes_params *p)
 {
     if (!p) return;
@@ -785,6 +1188,11 @@ void tng_stokes_params_default(tng_stokes_params *p)
     p->coarse_relative_tolerance = 1e-6;
     p->coarse_absolute_tolerance = 1e-16;
     p->seed = 12345;
+    p->gca_coarsest_only = 0;
+    p->gca_exact = 0;
+    p->viscous_pc_smoother = 0;
+    p->krylov_orthogonalization = 0;
+    p->rotation_correction = 0;
 }
 
 tng_status tng_stokes_solver_create(tng_ctx ctx, int min_level, int max_level, int radial_extra_levels, double r_min,

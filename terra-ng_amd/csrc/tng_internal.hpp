@@ -34,7 +34,7 @@ struct Error : std::runtime_error {
 
 // Shell boundary flags (grid/shell/bit_masks.hpp:11-19) and ownership flag
 // (grid/bit_masks.hpp:16-20), same bit values as the reference.
-enum : uint8_t { SBF_INNER = 1, SBF_CMB = 2 | 4, SBF_SURFACE = 2 | 8 };
+enum : uint8_t { SBF_INNER = 1, SBF_BOUNDARY = 2, SBF_CMB = 2 | 4, SBF_SURFACE = 2 | 8 };
 enum : uint8_t { OWNED = 1 };
 
 // Division by a launch-constant divisor through a multiply-high (round-up
@@ -107,8 +107,20 @@ struct DevBuf {
 
 }  // namespace tng
 
+// Launch-shape settings of the apply kernels (tng_ctx_set_tuning); every
+// setting computes the same operator.
+struct tng_tuning {
+    int apply_blocks = 0;        // FP64 applies: workgroups sharing the node rows (0: resident count)
+    int apply_seg = 0;           // FP32 and fused applies: node rows per workgroup (0: chosen per mesh)
+    int apply_radial_chunks = 0; // 1: 128 and 256 layers as two or four cells per lane
+    int apply_radial_split = 1;  // 0: 128 and 256 layers as radial chunks in one workgroup
+    int apply_waves = 8;         // 4: four-wave workgroups at 64 layers
+    int cheb_fuse = 1;           // 0: Chebyshev sub-steps as separate kernels (read at solver creation)
+};
+
 struct tng_context {
     int device = 0;
+    tng_tuning tuning;
     hipStream_t stream = nullptr;
     bool owns_stream = true;  // false: the caller's stream (tng_ctx_create_with_stream)
     std::string last_error;
@@ -145,7 +157,7 @@ struct tng_domain {
     std::vector<uint8_t> sd_has_cmb, sd_has_surf;
 
     // device copies
-    tng::DevBuf d_coords, d_radii, d_omask, d_sd_cmb, d_sd_surf;
+    tng::DevBuf d_coords, d_radii, d_omask, d_bmask, d_sd_cmb, d_sd_surf;
     tng::DevBuf d_coords_f, d_radii_f;
     // lateral wedge geometry: (sd, xc, yc, wedge, 10) = (M0, M1, M2, |det L|),
     // M = columns of L^{-T}, L = [p1-p0, p2-p0, (p0+p1+p2)/3]
@@ -230,7 +242,10 @@ enum OpKind {
     OP_PROLONGATION,
     OP_RESTRICTION,
     OP_KMASS_LUMPED,
-    OP_VECTOR_MASS
+    OP_VECTOR_MASS,
+    OP_SUPG,    // UnsteadyAdvectionDiffusionSUPG (scalar temperature operator)
+    OP_STOKES,  // EpsDivDivStokes: owns its blocks A, B^T, B and the zero block
+    OP_ZERO     // Zero (pressure -> pressure): Replace writes 0, Add leaves dst
 };
 
 struct tng_operator {
@@ -249,10 +264,19 @@ struct tng_operator {
     bool btab_valid = false;
     tng::DevBuf d_penalty_b;  // 3 doubles: eps * G^{-1} m, written on device
     tng::DevBuf d_iface;      // partial sums between radial parts (apply sweeps of 256 layers)
+    // OP_SUPG (UnsteadyAdvectionDiffusionSUPG): velocity (captured by reference), diffusivity,
+    // dt, mass scaling and flags
+    tng_vector *vel = nullptr;
+    double supg_kappa = 0.0, supg_dt = 0.0, supg_mass_scaling = 1.0;
+    bool supg_treat_boundary = true, supg_lumped = false, supg_enabled = true;
     // stored local matrices (OperatorStoredMatrixMode::Full): (sd, xc, yc, rc, wedge, 18, 18)
     bool stored = false;
     tng_dtype lmat_dtype = TNG_F64;
     tng::DevBuf d_lmat;
+    // OP_STOKES: blocks 11, 12, 21, 22 (owned; handed out as borrowed handles)
+    tng_operator *blocks[4] = {nullptr, nullptr, nullptr, nullptr};
+    tng_operator *parent = nullptr;  // a block of a Stokes operator (not destroyable on its own)
+    ~tng_operator();
 };
 
 namespace tng {
@@ -307,6 +331,8 @@ double *device_slots(tng_context &ctx, int first);  // ctx.red_result + first (d
 
 // generic operator apply (dispatch on kind)
 void op_apply(tng_operator &op, const tng_vector &src, tng_vector &dst);
+// EpsDivDivStokes apply: [u; p] -> [A u + B^T p; B u]
+void stokes_op_apply(tng_operator &op, const tng_vector &u, const tng_vector &p, tng_vector &out_u, tng_vector &out_p);
 
 // object factories (throw tng::Error; the caller owns the result)
 tng_domain *new_mesh(tng_context *ctx, int lat_level, int rad_level, double r_min, double r_max, int lat_sdr,
@@ -319,15 +345,26 @@ tng_operator *new_kmass(tng_domain *mesh, tng_vector *k);
 // stored local matrices and GCA (kernels_gca.hip)
 size_t stored_matrix_count(const tng_domain &m);  // wedges = matrices
 void epsdivdiv_local_matrices(const tng_operator &op, DevBuf &out, hipStream_t s);  // FP64, from k
-void gca_assemble(const tng_operator &fine, tng_operator &coarse, hipStream_t s);
+// exact = true: the Galerkin product R A P (P interpolates top-face nodes of
+// interior coarse cells from the coarse top layer); false: the reference's rule
+void gca_assemble(const tng_operator &fine, tng_operator &coarse, hipStream_t s, bool exact = false);
 // adaptive GCA: flags of the coarsest cells from the fine coefficient, and the
 // selective assembly (Galerkin on flagged cells, on-the-fly matrices elsewhere)
 void gca_collect(const tng_vector &k_fine, int level_range, const tng_domain &coarsest, DevBuf &flags, hipStream_t s);
 void gca_assemble_selective(const tng_operator &fine, tng_operator &coarse, int level_range,
-                            const tng_domain &coarsest, const DevBuf &flags, hipStream_t s);
+                            const tng_domain &coarsest, const DevBuf &flags, hipStream_t s, bool exact = false);
 void copy_stored_matrices(tng_operator &dst, const tng_operator &src, tng_dtype dtype, hipStream_t s);
 template <typename T>
 void stored_apply(tng_operator &op, const tng_vector &src, tng_vector &dst, const double *pen, bool add, hipStream_t s);
+
+// energy equation (kernels_energy.hip, kernels_stokes.hip)
+void supg_apply(tng_operator &op, const tng_vector &src, tng_vector &dst, hipStream_t s);
+void launch_scalar_mass(const tng_vector &src, tng_vector &dst, hipStream_t s);
+void launch_wedge_gather(const double *partial, tng_vector &dst, bool add, hipStream_t s);
+void launch_boundary_values(tng_vector &g, double t_cmb, double t_surf, hipStream_t s);
+void launch_assign_masked(tng_vector &b, const tng_vector &src, uint8_t flag, hipStream_t s);
+void launch_viscosity_fk(tng_vector &eta, const tng_vector &temp, double rmu, hipStream_t s);
+double max_vector_magnitude(const tng_vector &u, hipStream_t s);
 
 // viscosity adjoint kernel (kernels_adjoint.hip): g = d(lam^T A(k) u)/d log k
 void epsdivdiv_log_k_gradient(tng_operator &op, const tng_vector &lam, const tng_vector &u, tng_vector &g,

@@ -15,8 +15,12 @@ int pcg_solve(tng_operator &A, tng_vector &x, const tng_vector &b, tng_vector &r
               tng_vector &z, int max_iterations, double rtol, double atol, double *final_rel,
               bool x_zero = false);
 
-// Chebyshev-accelerated Jacobi smoother on D^{-1} A.
+// Chebyshev-accelerated Jacobi smoother on D^{-1} A, or (jacobi = true) the
+// damped Jacobi smoother x += omega D^{-1} (b - A x) with
+// omega = 2 / (1.3 lambda_max) (jacobi.hpp:16-60).
 struct Chebyshev {
+    bool jacobi = false;
+    double omega = 0.0;  // Jacobi relaxation; 0: 2 / (1.3 lambda_max) from the estimate
     int order = 2, iterations = 1, power_iterations = 10;
     tng_vector *inv_diag = nullptr;
     tng_vector *d = nullptr, *z = nullptr;  // work vectors
@@ -73,11 +77,32 @@ struct StokesSolver {
     StokesSolver(tng_context *ctx, int min_level, int max_level, int radial_extra, double r_min, double r_max,
                  int lat_sdr, int rad_sdr, int bc_cmb, int bc_surf, const tng_stokes_params &prm);
     void set_viscosity(const tng_vector &eta_fine);
+    void refresh_viscosity_hierarchy();
     void viscosity_updated();
     void apply_stokes(const SVec &src, SVec &dst);
+    // level l's multigrid operator uses Galerkin matrices
+    bool galerkin_level(int l) const
+    {
+        return params.gca >= 1 && l < (int)meshes.size() - 1 && (l == 0 || !params.gca_coarsest_only);
+    }
     void precondition_velocity(tng_vector &x, const tng_vector &b, bool x_zero = false);
     void precondition(SVec &x, const SVec &b, bool x_zero = false);
     void fgmres(SVec &x, const SVec &b, tng_solve_report &rep);
+    void fgmres_cgs2(SVec &x, const SVec &b, tng_solve_report &rep);
+    DevBuf gs_table, gs_part, gs_out;  // CGS2: basis pointer table, multi-dot partials and results
+    // rotation correction: Z (3 rotations, pressure 0), K Z, E = Z^T K Z (LU), pointer tables
+    std::vector<SVec> rc_z, rc_kz;
+    bool rc_dirty = true;
+    double rc_lu[9] = {0};
+    int rc_piv[3] = {0, 1, 2};
+    DevBuf rc_ptrs;
+    void rotation_setup();
+    // the app's viscosity update (stokes_solver.hpp:623-638): eta = rmu^(0.5 - T)
+    // on the velocity level only
+    void update_viscosity_velocity_level(const tng_vector &temp, double rmu);
+    tng_vector *make_vec(tng_domain *m, int ncomp);
+    tng_vector *rhs_f = nullptr, *rhs_g = nullptr;  // the time step's Stokes RHS
+    void rotation_correct(int j, SVec &v, SVec &z, SVec &w);
     void solve(SVec &x, const SVec &b, tng_solve_report &rep);
     void enforce_rhs(tng_vector &f_u);
     // f_u = M (Ra T x/|x|) with the consistent vector mass, then enforce_rhs (stokes_solver.hpp:650-674)
@@ -93,6 +118,7 @@ struct StokesSolver {
 
 }  // namespace tng
 
+struct tng_energy_s;
 struct tng_stokes_solver_s {
     std::unique_ptr<tng::StokesSolver> s;
 };

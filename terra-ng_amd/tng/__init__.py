@@ -11,6 +11,7 @@ fallback: loading fails loudly when it is missing.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 
@@ -58,6 +59,7 @@ EXPORTS = [
     ("tng_ctx_destroy", _i, [_vp]),
     ("tng_ctx_synchronize", _i, [_vp]),
     ("tng_ctx_set_profiling", _i, [_vp, _i]),
+    ("tng_ctx_set_tuning", _i, [_vp, C.c_char_p, _i]),
     ("tng_mesh_create_uniform", _i, [_vp, _i, _i, _d, _d, _i, _i, _pv]),
     ("tng_mesh_destroy", _i, [_vp]),
     ("tng_mesh_get_info", _i, [_vp, C.POINTER(_MeshInfo)]),
@@ -100,6 +102,7 @@ EXPORTS = [
     ("tng_op_clear_local_matrices", _i, [_vp]),
     ("tng_op_copy_local_matrices", _i, [_vp, _vp]),
     ("tng_gca_assemble", _i, [_vp, _vp]),
+    ("tng_gca_assemble_exact", _i, [_vp, _vp]),
     ("tng_gca_elements_collect", _i, [_vp, _i, _vp, _vp]),
     ("tng_xdmf_create", _i, [_vp, C.c_char_p, _i, _i, _pv]),
     ("tng_xdmf_add", _i, [_vp, _vp, C.c_char_p, _i]),
@@ -113,6 +116,7 @@ EXPORTS = [
     ("tng_pcg_solve", _i, [_vp, _vp, _vp, _i, _d, _d, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     ("tng_power_iteration", _i, [_vp, _vp, _i, C.c_ulonglong, C.POINTER(C.c_double)]),
     ("tng_chebyshev_smooth", _i, [_vp, _vp, _i, _i, _d, _vp, _vp]),
+    ("tng_jacobi_smooth", _i, [_vp, _vp, _i, _d, _vp, _vp]),
     ("tng_stokes_params_default", None, [_vp]),
     ("tng_stokes_solver_create", _i, [_vp, _i, _i, _i, _d, _d, _i, _i, _i, _i, _vp, _pv]),
     ("tng_stokes_solver_destroy", _i, [_vp]),
@@ -123,6 +127,26 @@ EXPORTS = [
     ("tng_stokes_precondition_velocity", _i, [_vp, _vp, _vp]),
     ("tng_stokes_enforce_rhs", _i, [_vp, _vp]),
     ("tng_stokes_solve", _i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    ("tng_vec_norm2", _i, [_vp, C.POINTER(C.c_double)]),
+    ("tng_vec_dot_device", _i, [_vp, _vp, _vp]),
+    ("tng_vec_fill_seeded", _i, [_vp, C.c_ulonglong]),
+    ("tng_stokes_create", _i, [_vp, _vp, _vp, _i, _i, _i, _pv]),
+    ("tng_stokes_op_apply", _i, [_vp, _vp, _vp, _vp, _vp]),
+    ("tng_stokes_block", _i, [_vp, _i, _i, _pv]),
+    ("tng_op_set_stored_matrix_mode", _i, [_vp, _i]),
+    ("tng_op_get_local_matrix", _i, [_vp, _i, _i, _i, _i, _i, _vp]),
+    ("tng_op_set_local_matrix", _i, [_vp, _i, _i, _i, _i, _i, _vp]),
+    ("tng_supg_create", _i, [_vp, _vp, _d, _d, _i, _i, _d, _i, _pv]),
+    ("tng_supg_set_dt", _i, [_vp, _d]),
+    ("tng_supg_set_enabled", _i, [_vp, _i]),
+    ("tng_mass_apply", _i, [_vp, _vp]),
+    ("tng_energy_params_default", None, [_vp]),
+    ("tng_energy_create", _i, [_vp, _vp, _vp, _vp, _pv]),
+    ("tng_energy_destroy", _i, [_vp]),
+    ("tng_energy_compute_dt", _i, [_vp, C.POINTER(C.c_double)]),
+    ("tng_energy_step", _i, [_vp, _d, _vp]),
+    ("tng_stokes_solver_update_viscosity", _i, [_vp, _vp, _d]),
+    ("tng_timestep", _i, [_vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_double), _vp, _vp]),
 ]
 
 class StokesParams(C.Structure):
@@ -133,7 +157,9 @@ class StokesParams(C.Structure):
                 ("viscous_pc_num_smoothing_steps_prepost", C.c_int), ("viscous_pc_num_power_iterations", C.c_int),
                 ("gca", C.c_int), ("viscous_pc_dtype", C.c_int), ("coarse_max_iterations", C.c_int),
                 ("coarse_relative_tolerance", C.c_double), ("coarse_absolute_tolerance", C.c_double),
-                ("seed", C.c_ulonglong)]
+                ("seed", C.c_ulonglong), ("gca_coarsest_only", C.c_int), ("gca_exact", C.c_int),
+                ("viscous_pc_smoother", C.c_int), ("krylov_orthogonalization", C.c_int),
+                ("rotation_correction", C.c_int)]
 
     @classmethod
     def defaults(cls, **kw):
@@ -142,6 +168,28 @@ class StokesParams(C.Structure):
         for k, v in kw.items():
             setattr(p, k, v)
         return p
+
+
+class EnergyParams(C.Structure):
+    """apps/mantlecirculation/src/parameters.hpp EnergySolverParameters + diffusivity, temperature BCs,
+    dt_scaling and energy_substeps."""
+    _fields_ = [("krylov_restart", C.c_int), ("krylov_max_iterations", C.c_int),
+                ("krylov_relative_tolerance", C.c_double), ("krylov_absolute_tolerance", C.c_double),
+                ("diffusivity", C.c_double), ("temperature_cmb", C.c_double), ("temperature_surface", C.c_double),
+                ("dt_scaling", C.c_double), ("energy_substeps", C.c_int)]
+
+    @classmethod
+    def defaults(cls, **kw):
+        p = cls()
+        load_library().tng_energy_params_default(C.byref(p))
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+
+class TimestepParams(C.Structure):
+    _fields_ = [("picard_iterations", C.c_int), ("viscosity_law", C.c_int), ("rmu", C.c_double),
+                ("rayleigh_number", C.c_double), ("refresh_hierarchy", C.c_int)]
 
 
 class SolveReport(C.Structure):
@@ -187,6 +235,21 @@ class Context:
 
     def synchronize(self):
         self.check(load_library().tng_ctx_synchronize(self.h))
+
+    def set_tuning(self, name, value):
+        """Launch-shape setting of the apply kernels (negative: the default)."""
+        self.check(load_library().tng_ctx_set_tuning(self.h, name.encode(), int(value)))
+
+    @contextlib.contextmanager
+    def tuning(self, **settings):
+        """Settings for the duration of a with-block, then the defaults."""
+        try:
+            for k, v in settings.items():
+                self.set_tuning(k, v)
+            yield self
+        finally:
+            for k in settings:
+                self.set_tuning(k, -1)
 
     def set_profiling(self, on=True):
         self.check(load_library().tng_ctx_set_profiling(self.h, int(on)))
@@ -403,10 +466,79 @@ class EpsilonDivDivKerngen(_Operator):
     def copy_local_matrices(self, other):
         self.domain.ctx.check(load_library().tng_op_copy_local_matrices(self.h, other.h))
 
+    def set_stored_matrix_mode(self, full):
+        """OperatorStoredMatrixMode::Full (zero matrices) or Off."""
+        self.domain.ctx.check(load_library().tng_op_set_stored_matrix_mode(self.h, int(bool(full))))
 
-def gca_assemble(fine_op, coarse_op):
-    """TwoGridGCA (linalg/solvers/gca/gca.hpp): coarse stored matrices from the fine operator."""
-    coarse_op.domain.ctx.check(load_library().tng_gca_assemble(fine_op.h, coarse_op.h))
+    def get_local_matrix(self, sd, x_cell, y_cell, r_cell, wedge):
+        """GCACapable get_local_matrix: an 18 x 18 array."""
+        out = np.empty((18, 18))
+        self.domain.ctx.check(load_library().tng_op_get_local_matrix(self.h, sd, x_cell, y_cell, r_cell, wedge,
+                                                                     out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def set_local_matrix(self, sd, x_cell, y_cell, r_cell, wedge, mat):
+        a = np.ascontiguousarray(mat, dtype=np.float64).reshape(18, 18)
+        self.domain.ctx.check(load_library().tng_op_set_local_matrix(self.h, sd, x_cell, y_cell, r_cell, wedge,
+                                                                     a.ctypes.data_as(C.c_void_p)))
+
+
+class _Block(_Operator):
+    """A block of a Stokes operator: a borrowed handle that the Stokes operator releases."""
+
+    def __init__(self, domain, h, owner):
+        super().__init__(domain, h)
+        self.owner = owner
+
+    def __del__(self):
+        self.h = None
+
+
+class EpsDivDivStokes:
+    """fe/wedge/operators/shell/epsilon_divdiv_stokes.hpp:18-133: K = [A B^T; B 0]."""
+
+    def __init__(self, domain_fine, domain_coarse, k, bcs=(DIRICHLET, DIRICHLET), diagonal=False):
+        self.domain, self.domain_coarse, self.k = domain_fine, domain_coarse, k
+        self.h = _create(domain_fine, "tng_stokes_create", domain_fine.h, domain_coarse.h, k.h, int(bcs[0]),
+                         int(bcs[1]), int(diagonal))
+
+    def apply(self, u, p, out_u, out_p):
+        self.domain.ctx.check(load_library().tng_stokes_op_apply(self.h, u.h, p.h, out_u.h, out_p.h))
+
+    def _block(self, row, col):
+        h = C.c_void_p()
+        self.domain.ctx.check(load_library().tng_stokes_block(self.h, row, col, C.byref(h)))
+        return _Block(self.domain if col == 1 or row == 1 else self.domain_coarse, h, self)
+
+    def block_11(self):
+        return self._block(1, 1)
+
+    def block_12(self):
+        return self._block(1, 2)
+
+    def block_21(self):
+        return self._block(2, 1)
+
+    def block_22(self):
+        return self._block(2, 2)
+
+    def __del__(self):
+        try:
+            if self.h:
+                load_library().tng_op_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def gca_assemble(fine_op, coarse_op, exact=False):
+    """TwoGridGCA (linalg/solvers/gca/gca.hpp): coarse stored matrices from the fine operator.
+    exact=True forms the Galerkin product R A P instead of the reference's top-face rule."""
+    lib = load_library()
+    if exact:
+        coarse_op.domain.ctx.check(lib.tng_gca_assemble_exact(fine_op.h, coarse_op.h))
+    else:
+        coarse_op.domain.ctx.check(lib.tng_gca_assemble(fine_op.h, coarse_op.h))
 
 
 def _cell_shape(domain):
@@ -577,6 +709,28 @@ class KMassLumped(_Operator):
         self.k = k
 
 
+class UnsteadyAdvectionDiffusionSUPG(_Operator):
+    """fe/wedge/operators/shell/unsteady_advection_diffusion_supg.hpp:128-433 (scalar, FP64)."""
+
+    def __init__(self, domain, velocity, diffusivity, dt, treat_boundary=True, diagonal=False, mass_scaling=1.0,
+                 lumped_mass=False):
+        super().__init__(domain, _create(domain, "tng_supg_create", domain.h, velocity.h, float(diffusivity),
+                                         float(dt), int(treat_boundary), int(diagonal), float(mass_scaling),
+                                         int(lumped_mass)))
+        self.velocity = velocity
+
+    def set_dt(self, dt):
+        self.domain.ctx.check(load_library().tng_supg_set_dt(self.h, float(dt)))
+
+    def set_supg_enabled(self, on):
+        self.domain.ctx.check(load_library().tng_supg_set_enabled(self.h, int(bool(on))))
+
+
+def mass_apply(src, dst):
+    """Mass (mass.hpp): dst = M src, consistent scalar mass."""
+    src.domain.ctx.check(load_library().tng_mass_apply(src.h, dst.h))
+
+
 class VectorMass(_Operator):
     """fe/wedge/operators/shell/vector_mass.hpp:17-130 (consistent, per component)."""
 
@@ -611,7 +765,19 @@ def dot(x, y):
 
 
 def norm_2(x):
-    return float(np.sqrt(dot(x, x)))
+    out = C.c_double()
+    x.domain.ctx.check(load_library().tng_vec_norm2(x.h, C.byref(out)))
+    return out.value
+
+
+def dot_device(x, y, out_device_ptr):
+    """dot(x, y) written to a device address (8 bytes), without a host synchronisation."""
+    x.domain.ctx.check(load_library().tng_vec_dot_device(x.h, y.h, C.c_void_p(out_device_ptr)))
+
+
+def randomize(x, seed=12345):
+    """Uniform [0, 1) entries, reproducible from the seed (linalg/vector.hpp randomize_impl)."""
+    x.domain.ctx.check(load_library().tng_vec_fill_seeded(x.h, C.c_ulonglong(seed)))
 
 
 def norm_inf(x):
@@ -639,6 +805,11 @@ def chebyshev_smooth(op, inv_diag, x, b, order=2, iterations=1, lambda_max_estim
     """Chebyshev sweeps with a given eigenvalue estimate (linalg/solvers/chebyshev.hpp)."""
     op.domain.ctx.check(load_library().tng_chebyshev_smooth(op.h, inv_diag.h, order, iterations,
                                                             float(lambda_max_estimate), x.h, b.h))
+
+
+def jacobi_smooth(op, inv_diag, x, b, iterations=1, omega=1.0):
+    """Damped Jacobi sweeps x += omega D^{-1} (b - A x) (linalg/solvers/jacobi.hpp)."""
+    op.domain.ctx.check(load_library().tng_jacobi_smooth(op.h, inv_diag.h, iterations, float(omega), x.h, b.h))
 
 
 class _BorrowedDomain(DistributedDomain):
@@ -740,3 +911,61 @@ def has_nan_or_inf(x):
 def apply(op, src, dst):
     """linalg::apply (linalg/operator.hpp:155-159)."""
     op.apply(src, dst)
+
+
+class SUPGEnergySolver:
+    """SUPGSolver (apps/mantlecirculation/src/energy_solver.hpp:85-276) on a Stokes solver's velocity mesh."""
+
+    def __init__(self, stokes, velocity, temperature, params=None):
+        self.stokes, self.velocity, self.temperature = stokes, velocity, temperature
+        self.params = params if params is not None else EnergyParams.defaults()
+        h = C.c_void_p()
+        stokes.ctx.check(load_library().tng_energy_create(stokes.h, velocity.h, temperature.h, C.byref(self.params),
+                                                          C.byref(h)))
+        self.h = h
+
+    def compute_dt(self):
+        out = C.c_double()
+        self.stokes.ctx.check(load_library().tng_energy_compute_dt(self.h, C.byref(out)))
+        return out.value
+
+    def step(self, dt, history_capacity=1000):
+        rep = SolveReport()
+        hist = (C.c_double * history_capacity)()
+        rep.history = C.cast(hist, C.POINTER(C.c_double))
+        rep.history_capacity = history_capacity
+        self.stokes.ctx.check(load_library().tng_energy_step(self.h, float(dt), C.byref(rep)))
+        return {name: getattr(rep, name) for name, _ in SolveReport._fields_ if name != "history"}
+
+    def __del__(self):
+        try:
+            if self.h:
+                load_library().tng_energy_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def update_viscosity(stokes, temperature, rmu):
+    """StokesContext::update_viscosity with the Frank-Kamenetskii law (velocity level only)."""
+    stokes.ctx.check(load_library().tng_stokes_solver_update_viscosity(stokes.h, temperature.h, float(rmu)))
+
+
+def timestep(stokes, energy, u, p, picard_iterations=1, viscosity_law=0, rmu=1.0, rayleigh_number=1e5,
+             refresh_hierarchy=False):
+    """One step of the mantle-circulation time loop (mantlecirculation.cpp:413-445); returns
+    (dt, Stokes report, energy report).  refresh_hierarchy: after the viscosity update,
+    also rebuild the coarse levels (the reference updates the velocity level only)."""
+    prm = TimestepParams(picard_iterations, viscosity_law, float(rmu), float(rayleigh_number),
+                         int(bool(refresh_hierarchy)))
+    dt = C.c_double()
+    reps = []
+    for _ in range(2):
+        rep = SolveReport()
+        rep.history = None
+        rep.history_capacity = 0
+        reps.append(rep)
+    stokes.ctx.check(load_library().tng_timestep(stokes.h, energy.h, u.h, p.h, C.byref(prm), C.byref(dt),
+                                                 C.byref(reps[0]), C.byref(reps[1])))
+    conv = [{name: getattr(r, name) for name, _ in SolveReport._fields_ if name != "history"} for r in reps]
+    return dt.value, conv[0], conv[1]

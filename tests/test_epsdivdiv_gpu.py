@@ -87,25 +87,24 @@ def test_parity_shapes(gpu_ctx, L, Lr, ls, rs, bcs):
     assert rel_err(g, ref) < 1e-12
 
 
-@pytest.mark.parametrize("L,Lr,env", [(2, 7, {"TNG_APPLY_RADIAL_CHUNKS": "1"}), (3, 6, {"TNG_APPLY_WAVES": "4"}),
-                                       (1, 8, {"TNG_APPLY_RADIAL_CHUNKS": "1"}), (1, 8, {"TNG_APPLY_RADIAL_SPLIT": "0"}),
-                                       (1, 8, {}), (2, 8, {}), (2, 7, {"TNG_APPLY_RADIAL_SPLIT": "1"}), (1, 9, {}),
+@pytest.mark.parametrize("L,Lr,env", [(2, 7, {"apply_radial_chunks": 1}), (3, 6, {"apply_waves": 4}),
+                                       (1, 8, {"apply_radial_chunks": 1}), (1, 8, {"apply_radial_split": 0}),
+                                       (1, 8, {}), (2, 8, {}), (2, 7, {"apply_radial_split": 1}), (1, 9, {}),
                                        (3, 7, {}), (3, 6, {})])
 @pytest.mark.parametrize("bcs", [(D, D), (F, F), (N, N), (D, F), (N, F), (F, D)])
-def test_parity_launch_variants(gpu_ctx, L, Lr, env, bcs, monkeypatch):
+def test_parity_launch_variants(gpu_ctx, L, Lr, env, bcs):
     """Launch shapes chosen by layer count or tuning variable: 128, 256 and
     512 layers as radial parts of 64 cells swept separately; 128 and 256 as
     radial chunks of one wave each in one workgroup, or as two or four cells
     per lane; 4-wave workgroups at 64 layers; every boundary-condition mix of
     the one-column kernels."""
-    for k, v in env.items():
-        monkeypatch.setenv(k, v)
     dom, om, w = setup(L, Lr)
     k = w.eta_lognormal(102)
     u = w.velocity(101)
     for add in (False, True):
         d0 = w.velocity(103)
-        g = gpu_apply(dom, k, u, bcs, eps=1.0, apply_mode=tng.ADD if add else tng.REPLACE, dst0=d0)
+        with tng.default_context().tuning(**env):
+            g = gpu_apply(dom, k, u, bcs, eps=1.0, apply_mode=tng.ADD if add else tng.REPLACE, dst0=d0)
         ref = om.epsdivdiv(k, u, *bcs, path="fast", penalty_eps=1.0, apply_mode=tng.ADD if add else tng.REPLACE,
                            dst=d0)
         assert rel_err(g, ref) < 1e-12
@@ -114,7 +113,7 @@ def test_parity_launch_variants(gpu_ctx, L, Lr, env, bcs, monkeypatch):
 @pytest.mark.parametrize("Lr", [7, 8])
 @pytest.mark.parametrize("bcs", [(D, D), (F, F), (F, D)])
 @pytest.mark.parametrize("dtype", [tng.F64, tng.F32])
-def test_radial_parts_match_radial_chunks(gpu_ctx, Lr, bcs, dtype, monkeypatch):
+def test_radial_parts_match_radial_chunks(gpu_ctx, Lr, bcs, dtype):
     """128 and 256 layers: the separately swept radial parts plus the
     interface kernel give the same bits as radial chunks in one workgroup
     (same sums in the same order), in both apply modes."""
@@ -123,11 +122,11 @@ def test_radial_parts_match_radial_chunks(gpu_ctx, Lr, bcs, dtype, monkeypatch):
     u = w.velocity(101)
     d0 = w.velocity(103)
     out = {}
-    for env in ("1", "0"):
-        monkeypatch.setenv("TNG_APPLY_RADIAL_SPLIT", env)
-        out[env] = [gpu_apply(dom, k, u, bcs, eps=1.0, dtype=dtype, apply_mode=m, dst0=d0)
-                    for m in (tng.REPLACE, tng.ADD)]
-    for a, b in zip(out["1"], out["0"]):
+    for split in (1, 0):
+        with tng.default_context().tuning(apply_radial_split=split):
+            out[split] = [gpu_apply(dom, k, u, bcs, eps=1.0, dtype=dtype, apply_mode=m, dst0=d0)
+                          for m in (tng.REPLACE, tng.ADD)]
+    for a, b in zip(out[1], out[0]):
         assert np.array_equal(a, b)
 
 
@@ -201,16 +200,16 @@ def test_config2_level6_freeslip_timed_size(gpu_ctx):
 
 @pytest.mark.parametrize("L,blocks", [(3, 5), (3, 7), (4, 3), (4, 37)])
 @pytest.mark.parametrize("bcs", [(F, F), (D, D), (F, D)])
-def test_balanced_shares_cross_strips(gpu_ctx, L, blocks, bcs, monkeypatch):
+def test_balanced_shares_cross_strips(gpu_ctx, L, blocks, bcs):
     """Few workgroups: every share of node rows runs over several strip
     boundaries (and subdomain boundaries inside a strip)."""
-    monkeypatch.setenv("TNG_APPLY_BLOCKS", str(blocks))
     dom, om, w = setup(L)
     k = w.eta_layered(102)
     u = w.velocity(101)
     d0 = w.velocity(103)
     for mode in (tng.REPLACE, tng.ADD):
-        g = gpu_apply(dom, k, u, bcs, eps=1.0, apply_mode=mode, dst0=d0)
+        with tng.default_context().tuning(apply_blocks=blocks):
+            g = gpu_apply(dom, k, u, bcs, eps=1.0, apply_mode=mode, dst0=d0)
         ref = om.epsdivdiv(k, u, *bcs, path="fast", penalty_eps=1.0, apply_mode=mode, dst=d0)
         assert rel_err(g, ref) < 1e-12
 

@@ -138,6 +138,26 @@ def test_chebyshev_closed_form(order, level1, gpu_ctx):
     assert rel_err(err, err_exp) < 1e-11
 
 
+@pytest.mark.parametrize("iterations,omega", [(1, 1.0), (3, 0.7)])
+def test_jacobi_closed_form(iterations, omega, level1, gpu_ctx):
+    """Damped Jacobi (jacobi.hpp:16-60) from a nonzero start on A x = A x*:
+    the error after n sweeps is (I - omega D^{-1} A)^n (x0 - x*)."""
+    om, dom, k, A, diag = level1
+    M = A / diag[:, None]
+    xs, x0 = field(om, 3, 9), field(om, 3, 10)
+    e = to_unique(om, x0) - to_unique(om, xs)
+    for _ in range(iterations):
+        e = e - omega * (M @ e)
+    op, kv = gpu_op(dom, k)
+    invd = inv_diag_vec(dom, op)
+    b = vec(dom, 3)
+    op.apply(vec(dom, 3, xs), b)
+    x = vec(dom, 3, x0)
+    tng.jacobi_smooth(op, invd, x, b, iterations=iterations, omega=omega)
+    err = to_unique(om, x.download()) - to_unique(om, xs)
+    assert rel_err(err, e) < 1e-11
+
+
 def test_pcg_matches_textbook_cg(level1, gpu_ctx):
     om, dom, k, A, diag = level1
     xs = field(om, 3, 8)
@@ -251,7 +271,7 @@ def test_vcycle_matches_restatement(dt, tol, gpu_ctx):
 
 @pytest.mark.parametrize("dt", [tng.F64, tng.F32])
 @pytest.mark.parametrize("bcs", [(D, D), (F, F), (D, F)])
-def test_fused_chebyshev_matches_separate_kernels(dt, bcs, gpu_ctx, monkeypatch):
+def test_fused_chebyshev_matches_separate_kernels(dt, bcs, gpu_ctx):
     """Chebyshev sub-steps fused into the one-column apply kernels (levels with
     64 and 128 radial cells) and the shared-node update after the halo sum give
     bit-identical V-cycles and FGMRES iterates to the separate apply, halo sum
@@ -262,10 +282,10 @@ def test_fused_chebyshev_matches_separate_kernels(dt, bcs, gpu_ctx, monkeypatch)
     fp = field(op_, 1, 33)
     out = {}
     for fuse in ("0", "1"):
-        monkeypatch.setenv("TNG_CHEB_FUSE", fuse)
-        S = tng.StokesSolver(3, 5, 0.55, 1.0, bcs, tng.StokesParams.defaults(
-            viscous_pc_dtype=dt, krylov_max_iterations=6, krylov_restart=4, krylov_relative_tolerance=1e-30),
-            radial_extra_levels=2)
+        with tng.default_context().tuning(cheb_fuse=int(fuse)):
+            S = tng.StokesSolver(3, 5, 0.55, 1.0, bcs, tng.StokesParams.defaults(
+                viscous_pc_dtype=dt, krylov_max_iterations=6, krylov_restart=4, krylov_relative_tolerance=1e-30),
+                radial_extra_levels=2)
         S.set_viscosity(vec(S.domain_velocity, 1, eta))
         x = vec(S.domain_velocity, 3)
         bv = vec(S.domain_velocity, 3, b)
@@ -349,11 +369,12 @@ def stokes_dense():
     return of, oc, eta, K, nu, npr
 
 
+@pytest.mark.parametrize("orth", [0, 1])
 @pytest.mark.parametrize("dt", [tng.F64, tng.F32])
-def test_stokes_solve_vs_dense(dt, stokes_dense, gpu_ctx):
+def test_stokes_solve_vs_dense(dt, orth, stokes_dense, gpu_ctx):
     of, oc, eta, K, nu, npr = stokes_dense
     S = stokes_solver(1, 2, (D, D), krylov_restart=30, krylov_max_iterations=600, krylov_relative_tolerance=1e-10,
-                      krylov_absolute_tolerance=1e-30, viscous_pc_dtype=dt)
+                      krylov_absolute_tolerance=1e-30, viscous_pc_dtype=dt, krylov_orthogonalization=orth)
     S.set_viscosity(vec(S.domain_velocity, 1, eta))
     xs_u, xs_p = field(of, 3, 13), field(oc, 1, 14)
     fu, fp = vec(S.domain_velocity, 3), vec(S.domain_pressure, 1)
@@ -416,6 +437,42 @@ def test_stokes_solve_freeslip_buoyancy(gpu_ctx):
     assert rep["converged"] == 1, rep
 
 
+def test_rotation_correction_reaches_the_same_solution(gpu_ctx):
+    """Free slip on both surfaces: with the rigid-rotation correction after the
+    block preconditioner the solve converges to the same solution as without;
+    the correction makes every preconditioned step exact on the rotations
+    (Z^T (v_j - K z_j) = 0), so the residual's rotational components end up at
+    rounding level."""
+    of = O.Mesh(4)
+    r = np.broadcast_to(of.radii[:, None, None, :], of.shape)
+    eta = np.where(r < 0.85, 10.0, 1.0) * np.exp(0.5 * field(of, 1, 43))
+    T = field(of, 1, 44)
+    out = {}
+    for rc in (0, 1):
+        S = stokes_solver(2, 4, (F, F), krylov_restart=30, krylov_max_iterations=300, krylov_relative_tolerance=1e-6,
+                          krylov_absolute_tolerance=1e-30, rotation_correction=rc)
+        S.set_viscosity(vec(S.domain_velocity, 1, eta))
+        fu = vec(S.domain_velocity, 3)
+        S.buoyancy_rhs(vec(S.domain_velocity, 1, T), 100.0, fu)
+        u, p = vec(S.domain_velocity, 3), vec(S.domain_pressure, 1)
+        rep = S.solve(u, p, fu, vec(S.domain_pressure, 1))
+        assert rep["converged"] == 1, (rc, rep)
+        ru, rp = vec(S.domain_velocity, 3), vec(S.domain_pressure, 1)
+        S.apply(u, p, ru, rp)
+        res = fu.download() - ru.download()
+        out[rc] = (u.download(), res)
+    own = of.ownership_mask == 1
+    assert rel_err(out[1][0][:, own], out[0][0][:, own]) < 1e-4
+    X = of.coords[:, :, :, None, :] * of.radii[:, None, None, :, None]
+    res = np.moveaxis(out[1][1], 0, -1)[own]
+    for i in range(3):
+        e = np.zeros(3)
+        e[i] = 1.0
+        m = np.cross(e, X[own])
+        m /= np.linalg.norm(m)
+        assert abs(np.sum(res * m)) < 1e-6 * np.linalg.norm(res), i
+
+
 def test_enforce_rhs_first_boundary_only(gpu_ctx):
     """Dirichlet on both surfaces: only the CMB rows are zeroed, as the
     reference's get_shell_boundary_flag returns the first match."""
@@ -435,3 +492,34 @@ def test_enforce_rhs_first_boundary_only(gpu_ctx):
     assert np.all(g[..., 0] == 0)
     assert np.abs(np.einsum("csxy,sxyc->sxy", g[..., -1], n)).max() < 1e-15
     assert np.array_equal(g[..., 1:-1], f[..., 1:-1])
+
+
+@pytest.mark.parametrize("bcs,rtol,restart", [((D, D), 1e-9, 30), ((D, D), 1e-9, 8), ((F, D), 1e-7, 30)])
+def test_cgs2_matches_mgs(bcs, rtol, restart, gpu_ctx):
+    """Two-pass CGS2 orthogonalisation (one reduction per Arnoldi step, the
+    re-orthogonalisation delayed by one step) against the reference's modified
+    Gram-Schmidt: same iteration count within one, the same residual history
+    to 1e-6 relative while the residual is above 1e3 rtol, the same solution."""
+    of = O.Mesh(4)
+    r = np.broadcast_to(of.radii[:, None, None, :], of.shape)
+    eta = np.where(r < 0.85, 10.0, 1.0) * np.exp(0.5 * field(of, 1, 41))
+    T = field(of, 1, 42)
+    out = {}
+    for orth in (0, 1):
+        S = stokes_solver(2, 4, bcs, krylov_restart=restart, krylov_max_iterations=600, krylov_relative_tolerance=rtol,
+                          krylov_absolute_tolerance=1e-30, viscous_pc_chebyshev_order=3, krylov_orthogonalization=orth)
+        S.set_viscosity(vec(S.domain_velocity, 1, eta))
+        fu = vec(S.domain_velocity, 3)
+        S.buoyancy_rhs(vec(S.domain_velocity, 1, T), 100.0, fu)
+        u, p = vec(S.domain_velocity, 3), vec(S.domain_pressure, 1)
+        rep = S.solve(u, p, fu, vec(S.domain_pressure, 1), history_capacity=2000)
+        assert rep["converged"] == 1 and rep["relative_residual"] <= rtol, (orth, rep)
+        out[orth] = (rep, u.download())
+    (r0, u0), (r1, u1) = out[0], out[1]
+    assert abs(r0["iterations"] - r1["iterations"]) <= 1, (r0["iterations"], r1["iterations"])
+    h0, h1 = np.array(r0["history"]), np.array(r1["history"])
+    n = min(len(h0), len(h1))
+    keep = h0[:n] > 1e3 * rtol * h0[0]
+    assert np.all(np.abs(h1[:n][keep] - h0[:n][keep]) <= 1e-6 * h0[:n][keep])
+    own = of.ownership_mask == 1
+    assert rel_err(u1[:, own], u0[:, own]) < 1e2 * rtol

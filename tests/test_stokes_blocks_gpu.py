@@ -44,8 +44,9 @@ def vec(dom, ncomp, data=None, dtype=tng.F64):
 
 BC_SETS = [(N, N), (D, D), (F, F), (D, F), (F, N)]
 SHAPES = [(3, 0, 0), (4, 0, 0), (3, 1, 1), (4, 1, 0)]
-# tall columns (65 and 129 nodes per subdomain column): the LDS-staged kernels
-TALL = [(3, 0, 0, 6), (3, 0, 0, 7), (3, 1, 1, 7), (2, 0, 0, 7)]
+# tall columns (65, 129 and 257 nodes per subdomain column): the LDS-staged
+# kernels and the one-wave-per-column gradient (1, 2 and 4 cells per lane)
+TALL = [(3, 0, 0, 6), (3, 0, 0, 7), (3, 1, 1, 7), (2, 0, 0, 7), (2, 0, 0, 8)]
 
 
 @pytest.mark.parametrize("bcs", BC_SETS)
@@ -72,10 +73,11 @@ def test_gradient(bcs, L, ls, rs, R, gpu_ctx):
     assert rel_err(out.download(), ref) < 1e-12
 
 
+@pytest.mark.parametrize("shape", [(3, 1, 0, None), (3, 1, 0, 7), (2, 0, 1, 7)])
 @pytest.mark.parametrize("mode,comm", [(tng.ADD, tng.COMMUNICATE_ADDITIVELY), (tng.ADD, tng.SKIP_COMMUNICATION),
                                        (tng.REPLACE, tng.SKIP_COMMUNICATION)])
-def test_divergence_gradient_modes(mode, comm, gpu_ctx):
-    df, dc, of, oc = levels(3, 1, 0)
+def test_divergence_gradient_modes(mode, comm, shape, gpu_ctx):
+    df, dc, of, oc = levels(*shape)
     u, p = field(of, 3, 3), field(oc, 1, 4)
     b0, g0 = field(oc, 1, 5), field(of, 3, 6)
     B = tng.DivergenceKerngen(df, dc, (D, F))
@@ -111,6 +113,17 @@ def test_divergence_gradient_fp32(gpu_ctx):
     tng.GradientKerngen(df, dc, (D, F)).apply(vec(dc, 1, p, tng.F32), og)
     assert rel_err(ob.download(), O.divergence(of, oc, u, D, F)) < 2e-6
     assert rel_err(og.download(), O.gradient(of, oc, p, D, F)) < 2e-6
+
+
+@pytest.mark.parametrize("L,ls,rs,R", [(3, 0, 0, 6), (3, 1, 1, 7)])
+def test_divergence_gradient_fp32_tall(L, ls, rs, R, gpu_ctx):
+    df, dc, of, oc = levels(L, ls, rs, R)
+    u, p = field(of, 3, 13), field(oc, 1, 14)
+    ob, og = vec(dc, 1, dtype=tng.F32), vec(df, 3, dtype=tng.F32)
+    tng.DivergenceKerngen(df, dc, (F, D)).apply(vec(df, 3, u, tng.F32), ob)
+    tng.GradientKerngen(df, dc, (F, D)).apply(vec(dc, 1, p, tng.F32), og)
+    assert rel_err(ob.download(), O.divergence(of, oc, u, F, D)) < 2e-6
+    assert rel_err(og.download(), O.gradient(of, oc, p, F, D)) < 2e-6
 
 
 @pytest.mark.parametrize("ncomp", [1, 3])

@@ -147,3 +147,32 @@ def test_writer_outlives_its_mesh(tmp_path, gpu_ctx):
     tng.load_library().tng_mesh_destroy(dom.h)
     dom.h = None
     assert tng.load_library().tng_xdmf_destroy(h) == 0
+
+
+def test_checkpoint_metadata_rejects_malformed_fields(tmp_path, gpu_ctx):
+    """Negative or inconsistent fields of the checkpoint metadata are refused
+    before any allocation sized from them."""
+    import struct
+
+    dom, om = meshes(2, 2, 0, 0)
+    w = tng.XDMFOutput(dom, str(tmp_path))
+    w.add(vec(dom, 1, field(om, 1, 12)), "eta", 8)
+    w.write()
+    w.close()
+    good = (tmp_path / "checkpoint_metadata.bin").read_bytes()
+    version, S, Sr, sx, sy, sr = struct.unpack_from("<6i", good, 0)
+    off_files = 24 + 8 * (Sr * (sr - 1) + 1) + (4 if version > 0 else 0)
+    bad = []
+    for off, val in [(4, -1), (8, 0), (20, 1), (4, 1 << 30), (off_files + 4, -5)]:
+        b = bytearray(good)
+        struct.pack_into("<i", b, off, val)
+        bad.append(bytes(b))
+    bad.append(good[: len(good) // 2])  # truncated
+    for b in bad:
+        (tmp_path / "checkpoint_metadata.bin").write_bytes(b)
+        with pytest.raises(tng.TngError) as e:
+            tng.read_xdmf_checkpoint(str(tmp_path), "eta", 0, vec(dom, 1))
+        assert e.value.status == tng.IO_ERROR
+    (tmp_path / "checkpoint_metadata.bin").write_bytes(good)
+    r = vec(dom, 1)
+    tng.read_xdmf_checkpoint(str(tmp_path), "eta", 0, r)
