@@ -164,6 +164,19 @@ tng_status tng_op_coefficient_updated(tng_op op);
 /* linalg::apply (linalg/operator.hpp:155-159) -> apply_impl
  * (epsilon_divdiv_kerngen.hpp:577-792). */
 tng_status tng_op_apply(tng_op op, tng_vec src, tng_vec dst);
+/* No counterpart in the reference.  tng_op_apply of an EpsilonDivDiv operator
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): the same
+ * kernels and results, with the fs/fs penalty's per-block moments and the
+ * partial sums between radial parts in a fixed scratch array of 8 MiB per
+ * GPU instead of library allocations.  A mesh whose radial-part sums exceed
+ * the scratch (8 MiB less 24 KiB: e.g. more than three parts of 64 layers at
+ * lateral level 7) is rejected with TNG_INVALID_ARGUMENT.  The operator's
+ * boundary-row table is allocated when the operator is created, so the first
+ * apply may be this one.  No allocation, no host synchronisation; profiling
+ * events are not recorded.  Every call on a GPU shares that GPU's scratch:
+ * calls that overlap on one GPU (two streams, two host threads) race. */
+tng_status tng_epsdivdiv_apply_noalloc(tng_op op, tng_vec src, tng_vec dst);
 tng_status tng_op_destroy(tng_op op);
 
 /* Stored local matrices (epsilon_divdiv_kerngen.hpp:503-575

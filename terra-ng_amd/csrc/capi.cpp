@@ -59,6 +59,7 @@ tng_status tng_ctx_create(int device_ordinal, tng_ctx *out)
         TNG_HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         ctx->red_partials.alloc(4096 * 3 * sizeof(double));
         ctx->red_result.alloc(4096);
+        noalloc_scratch_init(device_ordinal);
         TNG_HIP_CHECK(hipDeviceSynchronize());
     });
     if (st != TNG_OK) {
@@ -85,6 +86,7 @@ tng_status tng_ctx_create_with_stream(int device_ordinal, void *hip_stream, tng_
         ctx->owns_stream = false;
         ctx->red_partials.alloc(4096 * 3 * sizeof(double));
         ctx->red_result.alloc(4096);
+        noalloc_scratch_init(device_ordinal);
         TNG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     });
     if (st != TNG_OK) {
@@ -372,6 +374,22 @@ tng_status tng_op_apply(tng_op op, tng_vec src, tng_vec dst)
     });
 }
 
+tng_status tng_epsdivdiv_apply_noalloc(tng_op op, tng_vec src, tng_vec dst)
+{
+    if (!op || !src || !dst) return TNG_INVALID_ARGUMENT;
+    tng_context *ctx = op->mesh->ctx;
+    return guarded(ctx, [&] {
+        TNG_REQUIRE(op->kind == OP_EPSDIVDIV, "allocation-free apply: an EpsilonDivDiv operator is required");
+        TNG_REQUIRE(src != dst, "apply: src and dst must be different vectors");
+        TNG_REQUIRE(src->mesh == op->mesh && dst->mesh == op->mesh && src->ncomp == 3 && dst->ncomp == 3,
+                    "apply: vectors do not live on the operator's mesh");
+        TNG_REQUIRE(src->dtype == dst->dtype && op->k->dtype == src->dtype,
+                    "apply: coefficient and vectors must share one dtype");
+        set_device(ctx);
+        epsdivdiv_apply_noalloc(*op, *src, *dst, ctx->stream);
+    });
+}
+
 tng_status tng_op_destroy(tng_op op)
 {
     if (!op) return TNG_INVALID_ARGUMENT;
@@ -450,6 +468,8 @@ tng_operator *new_epsdivdiv(tng_domain *mesh, tng_vector *k, int bc_cmb, int bc_
     op->bc_surf = bc_surf;
     op->diagonal = diagonal;
     op->penalty_active = bc_cmb == TNG_FREESLIP && bc_surf == TNG_FREESLIP;
+    // the boundary-row table, allocated here so that applies never allocate it
+    op->d_btab.alloc((size_t)mesh->nlat * 2 * 3 * k->elem_size());
     op->d_penalty_b.alloc(4 * sizeof(double));
     TNG_HIP_CHECK(hipMemsetAsync(op->d_penalty_b.p, 0, 4 * sizeof(double), mesh->ctx->stream));
     epsdivdiv_update_boundary_table(*op, mesh->ctx->stream);

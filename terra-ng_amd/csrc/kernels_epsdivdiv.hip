@@ -1384,6 +1384,18 @@ template <typename T> void run_diag_kernel(tng_operator &op, const tng_vector *s
 
 template <typename T> PenArgs<T> pen_args(tng_operator &op, const tng_vector &src);
 
+// Scratch of the allocation-free apply: the penalty's per-block rotation
+// moments, then the partial sums of the levels between radial parts.  One
+// copy per GPU, resident until the process ends; every entry a call reads
+// was written earlier in the same call, so nothing needs clearing.  Calls
+// that overlap on one GPU race on it.
+constexpr size_t kNoAllocScratchBytes = (size_t)8 << 20;
+constexpr size_t kNoAllocPenDoubles = 3 * 1024;  // kPenBlocks x 3, rounded to a 256-byte boundary
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
 // fs/fs penalty: the per-block rotation moments of src into the context's
 // partials (kPenBlocks x 3 doubles)
 template <typename T> void penalty_partials(tng_operator &op, const tng_vector &src, hipStream_t s, double *partials)
@@ -1440,14 +1452,17 @@ ApplyArgs<T> apply_args(const tng_operator &op, const tng_vector &src, tng_vecto
     return a;
 }
 
-template <typename T> void apply_impl(tng_operator &op, const tng_vector &src, tng_vector &dst, hipStream_t s)
+// ws: the scratch array of the allocation-free apply, or nullptr (the
+// context's partials and the operator's interface buffer)
+template <typename T>
+void apply_impl(tng_operator &op, const tng_vector &src, tng_vector &dst, hipStream_t s, double *ws = nullptr)
 {
     tng_domain &m = *op.mesh;
     tng_context &ctx = *m.ctx;
     const bool add = op.apply_mode == TNG_ADD;
-    hipEvent_t *ev = ctx.profiling ? ctx.prof_next_set() : nullptr;
+    hipEvent_t *ev = ctx.profiling && !ws ? ctx.prof_next_set() : nullptr;
     if (ev) TNG_HIP_CHECK(hipEventRecord(ev[0], s));
-    double *partials = m.ctx->red_partials.as<double>();
+    double *partials = ws ? ws : m.ctx->red_partials.as<double>();
     auto penalty_moment = [&]() { tng::penalty_moment<T>(op, src, s, partials); };
     if (op.stored) {
         // slow path on stored matrices (Dirichlet/free-slip handled per row)
@@ -1469,8 +1484,17 @@ template <typename T> void apply_impl(tng_operator &op, const tng_vector &src, t
         if (sh.S > 1) {
             // partial sums of the levels between radial parts
             const size_t bytes = sizeof(T) * 6 * (sh.S - 1) * (size_t)m.nsd * m.nx * m.nx;
-            if (op.d_iface.bytes < bytes) op.d_iface.alloc(bytes);
-            a.iface = op.d_iface.as<T>();
+            if (ws) {
+                TNG_REQUIRE(bytes <= kNoAllocScratchBytes - kNoAllocPenDoubles * sizeof(double),
+                            "allocation-free apply: the partial sums between radial parts need " +
+                                std::to_string(bytes) + " bytes, more than the " +
+                                std::to_string(kNoAllocScratchBytes - kNoAllocPenDoubles * sizeof(double)) +
+                                " bytes of scratch");
+                a.iface = reinterpret_cast<T *>(ws + kNoAllocPenDoubles);
+            } else {
+                if (op.d_iface.bytes < bytes) op.d_iface.alloc(bytes);
+                a.iface = op.d_iface.as<T>();
+            }
         }
         if (pen) {
             penalty_partials<T>(op, src, s, partials);
@@ -1541,6 +1565,33 @@ void epsdivdiv_apply(tng_operator &op, const tng_vector &src, tng_vector &dst, h
 {
     if (src.dtype == TNG_F64) apply_impl<double>(op, src, dst, s);
     else apply_impl<float>(op, src, dst, s);
+}
+
+void noalloc_scratch_init(int device)
+{
+    TNG_REQUIRE(device >= 0 && device < kMaxDevices, "allocation-free apply: device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void *p = nullptr;
+    TNG_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    TNG_REQUIRE(p != nullptr, "allocation-free apply: no device address for the scratch array");
+    g_scratch_ptr[device] = static_cast<double *>(p);
+}
+
+void epsdivdiv_apply_noalloc(tng_operator &op, const tng_vector &src, tng_vector &dst, hipStream_t s)
+{
+    const int dev = op.mesh->ctx->device;
+    double *ws = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (dev >= 0 && dev < kMaxDevices) ws = g_scratch_ptr[dev];
+    }
+    TNG_REQUIRE(ws != nullptr, "allocation-free apply: the context's device has no scratch address");
+    const size_t btab_bytes = (size_t)op.mesh->nlat * 2 * 3 * op.k->elem_size();
+    TNG_REQUIRE(op.btab_valid || op.d_btab.bytes == btab_bytes,
+                "allocation-free apply: the boundary table would need a new allocation");
+    if (src.dtype == TNG_F64) apply_impl<double>(op, src, dst, s, ws);
+    else apply_impl<float>(op, src, dst, s, ws);
 }
 
 bool epsdivdiv_apply_cheb(tng_operator &op, const tng_vector &x, tng_vector &ax, tng_vector &x_out, tng_vector &d,

@@ -291,6 +291,11 @@ void launch_copy(tng_vector &dst, const tng_vector &src, hipStream_t s);
 void launch_halo_sum(const tng_domain &m, tng_vector &v, hipStream_t s);
 void epsdivdiv_update_boundary_table(tng_operator &op, hipStream_t s);
 void epsdivdiv_apply(tng_operator &op, const tng_vector &src, tng_vector &dst, hipStream_t s);
+// The same apply with its scratch (penalty partials, partial sums between
+// radial parts) in a fixed per-GPU device array: no device allocation.
+// noalloc_scratch_init records the array's address on the current device.
+void noalloc_scratch_init(int device);
+void epsdivdiv_apply_noalloc(tng_operator &op, const tng_vector &src, tng_vector &dst, hipStream_t s);
 // One Chebyshev sub-step fused into the apply: ax = A x on shared node copies
 // (scratch), d = az (b - A x) invd [+ ad d], x_out = x + d everywhere.
 // Returns false, launching nothing, where the fused kernels do not apply.

@@ -136,6 +136,7 @@ EXPORTS = [
     ("tng_op_set_stored_matrix_mode", _i, [_vp, _i]),
     ("tng_op_get_local_matrix", _i, [_vp, _i, _i, _i, _i, _i, _vp]),
     ("tng_op_set_local_matrix", _i, [_vp, _i, _i, _i, _i, _i, _vp]),
+    ("tng_epsdivdiv_apply_noalloc", _i, [_vp, _vp, _vp]),
     ("tng_supg_create", _i, [_vp, _vp, _d, _d, _i, _i, _d, _i, _pv]),
     ("tng_supg_set_dt", _i, [_vp, _d]),
     ("tng_supg_set_enabled", _i, [_vp, _i]),
