@@ -165,6 +165,89 @@ def adjoint_leg(args, tng, inputs, np):
             "gradient_norm": float(np.linalg.norm(g.download()))}
 
 
+def largest_leg(args, tng, inputs, np):
+    """The largest single-GPU case: velocity level 7 / pressure level 6, free slip on both surfaces,
+    viscosity 1e2 radial jump x 1e2 plume columns x lognormal(1), buoyancy RHS; FGMRES(30) with the FP32
+    and with the FP64 V-cycle to the tolerance both reach (the free-slip constant-pressure mode stalls
+    restarted FGMRES near 1e-5 at this size), then one adjoint viscosity-gradient evaluation (J = 1/2 |u|^2)
+    from the kept FP32-MG forward solution, Jacobians recomputed."""
+    lv = args.largest_level
+    F = tng.FREESLIP
+    out = {"workload": f"Stokes solve, velocity level {lv} / pressure level {lv - 1}, free slip both surfaces, "
+                       f"eta = 1e2 radial jump x 1e2 plume columns x lognormal(1), buoyancy RHS, FGMRES(30) to "
+                       f"{args.largest_rtol:g} rel, CGS2, Chebyshev order 3, re-discretised coarse operators; "
+                       f"then one adjoint gradient (FP32-MG solver)"}
+    fields = {}
+    for name, dt in (("fp64_mg", tng.F64), ("fp32_mg", tng.F32)):
+        prm = tng.StokesParams.defaults(krylov_restart=30, krylov_max_iterations=args.largest_max_iterations,
+                                        krylov_relative_tolerance=args.largest_rtol, krylov_absolute_tolerance=1e-30,
+                                        viscous_pc_chebyshev_order=3, viscous_pc_dtype=dt, krylov_orthogonalization=1)
+        print(f"[bench] largest {name}: building levels 2..{lv}", file=sys.stderr, flush=True)
+        S = tng.StokesSolver(2, lv, 0.55, 1.0, (F, F), prm)
+        dv, dp = S.domain_velocity, S.domain_pressure
+        if not fields:
+            w = inputs.Workload(dv.coords(), dv.radii(), lv, lv)
+            fields["eta"] = w.eta_plumes(105)
+            fields["T"] = w.temperature_harmonics(104, 8)
+            del w
+        S.set_viscosity(tng.VectorQ1Scalar(dv).upload(fields["eta"]))
+        fu = tng.VectorQ1Vec(dv)
+        S.buoyancy_rhs(tng.VectorQ1Scalar(dv).upload(fields["T"]), 1e5, fu)
+        u, p = tng.VectorQ1Vec(dv), tng.VectorQ1Scalar(dp)
+        dv.ctx.synchronize()
+        t0 = time.perf_counter()
+        rep = S.solve(u, p, fu, tng.VectorQ1Scalar(dp))
+        dv.ctx.synchronize()
+        sec = time.perf_counter() - t0
+        out[name] = {"seconds": round(sec, 4), "iterations": rep["iterations"], "converged": bool(rep["converged"]),
+                     "relative_residual": rep["relative_residual"]}
+        print(f"[bench] largest {name}: {sec:.3f} s, {rep['iterations']} iterations, "
+              f"rel residual {rep['relative_residual']:.3e}", file=sys.stderr, flush=True)
+        if name == "fp32_mg":
+            g = tng.VectorQ1Scalar(dv)
+            t1 = time.perf_counter()
+            arep = S.adjoint_gradient(u, u, g)
+            dv.ctx.synchronize()
+            out["adjoint"] = {"seconds": round(time.perf_counter() - t1, 4), "iterations": arep["iterations"],
+                              "relative_residual": arep["relative_residual"],
+                              "gradient_norm": float(np.linalg.norm(g.download()))}
+        del S, u, p, fu
+    out["speedup_fp32_mg"] = round(out["fp64_mg"]["seconds"] / out["fp32_mg"]["seconds"], 3)
+    return out
+
+
+def blocks_leg(args, tng, np):
+    """Divergence B (velocity -> pressure) and gradient B^T (pressure -> velocity) of the solve's top level
+    (velocity level L, pressure level L - 1), free slip on both surfaces, FP64 and FP32: seconds per apply
+    (wall clock over 20 applies after 3 warm-ups, one synchronisation) and the algorithmic bytes (24 B read per
+    velocity node copy and 8 B written per pressure node copy for B; the reverse for B^T)."""
+    lv = args.solve_level
+    df = tng.DistributedDomain.create_uniform(lv, lv, 0.55, 1.0)
+    dc = tng.DistributedDomain.create_uniform(lv - 1, lv - 1, 0.55, 1.0)
+    F = tng.FREESLIP
+    out = {"workload": f"B and B^T, velocity level {lv} ({df.num_alloc} node copies) / pressure level {lv - 1} "
+                       f"({dc.num_alloc}), free slip both surfaces"}
+    for name, dt, es in (("f64", tng.F64, 8), ("f32", tng.F32, 4)):
+        u, p = tng.VectorQ1Vec(df, dt), tng.VectorQ1Scalar(dc, dt)
+        tng.randomize(u, 1)
+        tng.randomize(p, 2)
+        ob, og = tng.VectorQ1Scalar(dc, dt), tng.VectorQ1Vec(df, dt)
+        B, Bt = tng.DivergenceKerngen(df, dc, (F, F)), tng.GradientKerngen(df, dc, (F, F))
+        alg = 3 * es * df.num_alloc + es * dc.num_alloc
+        for key, f in (("B", lambda: B.apply(u, ob)), ("Bt", lambda: Bt.apply(p, og))):
+            for _ in range(3):
+                f()
+            df.ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(20):
+                f()
+            df.ctx.synchronize()
+            sec = (time.perf_counter() - t0) / 20
+            out[f"{key}_{name}"] = {"ms_per_apply": round(sec * 1e3, 4), "algorithmic_GBps": round(alg / sec / 1e9, 1)}
+        del u, p, ob, og, B, Bt
+    return out
+
+
 def io_leg(args, tng, np):
     """XDMF output (io/xdmf.hpp) of a velocity-level field set: FP64 velocity and viscosity written in
     FP32 (the reference's default field precision), node coordinates FP32, connectivity int64."""
@@ -220,14 +303,19 @@ def main():
                     help="1: the Galerkin operator is used on the coarsest level only")
     ap.add_argument("--solve-gca-exact", type=int, default=0)
     ap.add_argument("--solve-smoother", type=int, default=0, help="0 Chebyshev, 1 damped Jacobi")
-    ap.add_argument("--solve-orth", type=int, default=0, help="0 modified Gram-Schmidt, 1 CGS2")
+    ap.add_argument("--solve-orth", type=int, default=1, help="0 modified Gram-Schmidt, 1 CGS2")
     ap.add_argument("--only-solve", action="store_true", help="run the Stokes solve leg only")
+    ap.add_argument("--only-blocks", action="store_true", help="time B and B^T of the solve's top level only")
     ap.add_argument("--solve-restart", type=int, default=30)
     ap.add_argument("--solve-max-iterations", type=int, default=300)
     ap.add_argument("--solve-rtol", type=float, default=1e-8)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the viscosity adjoint timing")
     ap.add_argument("--adjoint-level", type=int, default=5)
     ap.add_argument("--adjoint-rtol", type=float, default=1e-8)
+    ap.add_argument("--no-largest", action="store_true", help="skip the largest-case solve + adjoint")
+    ap.add_argument("--largest-level", type=int, default=7)
+    ap.add_argument("--largest-rtol", type=float, default=3e-5)
+    ap.add_argument("--largest-max-iterations", type=int, default=600)
     ap.add_argument("--io", action="store_true", help="also time XDMF output of a field set")
     ap.add_argument("--io-level", type=int, default=7)
     ap.add_argument("--io-steps", type=int, default=3)
@@ -247,6 +335,9 @@ def main():
 
     if args.only_solve:
         print(json.dumps({"solve": solve_leg(args, tng, inputs, np)}))
+        return
+    if args.only_blocks:
+        print(json.dumps({"blocks": blocks_leg(args, tng, np)}))
         return
     L = args.level
     bc = tng.FREESLIP if args.bcs == "freeslip" else tng.DIRICHLET
@@ -392,6 +483,11 @@ def main():
             result["adjoint"] = adjoint_leg(args, tng, inputs, np)
         except Exception as e:  # the apply measurement stands on its own
             result["adjoint"] = {"error": f"{type(e).__name__}: {e}"}
+    if not args.no_largest:
+        try:
+            result["largest"] = largest_leg(args, tng, inputs, np)
+        except Exception as e:  # the apply measurement stands on its own
+            result["largest"] = {"error": f"{type(e).__name__}: {e}"}
     if args.io:
         print(f"[bench] io: XDMF output at level {args.io_level}", file=sys.stderr, flush=True)
         result["io"] = io_leg(args, tng, np)

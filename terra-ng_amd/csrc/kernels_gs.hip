@@ -27,82 +27,111 @@ __device__ __forceinline__ double gs_wsum(double v)
 }
 
 // Rows: the velocity part (3 components per node copy) then the pressure part
-// (1 component).  Slot i of the basis table holds 4 component pointers.
-template <int JB>
-__global__ __launch_bounds__(256) void gs_dots_kernel(const GsArgs a, double *partials)
+// (1 component), in two loops so that the component of every access is known
+// at compile time and the basis pointers are wave-uniform (scalar loads).
+// Slot i of the basis table holds 4 component pointers.  Workgroup row
+// blockIdx.y takes the basis vectors 8 y .. 8 y + 7 (19 sums per thread); the
+// first row also forms u.u, u.w, w.w.
+constexpr int kGsChunk = 8, kGsND = 2 * kGsChunk + 3, kGsMaxChunks = 4;
+
+template <int C0, int NC>
+__device__ __forceinline__ void gs_dots_rows(const GsArgs &a, const uint8_t *mask, int64_t nrows, int64_t off,
+                                             int i0, int nq, bool first, double (&acc)[kGsND])
 {
-    constexpr int ND = 2 * JB + 3;
-    __shared__ double red[4][ND];
-    double acc[ND];
+    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < nrows; n += (int64_t)gridDim.x * blockDim.x) {
+        double v[kGsND];
 #pragma unroll
-    for (int k = 0; k < ND; ++k) acc[k] = 0.0;
-    const int j = a.j;
-    const int64_t nrows = a.nv + a.np;
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += (int64_t)gridDim.x * blockDim.x) {
-        const bool vel = r < a.nv;
-        const int64_t n = vel ? r : r - a.nv;
-        const bool own = (vel ? a.mask_v[n] : a.mask_p[n]) == OWNED;
-        const int c0 = vel ? 0 : 3, c1 = vel ? 3 : 4;
-        double v[ND];
+        for (int k = 0; k < kGsND; ++k) v[k] = 0.0;
 #pragma unroll
-        for (int k = 0; k < ND; ++k) v[k] = 0.0;
-        for (int c = c0; c < c1; ++c) {
+        for (int c = C0; c < C0 + NC; ++c) {
             const double wc = a.w[c][n];
-            const double uc = a.V[4 * j + c][n];
+            const double uc = a.V[4 * a.j + c][n];
 #pragma unroll
-            for (int i = 0; i < JB; ++i) {
-                if (i < j) {
-                    const double q = a.V[4 * i + c][n];
+            for (int i = 0; i < kGsChunk; ++i) {
+                if (i < nq) {
+                    const double q = a.V[4 * (i0 + i) + c][n];
                     v[i] = fma(q, uc, v[i]);
-                    v[JB + i] = fma(q, wc, v[JB + i]);
+                    v[kGsChunk + i] = fma(q, wc, v[kGsChunk + i]);
                 }
             }
-            v[2 * JB] = fma(uc, uc, v[2 * JB]);
-            v[2 * JB + 1] = fma(uc, wc, v[2 * JB + 1]);
-            v[2 * JB + 2] = fma(wc, wc, v[2 * JB + 2]);
+            if (first) {
+                v[2 * kGsChunk] = fma(uc, uc, v[2 * kGsChunk]);
+                v[2 * kGsChunk + 1] = fma(uc, wc, v[2 * kGsChunk + 1]);
+                v[2 * kGsChunk + 2] = fma(wc, wc, v[2 * kGsChunk + 2]);
+            }
         }
-        if (own) {
+        if (mask[n] == OWNED) {
 #pragma unroll
-            for (int k = 0; k < ND; ++k) acc[k] += v[k];
+            for (int k = 0; k < kGsND; ++k) acc[k] += v[k];
         }
     }
+    (void)off;
+}
+
+__global__ __launch_bounds__(256) void gs_dots_kernel(const GsArgs a, double *partials)
+{
+    __shared__ double red[4][kGsND];
+    const int i0 = blockIdx.y * kGsChunk;
+    const int nq = a.j - i0;  // basis vectors of this row of workgroups: min(8, nq)
+    const bool first = blockIdx.y == 0;
+    double acc[kGsND];
+#pragma unroll
+    for (int k = 0; k < kGsND; ++k) acc[k] = 0.0;
+    gs_dots_rows<0, 3>(a, a.mask_v, a.nv, 0, i0, nq, first, acc);
+    gs_dots_rows<3, 1>(a, a.mask_p, a.np, 0, i0, nq, first, acc);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-    for (int k = 0; k < ND; ++k) {
+    for (int k = 0; k < kGsND; ++k) {
         const double s = gs_wsum(acc[k]);
         if (lane == 0) red[wave][k] = s;
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < ND; k += blockDim.x)
-        partials[(int64_t)blockIdx.x * ND + k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+    double *o = partials + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * kGsND;
+    for (int k = threadIdx.x; k < kGsND; k += blockDim.x) o[k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
 }
 
-// Sums the per-block partials of every dot in a fixed order.
-__global__ __launch_bounds__(256) void gs_final_kernel(const double *partials, int nb, int nd, double *out)
+// Sums the per-workgroup partials of every dot in a fixed order into
+// out[0 .. 2B+2]: Q^T u (B slots), Q^T w (B slots), u.u, u.w, w.w.
+__global__ __launch_bounds__(256) void gs_final_kernel(const double *partials, int nbx, int nch, int B, double *out)
 {
-    for (int k = threadIdx.x; k < nd; k += blockDim.x) {
+    for (int o = threadIdx.x; o < 2 * B + 3; o += blockDim.x) {
+        int ch, k;
+        if (o < B) {
+            ch = o / kGsChunk;
+            k = o % kGsChunk;
+        } else if (o < 2 * B) {
+            ch = (o - B) / kGsChunk;
+            k = kGsChunk + (o - B) % kGsChunk;
+        } else {
+            ch = 0;
+            k = 2 * kGsChunk + (o - 2 * B);
+        }
         double s = 0.0;
-        for (int b = 0; b < nb; ++b) s += partials[(int64_t)b * nd + k];
-        out[k] = s;
+        if (ch < nch)
+            for (int b = 0; b < nbx; ++b) s += partials[((int64_t)ch * nbx + b) * kGsND + k];
+        out[o] = s;
     }
 }
 
-template <int JB> __global__ __launch_bounds__(256) void gs_update_kernel(const GsArgs a, const GsCoef cf)
+// The basis vectors in runs of 8 (their pointers and coefficients are
+// wave-uniform scalar loads per run).
+template <int C0, int NC>
+__device__ __forceinline__ void gs_update_rows(const GsArgs &a, const GsCoef &cf, int64_t nrows)
 {
     const int j = a.j;
-    const int64_t nrows = a.nv + a.np;
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += (int64_t)gridDim.x * blockDim.x) {
-        const bool vel = r < a.nv;
-        const int64_t n = vel ? r : r - a.nv;
-        const int c0 = vel ? 0 : 3, c1 = vel ? 3 : 4;
-        for (int c = c0; c < c1; ++c) {
-            double ts = 0.0, tc = 0.0;
+    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < nrows; n += (int64_t)gridDim.x * blockDim.x) {
 #pragma unroll
-            for (int i = 0; i < JB; ++i) {
-                if (i < j) {
-                    const double q = a.V[4 * i + c][n];
-                    ts = fma(cf.s[i], q, ts);
-                    tc = fma(cf.c[i], q, tc);
+        for (int c = C0; c < C0 + NC; ++c) {
+            double ts = 0.0, tc = 0.0;
+            for (int i0 = 0; i0 < j; i0 += 8) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int i = i0 + k;
+                    if (i < j) {
+                        const double q = a.V[4 * i + c][n];
+                        ts = fma(cf.s[i], q, ts);
+                        tc = fma(cf.c[i], q, tc);
+                    }
                 }
             }
             double *uslot = a.V[4 * j + c];
@@ -111,6 +140,12 @@ template <int JB> __global__ __launch_bounds__(256) void gs_update_kernel(const 
             a.V[4 * (j + 1) + c][n] = ((wc - tc) - cf.r * u) * cf.inv_nu;
         }
     }
+}
+
+__global__ __launch_bounds__(256) void gs_update_kernel(const GsArgs a, const GsCoef cf)
+{
+    gs_update_rows<0, 3>(a, cf, a.nv);
+    gs_update_rows<3, 1>(a, cf, a.np);
 }
 
 // y += sum_{i < k} coef_i X_i over the Stokes rows (X_i: table slots 0..k-1)
@@ -150,40 +185,28 @@ __global__ __launch_bounds__(256) void rotation_field_kernel(double *v0, double 
 
 constexpr int kGsBlocks = 512;
 
-template <int JB> void gs_dots_jb(const GsArgs &a, double *partials, double *out, hipStream_t s)
-{
-    hipLaunchKernelGGL(gs_dots_kernel<JB>, dim3(kGsBlocks), dim3(256), 0, s, a, partials);
-    hipLaunchKernelGGL(gs_final_kernel, dim3(1), dim3(256), 0, s, (const double *)partials, kGsBlocks, 2 * JB + 3,
-                       out);
-}
-
 }  // namespace
 
 int gs_bucket(int j) { return j <= 8 ? 8 : (j <= 16 ? 16 : 32); }
 
-size_t gs_partials_doubles() { return (size_t)kGsBlocks * (2 * 32 + 3); }
+size_t gs_partials_doubles() { return (size_t)kGsBlocks * kGsMaxChunks * kGsND; }
 
 void gs_dots(const GsArgs &a, double *partials, double *out, hipStream_t s)
 {
-    TNG_REQUIRE(a.j >= 0 && a.j <= 32, "Gram-Schmidt: at most 32 basis vectors");
-    switch (gs_bucket(a.j)) {
-    case 8: gs_dots_jb<8>(a, partials, out, s); break;
-    case 16: gs_dots_jb<16>(a, partials, out, s); break;
-    default: gs_dots_jb<32>(a, partials, out, s); break;
-    }
+    TNG_REQUIRE(a.j >= 0 && a.j <= kGsChunk * kGsMaxChunks, "Gram-Schmidt: at most 32 basis vectors");
+    const int nch = std::max(1, (a.j + kGsChunk - 1) / kGsChunk);
+    hipLaunchKernelGGL(gs_dots_kernel, dim3(kGsBlocks, nch), dim3(256), 0, s, a, partials);
+    hipLaunchKernelGGL(gs_final_kernel, dim3(1), dim3(256), 0, s, (const double *)partials, kGsBlocks, nch,
+                       gs_bucket(a.j), out);
     TNG_HIP_CHECK(hipGetLastError());
 }
 
 void gs_update(const GsArgs &a, const GsCoef &cf, hipStream_t s)
 {
     TNG_REQUIRE(a.j >= 0 && a.j <= 32, "Gram-Schmidt: at most 32 basis vectors");
-    const int64_t rows = a.nv + a.np;
+    const int64_t rows = std::max(a.nv, a.np);
     const unsigned g = (unsigned)std::min<int64_t>((rows + 255) / 256, 4096);
-    switch (gs_bucket(a.j)) {
-    case 8: hipLaunchKernelGGL(gs_update_kernel<8>, dim3(g), dim3(256), 0, s, a, cf); break;
-    case 16: hipLaunchKernelGGL(gs_update_kernel<16>, dim3(g), dim3(256), 0, s, a, cf); break;
-    default: hipLaunchKernelGGL(gs_update_kernel<32>, dim3(g), dim3(256), 0, s, a, cf); break;
-    }
+    hipLaunchKernelGGL(gs_update_kernel, dim3(g), dim3(256), 0, s, a, cf);
     TNG_HIP_CHECK(hipGetLastError());
 }
 
