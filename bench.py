@@ -71,6 +71,7 @@ def solve_leg(args, tng, inputs, np):
         coarse = f"GCA (gca={args.solve_gca}): {where}, {rule}"
     smoother = "damped Jacobi" if args.solve_smoother == 1 else "Chebyshev order 3"
     orth = "CGS2 orthogonalisation" if args.solve_orth == 1 else "modified Gram-Schmidt"
+    orth += ", direct coarsest-level solve" if args.solve_coarse == 1 else ", PCG coarsest-level solve"
     out = {"workload": f"Stokes solve, velocity level {lv} / pressure level {lv - 1}, multigrid levels "
                        f"{args.solve_min_level}..{lv}, no-slip both surfaces, eta = 1e4 radial contrast x "
                        f"lognormal(1), buoyancy RHS (T: spherical harmonics l<=8), FGMRES({args.solve_restart}) "
@@ -83,7 +84,7 @@ def solve_leg(args, tng, inputs, np):
                                         viscous_pc_chebyshev_order=3, viscous_pc_dtype=dt, gca=args.solve_gca,
                                         gca_coarsest_only=args.solve_gca_coarsest_only,
                                         gca_exact=args.solve_gca_exact, viscous_pc_smoother=args.solve_smoother,
-                                        krylov_orthogonalization=args.solve_orth)
+                                        krylov_orthogonalization=args.solve_orth, coarse_solver=args.solve_coarse)
         print(f"[bench] solve {name}: building levels {args.solve_min_level}..{lv}", file=sys.stderr, flush=True)
         c0 = time.perf_counter()
         S = tng.StokesSolver(args.solve_min_level, lv, 0.55, 1.0, (tng.DIRICHLET, tng.DIRICHLET), prm)
@@ -304,6 +305,7 @@ def main():
     ap.add_argument("--solve-gca-exact", type=int, default=0)
     ap.add_argument("--solve-smoother", type=int, default=0, help="0 Chebyshev, 1 damped Jacobi")
     ap.add_argument("--solve-orth", type=int, default=1, help="0 modified Gram-Schmidt, 1 CGS2")
+    ap.add_argument("--solve-coarse", type=int, default=1, help="0 PCG on the coarsest level, 1 direct")
     ap.add_argument("--only-solve", action="store_true", help="run the Stokes solve leg only")
     ap.add_argument("--only-blocks", action="store_true", help="time B and B^T of the solve's top level only")
     ap.add_argument("--solve-restart", type=int, default=30)

@@ -366,6 +366,10 @@ typedef struct {
     int rotation_correction; /* 0; 1 (free slip on both surfaces): after the block preconditioner, a
                                 correction on the three rigid rotations, the near-null modes of the
                                 penalised free-slip operator: z += Z E^{-1} Z^T (v - K z), E = Z^T K Z */
+    int coarse_solver;       /* 0: PCG on the coarsest level (pcg.hpp, the coarse_* fields); 1: direct,
+                                the coarsest operator (boundary rows and penalty included) inverted
+                                densely over the unique node components when the viscosity is set (at
+                                most 6000 unknowns: a lateral level-2 coarsest level has 2430) */
 } tng_stokes_params;
 
 /* Outcome of a solve.  history (caller memory, may be NULL) receives the

@@ -91,26 +91,31 @@ __global__ __launch_bounds__(256) void gs_dots_kernel(const GsArgs a, double *pa
 }
 
 // Sums the per-workgroup partials of every dot in a fixed order into
-// out[0 .. 2B+2]: Q^T u (B slots), Q^T w (B slots), u.u, u.w, w.w.
+// out[0 .. 2B+2]: Q^T u (B slots), Q^T w (B slots), u.u, u.w, w.w.  One
+// workgroup per output; each thread sums a fixed strided subset, then a fixed
+// tree over the threads.
 __global__ __launch_bounds__(256) void gs_final_kernel(const double *partials, int nbx, int nch, int B, double *out)
 {
-    for (int o = threadIdx.x; o < 2 * B + 3; o += blockDim.x) {
-        int ch, k;
-        if (o < B) {
-            ch = o / kGsChunk;
-            k = o % kGsChunk;
-        } else if (o < 2 * B) {
-            ch = (o - B) / kGsChunk;
-            k = kGsChunk + (o - B) % kGsChunk;
-        } else {
-            ch = 0;
-            k = 2 * kGsChunk + (o - 2 * B);
-        }
-        double s = 0.0;
-        if (ch < nch)
-            for (int b = 0; b < nbx; ++b) s += partials[((int64_t)ch * nbx + b) * kGsND + k];
-        out[o] = s;
+    __shared__ double red[4];
+    const int o = blockIdx.x;
+    int ch, k;
+    if (o < B) {
+        ch = o / kGsChunk;
+        k = o % kGsChunk;
+    } else if (o < 2 * B) {
+        ch = (o - B) / kGsChunk;
+        k = kGsChunk + (o - B) % kGsChunk;
+    } else {
+        ch = 0;
+        k = 2 * kGsChunk + (o - 2 * B);
     }
+    double s = 0.0;
+    if (ch < nch)
+        for (int b = threadIdx.x; b < nbx; b += blockDim.x) s += partials[((int64_t)ch * nbx + b) * kGsND + k];
+    s = gs_wsum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[o] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 // The basis vectors in runs of 8 (their pointers and coefficients are
@@ -196,8 +201,8 @@ void gs_dots(const GsArgs &a, double *partials, double *out, hipStream_t s)
     TNG_REQUIRE(a.j >= 0 && a.j <= kGsChunk * kGsMaxChunks, "Gram-Schmidt: at most 32 basis vectors");
     const int nch = std::max(1, (a.j + kGsChunk - 1) / kGsChunk);
     hipLaunchKernelGGL(gs_dots_kernel, dim3(kGsBlocks, nch), dim3(256), 0, s, a, partials);
-    hipLaunchKernelGGL(gs_final_kernel, dim3(1), dim3(256), 0, s, (const double *)partials, kGsBlocks, nch,
-                       gs_bucket(a.j), out);
+    hipLaunchKernelGGL(gs_final_kernel, dim3(2 * gs_bucket(a.j) + 3), dim3(256), 0, s, (const double *)partials,
+                       kGsBlocks, nch, gs_bucket(a.j), out);
     TNG_HIP_CHECK(hipGetLastError());
 }
 

@@ -198,6 +198,12 @@ int pcg_solve(tng_operator &A, tng_vector &x, const tng_vector &b, tng_vector &r
 // ---------------------------------------------------------------------------
 void Multigrid::coarse_solve(tng_vector &e0, const tng_vector &r0, bool x_zero)
 {
+    if (use_direct) {
+        // exact solve: no initial guess to use
+        direct.solve(e0, r0, stream_of(e0));
+        coarse_iterations += 1;
+        return;
+    }
     // The coarsest solve runs in FP64 whatever the precision of the levels.
     tng_vector &x = e0.dtype == TNG_F64 ? e0 : *cg_x;
     const tng_vector *b = &r0;
@@ -255,6 +261,7 @@ StokesSolver::StokesSolver(tng_context *ctx, int min_level, int max_level, int r
     TNG_REQUIRE(prm.rotation_correction == 0 ||
                     (prm.rotation_correction == 1 && bc_cmb_ == TNG_FREESLIP && bc_surf_ == TNG_FREESLIP),
                 "rotation_correction needs free slip on both surfaces");
+    TNG_REQUIRE(prm.coarse_solver == 0 || prm.coarse_solver == 1, "coarse_solver must be 0 (PCG) or 1 (direct)");
     TNG_REQUIRE(prm.rotation_correction == 0 || prm.krylov_restart <= 64,
                 "rotation_correction supports krylov_restart <= 64");
     for (int l = min_level; l <= max_level; ++l) {
@@ -334,6 +341,7 @@ StokesSolver::StokesSolver(tng_context *ctx, int min_level, int max_level, int r
         }
     // FP64 coarsest solve
     mg.coarse_max_it = prm.coarse_max_iterations;
+    mg.use_direct = prm.coarse_solver == 1;
     mg.coarse_rtol = prm.coarse_relative_tolerance;
     mg.coarse_atol = prm.coarse_absolute_tolerance;
     if (mgt == TNG_F64) {
@@ -438,6 +446,7 @@ void StokesSolver::viscosity_updated()
         mg.pre[l].need_estimate = true;
         mg.post[l].need_estimate = true;
     }
+    if (mg.use_direct) mg.direct.build(*mg.coarse_op, *mg.cg_tmp[0], *mg.cg_tmp[1], stream_of(*mg.cg_tmp[0]));
     // k_pm = 1 / eta at the pressure level; lumped diagonal, inverted (DiagonalSolver)
     assign(*k_pm, *eta[nl - 2]);
     launch_invert(*k_pm, stream_of(*k_pm));
@@ -1193,6 +1202,7 @@ void tng_stokes_params_default(tng_stokes_params *p)
     p->viscous_pc_smoother = 0;
     p->krylov_orthogonalization = 0;
     p->rotation_correction = 0;
+    p->coarse_solver = 0;
 }
 
 tng_status tng_stokes_solver_create(tng_ctx ctx, int min_level, int max_level, int radial_extra_levels, double r_min,

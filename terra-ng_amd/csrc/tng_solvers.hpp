@@ -4,6 +4,7 @@
 #include <memory>
 #include <vector>
 
+#include "tng_coarse.hpp"
 #include "tng_internal.hpp"
 
 namespace tng {
@@ -48,6 +49,9 @@ struct Multigrid {
     int coarse_max_it = 50;
     double coarse_rtol = 1e-6, coarse_atol = 1e-16;
     long coarse_iterations = 0;
+    // direct coarse solve (coarse_solver = 1), built with the viscosity
+    bool use_direct = false;
+    CoarseDirect direct;
 
     void solve(tng_vector &x, const tng_vector &b, bool x_zero = false);
     void recurse(int level, tng_vector &x, const tng_vector &b, bool x_zero);

@@ -160,7 +160,7 @@ class StokesParams(C.Structure):
                 ("coarse_relative_tolerance", C.c_double), ("coarse_absolute_tolerance", C.c_double),
                 ("seed", C.c_ulonglong), ("gca_coarsest_only", C.c_int), ("gca_exact", C.c_int),
                 ("viscous_pc_smoother", C.c_int), ("krylov_orthogonalization", C.c_int),
-                ("rotation_correction", C.c_int)]
+                ("rotation_correction", C.c_int), ("coarse_solver", C.c_int)]
 
     @classmethod
     def defaults(cls, **kw):

@@ -211,11 +211,13 @@ def _solve_level4(dt, **opts):
 @pytest.mark.parametrize("opts", [dict(gca=1), dict(gca=2), dict(gca=1, gca_exact=1),
                                   dict(gca=1, gca_coarsest_only=1), dict(gca=2, gca_coarsest_only=1, gca_exact=1),
                                   dict(viscous_pc_smoother=1), dict(krylov_orthogonalization=1),
-                                  dict(gca=1, gca_coarsest_only=1, krylov_orthogonalization=1)])
+                                  dict(gca=1, gca_coarsest_only=1, krylov_orthogonalization=1),
+                                  dict(coarse_solver=1), dict(gca=1, gca_coarsest_only=1, coarse_solver=1)])
 def test_stokes_solve_options(dt, opts, gpu_ctx):
     """Galerkin preconditioners (all coarse levels, adaptive, exact product,
-    coarsest level only), the damped-Jacobi smoother and the two-pass CGS2
-    orthogonalisation reach the solution of the re-discretised FP64 default."""
+    coarsest level only), the damped-Jacobi smoother, the two-pass CGS2
+    orthogonalisation and the direct coarsest-level solve reach the solution
+    of the re-discretised FP64 default."""
     rep0, u0, p0, of = _solve_level4(tng.F64)
     rep, u, p, _ = _solve_level4(dt, **opts)
     assert rep0["converged"] == 1 and rep["converged"] == 1, (opts, rep)
