@@ -176,13 +176,15 @@ def largest_leg(args, tng, inputs, np):
     F = tng.FREESLIP
     out = {"workload": f"Stokes solve, velocity level {lv} / pressure level {lv - 1}, free slip both surfaces, "
                        f"eta = 1e2 radial jump x 1e2 plume columns x lognormal(1), buoyancy RHS, FGMRES(30) to "
-                       f"{args.largest_rtol:g} rel, CGS2, Chebyshev order 3, re-discretised coarse operators; "
+                       f"{args.largest_rtol:g} rel, CGS2, Chebyshev order 3, re-discretised coarse operators, "
+                       + ("direct" if args.solve_coarse == 1 else "PCG") + " coarsest-level solve; "
                        f"then one adjoint gradient (FP32-MG solver)"}
     fields = {}
     for name, dt in (("fp64_mg", tng.F64), ("fp32_mg", tng.F32)):
         prm = tng.StokesParams.defaults(krylov_restart=30, krylov_max_iterations=args.largest_max_iterations,
                                         krylov_relative_tolerance=args.largest_rtol, krylov_absolute_tolerance=1e-30,
-                                        viscous_pc_chebyshev_order=3, viscous_pc_dtype=dt, krylov_orthogonalization=1)
+                                        viscous_pc_chebyshev_order=3, viscous_pc_dtype=dt, krylov_orthogonalization=1,
+                                        coarse_solver=args.solve_coarse)
         print(f"[bench] largest {name}: building levels 2..{lv}", file=sys.stderr, flush=True)
         S = tng.StokesSolver(2, lv, 0.55, 1.0, (F, F), prm)
         dv, dp = S.domain_velocity, S.domain_pressure

@@ -67,6 +67,44 @@ __global__ __launch_bounds__(256) void gj_update_kernel(double *M, int n, int k,
     M[e] = v;
 }
 
+// U = M Z and V = M^T W (n x 3 each, row-major), one thread per row
+__global__ __launch_bounds__(256) void proj_vectors_kernel(const double *M, int n, const double *Z, const double *W,
+                                                           double *U, double *V)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double u[3] = {0.0, 0.0, 0.0}, v[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < n; ++j) {
+        const double mij = M[(int64_t)i * n + j], mji = M[(int64_t)j * n + i];
+        for (int k = 0; k < 3; ++k) {
+            u[k] = fma(mij, Z[3 * j + k], u[k]);
+            v[k] = fma(mji, W[3 * j + k], v[k]);
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        U[3 * i + k] = u[k];
+        V[3 * i + k] = v[k];
+    }
+}
+
+struct Proj3 {
+    double C[9];  // W^T M Z
+};
+
+// M <- P M P with P = I - Z W^T:  M - Z V^T - U W^T + Z C W^T
+__global__ __launch_bounds__(256) void proj_update_kernel(double *M, int n, const double *Z, const double *W,
+                                                          const double *U, const double *V, const Proj3 c)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)n * n) return;
+    const int i = (int)(e / n), j = (int)(e - (int64_t)i * n);
+    double s = M[e];
+    for (int k = 0; k < 3; ++k) s -= Z[3 * i + k] * V[3 * j + k] + U[3 * i + k] * W[3 * j + k];
+    for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l) s += Z[3 * i + k] * c.C[3 * k + l] * W[3 * j + l];
+    M[e] = s;
+}
+
 __global__ __launch_bounds__(256) void diag_kernel(const double *M, int n, double *d)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -151,7 +189,7 @@ void CoarseDirect::numbering(const tng_domain &m)
     d_x.alloc(n * sizeof(double));
 }
 
-void CoarseDirect::build(tng_operator &A, tng_vector &x, tng_vector &y, hipStream_t s)
+void CoarseDirect::build(tng_operator &A, tng_vector &x, tng_vector &y, bool fp32_levels, hipStream_t s)
 {
     TNG_REQUIRE(x.dtype == TNG_F64 && y.dtype == TNG_F64 && x.ncomp == 3 && y.ncomp == 3,
                 "direct coarse solve: FP64 work vectors required");
@@ -181,7 +219,82 @@ void CoarseDirect::build(tng_operator &A, tng_vector &x, tng_vector &y, hipStrea
     for (int i = 0; i < n; ++i)
         TNG_REQUIRE(std::isfinite(diag[i]) && diag[i] > 0.0,
                     "direct coarse solve: the coarsest operator is not positive definite");
+    if (A.penalty_active && fp32_levels) project_rotations(*A.mesh, s);
     ready = true;
+}
+
+// With the fs/fs penalty the three rigid rotations are near-null modes whose
+// inverse gain is 1 / eps.  FP32 levels do not resolve the eps-sized
+// penalty, so the rotational part of their restricted residual is rounding
+// noise that an exact inverse would amplify: there the coarse correction is
+// taken in the rotations' Euclidean complement, M <- P M P with
+// P = I - Z (Z^T Z)^{-1} Z^T (Z: the rotations e_k x x at the unique nodes),
+// and the rotations are left to the fine levels and the Krylov method.
+void CoarseDirect::project_rotations(const tng_domain &m, hipStream_t s)
+{
+    std::vector<int32_t> rep(nu);
+    TNG_HIP_CHECK(hipMemcpy(rep.data(), d_rep.p, nu * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<double> Z((size_t)n * 3), W((size_t)n * 3);
+    for (int u = 0; u < nu; ++u) {
+        const int64_t i = rep[u], lcol = i / m.nr;
+        const int l = (int)(i - lcol * m.nr);
+        const int64_t sd = lcol / ((int64_t)m.nx * m.nx);
+        const double r = m.radii[sd * m.nr + l];
+        const double x[3] = {m.coords[3 * lcol] * r, m.coords[3 * lcol + 1] * r, m.coords[3 * lcol + 2] * r};
+        for (int k = 0; k < 3; ++k) {
+            // e_k x x, component c at row 3 u + c
+            const double e[3] = {k == 0 ? 1.0 : 0.0, k == 1 ? 1.0 : 0.0, k == 2 ? 1.0 : 0.0};
+            const double z[3] = {e[1] * x[2] - e[2] * x[1], e[2] * x[0] - e[0] * x[2], e[0] * x[1] - e[1] * x[0]};
+            for (int c = 0; c < 3; ++c) Z[(size_t)(3 * u + c) * 3 + k] = z[c];
+        }
+    }
+    // W = Z (Z^T Z)^{-1}
+    double G[9] = {0}, Gi[9];
+    for (int r = 0; r < n; ++r)
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) G[3 * a + b] += Z[(size_t)r * 3 + a] * Z[(size_t)r * 3 + b];
+    const double det = G[0] * (G[4] * G[8] - G[5] * G[7]) - G[1] * (G[3] * G[8] - G[5] * G[6]) +
+                       G[2] * (G[3] * G[7] - G[4] * G[6]);
+    TNG_REQUIRE(std::isfinite(det) && det > 0.0, "direct coarse solve: degenerate rotation fields");
+    Gi[0] = (G[4] * G[8] - G[5] * G[7]) / det;
+    Gi[1] = (G[2] * G[7] - G[1] * G[8]) / det;
+    Gi[2] = (G[1] * G[5] - G[2] * G[4]) / det;
+    Gi[3] = (G[5] * G[6] - G[3] * G[8]) / det;
+    Gi[4] = (G[0] * G[8] - G[2] * G[6]) / det;
+    Gi[5] = (G[2] * G[3] - G[0] * G[5]) / det;
+    Gi[6] = (G[3] * G[7] - G[4] * G[6]) / det;
+    Gi[7] = (G[1] * G[6] - G[0] * G[7]) / det;
+    Gi[8] = (G[0] * G[4] - G[1] * G[3]) / det;
+    for (int r = 0; r < n; ++r)
+        for (int a = 0; a < 3; ++a) {
+            double v = 0.0;
+            for (int b = 0; b < 3; ++b) v += Z[(size_t)r * 3 + b] * Gi[3 * b + a];
+            W[(size_t)r * 3 + a] = v;
+        }
+    DevBuf dZ, dW, dU, dV;
+    dZ.alloc(Z.size() * sizeof(double));
+    dW.alloc(W.size() * sizeof(double));
+    dU.alloc(Z.size() * sizeof(double));
+    dV.alloc(Z.size() * sizeof(double));
+    TNG_HIP_CHECK(hipMemcpyAsync(dZ.p, Z.data(), Z.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    TNG_HIP_CHECK(hipMemcpyAsync(dW.p, W.data(), W.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    double *M = d_M.as<double>();
+    hipLaunchKernelGGL(proj_vectors_kernel, dim3(blocks(n)), dim3(256), 0, s, (const double *)M, n,
+                       (const double *)dZ.p, (const double *)dW.p, dU.as<double>(), dV.as<double>());
+    std::vector<double> U((size_t)n * 3);
+    TNG_HIP_CHECK(hipMemcpyAsync(U.data(), dU.p, U.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    TNG_HIP_CHECK(hipStreamSynchronize(s));
+    Proj3 c{};
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            double v = 0.0;
+            for (int r = 0; r < n; ++r) v += W[(size_t)r * 3 + a] * U[(size_t)r * 3 + b];
+            c.C[3 * a + b] = v;
+        }
+    hipLaunchKernelGGL(proj_update_kernel, dim3(blocks((int64_t)n * n)), dim3(256), 0, s, M, n, (const double *)dZ.p,
+                       (const double *)dW.p, (const double *)dU.p, (const double *)dV.p, c);
+    TNG_HIP_CHECK(hipGetLastError());
+    TNG_HIP_CHECK(hipStreamSynchronize(s));
 }
 
 void CoarseDirect::solve(tng_vector &e0, const tng_vector &r0, hipStream_t s)

@@ -446,7 +446,9 @@ void StokesSolver::viscosity_updated()
         mg.pre[l].need_estimate = true;
         mg.post[l].need_estimate = true;
     }
-    if (mg.use_direct) mg.direct.build(*mg.coarse_op, *mg.cg_tmp[0], *mg.cg_tmp[1], stream_of(*mg.cg_tmp[0]));
+    if (mg.use_direct)
+        mg.direct.build(*mg.coarse_op, *mg.cg_tmp[0], *mg.cg_tmp[1], params.viscous_pc_dtype != TNG_F64,
+                        stream_of(*mg.cg_tmp[0]));
     // k_pm = 1 / eta at the pressure level; lumped diagonal, inverted (DiagonalSolver)
     assign(*k_pm, *eta[nl - 2]);
     launch_invert(*k_pm, stream_of(*k_pm));
