@@ -31,41 +31,66 @@ __device__ __forceinline__ double gs_wsum(double v)
 // at compile time and the basis pointers are wave-uniform (scalar loads).
 // Slot i of the basis table holds 4 component pointers.  Workgroup row
 // blockIdx.y takes the basis vectors 8 y .. 8 y + 7 (19 sums per thread); the
-// first row also forms u.u, u.w, w.w.
+// first row also forms u.u, u.w, w.w.  A thread takes two consecutive rows
+// per iteration with 16-byte loads (component arrays are 256-byte aligned);
+// rows that are not OWNED enter with u = w = 0, so all loads of an iteration
+// are independent of the mask.
 constexpr int kGsChunk = 8, kGsND = 2 * kGsChunk + 3, kGsMaxChunks = 4;
 
+__device__ __forceinline__ double2 ld2(const double *p, int64_t n) { return *reinterpret_cast<const double2 *>(p + n); }
+__device__ __forceinline__ void st2(double *p, int64_t n, double2 v) { *reinterpret_cast<double2 *>(p + n) = v; }
+
 template <int C0, int NC>
-__device__ __forceinline__ void gs_dots_rows(const GsArgs &a, const uint8_t *mask, int64_t nrows, int64_t off,
-                                             int i0, int nq, bool first, double (&acc)[kGsND])
+__device__ __forceinline__ void gs_dots_rows(const GsArgs &a, const uint8_t *mask, int64_t nrows, int i0, int nq,
+                                             bool first, double (&acc)[kGsND])
 {
-    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < nrows; n += (int64_t)gridDim.x * blockDim.x) {
-        double v[kGsND];
-#pragma unroll
-        for (int k = 0; k < kGsND; ++k) v[k] = 0.0;
+    const int64_t npairs = nrows >> 1, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t pr = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; pr < npairs; pr += stride) {
+        const int64_t n = 2 * pr;
+        const unsigned m2 = *reinterpret_cast<const uint16_t *>(mask + n);
+        const bool oa = (m2 & 0xffu) == OWNED, ob = (m2 >> 8) == OWNED;
 #pragma unroll
         for (int c = C0; c < C0 + NC; ++c) {
-            const double wc = a.w[c][n];
-            const double uc = a.V[4 * a.j + c][n];
+            const double2 w2 = ld2(a.w[c], n), u2 = ld2(a.V[4 * a.j + c], n);
+            const double ua = oa ? u2.x : 0.0, ub = ob ? u2.y : 0.0;
+            const double wa = oa ? w2.x : 0.0, wb = ob ? w2.y : 0.0;
+#pragma unroll
+            for (int i = 0; i < kGsChunk; ++i) {
+                if (i < nq) {
+                    const double2 q = ld2(a.V[4 * (i0 + i) + c], n);
+                    acc[i] = fma(q.y, ub, fma(q.x, ua, acc[i]));
+                    acc[kGsChunk + i] = fma(q.y, wb, fma(q.x, wa, acc[kGsChunk + i]));
+                }
+            }
+            if (first) {
+                acc[2 * kGsChunk] = fma(ub, ub, fma(ua, ua, acc[2 * kGsChunk]));
+                acc[2 * kGsChunk + 1] = fma(ub, wb, fma(ua, wa, acc[2 * kGsChunk + 1]));
+                acc[2 * kGsChunk + 2] = fma(wb, wb, fma(wa, wa, acc[2 * kGsChunk + 2]));
+            }
+        }
+    }
+    // an odd last row: the first thread of the grid
+    if ((nrows & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+        const int64_t n = nrows - 1;
+        const bool o = mask[n] == OWNED;
+#pragma unroll
+        for (int c = C0; c < C0 + NC; ++c) {
+            const double u = o ? a.V[4 * a.j + c][n] : 0.0, w = o ? a.w[c][n] : 0.0;
 #pragma unroll
             for (int i = 0; i < kGsChunk; ++i) {
                 if (i < nq) {
                     const double q = a.V[4 * (i0 + i) + c][n];
-                    v[i] = fma(q, uc, v[i]);
-                    v[kGsChunk + i] = fma(q, wc, v[kGsChunk + i]);
+                    acc[i] = fma(q, u, acc[i]);
+                    acc[kGsChunk + i] = fma(q, w, acc[kGsChunk + i]);
                 }
             }
             if (first) {
-                v[2 * kGsChunk] = fma(uc, uc, v[2 * kGsChunk]);
-                v[2 * kGsChunk + 1] = fma(uc, wc, v[2 * kGsChunk + 1]);
-                v[2 * kGsChunk + 2] = fma(wc, wc, v[2 * kGsChunk + 2]);
+                acc[2 * kGsChunk] = fma(u, u, acc[2 * kGsChunk]);
+                acc[2 * kGsChunk + 1] = fma(u, w, acc[2 * kGsChunk + 1]);
+                acc[2 * kGsChunk + 2] = fma(w, w, acc[2 * kGsChunk + 2]);
             }
         }
-        if (mask[n] == OWNED) {
-#pragma unroll
-            for (int k = 0; k < kGsND; ++k) acc[k] += v[k];
-        }
     }
-    (void)off;
 }
 
 __global__ __launch_bounds__(256) void gs_dots_kernel(const GsArgs a, double *partials)
@@ -77,8 +102,8 @@ __global__ __launch_bounds__(256) void gs_dots_kernel(const GsArgs a, double *pa
     double acc[kGsND];
 #pragma unroll
     for (int k = 0; k < kGsND; ++k) acc[k] = 0.0;
-    gs_dots_rows<0, 3>(a, a.mask_v, a.nv, 0, i0, nq, first, acc);
-    gs_dots_rows<3, 1>(a, a.mask_p, a.np, 0, i0, nq, first, acc);
+    gs_dots_rows<0, 3>(a, a.mask_v, a.nv, i0, nq, first, acc);
+    gs_dots_rows<3, 1>(a, a.mask_p, a.np, i0, nq, first, acc);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
     for (int k = 0; k < kGsND; ++k) {
@@ -119,25 +144,47 @@ __global__ __launch_bounds__(256) void gs_final_kernel(const double *partials, i
 }
 
 // The basis vectors in runs of 8 (their pointers and coefficients are
-// wave-uniform scalar loads per run).
+// wave-uniform scalar loads per run); two consecutive rows per thread with
+// 16-byte loads and stores, an odd last row by the first thread.
 template <int C0, int NC>
 __device__ __forceinline__ void gs_update_rows(const GsArgs &a, const GsCoef &cf, int64_t nrows)
 {
     const int j = a.j;
-    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < nrows; n += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t npairs = nrows >> 1, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t pr = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; pr < npairs; pr += stride) {
+        const int64_t n = 2 * pr;
 #pragma unroll
         for (int c = C0; c < C0 + NC; ++c) {
-            double ts = 0.0, tc = 0.0;
+            double tsa = 0.0, tsb = 0.0, tca = 0.0, tcb = 0.0;
             for (int i0 = 0; i0 < j; i0 += 8) {
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
                     const int i = i0 + k;
                     if (i < j) {
-                        const double q = a.V[4 * i + c][n];
-                        ts = fma(cf.s[i], q, ts);
-                        tc = fma(cf.c[i], q, tc);
+                        const double2 q = ld2(a.V[4 * i + c], n);
+                        tsa = fma(cf.s[i], q.x, tsa);
+                        tsb = fma(cf.s[i], q.y, tsb);
+                        tca = fma(cf.c[i], q.x, tca);
+                        tcb = fma(cf.c[i], q.y, tcb);
                     }
                 }
+            }
+            double *uslot = a.V[4 * j + c];
+            const double2 u = ld2(uslot, n), w = ld2(a.w[c], n);
+            st2(uslot, n, make_double2((u.x - tsa) * cf.inv_alpha, (u.y - tsb) * cf.inv_alpha));
+            st2(a.V[4 * (j + 1) + c], n,
+                make_double2(((w.x - tca) - cf.r * u.x) * cf.inv_nu, ((w.y - tcb) - cf.r * u.y) * cf.inv_nu));
+        }
+    }
+    if ((nrows & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+        const int64_t n = nrows - 1;
+#pragma unroll
+        for (int c = C0; c < C0 + NC; ++c) {
+            double ts = 0.0, tc = 0.0;
+            for (int i = 0; i < j; ++i) {
+                const double q = a.V[4 * i + c][n];
+                ts = fma(cf.s[i], q, ts);
+                tc = fma(cf.c[i], q, tc);
             }
             double *uslot = a.V[4 * j + c];
             const double u = uslot[n], wc = a.w[c][n];
@@ -188,7 +235,10 @@ __global__ __launch_bounds__(256) void rotation_field_kernel(double *v0, double 
     }
 }
 
-constexpr int kGsBlocks = 512;
+#ifndef TNG_GS_BLOCKS
+#define TNG_GS_BLOCKS 2048
+#endif
+constexpr int kGsBlocks = TNG_GS_BLOCKS;
 
 }  // namespace
 
@@ -200,17 +250,21 @@ void gs_dots(const GsArgs &a, double *partials, double *out, hipStream_t s)
 {
     TNG_REQUIRE(a.j >= 0 && a.j <= kGsChunk * kGsMaxChunks, "Gram-Schmidt: at most 32 basis vectors");
     const int nch = std::max(1, (a.j + kGsChunk - 1) / kGsChunk);
-    hipLaunchKernelGGL(gs_dots_kernel, dim3(kGsBlocks, nch), dim3(256), 0, s, a, partials);
+    // enough workgroups per chunk row for eight waves per SIMD: with fewer,
+    // the pass is bound by the loads each wave has in flight, not by HBM
+    const int64_t pairs = std::max(a.nv, a.np) / 2;
+    const int nbx = (int)std::max<int64_t>(1, std::min<int64_t>(kGsBlocks, (pairs + 255) / 256));
+    hipLaunchKernelGGL(gs_dots_kernel, dim3(nbx, nch), dim3(256), 0, s, a, partials);
     hipLaunchKernelGGL(gs_final_kernel, dim3(2 * gs_bucket(a.j) + 3), dim3(256), 0, s, (const double *)partials,
-                       kGsBlocks, nch, gs_bucket(a.j), out);
+                       nbx, nch, gs_bucket(a.j), out);
     TNG_HIP_CHECK(hipGetLastError());
 }
 
 void gs_update(const GsArgs &a, const GsCoef &cf, hipStream_t s)
 {
     TNG_REQUIRE(a.j >= 0 && a.j <= 32, "Gram-Schmidt: at most 32 basis vectors");
-    const int64_t rows = std::max(a.nv, a.np);
-    const unsigned g = (unsigned)std::min<int64_t>((rows + 255) / 256, 4096);
+    const int64_t pairs = std::max(a.nv, a.np) / 2;
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((pairs + 255) / 256, 4096));
     hipLaunchKernelGGL(gs_update_kernel, dim3(g), dim3(256), 0, s, a, cf);
     TNG_HIP_CHECK(hipGetLastError());
 }
