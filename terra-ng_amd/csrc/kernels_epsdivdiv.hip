@@ -512,11 +512,10 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
                 rec_stride = nr, rec_ymax = nx - 1;
             }
         }
-        constexpr int rec_len = FUSE ? REC_LEN_FUSE : REC_LEN;
         auto load_record = [&](int yc) -> T {
             const int y = min(max(yc + rec_dy, 0), rec_ymax);
-            const T val = rec_base[(int64_t)y * rec_stride];
-            return lane < rec_len ? val : T(0);
+            // lanes past the record load from a valid address and are never read
+            return rec_base[(int64_t)y * rec_stride];
         };
 
         NodeCol<T, K> B0, B1, T0, T1;
@@ -754,8 +753,12 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
                         u_l[c] = lane_fetch(rec, REC_UCMB + off + c);
                     }
                 }
+                // free-slip kernels add the penalty term unconditionally: with
+                // the penalty off its axis is zero and r (b x n) = 0 leaves val
+                // unchanged, while a runtime condition becomes per-lane selects
+                constexpr bool pen_uncond = BCM != BCM_RUNTIME && (BCM & 1) != 0;
                 T pbn[3] = {T(0), T(0), T(0)};
-                if (pen_on) {
+                if (pen_uncond || pen_on) {
     #pragma clang fp contract(off)
                     pbn[0] = fma(pb[1], cc[2], -(pb[2] * cc[1]));
                     pbn[1] = fma(pb[2], cc[0], -(pb[0] * cc[2]));
@@ -785,7 +788,18 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
                     }
                     const bool at_cmb = p == 0 && rl == 0 && bot_chunk && has_cmb && a.bc_cmb != TNG_NEUMANN;
                     const bool at_surf = p == K && top_chunk && has_surf && a.bc_surf != TNG_NEUMANN;
-                    if (at_cmb || at_surf) {
+                    if constexpr (hoist && BCM == 1) {
+                        // free slip only: the boundary row on every lane, scaled by
+                        // g (1 on the boundary lane, 0 elsewhere) instead of a branch
+                        // and its merges.  g = 1 gives the branch's bits, g = 0
+                        // leaves val unchanged.
+    #pragma clang fp contract(off)
+                        const T g = (at_cmb || at_surf) ? T(1) : T(0);
+                        const T dv = g * fma(cc[2], val[2], fma(cc[1], val[1], cc[0] * val[0]));
+                        const T un = g * (bt_l[0] * fma(cc[2], u_l[2], fma(cc[1], u_l[1], cc[0] * u_l[0])));
+    #pragma unroll
+                        for (int c = 0; c < 3; ++c) val[c] = fma(un, cc[c], fma(-dv, cc[c], val[c]));
+                    } else if (at_cmb || at_surf) {
                         const int bcv = at_cmb ? a.bc_cmb : a.bc_surf;
                         T bt[3], u[3];
                         if constexpr (R == 64 && !hoist) {
@@ -827,7 +841,7 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
                             for (int c = 0; c < 3; ++c) val[c] = fma(un, cc[c], fma(-dv, cc[c], val[c]));
                         }
                     }
-                    if (pen_on) {
+                    if (pen_uncond || pen_on) {
     #pragma clang fp contract(off)
                         // b x (r n) = r (b x n), b x n once per output column
                         const T rad = radl[p];
