@@ -518,7 +518,7 @@ __global__ __launch_bounds__(NW * 64) void epsdivdiv_apply_kernel(const ApplyArg
             return rec_base[(int64_t)y * rec_stride];
         };
 
-        NodeCol<T, K> B0, B1, T0, T1;
+        NodeCol<T, K> B0, B1;
         RawCol<T, K> P0, P1;
         T rec = T(0);
         {
