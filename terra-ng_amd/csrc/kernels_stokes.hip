@@ -251,9 +251,10 @@ __global__ __launch_bounds__(256) void div_cell_kernel(const DivArgs<T> a, T *cd
 // forms one coarse cell, and the cell below's top part arrives by a DPP
 // shift (across radial parts from the lanes 62, 63 sums in LDS).  The fine
 // node columns are swept along x, three rows at a time, the next x-column's
-// loads in flight while the current one is used; nothing is staged in HBM.
-// Trial-side boundary values are projected on loading (node level 0 belongs
-// only to the CMB cell, level ncr only to the surface cell).
+// loads in flight while the current one is used; nothing is staged in HBM.  The cell
+// geometry (made wave-uniform) and the node normals come from LDS tiles.  Trial-side boundary values are projected on every lane with 0/1
+// lane factors (node level 0 belongs only to the CMB cell, level ncr only to
+// the surface cell).
 constexpr int kDivMaxRows = 8;  // coarse rows per workgroup (cy) at most
 template <typename T, int RW, int CW>
 __global__ __launch_bounds__(64 * RW * CW) void div_column_kernel(const DivArgs<T> a, T *dst, int add, int cy)
@@ -262,6 +263,8 @@ __global__ __launch_bounds__(64 * RW * CW) void div_column_kernel(const DivArgs<
     __shared__ T xl[2][CW][RW][2];   // lanes 62, 63 of the node column Xc sums
     // lateral geometry of the workgroup's fine cells: 2 CW columns x 2 (cy + 1) rows
     __shared__ T geo[2 * CW][2 * (kDivMaxRows + 1)][20];
+    // unit normals of the workgroup's fine node columns (free-slip rows)
+    __shared__ T nrm[2 * CW + 1][2 * (kDivMaxRows + 1) + 1][3];
     const int nx = a.nx, nr = a.nr, nc = nx - 1, nxc = a.nxc, nrc = a.nrc;
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -298,6 +301,14 @@ __global__ __launch_bounds__(64 * RW * CW) void div_column_kernel(const DivArgs<
             const int xf = xf0 + i, yf = yf0 + j;
             if (xf >= 0 && xf < nc && j < nyf) geo[i][j][v] = a.latgeom[(((int64_t)sd * nc + xf) * nc + yf) * 20 + v];
         }
+        if (bcb == TNG_FREESLIP || bct == TNG_FREESLIP || (a.bc_cmb == TNG_FREESLIP && a.sd_cmb[sd] != 0) ||
+            (a.bc_surf == TNG_FREESLIP && a.sd_surf[sd] != 0)) {
+            for (int e = threadIdx.x; e < (2 * CW + 1) * (2 * (kDivMaxRows + 1) + 1) * 3; e += 64 * RW * CW) {
+                const int v = e % 3, j = (e / 3) % (2 * (kDivMaxRows + 1) + 1), i = e / (3 * (2 * (kDivMaxRows + 1) + 1));
+                const int xf = xf0 + i, yf = yf0 + j;
+                if (xf >= 0 && xf < nx && yf < nx) nrm[i][j][v] = a.coords[(((int64_t)sd * nx + xf) * nx + yf) * 3 + v];
+            }
+        }
         __syncthreads();
     }
     const T *src[3] = {a.u0, a.u1, a.u2};
@@ -320,6 +331,15 @@ __global__ __launch_bounds__(64 * RW * CW) void div_column_kernel(const DivArgs<
         }
     };
     if (cells && Ys <= nxc - 2) issue(2 * Xc, 2 * Ys);
+    // trial-side boundary factors: the bottom value of lane 0 (CMB) and the
+    // top value of lane 63 (surface); Dirichlet multiplies by z, free slip
+    // removes f (n . u) n
+    const bool fs_any = bcb == TNG_FREESLIP || bct == TNG_FREESLIP;
+    const bool dir_any = bcb == TNG_DIRICHLET || bct == TNG_DIRICHLET;
+    const T f_lo = (bcb == TNG_FREESLIP && lane == 0) ? T(1) : T(0);
+    const T f_hi = (bct == TNG_FREESLIP && lane == 63) ? T(1) : T(0);
+    const T z_lo = (bcb == TNG_DIRICHLET && lane == 0) ? T(0) : T(1);
+    const T z_hi = (bct == TNG_DIRICHLET && lane == 63) ? T(0) : T(1);
     T Lc = T(0), Ln = T(0), Rc = T(0), Rn = T(0);
     T U[2][3][3], Dv[2][3][3];  // [x slot][row][component]
     for (int Y = Ys; Y < Y1; ++Y) {
@@ -329,16 +349,32 @@ __global__ __launch_bounds__(64 * RW * CW) void div_column_kernel(const DivArgs<
         if (cells && Y <= nxc - 2) {
 #pragma unroll
             for (int xi = 0; xi < 3; ++xi) {
-                const int x = 2 * Xc + xi;
                 const int s = xi & 1;
                 // the column's sums and differences, then the next column's loads
 #pragma unroll
                 for (int row = 0; row < 3; ++row) {
                     T lo[3] = {pr[row][0], pr[row][1], pr[row][2]};
                     T hi[3] = {pt[row][0], pt[row][1], pt[row][2]};
-                    const int64_t lcol = ((int64_t)sd * nx + x) * nx + 2 * Y + row;
-                    if (bcb != TNG_NEUMANN && lane == 0) bc_project(a.coords, lcol, bcb, lo);
-                    if (bct != TNG_NEUMANN && lane == 63) bc_project(a.coords, lcol, bct, hi);
+                    if (fs_any) {
+#pragma clang fp contract(off)
+                        const T *n = nrm[2 * xs + xi][2 * (Y - Ys) + row];
+                        const T n0 = n[0], n1 = n[1], n2 = n[2];
+                        const T dl = f_lo * fma(n2, lo[2], fma(n1, lo[1], n0 * lo[0]));
+                        const T dh = f_hi * fma(n2, hi[2], fma(n1, hi[1], n0 * hi[0]));
+                        lo[0] = fma(-dl, n0, lo[0]);
+                        lo[1] = fma(-dl, n1, lo[1]);
+                        lo[2] = fma(-dl, n2, lo[2]);
+                        hi[0] = fma(-dh, n0, hi[0]);
+                        hi[1] = fma(-dh, n1, hi[1]);
+                        hi[2] = fma(-dh, n2, hi[2]);
+                    }
+                    if (dir_any) {
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) {
+                            lo[d] *= z_lo;
+                            hi[d] *= z_hi;
+                        }
+                    }
 #pragma unroll
                     for (int d = 0; d < 3; ++d) {
                         U[s][row][d] = lo[d] + hi[d];
@@ -346,45 +382,48 @@ __global__ __launch_bounds__(64 * RW * CW) void div_column_kernel(const DivArgs<
                     }
                 }
                 if (xi < 2)
-                    issue(x + 1, 2 * Y);
+                    issue(2 * Xc + xi + 1, 2 * Y);
                 else if (Y + 1 < Y1 && Y + 1 <= nxc - 2)
                     issue(2 * Xc, 2 * Y + 2);
                 if (xi == 0) continue;
-                // the two fine cells (x - 1, 2Y + fy)
+                // the two fine cells (2 Xc + xi - 1, 2Y + fy): corners c00, c10,
+                // c01, c11 = (sL, fy), (sR, fy), (sL, fy + 1), (sR, fy + 1);
+                // wedge 0 = (c00, c10, c01), wedge 1 = (c11, c01, c10)
                 const int fx = xi - 1, sL = (xi - 1) & 1, sR = s;
 #pragma unroll
                 for (int fy = 0; fy < 2; ++fy) {
-                    const T *gw = geo[2 * xs + fx][2 * (Y - Ys) + fy];
+                    // the cell's geometry as wave-uniform scalars (as vector
+                    // operands it takes ~70 more VGPRs and spills)
+                    const T *gl = geo[2 * xs + fx][2 * (Y - Ys) + fy];
+                    T gw[20];
 #pragma unroll
-                    for (int w = 0; w < 2; ++w) {
-                        T g[10];
+                    for (int v = 0; v < 20; ++v) gw[v] = uniform_value(gl[v]);
+                    T dm[3];
 #pragma unroll
-                        for (int v = 0; v < 10; ++v) g[v] = uniform_value(gw[w * 10 + v]);
-                        // corners k = 0, 1, 2: (0,0), (1,0), (0,1) / (1,1), (0,1), (1,0)
-                        const int s0 = w == 0 ? sL : sR, r0 = fy + (w == 0 ? 0 : 1);
-                        const int s1 = w == 0 ? sR : sL, r1 = fy + (w == 0 ? 0 : 1);
-                        const int s2 = w == 0 ? sL : sR, r2 = fy + (w == 0 ? 1 : 0);
-                        const int lat = fine_lat_idx(fx, fy, w);
-                        T lat1 = T(0), rad = T(0);
+                    for (int d = 0; d < 3; ++d) dm[d] = Dv[sR][fy][d] + Dv[sL][fy + 1][d];
+                    T l0 = T(0), r0 = T(0);
 #pragma unroll
-                        for (int d = 0; d < 3; ++d) {
-                            const T u0 = U[s0][r0][d];
-                            lat1 += g[d] * (U[s1][r1][d] - u0) + g[3 + d] * (U[s2][r2][d] - u0);
-                            rad += g[6 + d] * (Dv[s0][r0][d] + Dv[s1][r1][d] + Dv[s2][r2][d]);
-                        }
-                        const T f = (-g[9] * vol) * (hs * lat1 + tz * rad);
-                        // coarse wedge w's corners: w = 0 -> (Xc, Y), (Xc+1, Y), (Xc, Y+1);
-                        // w = 1 -> (Xc+1, Y+1), (Xc, Y+1), (Xc+1, Y)
-                        if (w == 0) {
-                            Lc += shape_lat_c<T>(0, lat) * f;
-                            Rc += shape_lat_c<T>(1, lat) * f;
-                            Ln += shape_lat_c<T>(2, lat) * f;
-                        } else {
-                            Rn += shape_lat_c<T>(0, lat) * f;
-                            Ln += shape_lat_c<T>(1, lat) * f;
-                            Rc += shape_lat_c<T>(2, lat) * f;
-                        }
+                    for (int d = 0; d < 3; ++d) {
+                        const T u00 = U[sL][fy][d], u10 = U[sR][fy][d], u01 = U[sL][fy + 1][d];
+                        l0 += gw[d] * (u10 - u00) + gw[3 + d] * (u01 - u00);
+                        r0 += gw[6 + d] * (Dv[sL][fy][d] + dm[d]);
                     }
+                    const T f0 = (-gw[9] * vol) * (hs * l0 + tz * r0);
+                    T l1 = T(0), r1 = T(0);
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        const T u10 = U[sR][fy][d], u01 = U[sL][fy + 1][d], u11 = U[sR][fy + 1][d];
+                        l1 += gw[10 + d] * (u01 - u11) + gw[13 + d] * (u10 - u11);
+                        r1 += gw[16 + d] * (Dv[sR][fy + 1][d] + dm[d]);
+                    }
+                    const T f1 = (-gw[19] * vol) * (hs * l1 + tz * r1);
+                    const int lat0 = fine_lat_idx(fx, fy, 0), lat1 = fine_lat_idx(fx, fy, 1);
+                    Lc += shape_lat_c<T>(0, lat0) * f0;
+                    Rc += shape_lat_c<T>(1, lat0) * f0;
+                    Ln += shape_lat_c<T>(2, lat0) * f0;
+                    Rn += shape_lat_c<T>(0, lat1) * f1;
+                    Ln += shape_lat_c<T>(1, lat1) * f1;
+                    Rc += shape_lat_c<T>(2, lat1) * f1;
                 }
             }
         }
