@@ -712,37 +712,36 @@ __global__ __launch_bounds__(256) void gradient_kernel(const GradArgs<T> a, cons
 // wave-uniform.  The parities (PX, PY) of the column fix which coarse columns
 // and lateral shape values each wedge uses, so they are compile-time.
 // Boundary nodes: Dirichlet 0, free slip the tangential part.
-template <typename T, int K, int PX, int PY>
-__device__ __forceinline__ void gradient_column(const GradArgs<T> &a, int lane, int X, int Y, int sd)
+// Coarse levels a lane reads per coarse column (K fine cells per lane).
+template <int K> constexpr int grad_np() { return K == 1 ? 2 : K / 2 + 1; }
+
+// Radial factors of the lane's K fine cells: 1 / r_mid, 1 / (6 h), r_mid^2 h.
+template <typename T, int K>
+__device__ __forceinline__ void grad_radial(const GradArgs<T> &a, int lane, int sd, T (&sfac)[K], T (&tz)[K],
+                                            T (&vol)[K])
 {
-    constexpr int NP = K == 1 ? 2 : K / 2 + 1;  // coarse levels a lane reads per column
-    const int nx = a.nx, nr = a.nr, nc = nx - 1, ncr = nr - 1, nxc = a.nxc;
-    // coarse columns cx0 + i, cy0 + j (i, j < 3) with cx0 = floor((X - 1) / 2);
-    // the clamped ones are never used
-    const int cx0 = (X - 1) >> 1, cy0 = (Y - 1) >> 1;
-    const int cb = K == 1 ? lane / 2 : lane * (K / 2);  // first coarse level of the lane
-    T pc[3][3][NP];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            const int cx = min(max(cx0 + i, 0), nxc - 1), cy = min(max(cy0 + j, 0), nxc - 1);
-            const T *pp = a.p + (((int64_t)sd * nxc + cx) * nxc + cy) * a.nrc + cb;
-#pragma unroll
-            for (int q = 0; q < NP; ++q) pc[i][j][q] = pp[q];
-        }
-    const bool has_cmb = a.sd_cmb[sd] != 0, has_surf = a.sd_surf[sd] != 0;
-    constexpr int corner_of[2][2][2] = {{{0, 2}, {1, -1}}, {{-1, 1}, {2, 0}}};
-    T sfac[K], tz[K], vol[K];
 #pragma unroll
     for (int p = 0; p < K; ++p) {
         const int c = lane * K + p;
-        const T r0 = a.radii[(int64_t)sd * nr + c], r1 = a.radii[(int64_t)sd * nr + c + 1];
+        const T r0 = a.radii[(int64_t)sd * a.nr + c], r1 = a.radii[(int64_t)sd * a.nr + c + 1];
         const T rmid = T(0.5) * (r0 + r1), h = T(0.5) * (r1 - r0);
         sfac[p] = T(1) / rmid;
         tz[p] = T(1) / (T(6) * h);
         vol[p] = rmid * rmid * h;
     }
+}
+
+// B^T at the fine node column (X, Y) (parities PX, PY) from the coarse values
+// pc[i][j][q] of the coarse columns (cx0 + i, cy0 + j), cx0 = floor((X - 1) / 2),
+// cy0 = floor((Y - 1) / 2), levels cb + q of the lane (the clamped columns are
+// never used).
+template <typename T, int K, int PX, int PY>
+__device__ __forceinline__ void gradient_node(const GradArgs<T> &a, int lane, int X, int Y, int sd, bool has_cmb,
+                                              bool has_surf, const T (&pc)[3][3][grad_np<K>()], const T (&sfac)[K],
+                                              const T (&tz)[K], const T (&vol)[K])
+{
+    const int nx = a.nx, nr = a.nr, nc = nx - 1, ncr = nr - 1;
+    constexpr int corner_of[2][2][2] = {{{0, 2}, {1, -1}}, {{-1, 1}, {2, 0}}};
     T o[K + 1][3];
 #pragma unroll
     for (int p = 0; p <= K; ++p)
@@ -828,23 +827,73 @@ __device__ __forceinline__ void gradient_column(const GradArgs<T> &a, int lane, 
 }
 
 constexpr int kGcolWaves = 4;
+// Fine rows per wave: FP64 2, FP32 4 (level 7: FP64 0.259 ms against 0.271
+// with 4 and 0.277 with 8 rows, the waves' FP64 work being the bound; FP32
+// 0.129 ms against 0.158 with one row).
+template <typename T> constexpr int grad_rows() { return sizeof(T) == 8 ? 2 : 4; }
+// B^T with one wave per fine node column and chunk of grad_rows rows (lane =
+// fine radial cells, K per lane).  The radial factors are formed once per
+// wave; the coarse rows the chunk needs (its nodes' 3 x 3 coarse columns
+// slide by one coarse row every two fine rows) are each loaded once, the
+// next one requested a node ahead of its first use.
+template <typename T, int K, int PX>
+__device__ __forceinline__ void gradient_sweep(const GradArgs<T> &a, int lane, int X, int chunk, int sd)
+{
+    constexpr int NP = grad_np<K>(), NROW = grad_rows<T>(), NWIN = NROW / 2 + 3;
+    const int nxc = a.nxc;
+    const int Ya = chunk * NROW;
+    const int cx0 = (X - 1) >> 1, cr0 = (Ya >> 1) - 1;  // window row j = coarse row cr0 + j
+    const int cb = K == 1 ? lane / 2 : lane * (K / 2);     // first coarse level of the lane
+    const bool has_cmb = a.sd_cmb[sd] != 0, has_surf = a.sd_surf[sd] != 0;
+    T sfac[K], tz[K], vol[K];
+    grad_radial<T, K>(a, lane, sd, sfac, tz, vol);
+    T win[NWIN][3][NP];
+    auto load_row = [&](int j) {
+        const int cy = min(max(cr0 + j, 0), nxc - 1);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const int cx = min(max(cx0 + i, 0), nxc - 1);
+            const T *pp = a.p + (((int64_t)sd * nxc + cx) * nxc + cy) * a.nrc + cb;
+#pragma unroll
+            for (int q = 0; q < NP; ++q) win[j][i][q] = pp[q];
+        }
+    };
+#pragma unroll
+    for (int j = 0; j < (NWIN < 4 ? NWIN : 4); ++j) load_row(j);
+#pragma unroll
+    for (int k = 0; k < NROW; ++k) {
+        // node k uses window rows j0 .. j0 + 2; at odd k the row that node
+        // k + 2 needs first is requested
+        const int j0 = k == 0 ? 0 : (k - 1) / 2 + 1;
+        if ((k & 1) == 1 && j0 + 3 < NWIN) load_row(j0 + 3);
+        const int Y = Ya + k;
+        if (Y >= a.nx) break;
+        T pc[3][3][NP];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+                for (int q = 0; q < NP; ++q) pc[i][jj][q] = win[j0 + jj][i][q];
+        if (k & 1) gradient_node<T, K, PX, 1>(a, lane, X, Y, sd, has_cmb, has_surf, pc, sfac, tz, vol);
+        else gradient_node<T, K, PX, 0>(a, lane, X, Y, sd, has_cmb, has_surf, pc, sfac, tz, vol);
+    }
+}
+
 template <typename T, int K>
 __global__ __launch_bounds__(64 * kGcolWaves) void gradient_column_kernel(const GradArgs<T> a)
 {
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int64_t ncol = (int64_t)a.nsd * a.nx * a.nx;
-    const int64_t col = (int64_t)xcd_block() * kGcolWaves + wv;
-    if (col >= ncol) return;
-    const int Y = (int)(col % a.nx);
-    const int X = (int)((col / a.nx) % a.nx);
-    const int sd = (int)(col / ((int64_t)a.nx * a.nx));
-    switch ((X & 1) * 2 + (Y & 1)) {
-    case 0: gradient_column<T, K, 0, 0>(a, lane, X, Y, sd); break;
-    case 1: gradient_column<T, K, 0, 1>(a, lane, X, Y, sd); break;
-    case 2: gradient_column<T, K, 1, 0>(a, lane, X, Y, sd); break;
-    default: gradient_column<T, K, 1, 1>(a, lane, X, Y, sd); break;
-    }
+    const int nyc = (a.nx + grad_rows<T>() - 1) / grad_rows<T>();
+    const int64_t nw = (int64_t)a.nsd * a.nx * nyc;
+    const int64_t w = (int64_t)xcd_block() * kGcolWaves + wv;
+    if (w >= nw) return;
+    const int chunk = (int)(w % nyc);
+    const int X = (int)((w / nyc) % a.nx);
+    const int sd = (int)(w / ((int64_t)nyc * a.nx));
+    if (X & 1) gradient_sweep<T, K, 1>(a, lane, X, chunk, sd);
+    else gradient_sweep<T, K, 0>(a, lane, X, chunk, sd);
 }
 
 // Constant prolongation (fe/wedge/shell/grid_transfer_constant.hpp:9-90):
@@ -1130,11 +1179,11 @@ template <typename T> void grad_impl(tng_operator &op, const tng_vector &src, tn
     a.bc_cmb = op.bc_cmb; a.bc_surf = op.bc_surf; a.add = op.apply_mode == TNG_ADD ? 1 : 0;
     a.fcell = Index4(f.nx - 1, f.nr - 1);
     a.fnode = Index4(f.nx, f.nr);
-    // one wave per node column: 64, 128 or 256 radial cells
+    // one wave per node column and chunk of rows: 64, 128 or 256 radial cells
     const int ncr = f.nr - 1;
     if (ncr == 64 || ncr == 128 || ncr == 256) {
-        const int64_t ncol = (int64_t)f.nsd * f.nx * f.nx;
-        const unsigned g = (unsigned)((ncol + kGcolWaves - 1) / kGcolWaves);
+        const int64_t nw = (int64_t)f.nsd * f.nx * ((f.nx + grad_rows<T>() - 1) / grad_rows<T>());
+        const unsigned g = (unsigned)((nw + kGcolWaves - 1) / kGcolWaves);
         if (ncr == 64)
             hipLaunchKernelGGL((gradient_column_kernel<T, 1>), dim3(g), dim3(64 * kGcolWaves), 0, s, a);
         else if (ncr == 128)
