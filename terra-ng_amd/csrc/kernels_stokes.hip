@@ -742,6 +742,23 @@ __device__ __forceinline__ void gradient_node(const GradArgs<T> &a, int lane, in
 {
     const int nx = a.nx, nr = a.nr, nc = nx - 1, ncr = nr - 1;
     constexpr int corner_of[2][2][2] = {{{0, 2}, {1, -1}}, {{-1, 1}, {2, 0}}};
+    // the coarse values interpolated radially to the centroid of each of the
+    // lane's cells, once per coarse column (the wedges share them)
+    T pcr[3][3][K];
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+        const int q = K == 1 ? 0 : p / 2;  // coarse level of the cell within the lane
+        const bool top = K == 1 ? (lane & 1) != 0 : (p & 1) != 0;
+        const T wb = top ? T(0.25) : T(0.75), wt = top ? T(0.75) : T(0.25);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) pcr[i][j][p] = wb * pc[i][j][q] + wt * pc[i][j][q + 1];
+    }
+    // 1 / (2 r_mid): the lateral shape gradients' factor 1/2 (exact) moves here
+    T hsfac[K];
+#pragma unroll
+    for (int p = 0; p < K; ++p) hsfac[p] = T(0.5) * sfac[p];
     T o[K + 1][3];
 #pragma unroll
     for (int p = 0; p <= K; ++p)
@@ -767,9 +784,10 @@ __device__ __forceinline__ void gradient_node(const GradArgs<T> &a, int lane, in
                 T g[10];
 #pragma unroll
                 for (int v = 0; v < 10; ++v) g[v] = gcell[w * 10 + v];
-                const T l0 = k == 0 ? T(-0.5) : (k == 1 ? T(0.5) : T(0));
-                const T l1 = k == 0 ? T(-0.5) : (k == 1 ? T(0) : T(0.5));
-                const T gl[3] = {l0 * g[0] + l1 * g[3], l0 * g[1] + l1 * g[4], l0 * g[2] + l1 * g[5]};
+                // the corner's lateral shape gradient times 2: -(M0 + M1), M0, M1
+                T gl[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) gl[d] = k == 0 ? -(g[d] + g[3 + d]) : (k == 1 ? g[d] : g[3 + d]);
                 const int lat = fine_lat_idx(pxc, pyc, w);
                 const T sl0 = shape_lat_c<T>(0, lat), sl1 = shape_lat_c<T>(1, lat), sl2 = shape_lat_c<T>(2, lat);
                 // coarse columns of the coarse wedge's corners 0, 1, 2
@@ -779,14 +797,9 @@ __device__ __forceinline__ void gradient_node(const GradArgs<T> &a, int lane, in
                 T av[K], bv[K];
 #pragma unroll
                 for (int p = 0; p < K; ++p) {
-                    const int q = K == 1 ? 0 : p / 2;  // coarse level of the cell within the lane
-                    const bool top = K == 1 ? (lane & 1) != 0 : (p & 1) != 0;
-                    const T wb = top ? T(0.25) : T(0.75), wt = top ? T(0.75) : T(0.25);
-                    const T pi = sl0 * (wb * pc[ax][ay][q] + wt * pc[ax][ay][q + 1]) +
-                                 sl1 * (wb * pc[bx][by][q] + wt * pc[bx][by][q + 1]) +
-                                 sl2 * (wb * pc[dx][dy][q] + wt * pc[dx][dy][q + 1]);
+                    const T pi = sl0 * pcr[ax][ay][p] + sl1 * pcr[bx][by][p] + sl2 * pcr[dx][dy][p];
                     const T f = -(g[9] * vol[p]) * pi;
-                    av[p] = f * sfac[p];
+                    av[p] = f * hsfac[p];
                     bv[p] = f * tz[p];
                 }
                 // the cell below the lane's first node level: the lane below's last cell
