@@ -200,21 +200,45 @@ __global__ __launch_bounds__(256) void gs_update_kernel(const GsArgs a, const Gs
     gs_update_rows<3, 1>(a, cf, a.np);
 }
 
-// y += sum_{i < k} coef_i X_i over the Stokes rows (X_i: table slots 0..k-1)
-__global__ __launch_bounds__(256) void gs_axpy_kernel(double *const *X, int k, const GsAxpy cf, double *const *y,
-                                                      int64_t nv, int64_t np)
+// y += sum_{i < k} coef_i X_i over the Stokes rows (X_i: table slots 0..k-1):
+// velocity rows then pressure rows (components known at compile time, the
+// pointers wave-uniform), two consecutive rows per thread with 16-byte loads.
+template <int C0, int NC>
+__device__ __forceinline__ void gs_axpy_rows(double *const *X, int k, const GsAxpy &cf, double *const *y, int64_t nrows)
 {
-    const int64_t nrows = nv + np;
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += (int64_t)gridDim.x * blockDim.x) {
-        const bool vel = r < nv;
-        const int64_t n = vel ? r : r - nv;
-        const int c0 = vel ? 0 : 3, c1 = vel ? 3 : 4;
-        for (int c = c0; c < c1; ++c) {
+    const int64_t npairs = nrows >> 1, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t pr = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; pr < npairs; pr += stride) {
+        const int64_t n = 2 * pr;
+#pragma unroll
+        for (int c = C0; c < C0 + NC; ++c) {
+            double2 t = ld2(y[c], n);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if (i < k) {
+                    const double2 q = ld2(X[4 * i + c], n);
+                    t.x = fma(cf.c[i], q.x, t.x);
+                    t.y = fma(cf.c[i], q.y, t.y);
+                }
+            }
+            st2(y[c], n, t);
+        }
+    }
+    if ((nrows & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+        const int64_t n = nrows - 1;
+#pragma unroll
+        for (int c = C0; c < C0 + NC; ++c) {
             double t = y[c][n];
             for (int i = 0; i < k; ++i) t = fma(cf.c[i], X[4 * i + c][n], t);
             y[c][n] = t;
         }
     }
+}
+
+__global__ __launch_bounds__(256) void gs_axpy_kernel(double *const *X, int k, const GsAxpy cf, double *const *y,
+                                                      int64_t nv, int64_t np)
+{
+    gs_axpy_rows<0, 3>(X, k, cf, y, nv);
+    gs_axpy_rows<3, 1>(X, k, cf, y, np);
 }
 
 // Rigid rotation e_axis x X at every node copy (X = radius * unit lateral position).
@@ -272,8 +296,8 @@ void gs_update(const GsArgs &a, const GsCoef &cf, hipStream_t s)
 void gs_axpy(double *const *X, int k, const GsAxpy &cf, double *const *y, int64_t nv, int64_t np, hipStream_t s)
 {
     TNG_REQUIRE(k >= 0 && k <= 8, "multi-axpy: at most 8 vectors");
-    const int64_t rows = nv + np;
-    const unsigned g = (unsigned)std::min<int64_t>((rows + 255) / 256, 4096);
+    const int64_t pairs = std::max(nv, np) / 2;
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((pairs + 255) / 256, 4096));
     hipLaunchKernelGGL(gs_axpy_kernel, dim3(g), dim3(256), 0, s, X, k, cf, y, nv, np);
     TNG_HIP_CHECK(hipGetLastError());
 }

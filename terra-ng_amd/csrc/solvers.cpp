@@ -610,13 +610,17 @@ void StokesSolver::rotation_setup()
 void StokesSolver::rotation_correct(int j, SVec &v, SVec &z, SVec &w)
 {
     (void)v;
-    (void)z;
     hipStream_t s = stream_of(*w.u);
     const int m = params.krylov_restart;
     double *const *tab = rc_ptrs.as<double *const>();
     double *const *t_dots = tab + 16 * j;           // [Z0, Z1, Z2, V_j]
     double *const *t_kz = tab + 16 * m;              // [KZ0, KZ1, KZ2]
     double *const *t_zj = tab + 16 * m + 12 + 4 * j; // [Z_j]
+    // the current storage of z: the fused Chebyshev smoother exchanges the
+    // storage of the vectors it updates (uploaded before the synchronisation
+    // below, which also keeps zp alive until the copy is done)
+    double *zp[4] = {(double *)z.u->comp(0), (double *)z.u->comp(1), (double *)z.u->comp(2), (double *)z.p->comp(0)};
+    TNG_HIP_CHECK(hipMemcpyAsync(rc_ptrs.as<double *>() + 16 * m + 12 + 4 * j, zp, sizeof(zp), hipMemcpyHostToDevice, s));
     GsArgs ga{};
     ga.V = t_dots;
     for (int c = 0; c < 3; ++c) ga.w[c] = (const double *)w.u->comp(c);
@@ -809,14 +813,17 @@ void StokesSolver::fgmres_cgs2(SVec &x, const SVec &b, tng_solve_report &rep)
         return;
     }
     const int offV = 1, offZ = offV + m + 1, idxW = offZ + m, idxAcc = idxW + 1;
-    // device table of the basis slots' component pointers
+    // device table of component pointers: the basis slots V_0..V_m, then the
+    // preconditioned directions Z_0..Z_{m-1}, then the update accumulator
     {
-        std::vector<double *> tab(4 * (size_t)(m + 1));
-        for (int i = 0; i <= m; ++i) {
-            const SVec &v = fg[offV + i];
-            for (int c = 0; c < 3; ++c) tab[4 * i + c] = (double *)v.u->comp(c);
-            tab[4 * i + 3] = (double *)v.p->comp(0);
-        }
+        std::vector<double *> tab(4 * (size_t)(2 * m + 2));
+        auto put = [&](int slot, const SVec &v) {
+            for (int c = 0; c < 3; ++c) tab[4 * (size_t)slot + c] = (double *)v.u->comp(c);
+            tab[4 * (size_t)slot + 3] = (double *)v.p->comp(0);
+        };
+        for (int i = 0; i <= m; ++i) put(i, fg[offV + i]);
+        for (int i = 0; i < m; ++i) put(m + 1 + i, fg[offZ + i]);
+        put(2 * m + 1, fg[idxAcc]);
         if (gs_table.bytes < tab.size() * sizeof(double *)) gs_table.alloc(tab.size() * sizeof(double *));
         TNG_HIP_CHECK(hipMemcpyAsync(gs_table.p, tab.data(), tab.size() * sizeof(double *), hipMemcpyHostToDevice, s));
         const size_t np = gs_partials_doubles();
@@ -974,9 +981,30 @@ void StokesSolver::fgmres_cgs2(SVec &x, const SVec &b, tng_solve_report &rep)
             for (int col = row + 1; col < inner; ++col) sum -= rr(row, col) * y[col];
             y[row] = sum / rr(row, row);
         }
+        // x += Z y, the directions summed in runs of 8 per pass.  The table's
+        // Z entries are refreshed first: the fused Chebyshev smoother exchanges
+        // the storage of the vectors it updates, so a preconditioned
+        // direction's components may have moved since the table was built.
         SVec &acc = fg[idxAcc];
         s_assign(acc, 0.0);
-        for (int i = 0; i < inner; ++i) s_lincomb(acc, {1.0, y[i]}, {&acc, &fg[offZ + i]});
+        {
+            std::vector<double *> zt(4 * (size_t)(m + 1));
+            for (int i = 0; i <= m; ++i) {
+                const SVec &v = i < m ? fg[offZ + i] : acc;
+                for (int c = 0; c < 3; ++c) zt[4 * (size_t)i + c] = (double *)v.u->comp(c);
+                zt[4 * (size_t)i + 3] = (double *)v.p->comp(0);
+            }
+            TNG_HIP_CHECK(hipMemcpyAsync(gs_table.as<double *>() + 4 * (m + 1), zt.data(), zt.size() * sizeof(double *),
+                                         hipMemcpyHostToDevice, s));
+            TNG_HIP_CHECK(hipStreamSynchronize(s));  // zt is released at the end of this block
+            double *const *tabd = gs_table.as<double *const>();
+            for (int i0 = 0; i0 < inner; i0 += 8) {
+                GsAxpy cf{};
+                const int k = std::min(8, inner - i0);
+                for (int i = 0; i < k; ++i) cf.c[i] = y[i0 + i];
+                gs_axpy(tabd + 4 * (m + 1 + i0), k, cf, tabd + 4 * (2 * m + 1), ga.nv, ga.np, s);
+            }
+        }
         s_lincomb(x, {1.0, 1.0}, {&x, &acc});
         rep.restarts += 1;
         apply_stokes(x, r);
