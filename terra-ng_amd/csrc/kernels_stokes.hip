@@ -234,51 +234,73 @@ __global__ __launch_bounds__(256) void div_cell_kernel(const DivArgs<T> a, T *cd
     }
 }
 
-// B in one pass.  A workgroup covers CW - 1 adjacent coarse node columns
-// X0 .. X0 + CW - 2 of a subdomain over a chunk of cy coarse rows; its waves
-// are the coarse cell columns Xc = X0 - 1 .. X0 + CW - 2 (the first one a
-// halo, evaluated again by the workgroup to the left) times RW radial parts
-// of 64 fine cells (lane = fine cell).  Sweeping the coarse rows Y, a wave
+// B in one pass.  A workgroup (wx, chunk) owns the coarse node columns
+// X0 = wx CW .. X0 + CW - 1 of a subdomain over a chunk of coarse rows
+// Y0 .. Y1 - 1; its waves are those node columns Xc times RW radial parts of
+// 64 fine cells (lane = fine cell).  Sweeping the coarse rows Y, a wave
 // evaluates the four fine cells of coarse cell (Xc, Y) -- per fine wedge
 // f = -|det J| vol div u, with
 //   div u = (M0 . (U_1 - U_0) + M1 . (U_2 - U_0)) / (2 r_mid) + M2 . sum_k D_k / (6h)
 // (U_k = bottom + top and D_k = top - bottom of the wedge's corner column k)
 // -- and adds f times the lateral coarse shape value of each corner to the
 // corner's node: (Xc, Y), (Xc + 1, Y), or carried to the next row (Xc, Y + 1),
-// (Xc + 1, Y + 1).  The sums for node column Xc + 1 pass to the next wave
-// through LDS.  The radial coarse weights (3/4, 1/4) split a lane's sum
-// between the bottom and top coarse node of its coarse cell, a lane pair
-// forms one coarse cell, and the cell below's top part arrives by a DPP
-// shift (across radial parts from the lanes 62, 63 sums in LDS).  The fine
-// node columns are swept along x, three rows at a time, the next x-column's
-// loads in flight while the current one is used; nothing is staged in HBM.  The cell
-// geometry (made wave-uniform) and the node normals come from LDS tiles.  Trial-side boundary values are projected on every lane with 0/1
-// lane factors (node level 0 belongs only to the CMB cell, level ncr only to
-// the surface cell).
+// (Xc + 1, Y + 1).  Node (Xc, Y) is the wave's own sums plus the sums for
+// column Xc + 1 of the wave to its left, passed through LDS.  The radial
+// coarse weights (3/4, 1/4) split a lane's sum between the bottom and top
+// coarse node of its coarse cell, a lane pair forms one coarse cell, and the
+// cell below's top part arrives by a DPP shift (across radial parts from the
+// lanes 62, 63 sums in LDS).  Every coarse cell is evaluated once: what the
+// workgroup's cells add to the nodes of its neighbours (column X0 + CW, row
+// Y1 and their corner) is written, as coarse node values, to the side buffer
+// and added by div_fixup_kernel, which also sets the last node column and row
+// where no workgroup owns them.  The fine node columns are swept along x,
+// three rows at a time, the next x-column's loads in flight while one is used
+// (uniform base address, lane offset).  The cell geometry (made wave-uniform)
+// and the node normals come from LDS tiles.  Trial-side boundary values are
+// projected on every lane with 0/1 lane factors (node level 0 belongs only to
+// the CMB cell, level ncr only to the surface cell).
 constexpr int kDivMaxRows = 8;  // coarse rows per workgroup (cy) at most
+#ifndef TNG_DIV_CW
+#define TNG_DIV_CW 4  // coarse node columns per workgroup of the one-pass B
+#endif
+// side buffer of the one-pass B: coarse node columns of nrc values
+struct DivSide {
+    int nxw, nchunk, cw, cy;
+    __host__ __device__ int64_t right(int sd, int wx, int Y, int nxc) const { return ((int64_t)sd * nxw + wx) * nxc + Y; }
+    __host__ __device__ int64_t bottom(int sd, int ch, int X, int nxc, int nsd) const
+    {
+        return (int64_t)nsd * nxw * nxc + ((int64_t)sd * nchunk + ch) * nxc + X;
+    }
+    __host__ __device__ int64_t corner(int sd, int wx, int ch, int nxc, int nsd) const
+    {
+        return (int64_t)nsd * (nxw + nchunk) * nxc + ((int64_t)sd * nxw + wx) * nchunk + ch;
+    }
+    __host__ __device__ int64_t columns(int nxc, int nsd) const { return (int64_t)nsd * ((nxw + nchunk) * (int64_t)nxc + nxw * nchunk); }
+};
 template <typename T, int RW, int CW>
-__global__ __launch_bounds__(64 * RW * CW) void div_column_kernel(const DivArgs<T> a, T *dst, int add, int cy)
+__global__ __launch_bounds__(64 * RW * CW) void div_column_kernel(const DivArgs<T> a, T *dst, int add, const DivSide sb,
+                                                                  T *side)
 {
     __shared__ T xr[2][CW][RW][64];  // node column Xc + 1 sums of each wave, by row parity
     __shared__ T xl[2][CW][RW][2];   // lanes 62, 63 of the node column Xc sums
-    // lateral geometry of the workgroup's fine cells: 2 CW columns x 2 (cy + 1) rows
-    __shared__ T geo[2 * CW][2 * (kDivMaxRows + 1)][20];
+    // lateral geometry of the workgroup's fine cells: 2 CW columns x 2 cy rows
+    __shared__ T geo[2 * CW][2 * kDivMaxRows][20];
     // unit normals of the workgroup's fine node columns (free-slip rows)
-    __shared__ T nrm[2 * CW + 1][2 * (kDivMaxRows + 1) + 1][3];
+    __shared__ T nrm[2 * CW + 1][2 * kDivMaxRows + 1][3];
     const int nx = a.nx, nr = a.nr, nc = nx - 1, nxc = a.nxc, nrc = a.nrc;
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int part = wv % RW, xs = wv / RW;
-    const int nxw = (nxc + CW - 2) / (CW - 1);
-    const int nchunk = (nxc + cy - 1) / cy;
+    const int cy = sb.cy;
     const int64_t wg = xcd_block();
-    const int wx = (int)(wg % nxw);
-    const int chunk = (int)((wg / nxw) % nchunk);
-    const int sd = (int)(wg / ((int64_t)nxw * nchunk));
-    const int Xc = wx * (CW - 1) - 1 + xs;
-    const bool cells = Xc >= 0 && Xc <= nxc - 2;
-    const bool out = xs >= 1 && Xc <= nxc - 1;
-    const int c = part * 64 + lane;  // fine radial cell of the lane
+    const int wx = (int)(wg % sb.nxw);
+    const int chunk = (int)((wg / sb.nxw) % sb.nchunk);
+    const int sd = (int)(wg / ((int64_t)sb.nxw * sb.nchunk));
+    const int X0 = wx * CW, Xc = X0 + xs;
+    const bool cells = Xc <= nxc - 2;
+    const bool out = Xc <= nxc - 1;
+    const bool right_side = X0 + CW <= nxc - 1;  // node column X0 + CW exists
+    const int c = part * 64 + lane;              // fine radial cell of the lane
     const int bcb = part == 0 && a.sd_cmb[sd] != 0 ? a.bc_cmb : TNG_NEUMANN;
     const int bct = part == RW - 1 && a.sd_surf[sd] != 0 ? a.bc_surf : TNG_NEUMANN;
     T hs, tz, vol;
@@ -291,30 +313,32 @@ __global__ __launch_bounds__(64 * RW * CW) void div_column_kernel(const DivArgs<
     }
     const bool odd = lane & 1;
     const T srb = odd ? T(0.25) : T(0.75), srt = odd ? T(0.75) : T(0.25);
-    const int Y0 = chunk * cy, Y1 = min(Y0 + cy, nxc), Ys = max(Y0 - 1, 0);
+    const int Y0 = chunk * cy, Y1 = min(Y0 + cy, nxc);
     {
-        // fine cells x = 2 (X0 - 1) + i, y = 2 Ys + j of existing coarse cells
-        const int xf0 = 2 * (wx * (CW - 1) - 1), yf0 = 2 * Ys;
-        const int nyf = 2 * (min(Y1 - 1, nxc - 2) - Ys + 1);
-        for (int e = threadIdx.x; e < 2 * CW * 2 * (kDivMaxRows + 1) * 20; e += 64 * RW * CW) {
-            const int v = e % 20, j = (e / 20) % (2 * (kDivMaxRows + 1)), i = e / (20 * 2 * (kDivMaxRows + 1));
+        // fine cells x = 2 X0 + i, y = 2 Y0 + j of existing coarse cells
+        const int xf0 = 2 * X0, yf0 = 2 * Y0;
+        const int nyf = 2 * (min(Y1 - 1, nxc - 2) - Y0 + 1);
+        for (int e = threadIdx.x; e < 2 * CW * 2 * kDivMaxRows * 20; e += 64 * RW * CW) {
+            const int v = e % 20, j = (e / 20) % (2 * kDivMaxRows), i = e / (20 * 2 * kDivMaxRows);
             const int xf = xf0 + i, yf = yf0 + j;
-            if (xf >= 0 && xf < nc && j < nyf) geo[i][j][v] = a.latgeom[(((int64_t)sd * nc + xf) * nc + yf) * 20 + v];
+            if (xf < nc && j < nyf) geo[i][j][v] = a.latgeom[(((int64_t)sd * nc + xf) * nc + yf) * 20 + v];
         }
         if (bcb == TNG_FREESLIP || bct == TNG_FREESLIP || (a.bc_cmb == TNG_FREESLIP && a.sd_cmb[sd] != 0) ||
             (a.bc_surf == TNG_FREESLIP && a.sd_surf[sd] != 0)) {
-            for (int e = threadIdx.x; e < (2 * CW + 1) * (2 * (kDivMaxRows + 1) + 1) * 3; e += 64 * RW * CW) {
-                const int v = e % 3, j = (e / 3) % (2 * (kDivMaxRows + 1) + 1), i = e / (3 * (2 * (kDivMaxRows + 1) + 1));
+            for (int e = threadIdx.x; e < (2 * CW + 1) * (2 * kDivMaxRows + 1) * 3; e += 64 * RW * CW) {
+                const int v = e % 3, j = (e / 3) % (2 * kDivMaxRows + 1), i = e / (3 * (2 * kDivMaxRows + 1));
                 const int xf = xf0 + i, yf = yf0 + j;
-                if (xf >= 0 && xf < nx && yf < nx) nrm[i][j][v] = a.coords[(((int64_t)sd * nx + xf) * nx + yf) * 3 + v];
+                if (xf < nx && yf < nx) nrm[i][j][v] = a.coords[(((int64_t)sd * nx + xf) * nx + yf) * 3 + v];
             }
         }
         __syncthreads();
     }
     const T *src[3] = {a.u0, a.u1, a.u2};
-    // node copies are indexed in 32 bits (checked on the host)
+    // node copies are indexed in 32 bits (checked on the host); the lane's
+    // byte offset is the only per-lane part of a load's address
     const uint32_t xstride = (uint32_t)nx * nr, ystride = (uint32_t)nr;
     const uint32_t sdbase = (uint32_t)sd * nx * xstride;
+    const uint32_t boff = (uint32_t)c * (uint32_t)sizeof(T);
     // raw values of the node column x at rows y0, y0 + 1, y0 + 2: levels c and
     // c + 1 of every lane; the next x-column's loads are in flight while one is used
     T pr[3][3], pt[3][3];
@@ -322,15 +346,15 @@ __global__ __launch_bounds__(64 * RW * CW) void div_column_kernel(const DivArgs<
         const uint32_t cb = sdbase + (uint32_t)x * xstride + (uint32_t)y0 * ystride;
 #pragma unroll
         for (int row = 0; row < 3; ++row) {
-            const uint32_t b = cb + row * ystride + c;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                pr[row][d] = src[d][b];
-                pt[row][d] = src[d][b + 1];
+                const T *q = (const T *)((const char *)(src[d] + (cb + row * ystride)) + boff);
+                pr[row][d] = q[0];
+                pt[row][d] = q[1];
             }
         }
     };
-    if (cells && Ys <= nxc - 2) issue(2 * Xc, 2 * Ys);
+    if (cells && Y0 <= nxc - 2) issue(2 * Xc, 2 * Y0);
     // trial-side boundary factors: the bottom value of lane 0 (CMB) and the
     // top value of lane 63 (surface); Dirichlet multiplies by z, free slip
     // removes f (n . u) n
@@ -340,9 +364,26 @@ __global__ __launch_bounds__(64 * RW * CW) void div_column_kernel(const DivArgs<
     const T f_hi = (bct == TNG_FREESLIP && lane == 63) ? T(1) : T(0);
     const T z_lo = (bcb == TNG_DIRICHLET && lane == 0) ? T(0) : T(1);
     const T z_hi = (bct == TNG_DIRICHLET && lane == 63) ? T(0) : T(1);
+    // a node column's radial sums (one per fine cell) as coarse node values:
+    // t62, t63 are lanes 62, 63 of the part below; stored to col[0 .. nrc)
+    auto put = [&](T tot, T t62, T t63, T *col, bool acc) {
+        const T b = tot * srb, t = tot * srt;
+        const T bs = b + pair_swap_t(b), ts = t + pair_swap_t(t);
+        T below = wave_shr1_t(ts);
+        if (part > 0 && lane == 0) below = t63 * T(0.75) + t62 * T(0.25);
+        if (!odd) {
+            T *p = col + part * 32 + (lane >> 1);
+            const T v = bs + below;
+            *p = acc ? *p + v : v;
+        }
+        if (part == RW - 1 && lane == 63) {
+            T *p = col + part * 32 + 32;
+            *p = acc ? *p + ts : ts;
+        }
+    };
     T Lc = T(0), Ln = T(0), Rc = T(0), Rn = T(0);
     T U[2][3][3], Dv[2][3][3];  // [x slot][row][component]
-    for (int Y = Ys; Y < Y1; ++Y) {
+    for (int Y = Y0; Y < Y1; ++Y) {
         Lc = Ln;
         Rc = Rn;
         Ln = Rn = T(0);
@@ -357,7 +398,7 @@ __global__ __launch_bounds__(64 * RW * CW) void div_column_kernel(const DivArgs<
                     T hi[3] = {pt[row][0], pt[row][1], pt[row][2]};
                     if (fs_any) {
 #pragma clang fp contract(off)
-                        const T *n = nrm[2 * xs + xi][2 * (Y - Ys) + row];
+                        const T *n = nrm[2 * xs + xi][2 * (Y - Y0) + row];
                         const T n0 = n[0], n1 = n[1], n2 = n[2];
                         const T dl = f_lo * fma(n2, lo[2], fma(n1, lo[1], n0 * lo[0]));
                         const T dh = f_hi * fma(n2, hi[2], fma(n1, hi[1], n0 * hi[0]));
@@ -394,7 +435,7 @@ __global__ __launch_bounds__(64 * RW * CW) void div_column_kernel(const DivArgs<
                 for (int fy = 0; fy < 2; ++fy) {
                     // the cell's geometry as wave-uniform scalars (as vector
                     // operands it takes ~70 more VGPRs and spills)
-                    const T *gl = geo[2 * xs + fx][2 * (Y - Ys) + fy];
+                    const T *gl = geo[2 * xs + fx][2 * (Y - Y0) + fy];
                     T gw[20];
 #pragma unroll
                     for (int v = 0; v < 20; ++v) gw[v] = uniform_value(gl[v]);
@@ -427,34 +468,97 @@ __global__ __launch_bounds__(64 * RW * CW) void div_column_kernel(const DivArgs<
                 }
             }
         }
-        if (Y < Y0) continue;
         // node (Xc, Y) = this wave's left sums + the left neighbour's right sums
         const int pb = Y & 1;
         xr[pb][xs][part][lane] = Rc;
         if (lane >= 62) xl[pb][xs][part][lane - 62] = Lc;
         __syncthreads();
         if (!out) continue;
-        const T tot = Lc + xr[pb][xs - 1][part][lane];
-        // bottom parts of the coarse cell above each level, top parts of the one below
-        const T b = tot * srb, t = tot * srt;
-        const T bs = b + pair_swap_t(b), ts = t + pair_swap_t(t);
-        T below = wave_shr1_t(ts);
-        if (part > 0 && lane == 0) {
-            const T t62 = (xl[pb][xs][part - 1][0] + xr[pb][xs - 1][part - 1][62]) * T(0.25);
-            const T t63 = (xl[pb][xs][part - 1][1] + xr[pb][xs - 1][part - 1][63]) * T(0.75);
-            below = t63 + t62;
+        T tot = Lc, t62 = T(0), t63 = T(0);
+        if (xs > 0) tot += xr[pb][xs - 1][part][lane];
+        if (part > 0) {
+            t62 = xl[pb][xs][part - 1][0];
+            t63 = xl[pb][xs][part - 1][1];
+            if (xs > 0) {
+                t62 += xr[pb][xs - 1][part - 1][62];
+                t63 += xr[pb][xs - 1][part - 1][63];
+            }
         }
-        const int64_t ob = (((int64_t)sd * nxc + Xc) * nxc + Y) * nrc + part * 32;
-        if (!odd) {
-            const T v = bs + below;
-            const int64_t i = ob + (lane >> 1);
-            dst[i] = add ? dst[i] + v : v;
-        }
-        if (part == RW - 1 && lane == 63) {
-            const int64_t i = ob + 32;
-            dst[i] = add ? dst[i] + ts : ts;
+        put(tot, t62, t63, dst + (((int64_t)sd * nxc + Xc) * nxc + Y) * nrc, add != 0);
+        // the right neighbour's node (X0 + CW, Y): the last wave's right sums
+        if (xs == CW - 1 && right_side) {
+            const T r62 = part > 0 ? xr[pb][xs][part - 1][62] : T(0);
+            const T r63 = part > 0 ? xr[pb][xs][part - 1][63] : T(0);
+            put(Rc, r62, r63, side + sb.right(sd, wx, Y, nxc) * nrc, false);
         }
     }
+    // node row Y1 (the next chunk's) from cell row Y1 - 1; the buffer of parity
+    // Y1 was last read before the barrier of row Y1 - 1
+    if (Y1 <= nxc - 1) {
+        const int pb = Y1 & 1;
+        xr[pb][xs][part][lane] = Rn;
+        if (lane >= 62) xl[pb][xs][part][lane - 62] = Ln;
+        __syncthreads();
+        if (out) {
+            T tot = Ln, t62 = T(0), t63 = T(0);
+            if (xs > 0) tot += xr[pb][xs - 1][part][lane];
+            if (part > 0) {
+                t62 = xl[pb][xs][part - 1][0];
+                t63 = xl[pb][xs][part - 1][1];
+                if (xs > 0) {
+                    t62 += xr[pb][xs - 1][part - 1][62];
+                    t63 += xr[pb][xs - 1][part - 1][63];
+                }
+            }
+            put(tot, t62, t63, side + sb.bottom(sd, chunk, Xc, nxc, a.nsd) * nrc, false);
+            if (xs == CW - 1 && right_side) {
+                const T r62 = part > 0 ? xr[pb][xs][part - 1][62] : T(0);
+                const T r63 = part > 0 ? xr[pb][xs][part - 1][63] : T(0);
+                put(Rn, r62, r63, side + sb.corner(sd, wx, chunk, nxc, a.nsd) * nrc, false);
+            }
+        }
+    }
+}
+
+// The side contributions of the one-pass B: node column X = k CW (k >= 1)
+// receives the right sums of workgroup column k - 1, node row Y = k cy the
+// bottom sums of chunk k - 1, and their crossing the corner sums.  The last
+// node column (row) has no owner when CW (cy) divides the coarse cell count:
+// there the side values are the node's whole sum and the apply mode applies.
+// One thread per coarse node value, the radial level fastest.
+template <typename T>
+__global__ __launch_bounds__(256) void div_fixup_kernel(const T *side, T *dst, int add, const DivSide sb, int nsd,
+                                                        int nxc, int nrc)
+{
+    const int ncl = (nxc - 1) / sb.cw, nrl = (nxc - 1) / sb.cy;
+    const int64_t per_sd = (int64_t)(ncl + nrl) * nxc;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= per_sd * nsd * nrc) return;
+    const int k = (int)(t % nrc);
+    const int64_t n = t / nrc;
+    const int sd = (int)(n / per_sd);
+    const int64_t j = n % per_sd;
+    int X, Y;
+    if (j < (int64_t)ncl * nxc) {
+        X = (int)(j / nxc + 1) * sb.cw;
+        Y = (int)(j % nxc);
+    } else {
+        const int64_t i = j - (int64_t)ncl * nxc;
+        Y = (int)(i / nxc + 1) * sb.cy;
+        X = (int)(i % nxc);
+        if (X > 0 && X % sb.cw == 0) return;  // on a column line: done there
+    }
+    // node columns (rows) below xcov (ycov) belong to a workgroup column
+    // (chunk), whose sweep wrote their right (bottom) sums
+    const int xcov = min(sb.nxw * sb.cw, nxc), ycov = min(sb.nchunk * sb.cy, nxc);
+    const bool col = X > 0 && X % sb.cw == 0, row = Y > 0 && Y % sb.cy == 0;
+    T v = T(0);
+    if (col && Y < ycov) v += side[sb.right(sd, X / sb.cw - 1, Y, nxc) * nrc + k];
+    if (row && X < xcov) v += side[sb.bottom(sd, Y / sb.cy - 1, X, nxc, nsd) * nrc + k];
+    if (col && row) v += side[sb.corner(sd, X / sb.cw - 1, Y / sb.cy - 1, nxc, nsd) * nrc + k];
+    const bool owned = X < xcov && Y < ycov;
+    T *p = dst + (((int64_t)sd * nxc + X) * nxc + Y) * nrc + k;
+    *p = (owned || add) ? *p + v : v;
 }
 
 // Phase 2 of B, per coarse cell and coarse wedge: the 6 nodal sums over the
@@ -1142,29 +1246,37 @@ template <typename T> void div_impl(tng_operator &op, const tng_vector &src, tng
     a.bc_cmb = op.bc_cmb; a.bc_surf = op.bc_surf;
     a.cidx = Index4(c.nx - 1, c.nr - 1);
     a.fidx = Index4(f.nx - 1, f.nr - 1);
-    // one pass for 128 radial cells, and for 64 in FP32: workgroups of 3
-    // coarse node columns (4 cell-column waves, times 2 radial parts for 128
-    // cells) and a chunk of coarse rows.  Level 7 FP64: 0.60 ms against 0.63 ms
-    // for the three phases below; level 6 FP64: 0.118 against 0.090 ms, so the
-    // phases stay there.
+    // one pass for 128 radial cells, and for 64 in FP32: workgroups of CW
+    // coarse node columns (times 2 radial parts for 128 cells) and a chunk of
+    // coarse rows, then the side contributions between workgroups
     const int ncr = f.nr - 1;
     if ((ncr == 128 || (ncr == 64 && sizeof(T) == 4)) && f.nalloc < ((int64_t)1 << 31)) {
-        constexpr int CW = 4;
-        const int64_t nxw = (c.nx + CW - 2) / (CW - 1);
-        // rows per workgroup: 8 unless that leaves fewer than 8192 waves (RW x CW
-        // per workgroup); level 7 FP64: 0.403 ms at 8 rows, 0.428 at 4, 0.435 at
-        // 16, 0.51 at 2
+        constexpr int CW = TNG_DIV_CW;
         const int RW = ncr == 128 ? 2 : 1;
-        int cy = 8;
-        while (cy > 2 && f.nsd * nxw * ((c.nx + cy - 1) / cy) * CW * RW < 8192) cy /= 2;
-        const unsigned g = (unsigned)(f.nsd * nxw * ((c.nx + cy - 1) / cy));
+        DivSide sb{};
+        sb.cw = CW;
+        sb.nxw = (c.nx - 1 + CW - 1) / CW;
+        // rows per workgroup: 8 unless that leaves fewer than 8192 waves
+        sb.cy = kDivMaxRows;
+        while (sb.cy > 2 && (int64_t)f.nsd * sb.nxw * ((c.nx - 1 + sb.cy - 1) / sb.cy) * CW * RW < 8192) sb.cy /= 2;
+        sb.nchunk = (c.nx - 1 + sb.cy - 1) / sb.cy;
+        const size_t bytes = (size_t)sb.columns(c.nx, f.nsd) * c.nr * sizeof(T);
+        if (op.scratch.bytes < bytes) op.scratch.alloc(bytes);
+        const unsigned g = (unsigned)(f.nsd * sb.nxw * sb.nchunk);
         const int add = op.apply_mode == TNG_ADD ? 1 : 0;
         if (ncr == 64)
-            hipLaunchKernelGGL((div_column_kernel<T, 1, CW>), dim3(g), dim3(64 * CW), 0, s, a, (T *)dst.comp(0), add, cy);
+            hipLaunchKernelGGL((div_column_kernel<T, 1, CW>), dim3(g), dim3(64 * CW), 0, s, a, (T *)dst.comp(0), add, sb,
+                               op.scratch.as<T>());
         else
-            hipLaunchKernelGGL((div_column_kernel<T, 2, CW>), dim3(g), dim3(128 * CW), 0, s, a, (T *)dst.comp(0), add,
-                               cy);
+            hipLaunchKernelGGL((div_column_kernel<T, 2, CW>), dim3(g), dim3(128 * CW), 0, s, a, (T *)dst.comp(0), add, sb,
+                               op.scratch.as<T>());
         TNG_HIP_CHECK(hipGetLastError());
+        const int64_t nfix = (int64_t)f.nsd * ((c.nx - 1) / CW + (c.nx - 1) / sb.cy) * c.nx * c.nr;
+        if (nfix > 0) {
+            hipLaunchKernelGGL(div_fixup_kernel<T>, dim3(blocks_for(nfix)), dim3(256), 0, s, (const T *)op.scratch.p,
+                               (T *)dst.comp(0), add, sb, f.nsd, c.nx, c.nr);
+            TNG_HIP_CHECK(hipGetLastError());
+        }
         return;
     }
     const size_t bytes = (size_t)(ncell_c * 12 + ncell_f * 2) * sizeof(T);
