@@ -836,33 +836,17 @@ __device__ __forceinline__ void grad_radial(const GradArgs<T> &a, int lane, int 
 }
 
 // B^T at the fine node column (X, Y) (parities PX, PY) from the coarse values
-// pc[i][j][q] of the coarse columns (cx0 + i, cy0 + j), cx0 = floor((X - 1) / 2),
-// cy0 = floor((Y - 1) / 2), levels cb + q of the lane (the clamped columns are
-// never used).
+// pcr[i][j][p] of the coarse columns (cx0 + i, cy0 + j), cx0 = floor((X - 1) / 2),
+// cy0 = floor((Y - 1) / 2), interpolated radially to the centroid of the lane's
+// cell p (the clamped columns are never used).  vh, vt: the lane's radial
+// factors -vol / (2 r_mid) and -vol / (6 h).
 template <typename T, int K, int PX, int PY>
 __device__ __forceinline__ void gradient_node(const GradArgs<T> &a, int lane, int X, int Y, int sd, bool has_cmb,
-                                              bool has_surf, const T (&pc)[3][3][grad_np<K>()], const T (&sfac)[K],
-                                              const T (&tz)[K], const T (&vol)[K])
+                                              bool has_surf, const T (&pcr)[3][3][K], const T (&vh)[K],
+                                              const T (&vt)[K])
 {
     const int nx = a.nx, nr = a.nr, nc = nx - 1, ncr = nr - 1;
     constexpr int corner_of[2][2][2] = {{{0, 2}, {1, -1}}, {{-1, 1}, {2, 0}}};
-    // the coarse values interpolated radially to the centroid of each of the
-    // lane's cells, once per coarse column (the wedges share them)
-    T pcr[3][3][K];
-#pragma unroll
-    for (int p = 0; p < K; ++p) {
-        const int q = K == 1 ? 0 : p / 2;  // coarse level of the cell within the lane
-        const bool top = K == 1 ? (lane & 1) != 0 : (p & 1) != 0;
-        const T wb = top ? T(0.25) : T(0.75), wt = top ? T(0.75) : T(0.25);
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) pcr[i][j][p] = wb * pc[i][j][q] + wt * pc[i][j][q + 1];
-    }
-    // 1 / (2 r_mid): the lateral shape gradients' factor 1/2 (exact) moves here
-    T hsfac[K];
-#pragma unroll
-    for (int p = 0; p < K; ++p) hsfac[p] = T(0.5) * sfac[p];
     T o[K + 1][3];
 #pragma unroll
     for (int p = 0; p <= K; ++p)
@@ -902,9 +886,9 @@ __device__ __forceinline__ void gradient_node(const GradArgs<T> &a, int lane, in
 #pragma unroll
                 for (int p = 0; p < K; ++p) {
                     const T pi = sl0 * pcr[ax][ay][p] + sl1 * pcr[bx][by][p] + sl2 * pcr[dx][dy][p];
-                    const T f = -(g[9] * vol[p]) * pi;
-                    av[p] = f * hsfac[p];
-                    bv[p] = f * tz[p];
+                    const T gp = g[9] * pi;
+                    av[p] = gp * vh[p];
+                    bv[p] = gp * vt[p];
                 }
                 // the cell below the lane's first node level: the lane below's last cell
                 const T ab = wave_shr1_t(av[K - 1]), bb = wave_shr1_t(bv[K - 1]);
@@ -962,8 +946,14 @@ __device__ __forceinline__ void gradient_sweep(const GradArgs<T> &a, int lane, i
     const int cx0 = (X - 1) >> 1, cr0 = (Ya >> 1) - 1;  // window row j = coarse row cr0 + j
     const int cb = K == 1 ? lane / 2 : lane * (K / 2);     // first coarse level of the lane
     const bool has_cmb = a.sd_cmb[sd] != 0, has_surf = a.sd_surf[sd] != 0;
-    T sfac[K], tz[K], vol[K];
+    T sfac[K], tz[K], vol[K], vh[K], vt[K];
     grad_radial<T, K>(a, lane, sd, sfac, tz, vol);
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+        // 1 / (2 r_mid): the lateral shape gradients' factor 1/2 (exact) moves here
+        vh[p] = -(vol[p] * (T(0.5) * sfac[p]));
+        vt[p] = -(vol[p] * tz[p]);
+    }
     T win[NWIN][3][NP];
     auto load_row = [&](int j) {
         const int cy = min(max(cr0 + j, 0), nxc - 1);
@@ -977,6 +967,19 @@ __device__ __forceinline__ void gradient_sweep(const GradArgs<T> &a, int lane, i
     };
 #pragma unroll
     for (int j = 0; j < (NWIN < 4 ? NWIN : 4); ++j) load_row(j);
+    // the window rows interpolated radially to the centroid of each of the
+    // lane's cells, once per row (the nodes and wedges share them)
+    T pw[NWIN][3][K];
+    auto interp_row = [&](int j) {
+#pragma unroll
+        for (int p = 0; p < K; ++p) {
+            const int q = K == 1 ? 0 : p / 2;  // coarse level of the cell within the lane
+            const bool top = K == 1 ? (lane & 1) != 0 : (p & 1) != 0;
+            const T wb = top ? T(0.25) : T(0.75), wt = top ? T(0.75) : T(0.25);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) pw[j][i][p] = wb * win[j][i][q] + wt * win[j][i][q + 1];
+        }
+    };
 #pragma unroll
     for (int k = 0; k < NROW; ++k) {
         // node k uses window rows j0 .. j0 + 2; at odd k the row that node
@@ -985,15 +988,20 @@ __device__ __forceinline__ void gradient_sweep(const GradArgs<T> &a, int lane, i
         if ((k & 1) == 1 && j0 + 3 < NWIN) load_row(j0 + 3);
         const int Y = Ya + k;
         if (Y >= a.nx) break;
-        T pc[3][3][NP];
+        // rows that node k - 1 did not use (it used up to row j0(k - 1) + 2)
+        const int j0prev = k <= 1 ? 0 : (k - 2) / 2 + 1;
+        const int jnew = k == 0 ? 0 : j0prev + 3;
+#pragma unroll
+        for (int j = jnew; j <= j0 + 2; ++j) interp_row(j);
+        T pcr[3][3][K];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int jj = 0; jj < 3; ++jj)
 #pragma unroll
-                for (int q = 0; q < NP; ++q) pc[i][jj][q] = win[j0 + jj][i][q];
-        if (k & 1) gradient_node<T, K, PX, 1>(a, lane, X, Y, sd, has_cmb, has_surf, pc, sfac, tz, vol);
-        else gradient_node<T, K, PX, 0>(a, lane, X, Y, sd, has_cmb, has_surf, pc, sfac, tz, vol);
+                for (int p = 0; p < K; ++p) pcr[i][jj][p] = pw[j0 + jj][i][p];
+        if (k & 1) gradient_node<T, K, PX, 1>(a, lane, X, Y, sd, has_cmb, has_surf, pcr, vh, vt);
+        else gradient_node<T, K, PX, 0>(a, lane, X, Y, sd, has_cmb, has_surf, pcr, vh, vt);
     }
 }
 
