@@ -908,20 +908,45 @@ __device__ __forceinline__ void gradient_node(const GradArgs<T> &a, int lane, in
     const int64_t lcol = ((int64_t)sd * nx + X) * nx + Y;
     const T *cn = a.coords + 3 * lcol;
     T *dst[3] = {a.d0, a.d1, a.d2};
-#pragma unroll
-    for (int p = 0; p <= K; ++p) {
-        if (p == K && lane != 63) continue;
-        const int l = p == K ? ncr : lane * K + p;
-        T v[3] = {o[p][0], o[p][1], o[p][2]};
-        const bool at_cmb = l == 0 && has_cmb, at_surf = l == ncr && has_surf;
-        const int bc = at_cmb ? a.bc_cmb : (at_surf ? a.bc_surf : TNG_NEUMANN);
+    // the test-side boundary rows: Dirichlet removes the node's row, free slip
+    // its normal component
+    auto boundary = [&](int bc, T (&v)[3]) {
         if (bc == TNG_DIRICHLET) {
             v[0] = v[1] = v[2] = T(0);
         } else if (bc == TNG_FREESLIP) {
             const T dd = cn[0] * v[0] + cn[1] * v[1] + cn[2] * v[2];
             v[0] -= dd * cn[0]; v[1] -= dd * cn[1]; v[2] -= dd * cn[2];
         }
-        const int64_t i = lcol * nr + l;
+    };
+    // levels lane K .. lane K + K - 1: one contiguous run per component (the
+    // CMB level is lane 0's first)
+    T run[3][K];
+#pragma unroll
+    for (int p = 0; p < K; ++p)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) run[d][p] = o[p][d];
+    if (lane == 0 && has_cmb) {
+        T v[3] = {run[0][0], run[1][0], run[2][0]};
+        boundary(a.bc_cmb, v);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) run[d][0] = v[d];
+    }
+    const int64_t i0 = lcol * nr + lane * K;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (a.add) {
+            T old[K];
+            __builtin_memcpy(old, dst[d] + i0, sizeof(old));
+#pragma unroll
+            for (int p = 0; p < K; ++p) run[d][p] += old[p];
+        }
+        __builtin_memcpy(dst[d] + i0, run[d], sizeof(run[d]));
+    }
+    // the top level (the surface level of the column): lane 63's last cell
+    if (lane == 63) {
+        T v[3] = {o[K][0], o[K][1], o[K][2]};
+        if (has_surf) boundary(a.bc_surf, v);
+        const int64_t i = lcol * nr + ncr;
 #pragma unroll
         for (int d = 0; d < 3; ++d) dst[d][i] = a.add ? dst[d][i] + v[d] : v[d];
     }
