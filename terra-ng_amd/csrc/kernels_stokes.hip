@@ -339,22 +339,27 @@ __global__ __launch_bounds__(64 * RW * CW) void div_column_kernel(const DivArgs<
     const uint32_t xstride = (uint32_t)nx * nr, ystride = (uint32_t)nr;
     const uint32_t sdbase = (uint32_t)sd * nx * xstride;
     const uint32_t boff = (uint32_t)c * (uint32_t)sizeof(T);
-    // raw values of the node column x at rows y0, y0 + 1, y0 + 2: levels c and
+    // FP32 keeps the sums of a coarse row's last fine row in registers for the
+    // next coarse row, where it is the first (level 7: 0.125 ms against 0.135);
+    // in FP64 the 33 more VGPRs and the copies between coarse rows cost what
+    // the saved loads gain
+    constexpr bool KEEP = sizeof(T) == 4;
+    constexpr int R0 = KEEP ? 1 : 0;  // first fine row loaded per coarse row
+    // raw values of a node column x at rows y0 + R0 .. y0 + 2: levels c and
     // c + 1 of every lane; the next x-column's loads are in flight while one is used
-    T pr[3][3], pt[3][3];
+    T pr[3 - R0][3], pt[3 - R0][3];
     auto issue = [&](int x, int y0) {
         const uint32_t cb = sdbase + (uint32_t)x * xstride + (uint32_t)y0 * ystride;
 #pragma unroll
-        for (int row = 0; row < 3; ++row) {
+        for (int row = R0; row < 3; ++row) {
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 const T *q = (const T *)((const char *)(src[d] + (cb + row * ystride)) + boff);
-                pr[row][d] = q[0];
-                pt[row][d] = q[1];
+                pr[row - R0][d] = q[0];
+                pt[row - R0][d] = q[1];
             }
         }
     };
-    if (cells && Y0 <= nxc - 2) issue(2 * Xc, 2 * Y0);
     // trial-side boundary factors: the bottom value of lane 0 (CMB) and the
     // top value of lane 63 (surface); Dirichlet multiplies by z, free slip
     // removes f (n . u) n
@@ -364,6 +369,57 @@ __global__ __launch_bounds__(64 * RW * CW) void div_column_kernel(const DivArgs<
     const T f_hi = (bct == TNG_FREESLIP && lane == 63) ? T(1) : T(0);
     const T z_lo = (bcb == TNG_DIRICHLET && lane == 0) ? T(0) : T(1);
     const T z_hi = (bct == TNG_DIRICHLET && lane == 63) ? T(0) : T(1);
+    // one node column row (normal-tile entry ni, nj): the boundary rows
+    // applied, then the radial sums U = bottom + top and differences
+    // D = top - bottom of the lane's cell.  The sweep below repeats this
+    // arithmetic inline: called there, the FP64 kernel is allocated 256 VGPRs
+    // and spills (193 inline)
+    auto node_row = [&](T (&lo)[3], T (&hi)[3], int ni, int nj, T (&u)[3], T (&dv)[3]) {
+        if (fs_any) {
+#pragma clang fp contract(off)
+            const T *n = nrm[ni][nj];
+            const T n0 = n[0], n1 = n[1], n2 = n[2];
+            const T dl = f_lo * fma(n2, lo[2], fma(n1, lo[1], n0 * lo[0]));
+            const T dh = f_hi * fma(n2, hi[2], fma(n1, hi[1], n0 * hi[0]));
+            lo[0] = fma(-dl, n0, lo[0]);
+            lo[1] = fma(-dl, n1, lo[1]);
+            lo[2] = fma(-dl, n2, lo[2]);
+            hi[0] = fma(-dh, n0, hi[0]);
+            hi[1] = fma(-dh, n1, hi[1]);
+            hi[2] = fma(-dh, n2, hi[2]);
+        }
+        if (dir_any) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                lo[d] *= z_lo;
+                hi[d] *= z_hi;
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            u[d] = lo[d] + hi[d];
+            dv[d] = hi[d] - lo[d];
+        }
+    };
+    // the last fine row of the previous coarse row, per column of the cell:
+    // the first row of the next (FP32: each fine row is loaded and processed once)
+    T U2[KEEP ? 3 : 1][3], Dv2[KEEP ? 3 : 1][3];
+    if (cells && Y0 <= nxc - 2) {
+        // the chunk's first fine row 2 Y0
+#pragma unroll
+        for (int xi = 0; xi < (KEEP ? 3 : 0); ++xi) {
+            const uint32_t b = sdbase + (uint32_t)(2 * Xc + xi) * xstride + (uint32_t)(2 * Y0) * ystride;
+            T lo[3], hi[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const T *q = (const T *)((const char *)(src[d] + b) + boff);
+                lo[d] = q[0];
+                hi[d] = q[1];
+            }
+            node_row(lo, hi, 2 * xs + xi, 0, U2[xi], Dv2[xi]);
+        }
+        issue(2 * Xc, 2 * Y0);
+    }
     // a node column's radial sums (one per fine cell) as coarse node values:
     // t62, t63 are lanes 62, 63 of the part below; stored to col[0 .. nrc)
     auto put = [&](T tot, T t62, T t63, T *col, bool acc) {
@@ -391,11 +447,19 @@ __global__ __launch_bounds__(64 * RW * CW) void div_column_kernel(const DivArgs<
 #pragma unroll
             for (int xi = 0; xi < 3; ++xi) {
                 const int s = xi & 1;
-                // the column's sums and differences, then the next column's loads
+                // the column's sums and differences (row 0 kept from the
+                // previous coarse row), then the next column's loads
+                if constexpr (KEEP) {
 #pragma unroll
-                for (int row = 0; row < 3; ++row) {
-                    T lo[3] = {pr[row][0], pr[row][1], pr[row][2]};
-                    T hi[3] = {pt[row][0], pt[row][1], pt[row][2]};
+                    for (int d = 0; d < 3; ++d) {
+                        U[s][0][d] = U2[xi][d];
+                        Dv[s][0][d] = Dv2[xi][d];
+                    }
+                }
+#pragma unroll
+                for (int row = R0; row < 3; ++row) {
+                    T lo[3] = {pr[row - R0][0], pr[row - R0][1], pr[row - R0][2]};
+                    T hi[3] = {pt[row - R0][0], pt[row - R0][1], pt[row - R0][2]};
                     if (fs_any) {
 #pragma clang fp contract(off)
                         const T *n = nrm[2 * xs + xi][2 * (Y - Y0) + row];
@@ -420,6 +484,13 @@ __global__ __launch_bounds__(64 * RW * CW) void div_column_kernel(const DivArgs<
                     for (int d = 0; d < 3; ++d) {
                         U[s][row][d] = lo[d] + hi[d];
                         Dv[s][row][d] = hi[d] - lo[d];
+                    }
+                }
+                if constexpr (KEEP) {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        U2[xi][d] = U[s][2][d];
+                        Dv2[xi][d] = Dv[s][2][d];
                     }
                 }
                 if (xi < 2)
