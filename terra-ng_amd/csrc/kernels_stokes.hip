@@ -993,7 +993,9 @@ __device__ __forceinline__ void gradient_node(const GradArgs<T> &a, int lane, in
         }
     }
     const int64_t lcol = ((int64_t)sd * nx + X) * nx + Y;
-    const T *cn = a.coords + 3 * lcol;
+    // the node's coordinates through the scalar cache (a constant address
+    // space: its loads need not wait for the stores of the previous node)
+    const CT *cn = (const CT *)a.coords + 3 * lcol;
     T *dst[3] = {a.d0, a.d1, a.d2};
     // the test-side boundary rows: Dirichlet removes the node's row, free slip
     // its normal component
