@@ -113,6 +113,45 @@ def test_local_matrix_per_element(gpu_ctx):
     assert rel_err(op.get_local_matrix(3, 1, 2, 3, 1), ref[3, 1, 2, 3, 1]) < 1e-13
 
 
+def test_set_boundary_conditions(gpu_ctx):
+    """set_boundary_conditions (epsilon_divdiv_kerngen.hpp:474-481) changes the
+    rows the next apply treats; the rotation penalty stays as armed at
+    creation (an operator created with Neumann rows has none)."""
+    dom = tng.DistributedDomain.create_uniform(3, 3, 0.55, 1.0)
+    om = O.Mesh(3)
+    k = np.exp(0.5 * field(om, 1, 20))
+    u = field(om, 3, 21)
+    op = tng.EpsilonDivDivKerngen(dom, vec(dom, 1, k), (N, N))
+    out = vec(dom, 3)
+    op.apply(vec(dom, 3, u), out)
+    assert rel_err(out.download(), om.epsdivdiv(k, u, N, N, path="fast")) < 1e-12
+    for bcs in [(D, F), (F, D), (D, D), (F, F)]:
+        op.set_boundary_conditions(bcs)
+        op.apply(vec(dom, 3, u), out)
+        assert rel_err(out.download(), om.epsdivdiv(k, u, *bcs, path="fast", penalty_eps=0.0)) < 1e-12
+    with pytest.raises(tng.TngError):
+        op.set_boundary_conditions((N, 7))
+
+
+def test_clear_local_matrices(gpu_ctx):
+    """clear_local_matrices drops the stored matrices: get returns the
+    assembled Neumann matrices again and the apply is matrix-free."""
+    dom = tng.DistributedDomain.create_uniform(2, 2, 0.55, 1.0)
+    om = O.Mesh(2)
+    k = np.exp(field(om, 1, 22))
+    op = tng.EpsilonDivDivKerngen(dom, vec(dom, 1, k), (D, F))
+    ref = O.local_matrices(om, k)
+    op.set_stored_matrix_mode(True)
+    op.set_local_matrix(2, 1, 0, 1, 1, 2.0 * ref[2, 1, 0, 1, 1])
+    assert rel_err(op.get_local_matrix(2, 1, 0, 1, 1), 2.0 * ref[2, 1, 0, 1, 1]) == 0.0
+    op.clear_local_matrices()
+    assert rel_err(op.get_local_matrix(2, 1, 0, 1, 1), ref[2, 1, 0, 1, 1]) < 1e-13
+    u = field(om, 3, 23)
+    out = vec(dom, 3)
+    op.apply(vec(dom, 3, u), out)
+    assert rel_err(out.download(), om.epsdivdiv(k, u, D, F, path="fast")) < 1e-12
+
+
 def test_vector_norm_randomize_device_dot(gpu_ctx):
     dom = tng.DistributedDomain.create_uniform(3, 3, 0.55, 1.0)
     om = O.Mesh(3)

@@ -63,6 +63,24 @@ def test_supg_apply_matches_oracle(L, Lr, ls, rs, flags, gpu_ctx):
     assert rel_err(out2.download(), ref2) < 1e-12
 
 
+def test_supg_set_dt(gpu_ctx):
+    """set_dt (unsteady_advection_diffusion_supg.hpp:206-207): the next apply
+    uses the new time step in the mass/advection-diffusion weighting and the
+    SUPG parameter."""
+    dom = tng.DistributedDomain.create_uniform(3, 3, 0.55, 1.0)
+    om = O.Mesh(3)
+    vel = field(om, 3, 24, 50.0)
+    T = field(om, 1, 25)
+    op = tng.UnsteadyAdvectionDiffusionSUPG(dom, vec(dom, 3, vel), 0.01, 0.02)
+    out = vec(dom, 1)
+    for dt in (0.05, 0.003):
+        op.set_dt(dt)
+        op.apply(vec(dom, 1, T), out)
+        ref = O.supg_apply(om, vel, T, kappa=0.01, dt=dt, mass_scaling=1.0, treat_boundary=True, diagonal=False,
+                           lumped=False, supg=True)
+        assert rel_err(out.download(), ref) < 1e-12
+
+
 def test_scalar_mass(gpu_ctx):
     dom = tng.DistributedDomain.create_uniform(3, 3, 0.55, 1.0, 1, 0)
     om = O.Mesh(3, 3, 0.55, 1.0, 1, 0)
