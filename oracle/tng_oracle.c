@@ -12,6 +12,21 @@
  * expression order); it is compiled with -ffp-contract=off so that no fused
  * multiply-add changes the rounding of the source expressions.
  *
+ * Parity unpinned, two conventions restated from the reference with no
+ * golden output of it to fix them:
+ *  - the centre-triangle convention of B and B^T (orc_divergence_apply,
+ *    orc_gradient_apply: the centre fine triangle of a coarse wedge is
+ *    integrated with the opposite coarse wedge's shape functions,
+ *    kernel_helpers.hpp:600-608, divergence_kerngen.hpp:758-775); the
+ *    transposition, volume and kernel checks pass under either assignment,
+ *    and the brute-force B^T of the tests restates the same reading;
+ *  - the GCA rule for fine nodes on the top face of a coarse cell below the
+ *    top layer (orc_gca, gca.hpp:206-208); it is checked equal to the exact
+ *    Galerkin product where the two coincide (the top layer, single-layer
+ *    shells), and only its difference is checked below.
+ * The other functions are checked against the reference's fixtures, closed
+ * forms or brute force (tests/test_oracle_*).
+ *
  * Layout conventions (identical to the reference's Kokkos views, unpadded):
  *   scalar field  : (sd, x, y, r)            r fastest
  *   vector field  : comp-major SoA, 3 x (sd, x, y, r)
@@ -1178,6 +1193,8 @@ int orc_num_threads(void)
 /* Fine wedge w of fine cell (x, y, r) contributes to coarse wedge w of the  */
 /* coarse cell (x/2, y/2, r/2), with the coarse shape functions of the       */
 /* fine wedge's position (fine_lateral_wedge_idx, kernel_helpers.hpp:600-608)*/
+/* Parity unpinned: the centre fine triangle's assignment to the opposite     */
+/* coarse wedge is restated, not fixed by a golden output (see the header).  */
 /* ------------------------------------------------------------------------- */
 /* integrands.hpp:266-393 shape_rad_coarse / shape_lat_coarse */
 static double shape_rad_coarse(int i, int rad, double zeta)
