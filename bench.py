@@ -43,19 +43,25 @@ def profiled_record(level, bcs, dtype):
     """The committed rocprof counter record of the apply kernel: HBM bytes per
     launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated against a copy
     kernel) and FP64 FLOP per launch (64 lanes x (ADD + MUL + TRANS + 2 FMA)
-    FP64 wave instructions).  Used only while the kernel source is the one that
-    was profiled."""
+    FP64 wave instructions).  Used only while the kernel source, and the build
+    file that sets its compiler flags, are the ones that were profiled."""
     if (level, bcs, dtype) != (6, "freeslip", "f64"):
         return None
     rec_path = os.path.join(ROOT, "profiles", "epsdivdiv_apply_L6_freeslip.traffic.json")
+
+    def sha256_of(rel):
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            return hashlib.sha256(f.read()).hexdigest()
+
     try:
         with open(rec_path) as f:
             rec = json.load(f)
-        with open(os.path.join(ROOT, rec["source"]), "rb") as f:
-            sha = hashlib.sha256(f.read()).hexdigest()
+        same = sha256_of(rec["source"]) == rec["source_sha256"]
+        if "build" in rec:
+            same = same and sha256_of(rec["build"]) == rec["build_sha256"]
     except (OSError, KeyError, ValueError):
         return None
-    return rec if sha == rec["source_sha256"] else None
+    return rec if same else None
 
 
 def solve_leg(args, tng, inputs, np):
